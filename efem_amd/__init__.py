@@ -1,0 +1,330 @@
+"""efem_amd -- MI355X (gfx950) Total-Lagrangian solve loop.
+
+Python binding of the C ABI declared in include/efem_gpu.h (libefem_gpu.so,
+built in-tree by efem_amd/_build.py).  The library is required: importing
+this package without it raises, and every compute call goes to the HIP
+kernels -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libefem_gpu.so"
+
+EFEM_OK = 0
+EFEM_ERR_ARG = -1
+EFEM_ERR_HIP = -2
+EFEM_ERR_NONFINITE = -3
+EFEM_ERR_STATE = -4
+
+NEOHOOKEAN, MOONEY_RIVLIN, YEOH = 0, 1, 2
+FP64, MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_ubyte)
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int), ("X", _dp), ("n_elems", C.c_int), ("nen", C.c_int),
+        ("conn", _ip), ("reduced", C.c_int), ("law", C.c_int), ("params", _dp),
+        ("params_per_elem", C.c_int), ("fixed", _up), ("n_faces", C.c_int), ("fnn", C.c_int),
+        ("faces", _ip), ("pressure", C.c_double), ("pdir", _dp), ("n_fnodes", C.c_int),
+        ("fnodes", _ip), ("fvec", _dp), ("volumes", _dp), ("precision", C.c_int),
+    ]
+
+
+class LbfgsParams(C.Structure):
+    _fields_ = [
+        ("epsilon", C.c_double), ("epsilon_rel", C.c_double), ("max_iterations", C.c_int),
+        ("max_linesearch", C.c_int), ("m", C.c_int), ("lns_fea", C.c_double),
+        ("use_precond", C.c_int), ("line_search", C.c_int),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("r_evals", C.c_int), ("ls_trials", C.c_int),
+        ("precond_failed", C.c_int), ("rinf", C.c_double), ("xinf", C.c_double),
+        ("factor_seconds", C.c_double), ("solve_seconds", C.c_double),
+        ("factor_nnz", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def load_library(path: Path | str | None = None) -> C.CDLL:
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is not built; run `python efem_amd/_build.py` (or __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    vp = C.c_void_p
+    sig = {
+        "efem_create": (C.c_int, [C.POINTER(vp), C.POINTER(Problem)]),
+        "efem_destroy": (C.c_int, [vp]),
+        "efem_last_error": (C.c_char_p, [vp]),
+        "efem_global_error": (C.c_char_p, []),
+        "efem_ndof": (C.c_int, [vp]),
+        "efem_precision": (C.c_int, [vp]),
+        "efem_residual": (C.c_int, [vp, _dp, _dp]),
+        "efem_residual_d": (C.c_int, [vp, vp, vp]),
+        "efem_factor": (C.c_int, [vp, _ip]),
+        "efem_factor_nnz": (C.c_longlong, [vp]),
+        "efem_apply_h0": (C.c_int, [vp, _dp, _dp]),
+        "efem_solve": (C.c_int, [vp, _dp, C.POINTER(LbfgsParams), C.POINTER(SolveStats)]),
+        "efem_trace": (C.c_int, [vp, _dp, C.c_int]),
+        "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
+        "efem_profile_reset": (C.c_int, [vp]),
+        "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
+        "efem_mesh_build": (vp, [C.c_int, _dp, C.c_int]),
+        "efem_mesh_free": (None, [vp]),
+        "efem_mesh_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "efem_mesh_arrays": (C.c_int, [vp, _dp, _ip, _dp, _up, _ip, _dp]),
+        "efem_mesh_problem": (C.c_int, [vp, C.POINTER(Problem), C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+def _ptr(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+class Mesh:
+    """A problem description held as numpy arrays (reference node order).
+
+    Fields mirror efem_problem: X [n,3], conn [E,nen], reduced, law,
+    params [4] or [E,4], fixed [n] (uint8), faces [F,fnn], pressure,
+    optional pdir [3], fnodes / fvec (constant nodal forces), volumes [E].
+    """
+
+    def __init__(self, X, conn, law, params, fixed=None, faces=None, pressure=0.0, reduced=0,
+                 pdir=None, fnodes=None, fvec=None, volumes=None):
+        self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.nen = int(self.conn.shape[1])
+        self.law = int(law)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self.params_per_elem = 1 if p.ndim == 2 else 0
+        self.params = p
+        n = self.X.shape[0]
+        self.fixed = (np.zeros(n, np.uint8) if fixed is None
+                      else np.ascontiguousarray(fixed, dtype=np.uint8))
+        self.faces = (np.zeros((0, 3), np.int32) if faces is None
+                      else np.ascontiguousarray(faces, dtype=np.int32))
+        self.pressure = float(pressure)
+        self.reduced = int(reduced)
+        self.pdir = None if pdir is None else np.ascontiguousarray(pdir, dtype=np.float64)
+        self.fnodes = None if fnodes is None else np.ascontiguousarray(fnodes, dtype=np.int32)
+        self.fvec = None if fvec is None else np.ascontiguousarray(fvec, dtype=np.float64)
+        self.volumes = None if volumes is None else np.ascontiguousarray(volumes, dtype=np.float64)
+
+    @property
+    def n_nodes(self):
+        return self.X.shape[0]
+
+    @property
+    def n_elems(self):
+        return self.conn.shape[0]
+
+    @property
+    def n_free(self):
+        return int((self.fixed == 0).sum())
+
+    @property
+    def ndof(self):
+        return 3 * self.n_free
+
+    @staticmethod
+    def workload(which: int, opts=None) -> "Mesh":
+        """Synthetic workloads generated by the C++ host code
+        (1 cantilever C3D8, 2 beam Kuhn C3D4, 3 cylinder C3D20R,
+        4 bar Kuhn C3D10 with per-element C10, 0 custom box)."""
+        L = lib()
+        o = None if opts is None else np.ascontiguousarray(opts, dtype=np.float64)
+        h = L.efem_mesh_build(which, _ptr(o, C.c_double), 0 if o is None else len(o))
+        if not h:
+            raise RuntimeError("efem_mesh_build failed")
+        try:
+            s = (C.c_longlong * 8)()
+            L.efem_mesh_sizes(h, s)
+            n, E, nen, red, law, ppe, F, fnn = (int(v) for v in s)
+            X = np.empty((n, 3))
+            conn = np.empty((E, nen), np.int32)
+            params = np.empty((E, 4) if ppe else (4,))
+            fixed = np.empty(n, np.uint8)
+            faces = np.empty((F, fnn), np.int32)
+            pr = C.c_double()
+            L.efem_mesh_arrays(h, _ptr(X, C.c_double), _ptr(conn, C.c_int), _ptr(params, C.c_double),
+                               _ptr(fixed, C.c_ubyte), _ptr(faces, C.c_int), C.byref(pr))
+        finally:
+            L.efem_mesh_free(h)
+        return Mesh(X, conn, law, params, fixed, faces, pr.value, red)
+
+    def problem(self, precision: int = FP64) -> Problem:
+        pb = Problem()
+        pb.n_nodes = self.n_nodes
+        pb.X = _ptr(self.X, C.c_double)
+        pb.n_elems = self.n_elems
+        pb.nen = self.nen
+        pb.conn = _ptr(self.conn, C.c_int)
+        pb.reduced = self.reduced
+        pb.law = self.law
+        pb.params = _ptr(self.params, C.c_double)
+        pb.params_per_elem = self.params_per_elem
+        pb.fixed = _ptr(self.fixed, C.c_ubyte)
+        pb.n_faces = int(self.faces.shape[0])
+        pb.fnn = int(self.faces.shape[1]) if self.faces.ndim == 2 else 0
+        pb.faces = _ptr(self.faces, C.c_int) if self.faces.size else None
+        pb.pressure = self.pressure
+        pb.pdir = _ptr(self.pdir, C.c_double)
+        pb.n_fnodes = 0 if self.fnodes is None else len(self.fnodes)
+        pb.fnodes = _ptr(self.fnodes, C.c_int)
+        pb.fvec = _ptr(self.fvec, C.c_double)
+        pb.volumes = _ptr(self.volumes, C.c_double)
+        pb.precision = int(precision)
+        return pb
+
+
+class EfemError(RuntimeError):
+    pass
+
+
+class Context:
+    """One device context (efem_create ... efem_destroy)."""
+
+    def __init__(self, mesh: Mesh, precision: int = FP64):
+        self._lib = lib()
+        self.mesh = mesh
+        self._pb = mesh.problem(precision)
+        h = C.c_void_p()
+        rc = self._lib.efem_create(C.byref(h), C.byref(self._pb))
+        if rc != EFEM_OK:
+            raise EfemError(f"efem_create failed ({rc}): {self._lib.efem_global_error().decode()}")
+        self._h = h
+        self.ndof = self._lib.efem_ndof(h)
+
+    def _check(self, rc, what):
+        if rc != EFEM_OK:
+            raise EfemError(f"{what} failed ({rc}): {self._lib.efem_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.efem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def residual(self, u) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.empty(self.ndof)
+        self._check(self._lib.efem_residual(self._h, _ptr(u, C.c_double), _ptr(r, C.c_double)),
+                    "efem_residual")
+        return r
+
+    def residual_device(self, u_ptr: int, r_ptr: int):
+        self._check(self._lib.efem_residual_d(self._h, C.c_void_p(u_ptr), C.c_void_p(r_ptr)),
+                    "efem_residual_d")
+
+    def factor(self) -> bool:
+        failed = C.c_int(0)
+        self._check(self._lib.efem_factor(self._h, C.byref(failed)), "efem_factor")
+        return failed.value == 0
+
+    @property
+    def factor_nnz(self) -> int:
+        return int(self._lib.efem_factor_nnz(self._h))
+
+    def apply_h0(self, v) -> np.ndarray:
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty(self.ndof)
+        self._check(self._lib.efem_apply_h0(self._h, _ptr(v, C.c_double), _ptr(out, C.c_double)),
+                    "efem_apply_h0")
+        return out
+
+    def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=0, max_linesearch=20,
+              m=6, lns_fea=1.0, use_precond=True):
+        """LBFGSSolver::solve.  epsilon_rel defaults to epsilon, as
+        Integrator.cpp:13-14 sets both from residual_criterion."""
+        u = np.zeros(self.ndof) if u0 is None else np.array(u0, dtype=np.float64)
+        prm = LbfgsParams(float(epsilon), float(epsilon if epsilon_rel is None else epsilon_rel),
+                          int(max_iterations), int(max_linesearch), int(m), float(lns_fea),
+                          1 if use_precond else 0, 0)
+        st = SolveStats()
+        rc = self._lib.efem_solve(self._h, _ptr(u, C.c_double), C.byref(prm), C.byref(st))
+        if rc not in (EFEM_OK, EFEM_ERR_NONFINITE):
+            self._check(rc, "efem_solve")
+        d = st.as_dict()
+        d["status"] = rc
+        return u, d
+
+    def trace(self) -> np.ndarray:
+        n = self._lib.efem_trace(self._h, None, 0)
+        out = np.empty((max(n, 0), 3))
+        if n > 0:
+            self._lib.efem_trace(self._h, _ptr(out, C.c_double), n)
+        return out
+
+    def profile_reset(self):
+        self._check(self._lib.efem_profile_reset(self._h), "efem_profile_reset")
+
+    def profile_read(self):
+        ms = C.c_double()
+        k = C.c_int()
+        self._check(self._lib.efem_profile_read(self._h, C.byref(ms), C.byref(k)), "efem_profile_read")
+        return ms.value, k.value
+
+
+def host_factor_solve(mesh: Mesh, b):
+    """Host-only: x = K^-1 b through the supernodal factor and the device
+    block-inverse algorithm run on the CPU (no GPU needed)."""
+    L = lib()
+    pb = mesh.problem(FP64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty_like(b)
+    info = (C.c_longlong * 5)()
+    rc = L.efem_host_factor_solve(C.byref(pb), _ptr(b, C.c_double), _ptr(x, C.c_double), info)
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_host_factor_solve failed ({rc}): {L.efem_global_error().decode()}")
+    keys = ("nnz", "supernodes", "levels", "flops", "microseconds")
+    return x, dict(zip(keys, (int(v) for v in info)))
+
+
+def free_ids(fixed) -> np.ndarray:
+    """node -> free id (or -1), Constraint.cpp:61-72."""
+    fixed = np.asarray(fixed)
+    fid = -np.ones(len(fixed), np.int64)
+    free = np.flatnonzero(fixed == 0)
+    fid[free] = np.arange(len(free))
+    return fid

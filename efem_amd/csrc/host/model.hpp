@@ -1,0 +1,55 @@
+// Host-side preprocessing of a problem into the arrays the device kernels
+// read.  Stands in for the reference's setup path: Constraint::Constraint
+// (src/fe/Constraint.cpp:22-91), IntegralElement::initial
+// (src/fe/element/IntegralElement.cpp:3-19), ElementConstraint ctor
+// (src/fe/ElementConstraint.cpp:69-83), ExternalPressureFace::initial
+// (src/fe/ExternalConstraint.cpp:255-287) and the constraint configs.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../../include/efem_gpu.h"
+#include "sparse.hpp"
+
+namespace efem {
+
+struct Model {
+    int nNodes = 0, nElems = 0, nen = 0, ng = 0, law = 0, nFree = 0;
+    int reduced = 0;
+    int paramsPerElem = 0;
+    std::vector<double> X;            // [n][3]
+    std::vector<int> conn;            // [E][nen] node ids
+    std::vector<int> fid;             // node -> free id or -1
+    std::vector<int> connDof;         // [nen][E] free id, or nFree for a fixed node
+    std::vector<double> vol;          // [E]
+    std::vector<double> gw;           // [ng] normalised weights
+    std::vector<double> gdN;          // [ng][nen][3] dN/dxi at the Gauss points
+    std::vector<double> ginv;         // [ng][9][E] J0^-1 per Gauss point (SoA)
+    std::vector<double> params;       // [4] or [E][4]
+    std::vector<double> coeff;        // [E] elasticity_coeff
+    // pressure faces
+    int nFaces = 0, fnn = 0, fq = 0;  // fq: Gauss points per face
+    double pressure = 0.0;
+    int directed = 0;
+    double pdir[3] = {0, 0, 0};
+    std::vector<int> faceElem;        // [F]
+    std::vector<double> fA0;          // [F][3]
+    std::vector<double> fWN;          // [F][fq][nen] w_q N_a(xi_q)
+    std::vector<double> fG;           // [F][fq][nen][3] grad_X N_a at xi_q
+    // constant nodal forces per free DOF (ExternalForceConstraint)
+    std::vector<double> fext;         // [3 nFree]
+    // deterministic gather: free node -> slots (e*nen + a, then E*nen + f*nen + a)
+    std::vector<int> gptr, gslot;
+
+    int ndof() const { return 3 * nFree; }
+};
+
+bool buildModel(const efem_problem& pb, Model& m, std::string& msg);
+
+// scalar approximate Jacobian over free nodes (full symmetric CSC)
+void scalarLaplacian(const Model& m, Csc& L);
+
+// free-node coordinates [nFree][3]
+std::vector<double> freeCoords(const Model& m);
+
+}  // namespace efem

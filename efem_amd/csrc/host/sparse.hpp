@@ -1,0 +1,63 @@
+// Sparse Cholesky of the approximate Jacobian for the device preconditioner.
+//
+// The reference factors the 3N_f x 3N_f interleaved matrix
+// K = sum_e sum_q V w c (T^T T) (x) I3 with Eigen's SimplicialLLT + AMD
+// (src/simulation/nlsolver/LBFGS.h:22,65-81, src/fe/ElementConstraint.cpp:
+// 164-205).  Because K = Pi (L (x) I3) Pi^T, this module factors the scalar
+// N_f x N_f Laplacian L once and applies it to the three interleaved
+// components: identical K^-1 up to rounding, 1/9 of the factor storage and
+// 1/27 of the factor flops.
+//
+// Pipeline (host): nested-dissection ordering -> elimination tree and
+// postorder -> column counts -> relaxed supernodes -> supernodal multifrontal
+// LL^T -> per-supernode block B_s = [L11^-1 ; L21 L11^-1] used by the device
+// level-scheduled triangular solves (forward y_s = L11^-1 b'_s and
+// t_s = L21 L11^-1 b'_s; backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace efem {
+
+struct Csc {            // full symmetric, sorted rows, no duplicates
+    int n = 0;
+    std::vector<int> p, i;
+    std::vector<double> x;
+};
+
+struct SupernodalFactor {
+    int n = 0;
+    std::vector<int> perm;       // perm[new] = old
+    int nsuper = 0;
+    std::vector<int> first;      // [nsuper+1] first column of each supernode
+    std::vector<int> rowPtr;     // [nsuper+1] offsets into rows
+    std::vector<int> rows;       // row set of each supernode (its columns first)
+    std::vector<long long> valPtr;  // [nsuper+1] offsets into blocks
+    std::vector<double> L;       // column-major m_s x ns blocks of L
+    std::vector<double> B;       // column-major m_s x ns blocks [L11^-1 ; L21 L11^-1]
+    std::vector<int> parent;     // supernodal elimination tree (-1 root)
+    std::vector<int> level;      // height from the leaves
+    int nlevels = 0;
+    long long nnzL = 0;          // scalar nonzeros of L incl. relaxed zeros
+    double flops = 0;
+    double seconds = 0;
+};
+
+// scalar Laplacian from element gradients (see model.cpp)
+// order: nested dissection on the node graph using coordinates xyz[n][3]
+std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz);
+
+// returns false if a pivot is not positive (the reference's fallback case)
+bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F,
+                        std::string& msg);
+
+// reference-form solve on the host (testing aid): x = A^-1 b, b/x length n
+void factorSolve(const SupernodalFactor& F, const double* b, double* x);
+
+// the device algorithm run on the host with 3 interleaved right-hand sides:
+// forward y_s = L11^-1 b'_s, t_s = (L21 L11^-1) b'_s with b'_s gathered from
+// the descendants' t; backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R
+void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x);
+
+}  // namespace efem

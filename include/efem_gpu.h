@@ -1,0 +1,171 @@
+/*
+ * efem_gpu.h -- C ABI of the MI355X (gfx950) Total-Lagrangian solve loop.
+ *
+ * Drop-in boundary for EFEM's solver concept.  The reference is in-process
+ * C++ (no FFI); each entry point below replaces the call named in its
+ * comment (paths relative to the SeanPei/EFEM source tree):
+ *
+ *   efem_create     Constraint::Constraint + IntegralElement::initial +
+ *                   ElementConstraint ctor + ExternalPressureFace::initial
+ *                   (src/fe/Constraint.cpp:22-91,
+ *                    src/fe/element/IntegralElement.cpp:3-19,
+ *                    src/fe/ElementConstraint.cpp:69-83,
+ *                    src/fe/ExternalConstraint.cpp:249-827)
+ *   efem_residual   SoftObjective::residual (src/simulation/Integrator.cpp:49-55)
+ *                   -> Constraint::getResidual (src/fe/Constraint.cpp:115-126)
+ *   efem_factor     SoftObjective::jacobi + LBFGSSolver::updateApproxJacobi
+ *                   (Integrator.cpp:57-70, src/simulation/nlsolver/LBFGS.h:65-81)
+ *   efem_apply_h0   SimplicialLLT::solve as called at LBFGS.h:130 and
+ *                   LBFGSMat.h:120 (external/Eigen/src/SparseCholesky/
+ *                   SimplicialCholesky.h:155-181)
+ *   efem_solve      LBFGSSolver<SoftObjective>::solve (LBFGS.h:96-174) as
+ *                   called by Integrator::integral (Integrator.cpp:22-42)
+ *
+ * Conventions: plain pointers and sizes; host pointers unless the name ends
+ * in _d (device pointers, ordered on the context's HIP stream).  Vectors on
+ * the free DOFs are interleaved: entry 3*id + c, id = rank of the node among
+ * free (non-fixed) nodes in node order (Constraint.cpp:61-72).  Every call
+ * returns EFEM_OK (0) or a negative code; efem_last_error() gives the text.
+ * No C++ exception crosses this boundary.  One context per host thread.
+ */
+#ifndef EFEM_GPU_H
+#define EFEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFEM_OK 0
+#define EFEM_ERR_ARG (-1)       /* invalid argument / unsupported input */
+#define EFEM_ERR_HIP (-2)       /* HIP runtime error (no device, OOM, ...) */
+#define EFEM_ERR_NONFINITE (-3) /* NaN/Inf in the result (Integrator.cpp:28-34) */
+#define EFEM_ERR_STATE (-4)     /* call out of order */
+
+/* constitutive law (src/material/Material.h:9-16) */
+#define EFEM_NEOHOOKEAN 0
+#define EFEM_MOONEY_RIVLIN 1
+#define EFEM_YEOH 2
+
+/* arithmetic of the element kernels and the preconditioner */
+#define EFEM_FP64 0
+#define EFEM_MIXED 1 /* FP32 elements + factor; FP64 J-1, volumetric term,
+                        reductions, L-BFGS scalars and residual assembly */
+
+/* line-search rule */
+#define EFEM_LS_PROJECTION 0 /* the reference's |r.d| rule (LineSearchFEA.h:37-68) */
+
+/*
+ * Problem description, copied at efem_create (the caller keeps ownership).
+ * Material parameters p[4]: NH {C10,0,0,D1}; MR {C10,C01,0,D1};
+ * Yeoh {C10,C20,C30,D1} (src/material/*.cpp initFromJson).
+ */
+typedef struct efem_problem {
+    int n_nodes;
+    const double* X;            /* [n_nodes][3] reference positions */
+    int n_elems;
+    int nen;                    /* 4 (C3D4), 8 (C3D8), 10 (C3D10), 20 (C3D20) */
+    const int* conn;            /* [n_elems][nen], 0-based, reference node order */
+    int reduced;                /* 0 full integration (the reference), 1 reduced */
+    int law;                    /* EFEM_NEOHOOKEAN / _MOONEY_RIVLIN / _YEOH */
+    const double* params;       /* [4] or [n_elems][4] */
+    int params_per_elem;        /* 1: params is per element */
+    const unsigned char* fixed; /* [n_nodes] 1 = fixed, or NULL */
+    int n_faces;
+    int fnn;                    /* pressure-face node count: 3, 6, 4 or 8 */
+    const int* faces;           /* [n_faces][fnn] */
+    double pressure;            /* follower pressure (> 0), ExternalConstraint.cpp:184-247 */
+    const double* pdir;         /* non-NULL: directed traction [3] (:937-940) */
+    int n_fnodes;
+    const int* fnodes;          /* nodes with a constant force (:132-175) */
+    const double* fvec;         /* [3] */
+    const double* volumes;      /* optional [n_elems] element volumes; NULL: computed */
+    int precision;              /* EFEM_FP64 or EFEM_MIXED */
+} efem_problem;
+
+/* L-BFGS parameters (src/simulation/nlsolver/Param.h:92-103 defaults) */
+typedef struct efem_lbfgs_params {
+    double epsilon;      /* absolute ||r||_inf tolerance (residual_criterion) */
+    double epsilon_rel;  /* relative tolerance, times ||x||_inf */
+    int max_iterations;  /* 0: unlimited */
+    int max_linesearch;  /* 20 */
+    int m;               /* history, 6 in the reference (1..64) */
+    double lns_fea;      /* 1.0 */
+    int use_precond;     /* 1: H0 = K^-1 (nl_solver "lbfgs_ah"), 0: H0 = I/theta */
+    int line_search;     /* EFEM_LS_PROJECTION */
+} efem_lbfgs_params;
+
+typedef struct efem_solve_stats {
+    int iterations;      /* LBFGSSolver::solve return value */
+    int r_evals;         /* SoftObjective::r_evalutes */
+    int ls_trials;
+    int precond_failed;  /* factorisation failed -> H0 = I/theta (LBFGS.h:76-80) */
+    double rinf;         /* final ||r||_inf */
+    double xinf;         /* final ||x||_inf */
+    double factor_seconds;
+    double solve_seconds;
+    long long factor_nnz; /* nnz of the (scalar) Cholesky factor */
+} efem_solve_stats;
+
+typedef struct efem_ctx efem_ctx;
+
+int efem_create(efem_ctx** out, const efem_problem* pb);
+int efem_destroy(efem_ctx* ctx);
+const char* efem_last_error(const efem_ctx* ctx);
+const char* efem_global_error(void); /* error text of a failed efem_create */
+
+int efem_ndof(const efem_ctx* ctx);
+int efem_precision(const efem_ctx* ctx);
+
+/* r(u) = f_int(u) - f_ext(u) on the free DOFs (host pointers, synchronous) */
+int efem_residual(efem_ctx* ctx, const double* u, double* r);
+/* device-pointer variant, ordered on the context stream (not synchronised) */
+int efem_residual_d(efem_ctx* ctx, const double* u_d, double* r_d);
+
+/* assemble the approximate Jacobian, order and factor it on the host,
+ * upload the factor.  Returns EFEM_OK; *failed = 1 if not SPD (fallback). */
+int efem_factor(efem_ctx* ctx, int* failed);
+long long efem_factor_nnz(const efem_ctx* ctx);
+/* out = K^-1 v (host pointers); requires a successful efem_factor */
+int efem_apply_h0(efem_ctx* ctx, const double* v, double* out);
+
+/* L-BFGS from u (in/out, host pointer).  Factors first when
+ * prm->use_precond and no factor is present. */
+int efem_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
+               efem_solve_stats* stats);
+
+/* host-only diagnostic (no GPU): assemble, order and factor the approximate
+ * Jacobian on the host, then run the device's block-inverse triangular-solve
+ * algorithm on the host: x = K^-1 b (interleaved).  info (may be NULL) gets
+ * {nnz(L), supernodes, levels, factor flops, factor microseconds}. */
+int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
+                           long long* info5);
+
+/* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */
+int efem_trace(const efem_ctx* ctx, double* out, int cap);
+
+/* kernel timing of the element-force kernel over the last efem_residual
+ * calls made after efem_profile_reset (HIP events on the context stream) */
+int efem_profile_reset(efem_ctx* ctx);
+int efem_profile_read(efem_ctx* ctx, double* elem_ms_total, int* elem_launches);
+
+/* ---- synthetic workloads (host only, no GPU needed) --------------------
+ * which: 1 cantilever 20x4x4 C3D8 NH; 2 beam 200x20x20 Kuhn C3D4 MR;
+ *        3 hollow cylinder 48x4x40 C3D20R Yeoh; 4 bar 64x16x12 Kuhn C3D10
+ *        NH per-element; 0 custom box (dims from opts).
+ * opts (may be NULL) = {nx, ny, nz, seed, ...} overrides, see efem_mesh_build. */
+typedef struct efem_mesh efem_mesh;
+efem_mesh* efem_mesh_build(int which, const double* opts, int n_opts);
+void efem_mesh_free(efem_mesh* m);
+/* sizes: {n_nodes, n_elems, nen, reduced, law, params_per_elem, n_faces, fnn} */
+int efem_mesh_sizes(const efem_mesh* m, long long* sizes8);
+/* copies: X [n_nodes*3], conn [n_elems*nen], params [4 or n_elems*4],
+ * fixed [n_nodes], faces [n_faces*fnn], scalars {pressure} */
+int efem_mesh_arrays(const efem_mesh* m, double* X, int* conn, double* params,
+                     unsigned char* fixed, int* faces, double* pressure);
+/* fill an efem_problem that points into the mesh (valid while m lives) */
+int efem_mesh_problem(const efem_mesh* m, efem_problem* pb, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

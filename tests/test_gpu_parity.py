@@ -1,0 +1,171 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the
+same seeded inputs.
+
+Tolerances: FP64 residual <= 1e-10 relative (L2) to the oracle, the
+north-star bound for the internal-force vector; mixed precision residual
+<= 2e-6 relative; H0 apply <= 1e-10 (FP64) / 1e-4 (mixed, FP32 factor);
+solves: same iteration counts within 2 and displacements within 1e-6.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import efem_amd as E
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (kind, law, reduced)
+    (8, 0, 0), (8, 1, 1), (4, 1, 0), (4, 2, 0), (10, 0, 0), (10, 2, 1), (20, 2, 1), (20, 1, 0),
+]
+
+
+def small(kind, law, reduced, p=800.0, jitter=0.05):
+    return E.Mesh.workload(0, [5, 3, 2, 0.005, 0.003, 0.002, kind, law, p, jitter if kind in (4, 10) else 0.0,
+                               11, reduced])
+
+
+def smooth_u(m, rng, amp=0.02):
+    # smooth bending + stretching field plus small noise, zero on fixed nodes
+    X = m.X
+    L = X.max(0) - X.min(0)
+    s = X[:, 0] / L[0]
+    u = np.stack([amp * L[0] * (0.3 * s + 0.1 * np.sin(3 * s)),
+                  amp * L[0] * 0.2 * s ** 2,
+                  -amp * L[0] * s ** 2], axis=1)
+    u += amp * 0.02 * L.min() * rng.standard_normal(u.shape)
+    return u[m.fixed == 0].ravel()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("kind,law,reduced", CASES)
+def test_residual_fp64_matches_oracle(kind, law, reduced, rng):
+    m = small(kind, law, reduced)
+    print(f"case kind={kind} law={law} reduced={reduced} ndof={m.ndof}", flush=True)
+    om = O.Model(m)
+    with E.Context(m, E.FP64) as ctx:
+        for amp in (0.0, 0.01, 0.2):
+            u = smooth_u(m, rng, amp)
+            r_or = om.residual(u)
+            r_gpu = ctx.residual(u)
+            assert rel(r_gpu, r_or) <= 1e-10, (amp, rel(r_gpu, r_or))
+
+
+@pytest.mark.parametrize("kind,law,reduced", CASES)
+def test_residual_mixed_matches_oracle(kind, law, reduced, rng):
+    m = small(kind, law, reduced)
+    om = O.Model(m)
+    with E.Context(m, E.MIXED) as ctx:
+        for amp in (0.01, 0.2):
+            u = smooth_u(m, rng, amp)
+            r_or = om.residual(u)
+            r_gpu = ctx.residual(u)
+            # error relative to the internal-force scale (the residual itself
+            # is a small difference of large element forces)
+            scale = np.linalg.norm(om.internal_force(u)) + np.linalg.norm(om.external_force(u))
+            assert np.linalg.norm(r_gpu - r_or) <= 2e-6 * scale
+
+
+@pytest.mark.parametrize("which,opts", [(1, None), (2, [40, 8, 8]), (3, [24, 2, 8]), (4, [16, 4, 4])])
+def test_residual_workloads_fp64(which, opts, rng):
+    m = E.Mesh.workload(which, opts)
+    print(f"workload {which} ndof={m.ndof}", flush=True)
+    om = O.Model(m)
+    with E.Context(m, E.FP64) as ctx:
+        u = smooth_u(m, rng, 0.05)
+        assert rel(ctx.residual(u), om.residual(u)) <= 1e-10
+
+
+def test_residual_full_beam_fp64_and_mixed(rng):
+    m = E.Mesh.workload(2)
+    om = O.Model(m)
+    u = smooth_u(m, rng, 0.05)
+    r_or = om.residual(u)
+    with E.Context(m, E.FP64) as ctx:
+        assert rel(ctx.residual(u), r_or) <= 1e-10
+    with E.Context(m, E.MIXED) as ctx:
+        scale = np.linalg.norm(om.internal_force(u))
+        assert np.linalg.norm(ctx.residual(u) - r_or) <= 2e-6 * scale
+
+
+def test_residual_directed_traction_and_nodal_force(rng):
+    m0 = small(8, 0, 0)
+    top = np.unique(m0.faces.ravel())
+    m = E.Mesh(m0.X, m0.conn, m0.law, m0.params, m0.fixed, m0.faces, 0.0, 0,
+               pdir=np.array([0.0, 4500.0, 0.0]), fnodes=top[:5], fvec=np.array([0.1, -0.2, 0.3]))
+    om = O.Model(m)
+    with E.Context(m) as ctx:
+        u = smooth_u(m, rng, 0.05)
+        assert rel(ctx.residual(u), om.residual(u)) <= 1e-10
+
+
+def test_residual_is_deterministic(rng):
+    m = E.Mesh.workload(2, [40, 8, 8])
+    u = smooth_u(m, rng, 0.05)
+    for prec in (E.FP64, E.MIXED):
+        with E.Context(m, prec) as ctx:
+            a = ctx.residual(u)
+            b = ctx.residual(u)
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
+def test_apply_h0_matches_direct_solve(prec, tol, rng):
+    m = E.Mesh.workload(2, [30, 6, 6])
+    K = O.Model(m).K(scalar=False)
+    b = rng.standard_normal(m.ndof)
+    x_ref = spla.spsolve(K.tocsc(), b)
+    with E.Context(m, prec) as ctx:
+        assert ctx.factor()
+        assert rel(ctx.apply_h0(b), x_ref) <= tol
+
+
+def test_solve_matches_oracle_small_load():
+    # a load the reference algorithm converges on
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    om = O.Model(m)
+    u_or, st_or = om.solve(epsilon=1e-8, max_iterations=3000)
+    with E.Context(m, E.FP64) as ctx:
+        u, st = ctx.solve(epsilon=1e-8, max_iterations=3000)
+    print("oracle", st_or, "gpu", st, flush=True)
+    assert st["status"] == 0 and st["precond_failed"] == 0
+    assert st["rinf"] <= 1e-8
+    assert abs(st["iterations"] - st_or["iterations"]) <= max(2, 0.02 * st_or["iterations"])
+    assert rel(u, u_or) <= 1e-6
+
+
+def test_solve_identical_trajectory_first_iterations():
+    # FP64 GPU vs oracle: the first iterations follow the same path
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    om = O.Model(m)
+    u_or, st_or = om.solve(epsilon=0.0, max_iterations=10, trace_cap=10)
+    with E.Context(m, E.FP64) as ctx:
+        u, st = ctx.solve(epsilon=0.0, max_iterations=10)
+        tr = ctx.trace()
+    assert st["iterations"] == 10 and st_or["iterations"] == 10
+    assert np.allclose(tr[:, 0], st_or["trace"][:, 0], rtol=1e-6)
+    assert np.array_equal(tr[:, 2], st_or["trace"][:, 2])
+    assert rel(u, u_or) <= 1e-8
+
+
+def test_solve_mixed_reaches_fp64_solution():
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    with E.Context(m, E.FP64) as ctx:
+        u64, st64 = ctx.solve(epsilon=1e-10, max_iterations=5000)
+    with E.Context(m, E.MIXED) as ctx:
+        u32, st32 = ctx.solve(epsilon=1e-8, max_iterations=5000)
+    print("fp64", st64, "mixed", st32, flush=True)
+    assert rel(u32, u64) <= 1e-5
+
+
+def test_solve_without_preconditioner_and_fallback():
+    m = E.Mesh.workload(1, [10, 2, 2, 0, 100.0])
+    with E.Context(m) as ctx:
+        u, st = ctx.solve(epsilon=1e-6, max_iterations=50, use_precond=False)
+        assert st["precond_failed"] == 0 and st["iterations"] == 50
+    om = O.Model(m)
+    u_or, st_or = om.solve(epsilon=1e-6, max_iterations=50, use_precond=False)
+    assert rel(u, u_or) <= 1e-8

@@ -1,0 +1,162 @@
+"""Pins the oracle's element library (shape functions, Gauss rules, element
+volume) by properties the mathematics fixes -- not by retyping formulas.
+
+Reference: src/fe/element/{TetrahedraElement,HexahedraElement}.cpp,
+IntegralElement.inl, src/model/Element.h:25-89.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+# nodal parametric coordinates implied by the reference's shape functions;
+# the Kronecker-delta test below checks the oracle against these positions
+TET4_NODES = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], float)
+TET10_NODES = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1],
+                        [.5, 0, 0], [.5, .5, 0], [0, .5, 0], [0, 0, .5], [.5, 0, .5], [0, .5, .5]])
+HEX8_NODES = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                       [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+# C3D20 in the reference's own parametrisation: node 0 at (1,-1,1)
+HEX20_NODES = np.array([[1, -1, 1], [1, 1, 1], [1, 1, -1], [1, -1, -1],
+                        [-1, -1, 1], [-1, 1, 1], [-1, 1, -1], [-1, -1, -1],
+                        [1, 0, 1], [1, 1, 0], [1, 0, -1], [1, -1, 0],
+                        [-1, 0, 1], [-1, 1, 0], [-1, 0, -1], [-1, -1, 0],
+                        [0, -1, 1], [0, 1, 1], [0, 1, -1], [0, -1, -1]], float)
+NODES = {4: TET4_NODES, 10: TET10_NODES, 8: HEX8_NODES, 20: HEX20_NODES}
+
+
+def _interior(nen, rng, k=10):
+    if nen in (4, 10):
+        p = rng.dirichlet(np.ones(4), size=k)
+        return p[:, :3]
+    return rng.uniform(-0.9, 0.9, size=(k, 3))
+
+
+@pytest.mark.parametrize("nen", [4, 8, 10, 20])
+def test_kronecker_delta_at_nodes(nen):
+    for a, xi in enumerate(NODES[nen]):
+        N, _ = O.shape(nen, xi)
+        e = np.zeros(nen)
+        e[a] = 1
+        assert np.allclose(N, e, atol=1e-15), (nen, a, N)
+
+
+@pytest.mark.parametrize("nen", [4, 8, 10, 20])
+def test_partition_of_unity_and_gradient_sum(nen, rng):
+    for xi in _interior(nen, rng):
+        N, dN = O.shape(nen, xi)
+        assert abs(N.sum() - 1) < 1e-14
+        assert np.abs(dN.sum(axis=0)).max() < 1e-14
+
+
+@pytest.mark.parametrize("nen", [4, 8, 10, 20])
+def test_gradients_match_finite_differences(nen, rng):
+    h = 1e-6
+    for xi in _interior(nen, rng, 5):
+        _, dN = O.shape(nen, xi)
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            Np, _ = O.shape(nen, xi + e)
+            Nm, _ = O.shape(nen, xi - e)
+            assert np.allclose((Np - Nm) / (2 * h), dN[:, d], atol=1e-8)
+
+
+@pytest.mark.parametrize("nen", [4, 8, 10, 20])
+def test_linear_completeness(nen, rng):
+    # sum_a N_a x_a reproduces any linear field, so the isoparametric map of
+    # an affine element is affine and sum_a x_a (x) dN_a = the affine matrix
+    A = rng.standard_normal((3, 3))
+    b = rng.standard_normal(3)
+    Xn = NODES[nen] @ A.T + b
+    for xi in _interior(nen, rng, 5):
+        N, dN = O.shape(nen, xi)
+        assert np.allclose(N @ Xn, A @ xi + b, atol=1e-13)
+        assert np.allclose(Xn.T @ dN, A, atol=1e-13)
+
+
+def test_quadratic_completeness():
+    # C3D10 and C3D20 reproduce every quadratic monomial the space contains
+    rng = np.random.default_rng(3)
+    for nen, monos in ((10, [(2, 0, 0), (1, 1, 0), (0, 1, 1), (0, 0, 2)]),
+                       (20, [(2, 0, 0), (1, 1, 0), (1, 1, 1), (2, 1, 0), (0, 2, 1)])):
+        for mono in monos:
+            f = lambda p: np.prod(p ** np.array(mono), axis=-1)
+            vals = f(NODES[nen])
+            for xi in _interior(nen, rng, 4):
+                N, _ = O.shape(nen, xi)
+                assert abs(N @ vals - f(xi)) < 1e-13
+
+
+def _monomial_integral_hex(e):
+    return np.prod([(1 - (-1) ** (k + 1)) / (k + 1) for k in e]) / 8.0
+
+
+def _monomial_integral_tet(e):
+    # int over the unit tet of x^a y^b z^c = a! b! c! / (a+b+c+3)!, normalised by volume 1/6
+    from math import factorial
+    a, b, c = e
+    return 6.0 * factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+
+
+@pytest.mark.parametrize("nen,reduced,degree", [(8, 0, 3), (8, 1, 1), (20, 0, 5), (20, 1, 3),
+                                                 (4, 0, 1), (10, 1, 1)])
+def test_gauss_rules_integrate_monomials(nen, reduced, degree):
+    pts, w = O.gauss(nen, reduced)
+    assert abs(w.sum() - 1) < 1e-15
+    exact = _monomial_integral_tet if nen in (4, 10) else _monomial_integral_hex
+    for e in itertools.product(range(degree + 1), repeat=3):
+        if sum(e) > degree and nen in (4, 10):
+            continue
+        if max(e) > degree:
+            continue
+        val = (w * np.prod(pts ** np.array(e), axis=1)).sum()
+        assert abs(val - exact(e)) < 1e-14, (nen, reduced, e)
+
+
+def test_tet4pt_rule_uses_truncated_constants():
+    # the reference hard-codes 0.58541020 / 0.13819660 (IntegralElement.inl:41-47)
+    pts, w = O.gauss(10, 0)
+    assert np.allclose(w, 0.25)
+    assert set(np.unique(pts)) == {0.58541020, 0.13819660}
+    # so the rule is exact for degree 2 only to ~1e-8, not to rounding
+    val = (w * pts[:, 0] ** 2).sum()
+    exact = _monomial_integral_tet((2, 0, 0))
+    assert 1e-10 < abs(val - exact) < 1e-7
+
+
+def test_hex_rules_orders_and_weights():
+    pts, w = O.gauss(8, 0)
+    g = 1.0 / np.sqrt(3.0)
+    assert np.allclose(pts, HEX8_NODES * g, rtol=0, atol=0)
+    pts, w = O.gauss(20, 0)
+    assert np.allclose(np.sort(np.unique(np.round(w * 5832))), [125, 200, 320, 512])
+    assert np.allclose(pts[1], [0, -np.sqrt(0.6), -np.sqrt(0.6)])  # xi varies fastest
+
+
+@pytest.mark.parametrize("nen", [4, 8, 10, 20])
+def test_volume_of_affine_elements_equals_jacobian(nen, rng):
+    for _ in range(5):
+        A = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+        if np.linalg.det(A) < 0.2:
+            continue
+        Xe = NODES[nen] @ A.T + rng.standard_normal(3)
+        ref_vol = (1 / 6 if nen in (4, 10) else 8.0) * abs(np.linalg.det(A))
+        assert abs(O.element_volume(nen, Xe) - ref_vol) < 1e-13 * max(1, ref_vol)
+
+
+def test_volume_of_distorted_hex_is_polyhedral():
+    # a hex with a warped (non-planar) face: the tessellated-surface volume
+    # equals the sum of the volumes of the tets of the face-triangle fan
+    X = HEX8_NODES.copy()
+    X[6] += [0.0, 0.0, 0.4]
+    v = O.element_volume(8, X)
+    c = X.mean(axis=0)
+    faces = [(0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2), (0, 3, 2, 1), (4, 5, 6, 7)]
+    tot = 0.0
+    for a, b, cc, d in faces:
+        for t in ((a, b, cc), (a, cc, d)):
+            tot += abs(np.linalg.det(np.array([X[t[0]] - c, X[t[1]] - c, X[t[2]] - c]))) / 6
+    assert abs(v - tot) < 1e-12
