@@ -302,6 +302,10 @@ int factorInternal(efem_ctx* c, int* failed)
     int nt = 0;
     for (int s = 0; s < F.nsuper; ++s) {
         w[s] = F.first[s + 1] - F.first[s];
+        if (w[s] > 256) {
+            c->err = "internal: supernode wider than the device staging buffer";
+            return EFEM_ERR_STATE;
+        }
         mm[s] = F.rowPtr[s + 1] - F.rowPtr[s];
         rowOff[s] = F.rowPtr[s];
         valOff[s] = F.valPtr[s];

@@ -247,13 +247,16 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     std::vector<int> nchild(n, 0);
     for (int j = 0; j < n; ++j)
         if (parent[j] != -1) nchild[parent[j]]++;
+    // (no supernode is wider than kMaxWidth: the device stages one
+    // supernode's right-hand sides in a kMaxWidth-wide LDS buffer)
+    const int kMaxWidth = 256;
     std::vector<int> fstart;
     for (int j = 0; j < n; ++j) {
-        const bool cont = j > 0 && parent[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1;
+        const bool cont = j > 0 && parent[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1 &&
+                          j - fstart.back() < kMaxWidth;
         if (!cont) fstart.push_back(j);
     }
     fstart.push_back(n);
-    const int kMaxWidth = 256;
     std::vector<int> first;
     {
         int g0 = fstart[0];
