@@ -44,7 +44,8 @@ class LbfgsParams(C.Structure):
     _fields_ = [
         ("epsilon", C.c_double), ("epsilon_rel", C.c_double), ("max_iterations", C.c_int),
         ("max_linesearch", C.c_int), ("m", C.c_int), ("lns_fea", C.c_double),
-        ("use_precond", C.c_int), ("line_search", C.c_int),
+        ("use_precond", C.c_int), ("line_search", C.c_int), ("x_target", _dp),
+        ("target_tol", C.c_double),
     ]
 
 
@@ -53,7 +54,7 @@ class SolveStats(C.Structure):
         ("iterations", C.c_int), ("r_evals", C.c_int), ("ls_trials", C.c_int),
         ("precond_failed", C.c_int), ("rinf", C.c_double), ("xinf", C.c_double),
         ("factor_seconds", C.c_double), ("solve_seconds", C.c_double),
-        ("factor_nnz", C.c_longlong),
+        ("factor_nnz", C.c_longlong), ("target_reached", C.c_int), ("target_err", C.c_double),
     ]
 
     def as_dict(self):
@@ -78,6 +79,8 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_residual_d": (C.c_int, [vp, vp, vp]),
         "efem_factor": (C.c_int, [vp, _ip]),
         "efem_factor_nnz": (C.c_longlong, [vp]),
+        "efem_factor_release": (C.c_int, [vp]),
+        "efem_profile_read_h0": (C.c_int, [vp, _dp, _ip]),
         "efem_apply_h0": (C.c_int, [vp, _dp, _dp]),
         "efem_solve": (C.c_int, [vp, _dp, C.POINTER(LbfgsParams), C.POINTER(SolveStats)]),
         "efem_trace": (C.c_int, [vp, _dp, C.c_int]),
@@ -274,13 +277,15 @@ class Context:
         return out
 
     def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=0, max_linesearch=20,
-              m=6, lns_fea=1.0, use_precond=True):
+              m=6, lns_fea=1.0, use_precond=True, x_target=None, target_tol=0.0):
         """LBFGSSolver::solve.  epsilon_rel defaults to epsilon, as
-        Integrator.cpp:13-14 sets both from residual_criterion."""
+        Integrator.cpp:13-14 sets both from residual_criterion.  x_target /
+        target_tol: also stop once ||x - x_target|| <= tol ||x_target||."""
         u = np.zeros(self.ndof) if u0 is None else np.array(u0, dtype=np.float64)
+        xt = None if x_target is None else np.ascontiguousarray(x_target, dtype=np.float64)
         prm = LbfgsParams(float(epsilon), float(epsilon if epsilon_rel is None else epsilon_rel),
                           int(max_iterations), int(max_linesearch), int(m), float(lns_fea),
-                          1 if use_precond else 0, 0)
+                          1 if use_precond else 0, 0, _ptr(xt, C.c_double), float(target_tol))
         st = SolveStats()
         rc = self._lib.efem_solve(self._h, _ptr(u, C.c_double), C.byref(prm), C.byref(st))
         if rc not in (EFEM_OK, EFEM_ERR_NONFINITE):
@@ -304,6 +309,15 @@ class Context:
         k = C.c_int()
         self._check(self._lib.efem_profile_read(self._h, C.byref(ms), C.byref(k)), "efem_profile_read")
         return ms.value, k.value
+
+    def profile_read_h0(self):
+        ms = C.c_double()
+        k = C.c_int()
+        self._check(self._lib.efem_profile_read_h0(self._h, C.byref(ms), C.byref(k)), "efem_profile_read_h0")
+        return ms.value, k.value
+
+    def factor_release(self):
+        self._check(self._lib.efem_factor_release(self._h), "efem_factor_release")
 
 
 def host_factor_solve(mesh: Mesh, b):

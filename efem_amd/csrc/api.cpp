@@ -68,10 +68,11 @@ struct efem_ctx {
     bool factorFailed = false;
     double factorSeconds = 0.0;
     std::vector<double> trace;
-    // profiling of the element kernel
+    // profiling of the element kernel and of H0 applies
     bool profiling = false;
-    std::vector<hipEvent_t> evs;
-    int evUsed = 0;
+    std::vector<hipEvent_t> evs, evsH0;
+    int evUsed = 0, evH0Used = 0;
+    double* xtarget = nullptr;
 };
 
 namespace {
@@ -265,11 +266,17 @@ int applyH0Internal(efem_ctx* c, const double* v, double* out)
 {
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
+    const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
+    if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
     for (int l = 0; l < f.nlevels; ++l)
         efem::dev::launchTrsvForward(a, v, f.items + f.fwdOff[l], f.fwdCnt[l], c->stream);
     for (int l = f.nlevels - 1; l >= 0; --l)
         efem::dev::launchTrsvBackward(a, f.items + f.bwdOff[l], f.bwdCnt[l], c->stream);
     efem::dev::launchUnpermute(a, out, c->stream);
+    if (prof) {
+        HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used + 1], c->stream));
+        c->evH0Used += 2;
+    }
     HIP_CHECK(c, hipGetLastError());
     return EFEM_OK;
 }
@@ -422,11 +429,12 @@ int efem_create(efem_ctx** out, const efem_problem* pb)
 int efem_destroy(efem_ctx* c)
 {
     if (!c) return EFEM_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* p : c->allocs) hipFree(p);
-    for (hipEvent_t e : c->evs) hipEventDestroy(e);
-    if (c->hostScal) hipHostFree(c->hostScal);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->evsH0) (void)hipEventDestroy(e);
+    if (c->hostScal) (void)hipHostFree(c->hostScal);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return EFEM_OK;
 }
@@ -463,6 +471,15 @@ int efem_factor(efem_ctx* c, int* failed)
 }
 
 long long efem_factor_nnz(const efem_ctx* c) { return c && c->fac.ready ? c->fac.nnz : 0; }
+
+int efem_factor_release(efem_ctx* c)
+{
+    if (!c) return EFEM_ERR_ARG;
+    // device buffers of the factor stay in c->allocs until efem_destroy;
+    // dropping `ready` makes the next solve factor again
+    c->fac.ready = false;
+    return EFEM_OK;
+}
 
 int efem_apply_h0(efem_ctx* c, const double* v, double* out)
 {
@@ -503,14 +520,20 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         c->mcap = mc;
     }
     c->trace.clear();
-    HIP_CHECK(c, hipMemcpyAsync(c->x, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     double* sc = c->scal;
+    if (prm->x_target) {
+        if (!c->xtarget && (rc = devAlloc(c, &c->xtarget, static_cast<size_t>(n) + 3))) return rc;
+        HIP_CHECK(c, hipMemcpyAsync(c->xtarget, prm->x_target, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        efem::dev::launchDot2(c->xtarget, c->xtarget, n, c->work, c->counter, sc + kOut + 4, sc + kOut + 5, c->stream);
+    }
+    HIP_CHECK(c, hipMemcpyAsync(c->x, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = residualInternal(c, c->x, c->r))) return rc;
     s.r_evals = 1;
     efem::dev::launchDotMax(c->x, c->x, c->r, n, c->work, c->counter, sc + kOut, c->stream);
-    if ((rc = readScalars(c, 3))) return rc;
+    if ((rc = readScalars(c, 6))) return rc;
     double rinf = c->hostScal[2], xinf = c->hostScal[1];
     const double xnorm2 = std::sqrt(c->hostScal[0]);
+    const double tnorm = std::sqrt(c->hostScal[5]);
     s.rinf = rinf;
     s.xinf = xinf;
     if (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xnorm2) {
@@ -565,6 +588,18 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         s.rinf = rinf;
         s.xinf = xinf;
         if (!std::isfinite(rinf) || !std::isfinite(xinf)) break;
+        if (prm->x_target) {
+            // ||x - x_target||: diff into tmp, then its norm
+            efem::dev::launchDiff2(c->tmp, c->tmp, c->x, c->xtarget, c->x, c->xtarget, n, c->stream);
+            efem::dev::launchDot2(c->tmp, c->tmp, n, c->work, c->counter, sc + kOut + 6, sc + kOut + 7, c->stream);
+            HIP_CHECK(c, hipMemcpyAsync(c->hostScal + 6, sc + kOut + 6, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            s.target_err = std::sqrt(c->hostScal[6]) / tnorm;
+            if (s.target_err <= prm->target_tol) {
+                s.target_reached = 1;
+                break;
+            }
+        }
         if (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xinf) break;
         if (prm->max_iterations != 0 && k >= prm->max_iterations) break;
         // add_correction (LBFGSMat.h:81-98)
@@ -671,9 +706,27 @@ int efem_profile_reset(efem_ctx* c)
     if (c->evs.empty()) {
         c->evs.resize(8192);
         for (auto& e : c->evs) HIP_CHECK(c, hipEventCreate(&e));
+        c->evsH0.resize(8192);
+        for (auto& e : c->evsH0) HIP_CHECK(c, hipEventCreate(&e));
     }
     c->evUsed = 0;
+    c->evH0Used = 0;
     c->profiling = true;
+    return EFEM_OK;
+}
+
+int efem_profile_read_h0(efem_ctx* c, double* ms, int* applies)
+{
+    if (!c) return EFEM_ERR_ARG;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int i = 0; i + 1 < c->evH0Used; i += 2) {
+        float t = 0.f;
+        HIP_CHECK(c, hipEventElapsedTime(&t, c->evsH0[i], c->evsH0[i + 1]));
+        tot += t;
+    }
+    if (ms) *ms = tot;
+    if (applies) *applies = c->evH0Used / 2;
     return EFEM_OK;
 }
 
@@ -689,7 +742,6 @@ int efem_profile_read(efem_ctx* c, double* ms, int* launches)
     }
     if (ms) *ms = tot;
     if (launches) *launches = c->evUsed / 2;
-    c->profiling = false;
     return EFEM_OK;
 }
 

@@ -57,7 +57,8 @@ extern "C" {
 /*
  * Problem description, copied at efem_create (the caller keeps ownership).
  * Material parameters p[4]: NH {C10,0,0,D1}; MR {C10,C01,0,D1};
- * Yeoh {C10,C20,C30,D1} (src/material/*.cpp initFromJson).
+ * Yeoh {C10,C20,C30,D1} (initFromJson of src/material/NeoHookean.cpp,
+ * MooneyRivlin.cpp, Yeoh.cpp).
  */
 typedef struct efem_problem {
     int n_nodes;
@@ -92,6 +93,12 @@ typedef struct efem_lbfgs_params {
     double lns_fea;      /* 1.0 */
     int use_precond;     /* 1: H0 = K^-1 (nl_solver "lbfgs_ah"), 0: H0 = I/theta */
     int line_search;     /* EFEM_LS_PROJECTION */
+    /* optional measurement hook (no counterpart in the reference): when
+     * x_target is non-NULL the solve also stops as soon as
+     * ||x - x_target||_2 <= target_tol ||x_target||_2 (checked on the device
+     * after every iteration); stats->target_reached reports it */
+    const double* x_target;
+    double target_tol;
 } efem_lbfgs_params;
 
 typedef struct efem_solve_stats {
@@ -104,6 +111,8 @@ typedef struct efem_solve_stats {
     double factor_seconds;
     double solve_seconds;
     long long factor_nnz; /* nnz of the (scalar) Cholesky factor */
+    int target_reached;   /* see efem_lbfgs_params.x_target */
+    double target_err;    /* last ||x - x_target|| / ||x_target|| */
 } efem_solve_stats;
 
 typedef struct efem_ctx efem_ctx;
@@ -125,6 +134,8 @@ int efem_residual_d(efem_ctx* ctx, const double* u_d, double* r_d);
  * upload the factor.  Returns EFEM_OK; *failed = 1 if not SPD (fallback). */
 int efem_factor(efem_ctx* ctx, int* failed);
 long long efem_factor_nnz(const efem_ctx* ctx);
+/* drop the factor (the next efem_solve with use_precond factors again) */
+int efem_factor_release(efem_ctx* ctx);
 /* out = K^-1 v (host pointers); requires a successful efem_factor */
 int efem_apply_h0(efem_ctx* ctx, const double* v, double* out);
 
@@ -143,10 +154,12 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
 /* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */
 int efem_trace(const efem_ctx* ctx, double* out, int cap);
 
-/* kernel timing of the element-force kernel over the last efem_residual
- * calls made after efem_profile_reset (HIP events on the context stream) */
+/* device timing (HIP events on the context stream) of the element-force
+ * kernel and of whole H0 applies (forward + backward triangular solves) made
+ * after efem_profile_reset, by efem_residual / efem_apply_h0 / efem_solve */
 int efem_profile_reset(efem_ctx* ctx);
 int efem_profile_read(efem_ctx* ctx, double* elem_ms_total, int* elem_launches);
+int efem_profile_read_h0(efem_ctx* ctx, double* h0_ms_total, int* h0_applies);
 
 /* ---- synthetic workloads (host only, no GPU needed) --------------------
  * which: 1 cantilever 20x4x4 C3D8 NH; 2 beam 200x20x20 Kuhn C3D4 MR;
