@@ -81,6 +81,10 @@ def main():
     ap.add_argument("--cpu-evals", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-switch", type=float, default=1e-3,
+                    help="mixed: element kernels go FP64 once |r|inf <= this x |r0|inf")
+    ap.add_argument("--profile-run", type=int, default=0,
+                    help="only run one mixed solve of this many iterations (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU design "
@@ -92,6 +96,18 @@ def main():
 
     mesh = E.Mesh.workload(2, [200, 20, 20, 20240531, args.pressure])
     log(f"beam: {mesh.n_nodes} nodes, {mesh.n_elems} C3D4, {mesh.ndof} DOF, p = {args.pressure} Pa")
+    if args.profile_run:
+        c = E.Context(mesh, E.MIXED)
+        c.profile_reset()
+        t = time.perf_counter()
+        u, st = c.solve(epsilon=0.0, epsilon_rel=0.0, max_iterations=args.profile_run)
+        dt = time.perf_counter() - t
+        ems, ek = c.profile_read()
+        hms, hk = c.profile_read_h0()
+        print(json.dumps({"profile_run": True, "iterations": st["iterations"], "r_evals": st["r_evals"],
+                          "wall_s": dt, "factor_s": st["factor_seconds"], "elem_us": 1e3 * ems / max(ek, 1),
+                          "elem_launches": ek, "h0_ms": hms / max(hk, 1), "h0_applies": hk}))
+        return
     ctx = {"mixed": E.Context(mesh, E.MIXED)}
     if not args.no_fp64:
         ctx["fp64"] = E.Context(mesh, E.FP64)
@@ -111,7 +127,8 @@ def main():
             c.profile_reset()
         t = time.perf_counter()
         u, st = c.solve(epsilon=0.0, epsilon_rel=0.0, max_iterations=args.max_iters,
-                        x_target=u_star, target_tol=args.target)
+                        x_target=u_star, target_tol=args.target,
+                        fp64_switch=args.fp64_switch if name == "mixed" else 0.0)
         dt = time.perf_counter() - t
         st["wall"] = dt
         if not st["target_reached"]:
@@ -129,7 +146,8 @@ def main():
             st = one_solve(name, profile=(k == args.steps - 1))
             res[name].append(st)
             log(f"{name}: {st['iterations']} it, {st['r_evals']} evals, err {st['target_err']:.2e}, "
-                f"factor {st['factor_seconds']:.2f}s, total {st['wall']:.3f}s")
+                f"factor {st['factor_seconds']:.2f}s, total {st['wall']:.3f}s, "
+                f"fp64 from it {st['fp64_switch_iteration']}")
             if k == args.steps - 1:
                 ems, ek = ctx[name].profile_read()
                 hms, hk = ctx[name].profile_read_h0()
@@ -170,6 +188,8 @@ def main():
         "r_evals_mixed": int(np.mean([s["r_evals"] for s in mix])),
         "factor_s_mixed": round(float(np.mean([s["factor_seconds"] for s in mix])), 3),
         "disp_err_mixed": float(max(s["target_err"] for s in mix)),
+        "fp64_switch": args.fp64_switch,
+        "fp64_switch_iteration": int(mix[-1]["fp64_switch_iteration"]),
         "u_star_iterations": st_star["iterations"],
     }
     if "fp64" in res:

@@ -45,7 +45,7 @@ class LbfgsParams(C.Structure):
         ("epsilon", C.c_double), ("epsilon_rel", C.c_double), ("max_iterations", C.c_int),
         ("max_linesearch", C.c_int), ("m", C.c_int), ("lns_fea", C.c_double),
         ("use_precond", C.c_int), ("line_search", C.c_int), ("x_target", _dp),
-        ("target_tol", C.c_double),
+        ("target_tol", C.c_double), ("fp64_switch", C.c_double),
     ]
 
 
@@ -55,6 +55,7 @@ class SolveStats(C.Structure):
         ("precond_failed", C.c_int), ("rinf", C.c_double), ("xinf", C.c_double),
         ("factor_seconds", C.c_double), ("solve_seconds", C.c_double),
         ("factor_nnz", C.c_longlong), ("target_reached", C.c_int), ("target_err", C.c_double),
+        ("fp64_switch_iteration", C.c_int),
     ]
 
     def as_dict(self):
@@ -277,7 +278,7 @@ class Context:
         return out
 
     def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=0, max_linesearch=20,
-              m=6, lns_fea=1.0, use_precond=True, x_target=None, target_tol=0.0):
+              m=6, lns_fea=1.0, use_precond=True, x_target=None, target_tol=0.0, fp64_switch=0.0):
         """LBFGSSolver::solve.  epsilon_rel defaults to epsilon, as
         Integrator.cpp:13-14 sets both from residual_criterion.  x_target /
         target_tol: also stop once ||x - x_target|| <= tol ||x_target||."""
@@ -285,7 +286,8 @@ class Context:
         xt = None if x_target is None else np.ascontiguousarray(x_target, dtype=np.float64)
         prm = LbfgsParams(float(epsilon), float(epsilon if epsilon_rel is None else epsilon_rel),
                           int(max_iterations), int(max_linesearch), int(m), float(lns_fea),
-                          1 if use_precond else 0, 0, _ptr(xt, C.c_double), float(target_tol))
+                          1 if use_precond else 0, 0, _ptr(xt, C.c_double), float(target_tol),
+                          float(fp64_switch))
         st = SolveStats()
         rc = self._lib.efem_solve(self._h, _ptr(u, C.c_double), C.byref(prm), C.byref(st))
         if rc not in (EFEM_OK, EFEM_ERR_NONFINITE):

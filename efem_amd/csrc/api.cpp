@@ -48,6 +48,9 @@ struct efem_ctx {
     void* ginv = nullptr;
     double* vol = nullptr;
     void* gdN = nullptr;
+    void* ginv64 = nullptr;  // FP64 copies used by EFEM_MIXED after its switch
+    void* gdN64 = nullptr;
+    bool elemFp64 = false;   // current element-kernel precision in EFEM_MIXED
     double* gw = nullptr;
     double* params = nullptr;
     void* eslots = nullptr;
@@ -142,14 +145,21 @@ int uploadModel(efem_ctx* c)
     if ((rc = upload(c, &c->gw, m.gw))) return rc;
     if ((rc = upload(c, &c->params, m.params))) return rc;
     const size_t nslot = static_cast<size_t>(m.nElems) * m.nen * 3;
-    if (c->precision == EFEM_MIXED) {
-        float* q = nullptr;
-        if ((rc = devAlloc(c, &q, nslot))) return rc;
-        c->eslots = q;
-    } else {
+    {
+        // sized for FP64 slots; the FP32 kernels use the first half
         double* q = nullptr;
         if ((rc = devAlloc(c, &q, nslot))) return rc;
         c->eslots = q;
+    }
+    if (c->precision == EFEM_MIXED) {
+        double* q = nullptr;
+        if ((rc = upload(c, &q, m.ginv))) return rc;
+        c->ginv64 = q;
+        if ((rc = upload(c, &q, m.gdN))) return rc;
+        c->gdN64 = q;
+    } else {
+        c->ginv64 = c->ginv;
+        c->gdN64 = c->gdN;
     }
     if (m.nFaces > 0) {
         std::vector<int> fdof(static_cast<size_t>(m.nen) * m.nFaces);
@@ -189,11 +199,12 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
     ea.ng = m.ng;
     ea.law = m.law;
     ea.paramsPerElem = m.paramsPerElem;
-    ea.mixed = c->precision == EFEM_MIXED;
+    const bool f32 = c->precision == EFEM_MIXED && !c->elemFp64;
+    ea.mixed = f32;
     ea.connDof = c->connDof;
-    ea.ginv = c->ginv;
+    ea.ginv = f32 ? c->ginv : c->ginv64;
     ea.vol = c->vol;
-    ea.gdN = c->gdN;
+    ea.gdN = f32 ? c->gdN : c->gdN64;
     ea.gw = c->gw;
     ea.params = c->params;
     ea.x = x;
@@ -226,7 +237,7 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
     efem::dev::GatherArgs ga;
     ga.nFree = m.nFree;
     ga.nElemSlots = m.nElems * m.nen;
-    ga.mixed = c->precision == EFEM_MIXED;
+    ga.mixed = f32;
     ga.gptr = c->gptr;
     ga.gslot = c->gslot;
     ga.eslots = c->eslots;
@@ -521,6 +532,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     }
     c->trace.clear();
     double* sc = c->scal;
+    c->elemFp64 = false;
     if (prm->x_target) {
         if (!c->xtarget && (rc = devAlloc(c, &c->xtarget, static_cast<size_t>(n) + 3))) return rc;
         HIP_CHECK(c, hipMemcpyAsync(c->xtarget, prm->x_target, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -536,6 +548,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     const double tnorm = std::sqrt(c->hostScal[5]);
     s.rinf = rinf;
     s.xinf = xinf;
+    const double rinf0 = rinf;
     if (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xnorm2) {
         s.iterations = 1;
         s.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -602,6 +615,14 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         }
         if (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xinf) break;
         if (prm->max_iterations != 0 && k >= prm->max_iterations) break;
+        if (c->precision == EFEM_MIXED && !c->elemFp64 && prm->fp64_switch > 0.0 &&
+            rinf <= prm->fp64_switch * rinf0) {
+            // FP64 finish: re-evaluate r(x) with the FP64 element kernels
+            c->elemFp64 = true;
+            s.fp64_switch_iteration = k;
+            if ((rc = residualInternal(c, c->x, c->r))) return rc;
+            ++s.r_evals;
+        }
         // add_correction (LBFGSMat.h:81-98)
         const int loc = ptr % mc;
         double* sl = c->S + static_cast<size_t>(loc) * n;
@@ -643,6 +664,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     }
     HIP_CHECK(c, hipMemcpyAsync(u, c->x, nb, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->elemFp64 = false;
     s.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!std::isfinite(s.rinf) || !std::isfinite(s.xinf)) {
         // Integrator.cpp:28-34: non-finite result -> dx zeroed, failure reported

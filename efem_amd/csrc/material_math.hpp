@@ -1,13 +1,17 @@
 // Hyperelastic PK1 stress for the three laws of src/material/*.cpp, written
-// in terms of the displacement gradient H = F - I so that the FP32 path does
-// not cancel:
+// in terms of the displacement gradient H = F - I so that FP32 evaluation
+// keeps its relative accuracy at small strain:
 //   * J - 1 = tr H + (tr^2 H - tr H^2)/2 + det H, always in FP64;
-//   * J F^-T = cof F (no division by J for the volumetric term);
-//   * P = c1 J^-2/3 (2F - 2/3 I1 cof F / J)                     [I1bar term]
-//       + C01 J^-4/3 (2F(I1 I - C) - 4/3 I2 cof F / J)          [I2bar term, MR]
-//       + (2/D1)(J - 1) cof F                                   [volumetric]
+//   * E = C - I = H + H^T + H^T H (twice the Green strain), from H directly;
+//   * the isochoric terms use F^-T dev(.) forms with no O(1) cancellation:
+//       2F - 2/3 I1 F^-T             = 2 F^-T dev(E)
+//       2F(I1 I - C) - 4/3 I2 F^-T   = 2 F^-T dev((1 + tr E) E - E^2)
+//   so P = 2 c1 J^-2/3 F^-T dev(E)                         [I1bar term]
+//        + 2 C01 J^-4/3 F^-T dev((1 + tr E) E - E^2)       [I2bar term, MR]
+//        + (2/D1)(J - 1) cof F                              [volumetric]
+// with c1 = C10 (NH, MR) or C10 + 2 C20 t + 3 C30 t^2, t = I1bar - 3 (Yeoh),
 // which equals NeoHookean.cpp:4-20, MooneyRivlin.cpp:4-24 and Yeoh.cpp:4-20
-// (dJ/dF = J F^-T, I1bar = I1 / I3^(1/3), I2bar = I2 / I3^(2/3)).
+// (dJ/dF = J F^-T = cof F, I1bar = I1 / I3^(1/3), I2bar = I2 / I3^(2/3)).
 // Strain energies (no counterpart in the reference, which never evaluates
 // one): W = c(I1bar-3)[+C01(I2bar-3)] + (J-1)^2/D1, Yeoh sum_i Ci0(I1bar-3)^i.
 #pragma once
@@ -45,7 +49,7 @@ EFEM_HD double detIpHm1(const double* h)
 }
 
 // P(F = I + H) for law with p = {C10, C01|C20, C30, D1}.  T = double or float;
-// J - 1 and the volumetric coefficient are FP64 in both cases.
+// J - 1, J^-2/3 - 1 and the volumetric coefficient are FP64 in both cases.
 template <typename T>
 EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
 {
@@ -53,6 +57,8 @@ EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
     for (int k = 0; k < 9; ++k) hd[k] = static_cast<double>(H[k]);
     const double jm1 = detIpHm1(hd);
     const double J = 1.0 + jm1;
+    // J^-2/3 - 1 = expm1(-2/3 log1p(J - 1)), accurate for J near 1
+    const double em1 = std::expm1(-2.0 / 3.0 * std::log1p(jm1));
     T F[9];
     for (int k = 0; k < 9; ++k) F[k] = H[k];
     F[0] += T(1);
@@ -69,73 +75,76 @@ EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
     cf[6] = F[1] * F[5] - F[2] * F[4];
     cf[7] = F[2] * F[3] - F[0] * F[5];
     cf[8] = F[0] * F[4] - F[1] * F[3];
-    T I1 = T(0);
-    for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
-    const T Jt = static_cast<T>(J);
-    const T invJ = T(1) / Jt;
-    const T jm23 = T(1) / static_cast<T>(std::cbrt(J * J));  // J^-2/3
+    // E = H + H^T + H^T H (symmetric)
+    T Em[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Em[3 * i + j] = H[3 * i + j] + H[3 * j + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
+    const T trE = Em[0] + Em[4] + Em[8];
+    T A[9];  // deviatoric argument of the I1bar (and I2bar) terms
+    for (int k = 0; k < 9; ++k) A[k] = Em[k];
+    const T jm23 = static_cast<T>(1.0 + em1);
     T c1 = static_cast<T>(p[0]);
     if (law == LAW_YEOH) {
-        const T t = I1 * jm23 - T(3);
-        c1 = static_cast<T>(p[0]) + T(2) * static_cast<T>(p[1]) * t +
-             T(3) * static_cast<T>(p[2]) * t * t;
+        // t = I1 J^-2/3 - 3 = 3 (J^-2/3 - 1) + tr E J^-2/3
+        const T t = static_cast<T>(3.0 * em1) + trE * jm23;
+        c1 = static_cast<T>(p[0]) + T(2) * static_cast<T>(p[1]) * t + T(3) * static_cast<T>(p[2]) * t * t;
     }
-    const T a = c1 * jm23;
-    const T b = T(2.0 / 3.0) * I1 * invJ;
-    const double kvol = 2.0 / p[3] * jm1;  // (2/D1)(J-1)
-    const T kv = static_cast<T>(kvol);
-    for (int k = 0; k < 9; ++k) P[k] = a * (T(2) * F[k] - b * cf[k]) + kv * cf[k];
+    const T invJ = static_cast<T>(1.0 / J);
+    T s1 = T(2) * c1 * jm23 * invJ;  // 2 c1 J^-2/3 / J  (F^-T = cof / J)
+    T dev[9];
+    {
+        const T m = trE * T(1.0 / 3.0);
+        for (int k = 0; k < 9; ++k) dev[k] = s1 * A[k];
+        dev[0] -= s1 * m;
+        dev[4] -= s1 * m;
+        dev[8] -= s1 * m;
+    }
     if (law == LAW_MR) {
-        // C = F^T F ; I2 = (I1^2 - tr C^2)/2
-        T C[9];
+        // + 2 C01 J^-4/3 / J dev((1 + tr E) E - E^2)
+        T E2[9];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
-                C[3 * i + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
-        T trCC = T(0);
-        for (int k = 0; k < 9; ++k) trCC += C[k] * C[k];
-        const T I2 = T(0.5) * (I1 * I1 - trCC);
-        const T jm43 = jm23 * jm23;
-        const T c01 = static_cast<T>(p[1]) * jm43;
-        const T b2 = T(4.0 / 3.0) * I2 * invJ;
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                // (F (I1 I - C))_ij
-                const T fm = I1 * F[3 * i + j] -
-                             (F[3 * i] * C[j] + F[3 * i + 1] * C[3 + j] + F[3 * i + 2] * C[6 + j]);
-                P[3 * i + j] += c01 * (T(2) * fm - b2 * cf[3 * i + j]);
-            }
+                E2[3 * i + j] = Em[3 * i] * Em[j] + Em[3 * i + 1] * Em[3 + j] + Em[3 * i + 2] * Em[6 + j];
+        T B[9];
+        for (int k = 0; k < 9; ++k) B[k] = (T(1) + trE) * Em[k] - E2[k];
+        const T mB = (B[0] + B[4] + B[8]) * T(1.0 / 3.0);
+        const T s2 = T(2) * static_cast<T>(p[1]) * jm23 * jm23 * invJ;
+        for (int k = 0; k < 9; ++k) dev[k] += s2 * B[k];
+        dev[0] -= s2 * mB;
+        dev[4] -= s2 * mB;
+        dev[8] -= s2 * mB;
     }
+    // P = cof(F) dev + (2/D1)(J-1) cof(F)
+    const T kv = static_cast<T>(2.0 / p[3] * jm1);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            P[3 * i + j] = cf[3 * i] * dev[j] + cf[3 * i + 1] * dev[3 + j] + cf[3 * i + 2] * dev[6 + j] +
+                           kv * cf[3 * i + j];
 }
 
 // strain energy density at F = I + H (FP64)
 EFEM_HD double energyFromH(int law, const double* p, const double* H)
 {
     const double jm1 = detIpHm1(H);
-    const double J = 1.0 + jm1;
-    double F[9];
-    for (int k = 0; k < 9; ++k) F[k] = H[k];
-    F[0] += 1.0;
-    F[4] += 1.0;
-    F[8] += 1.0;
-    double I1 = 0.0;
-    for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
-    const double jm23 = 1.0 / std::cbrt(J * J);
-    const double i1b3 = I1 * jm23 - 3.0;
+    const double em1 = std::expm1(-2.0 / 3.0 * std::log1p(jm1));
+    double E[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            E[3 * i + j] = H[3 * i + j] + H[3 * j + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
+    const double trE = E[0] + E[4] + E[8];
+    const double t = 3.0 * em1 + trE * (1.0 + em1);  // I1bar - 3
     double W = jm1 * jm1 / p[3];
-    if (law == LAW_YEOH) {
-        W += p[0] * i1b3 + p[1] * i1b3 * i1b3 + p[2] * i1b3 * i1b3 * i1b3;
-    } else {
-        W += p[0] * i1b3;
-        if (law == LAW_MR) {
-            double C[9];
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j)
-                    C[3 * i + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
-            double trCC = 0.0;
-            for (int k = 0; k < 9; ++k) trCC += C[k] * C[k];
-            const double I2 = 0.5 * (I1 * I1 - trCC);
-            W += p[1] * (I2 * jm23 * jm23 - 3.0);
-        }
+    if (law == LAW_YEOH) return W + p[0] * t + p[1] * t * t + p[2] * t * t * t;
+    W += p[0] * t;
+    if (law == LAW_MR) {
+        // I2 - 3 = 2 tr E + (tr^2 E - tr E^2)/2 ; I2bar - 3 = (I2 - 3) J^-4/3 + 3 (J^-4/3 - 1)
+        double trE2 = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) trE2 += E[3 * i + j] * E[3 * j + i];
+        const double i2m3 = 2.0 * trE + 0.5 * (trE * trE - trE2);
+        const double j43 = (1.0 + em1) * (1.0 + em1);
+        W += p[1] * (i2m3 * j43 + 3.0 * (j43 - 1.0));
     }
     return W;
 }

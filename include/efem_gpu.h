@@ -99,6 +99,10 @@ typedef struct efem_lbfgs_params {
      * after every iteration); stats->target_reached reports it */
     const double* x_target;
     double target_tol;
+    /* EFEM_MIXED only: the element kernels switch from FP32 to FP64 once
+     * ||r||_inf <= fp64_switch * ||r(x0)||_inf (0: stay FP32).  The FP32
+     * preconditioner is kept to the end. */
+    double fp64_switch;
 } efem_lbfgs_params;
 
 typedef struct efem_solve_stats {
@@ -113,6 +117,7 @@ typedef struct efem_solve_stats {
     long long factor_nnz; /* nnz of the (scalar) Cholesky factor */
     int target_reached;   /* see efem_lbfgs_params.x_target */
     double target_err;    /* last ||x - x_target|| / ||x_target|| */
+    int fp64_switch_iteration; /* iteration at which EFEM_MIXED went FP64 (0: never) */
 } efem_solve_stats;
 
 typedef struct efem_ctx efem_ctx;
