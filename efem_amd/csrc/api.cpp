@@ -118,10 +118,13 @@ int upload(efem_ctx* c, T** p, const std::vector<T>& h)
     return EFEM_OK;
 }
 
-// double or float copy depending on the precision mode
-int uploadReal(efem_ctx* c, void** p, const std::vector<double>& h)
+bool elem32(const efem_ctx* c) { return (c->precision & EFEM_F32_ELEMENTS) != 0; }
+bool fac32(const efem_ctx* c) { return (c->precision & EFEM_F32_PRECOND) != 0; }
+
+// double or float copy
+int uploadReal(efem_ctx* c, void** p, const std::vector<double>& h, bool f32)
 {
-    if (c->precision == EFEM_MIXED) {
+    if (f32) {
         std::vector<float> f(h.begin(), h.end());
         float* q = nullptr;
         int rc = upload(c, &q, f);
@@ -139,9 +142,9 @@ int uploadModel(efem_ctx* c)
     const efem::Model& m = c->model;
     int rc;
     if ((rc = upload(c, &c->connDof, m.connDof))) return rc;
-    if ((rc = uploadReal(c, &c->ginv, m.ginv))) return rc;
+    if ((rc = uploadReal(c, &c->ginv, m.ginv, elem32(c)))) return rc;
     if ((rc = upload(c, &c->vol, m.vol))) return rc;
-    if ((rc = uploadReal(c, &c->gdN, m.gdN))) return rc;
+    if ((rc = uploadReal(c, &c->gdN, m.gdN, elem32(c)))) return rc;
     if ((rc = upload(c, &c->gw, m.gw))) return rc;
     if ((rc = upload(c, &c->params, m.params))) return rc;
     const size_t nslot = static_cast<size_t>(m.nElems) * m.nen * 3;
@@ -151,7 +154,7 @@ int uploadModel(efem_ctx* c)
         if ((rc = devAlloc(c, &q, nslot))) return rc;
         c->eslots = q;
     }
-    if (c->precision == EFEM_MIXED) {
+    if (elem32(c)) {
         double* q = nullptr;
         if ((rc = upload(c, &q, m.ginv))) return rc;
         c->ginv64 = q;
@@ -199,7 +202,7 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
     ea.ng = m.ng;
     ea.law = m.law;
     ea.paramsPerElem = m.paramsPerElem;
-    const bool f32 = c->precision == EFEM_MIXED && !c->elemFp64;
+    const bool f32 = elem32(c) && !c->elemFp64;
     ea.mixed = f32;
     ea.connDof = c->connDof;
     ea.ginv = f32 ? c->ginv : c->ginv64;
@@ -254,7 +257,7 @@ efem::dev::TrsvArgs trsvArgs(efem_ctx* c)
     DeviceFactor& f = c->fac;
     efem::dev::TrsvArgs a;
     a.n = f.n;
-    a.mixed = c->precision == EFEM_MIXED;
+    a.mixed = fac32(c);
     a.perm = f.perm;
     a.snFirst = f.snFirst;
     a.snW = f.snW;
@@ -369,11 +372,11 @@ int factorInternal(efem_ctx* c, int* failed)
     if ((rc = upload(c, &d.snVal, valOff))) return rc;
     if ((rc = upload(c, &d.snT, tOff))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
-    if ((rc = uploadReal(c, &d.B, F.B))) return rc;
+    if ((rc = uploadReal(c, &d.B, F.B, fac32(c)))) return rc;
     if ((rc = upload(c, &d.cptr, cptr))) return rc;
     if ((rc = upload(c, &d.cidx, cidx))) return rc;
     if ((rc = upload(c, &d.items, items))) return rc;
-    const size_t es = c->precision == EFEM_MIXED ? sizeof(float) : sizeof(double);
+    const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
     double* q = nullptr;
     if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(F.n) * es + 7) / 8))) return rc;
     d.y = q;
@@ -407,7 +410,11 @@ int efem_create(efem_ctx** out, const efem_problem* pb)
     }
     *out = nullptr;
     auto c = std::make_unique<efem_ctx>();
-    c->precision = pb->precision == EFEM_MIXED ? EFEM_MIXED : EFEM_FP64;
+    if (pb->precision < 0 || pb->precision > EFEM_MIXED) {
+        g_error = "efem_create: unknown precision mode";
+        return EFEM_ERR_ARG;
+    }
+    c->precision = pb->precision;
     std::string msg;
     try {
         if (!efem::buildModel(*pb, c->model, msg)) {
@@ -615,7 +622,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         }
         if (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xinf) break;
         if (prm->max_iterations != 0 && k >= prm->max_iterations) break;
-        if (c->precision == EFEM_MIXED && !c->elemFp64 && prm->fp64_switch > 0.0 &&
+        if (elem32(c) && !c->elemFp64 && prm->fp64_switch > 0.0 &&
             rinf <= prm->fp64_switch * rinf0) {
             // FP64 finish: re-evaluate r(x) with the FP64 element kernels
             c->elemFp64 = true;

@@ -46,10 +46,12 @@ extern "C" {
 #define EFEM_MOONEY_RIVLIN 1
 #define EFEM_YEOH 2
 
-/* arithmetic of the element kernels and the preconditioner */
+/* arithmetic of the element kernels and the preconditioner (bit mask) */
 #define EFEM_FP64 0
-#define EFEM_MIXED 1 /* FP32 elements + factor; FP64 J-1, volumetric term,
-                        reductions, L-BFGS scalars and residual assembly */
+#define EFEM_F32_ELEMENTS 1 /* FP32 element kernels; FP64 J-1, volumetric
+                               coefficient and residual assembly */
+#define EFEM_F32_PRECOND 2  /* FP32 factor blocks and triangular solves */
+#define EFEM_MIXED 3        /* both; L-BFGS vectors, scalars and norms stay FP64 */
 
 /* line-search rule */
 #define EFEM_LS_PROJECTION 0 /* the reference's |r.d| rule (LineSearchFEA.h:37-68) */
@@ -80,7 +82,7 @@ typedef struct efem_problem {
     const int* fnodes;          /* nodes with a constant force (:132-175) */
     const double* fvec;         /* [3] */
     const double* volumes;      /* optional [n_elems] element volumes; NULL: computed */
-    int precision;              /* EFEM_FP64 or EFEM_MIXED */
+    int precision;              /* EFEM_FP64, EFEM_MIXED or a mask of EFEM_F32_* */
 } efem_problem;
 
 /* L-BFGS parameters (src/simulation/nlsolver/Param.h:92-103 defaults) */
@@ -99,7 +101,7 @@ typedef struct efem_lbfgs_params {
      * after every iteration); stats->target_reached reports it */
     const double* x_target;
     double target_tol;
-    /* EFEM_MIXED only: the element kernels switch from FP32 to FP64 once
+    /* EFEM_F32_ELEMENTS only: the element kernels switch from FP32 to FP64 once
      * ||r||_inf <= fp64_switch * ||r(x0)||_inf (0: stay FP32).  The FP32
      * preconditioner is kept to the end. */
     double fp64_switch;
