@@ -142,9 +142,9 @@ int uploadModel(efem_ctx* c)
     const efem::Model& m = c->model;
     int rc;
     if ((rc = upload(c, &c->connDof, m.connDof))) return rc;
-    if ((rc = uploadReal(c, &c->ginv, m.ginv, elem32(c)))) return rc;
+    if ((rc = uploadReal(c, &c->ginv, m.ginv, false))) return rc;  // J0^-1 stays FP64
     if ((rc = upload(c, &c->vol, m.vol))) return rc;
-    if ((rc = uploadReal(c, &c->gdN, m.gdN, elem32(c)))) return rc;
+    if ((rc = uploadReal(c, &c->gdN, m.gdN, false))) return rc;  // local gradients stay FP64
     if ((rc = upload(c, &c->gw, m.gw))) return rc;
     if ((rc = upload(c, &c->params, m.params))) return rc;
     const size_t nslot = static_cast<size_t>(m.nElems) * m.nen * 3;
@@ -154,16 +154,8 @@ int uploadModel(efem_ctx* c)
         if ((rc = devAlloc(c, &q, nslot))) return rc;
         c->eslots = q;
     }
-    if (elem32(c)) {
-        double* q = nullptr;
-        if ((rc = upload(c, &q, m.ginv))) return rc;
-        c->ginv64 = q;
-        if ((rc = upload(c, &q, m.gdN))) return rc;
-        c->gdN64 = q;
-    } else {
-        c->ginv64 = c->ginv;
-        c->gdN64 = c->gdN;
-    }
+    c->ginv64 = c->ginv;
+    c->gdN64 = c->gdN;
     if (m.nFaces > 0) {
         std::vector<int> fdof(static_cast<size_t>(m.nen) * m.nFaces);
         for (int f = 0; f < m.nFaces; ++f)

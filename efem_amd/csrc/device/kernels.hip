@@ -25,19 +25,26 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <typename T>
-__device__ __forceinline__ void stressMoment(int law, const double* p, const T* Gu, const T* Ji,
+// H = Gu J0^-1 is formed in FP64 in both modes: J - 1 (and with it the
+// near-incompressible volumetric stress (2/D1)(J-1) cof F, 2/D1 = 2e6 here)
+// inherits the rounding of H, so H must carry more than FP32's 24 bits.
+// The deviatoric stress and the force moments then run in T.
+template <typename T, typename GT>
+__device__ __forceinline__ void stressMoment(int law, const double* p, const GT* Gu, const double* Jd,
                                              double wv, T* Q)
 {
-    // H = Gu J0^-1 ; P = P(I + H) ; Q = wv P J0^-T
-    T H[9];
+    double Hd[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            H[3 * i + j] = Gu[3 * i] * Ji[j] + Gu[3 * i + 1] * Ji[3 + j] + Gu[3 * i + 2] * Ji[6 + j];
+            Hd[3 * i + j] = static_cast<double>(Gu[3 * i]) * Jd[j] + static_cast<double>(Gu[3 * i + 1]) * Jd[3 + j] +
+                            static_cast<double>(Gu[3 * i + 2]) * Jd[6 + j];
     T P[9];
-    piolaFromH<T>(law, p, H, P);
+    piolaFromHd<T>(law, p, Hd, P);
+    T Ji[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ji[k] = static_cast<T>(Jd[k]);
     const T w = static_cast<T>(wv);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -49,7 +56,7 @@ __device__ __forceinline__ void stressMoment(int law, const double* p, const T* 
 // ---- linear tetrahedra: one lane per element ------------------------------
 template <typename T, int LAW>
 __global__ __launch_bounds__(kThreads) void kElemTet4(int E, const int* __restrict__ connDof,
-                                                      const T* __restrict__ ginv,
+                                                      const double* __restrict__ ginv,
                                                       const double* __restrict__ vol, double w0,
                                                       const double* __restrict__ params, int ppe,
                                                       const double* __restrict__ x,
@@ -64,16 +71,16 @@ __global__ __launch_bounds__(kThreads) void kElemTet4(int E, const int* __restri
     double u3[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) u3[c] = x[3 * id[3] + c];
-    T Gu[9];
+    double Gu[9];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) Gu[3 * i + k] = static_cast<T>(x[3 * id[k] + i] - u3[i]);
-    T Ji[9];
+        for (int i = 0; i < 3; ++i) Gu[3 * i + k] = x[3 * id[k] + i] - u3[i];
+    double Jd[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Ji[k] = ginv[k * E + e];
+    for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + e];
     T Q[9];
-    stressMoment<T>(LAW, params + (ppe ? 4 * e : 0), Gu, Ji, vol[e] * w0, Q);
+    stressMoment<T, double>(LAW, params + (ppe ? 4 * e : 0), Gu, Jd, vol[e] * w0, Q);
     T* out = slots + 12 * static_cast<size_t>(e);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -92,49 +99,51 @@ struct Tile {
 
 template <typename T, int NEN, int NG, int LAW>
 __global__ __launch_bounds__(kThreads) void kElemMulti(int E, const int* __restrict__ connDof,
-                                                       const T* __restrict__ ginv,
+                                                       const double* __restrict__ ginv,
                                                        const double* __restrict__ vol,
-                                                       const T* __restrict__ gdN,
+                                                       const double* __restrict__ gdN,
                                                        const double* __restrict__ gw,
                                                        const double* __restrict__ params, int ppe,
                                                        const double* __restrict__ x,
                                                        T* __restrict__ slots)
 {
     constexpr int EPB = Tile<NG>::epb;
-    __shared__ T sdN[NG * NEN * 3];
-    __shared__ T su[EPB * NEN * 3];
+    // the kinematics (relative displacements, local gradients, H) are FP64 in
+    // both modes; the stress moments Q and the nodal forces use T
+    __shared__ double sdN[NG * NEN * 3];
+    __shared__ double su[EPB * NEN * 3];
     __shared__ T sQ[EPB * NG * 9];
     const int tid = threadIdx.x;
     const int e0 = blockIdx.x * EPB;
     for (int k = tid; k < NG * NEN * 3; k += kThreads) sdN[k] = gdN[k];
-    // relative displacements u_a - u_0 (FP64 difference, then rounded)
+    // relative displacements u_a - u_0
     for (int k = tid; k < EPB * NEN; k += kThreads) {
         const int el = k / NEN, a = k - el * NEN, e = e0 + el;
         if (e < E) {
             const int id = connDof[a * E + e], id0 = connDof[e];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) su[3 * k + c] = static_cast<T>(x[3 * id + c] - x[3 * id0 + c]);
+            for (int c = 0; c < 3; ++c) su[3 * k + c] = x[3 * id + c] - x[3 * id0 + c];
         }
     }
     __syncthreads();
     {
         const int el = tid / NG, q = tid - el * NG, e = e0 + el;
         if (el < EPB && e < E) {
-            T Gu[9];
+            double Gu[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) Gu[k] = T(0);
-            const T* ue = su + el * NEN * 3;
-            const T* dq = sdN + q * NEN * 3;
+            for (int k = 0; k < 9; ++k) Gu[k] = 0.0;
+            const double* ue = su + el * NEN * 3;
+            const double* dq = sdN + q * NEN * 3;
             for (int a = 1; a < NEN; ++a) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) Gu[3 * i + k] += ue[3 * a + i] * dq[3 * a + k];
             }
-            T Ji[9];
+            double Jd[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) Ji[k] = ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
-            stressMoment<T>(LAW, params + (ppe ? 4 * e : 0), Gu, Ji, vol[e] * gw[q], sQ + (el * NG + q) * 9);
+            for (int k = 0; k < 9; ++k) Jd[k] = ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
+            stressMoment<T, double>(LAW, params + (ppe ? 4 * e : 0), Gu, Jd, vol[e] * gw[q], sQ + (el * NG + q) * 9);
         }
     }
     __syncthreads();
@@ -144,9 +153,10 @@ __global__ __launch_bounds__(kThreads) void kElemMulti(int E, const int* __restr
         T f[3] = {T(0), T(0), T(0)};
         for (int q = 0; q < NG; ++q) {
             const T* Q = sQ + (el * NG + q) * 9;
-            const T* d = sdN + (q * NEN + a) * 3;
+            const double* d = sdN + (q * NEN + a) * 3;
+            const T d0 = static_cast<T>(d[0]), d1 = static_cast<T>(d[1]), d2 = static_cast<T>(d[2]);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) f[i] += Q[3 * i] * d[0] + Q[3 * i + 1] * d[1] + Q[3 * i + 2] * d[2];
+            for (int i = 0; i < 3; ++i) f[i] += Q[3 * i] * d0 + Q[3 * i + 1] * d1 + Q[3 * i + 2] * d2;
         }
         T* out = slots + (static_cast<size_t>(e) * NEN + a) * 3;
         out[0] = f[0];
@@ -160,8 +170,8 @@ void launchMultiLaw(const ElemArgs& a, hipStream_t s)
 {
     const int EPB = Tile<NG>::epb;
     const int grid = (a.nElems + EPB - 1) / EPB;
-    const T* ginv = static_cast<const T*>(a.ginv);
-    const T* gdN = static_cast<const T*>(a.gdN);
+    const double* ginv = static_cast<const double*>(a.ginv);
+    const double* gdN = static_cast<const double*>(a.gdN);
     T* slots = static_cast<T*>(a.slots);
     switch (a.law) {
         case LAW_NH:
@@ -182,7 +192,7 @@ template <typename T>
 void launchTet4Law(const ElemArgs& a, hipStream_t s)
 {
     const int grid = (a.nElems + kThreads - 1) / kThreads;
-    const T* ginv = static_cast<const T*>(a.ginv);
+    const double* ginv = static_cast<const double*>(a.ginv);
     T* slots = static_cast<T*>(a.slots);
     switch (a.law) {
         case LAW_NH:

@@ -48,13 +48,14 @@ EFEM_HD double detIpHm1(const double* h)
     return tr + 0.5 * (tr * tr - trH2) + det;
 }
 
-// P(F = I + H) for law with p = {C10, C01|C20, C30, D1}.  T = double or float;
-// J - 1, J^-2/3 - 1 and the volumetric coefficient are FP64 in both cases.
+// P(F = I + H) for law with p = {C10, C01|C20, C30, D1}, H given in FP64.
+// T = double or float for the deviatoric terms and the cofactor; J - 1,
+// J^-2/3 - 1 and the volumetric coefficient are FP64 in both cases.
 template <typename T>
-EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
+EFEM_HD void piolaFromHd(int law, const double* p, const double* hd, T* P)
 {
-    double hd[9];
-    for (int k = 0; k < 9; ++k) hd[k] = static_cast<double>(H[k]);
+    T H[9];
+    for (int k = 0; k < 9; ++k) H[k] = static_cast<T>(hd[k]);
     const double jm1 = detIpHm1(hd);
     const double J = 1.0 + jm1;
     // J^-2/3 - 1 = expm1(-2/3 log1p(J - 1)), accurate for J near 1
@@ -121,6 +122,14 @@ EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
         for (int j = 0; j < 3; ++j)
             P[3 * i + j] = cf[3 * i] * dev[j] + cf[3 * i + 1] * dev[3 + j] + cf[3 * i + 2] * dev[6 + j] +
                            kv * cf[3 * i + j];
+}
+
+template <typename T>
+EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
+{
+    double hd[9];
+    for (int k = 0; k < 9; ++k) hd[k] = static_cast<double>(H[k]);
+    piolaFromHd<T>(law, p, hd, P);
 }
 
 // strain energy density at F = I + H (FP64)
