@@ -85,6 +85,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_apply_h0": (C.c_int, [vp, _dp, _dp]),
         "efem_solve": (C.c_int, [vp, _dp, C.POINTER(LbfgsParams), C.POINTER(SolveStats)]),
         "efem_trace": (C.c_int, [vp, _dp, C.c_int]),
+        "efem_set_option": (C.c_int, [vp, C.c_char_p, C.c_double]),
         "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
         "efem_profile_reset": (C.c_int, [vp]),
         "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
@@ -317,6 +318,9 @@ class Context:
         k = C.c_int()
         self._check(self._lib.efem_profile_read_h0(self._h, C.byref(ms), C.byref(k)), "efem_profile_read_h0")
         return ms.value, k.value
+
+    def set_option(self, name: str, value: float):
+        self._check(self._lib.efem_set_option(self._h, name.encode(), float(value)), "efem_set_option")
 
     def factor_release(self):
         self._check(self._lib.efem_factor_release(self._h), "efem_factor_release")

@@ -29,10 +29,14 @@ struct DeviceFactor {
     long long* snVal = nullptr;
     int* rows = nullptr;
     void* B = nullptr;
-    int *cptr = nullptr, *cidx = nullptr;
+    int *fptr = nullptr, *fidx = nullptr;
     void *y = nullptr, *t = nullptr, *z = nullptr;
     int2* items = nullptr;
     std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level
+    // the whole apply captured as one HIP graph over fixed staging buffers
+    double *in = nullptr, *out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 }  // namespace
@@ -76,6 +80,8 @@ struct efem_ctx {
     std::vector<hipEvent_t> evs, evsH0;
     int evUsed = 0, evH0Used = 0;
     double* xtarget = nullptr;
+    bool useGraph = true;    // option "h0_graph"
+    bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
 };
 
 namespace {
@@ -87,7 +93,8 @@ constexpr int kYY = 72;       // [64]
 constexpr int kAlpha = 136;   // [64]
 constexpr int kCoef = 200;    // [64]
 constexpr int kTheta = 264;   // [1]
-constexpr int kNumScal = 272;
+constexpr int kGram = 272;    // [16][16] s_i . y_k of the compact two-loop
+constexpr int kNumScal = 528;
 
 #define HIP_CHECK(ctx, expr)                                                          \
     do {                                                                              \
@@ -259,26 +266,51 @@ efem::dev::TrsvArgs trsvArgs(efem_ctx* c)
     a.snT = f.snT;
     a.rows = f.rows;
     a.B = f.B;
-    a.cptr = f.cptr;
-    a.cidx = f.cidx;
+    a.fptr = f.fptr;
+    a.fidx = f.fidx;
     a.y = f.y;
     a.t = f.t;
     a.z = f.z;
     return a;
 }
 
-// out = K^-1 v on the device (v, out: interleaved free-DOF vectors)
-int applyH0Internal(efem_ctx* c, const double* v, double* out)
+void launchApplySweeps(efem_ctx* c, const double* v, double* out)
 {
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
-    const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
-    if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
     for (int l = 0; l < f.nlevels; ++l)
         efem::dev::launchTrsvForward(a, v, f.items + f.fwdOff[l], f.fwdCnt[l], c->stream);
     for (int l = f.nlevels - 1; l >= 0; --l)
         efem::dev::launchTrsvBackward(a, f.items + f.bwdOff[l], f.bwdCnt[l], c->stream);
     efem::dev::launchUnpermute(a, out, c->stream);
+}
+
+// out = K^-1 v on the device (v, out: interleaved free-DOF vectors).  With
+// the graph option the forward/backward level launches are replayed from one
+// captured HIP graph reading `in` and writing `out` staging buffers.
+int applyH0Internal(efem_ctx* c, const double* v, double* out)
+{
+    DeviceFactor& f = c->fac;
+    const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
+    if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
+    if (c->useGraph) {
+        const size_t nb = 3 * static_cast<size_t>(f.n) * sizeof(double);
+        if (!f.exec) {
+            HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            launchApplySweeps(c, f.in, f.out);
+            hipError_t e = hipStreamEndCapture(c->stream, &f.graph);
+            if (e != hipSuccess) {
+                c->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+                return EFEM_ERR_HIP;
+            }
+            HIP_CHECK(c, hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
+        }
+        HIP_CHECK(c, hipMemcpyAsync(f.in, v, nb, hipMemcpyDeviceToDevice, c->stream));
+        HIP_CHECK(c, hipGraphLaunch(f.exec, c->stream));
+        HIP_CHECK(c, hipMemcpyAsync(out, f.out, nb, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        launchApplySweeps(c, v, out);
+    }
     if (prof) {
         HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used + 1], c->stream));
         c->evH0Used += 2;
@@ -310,7 +342,7 @@ int factorInternal(efem_ctx* c, int* failed)
     d.nsuper = F.nsuper;
     d.nlevels = F.nlevels;
     d.nnz = F.nnzL;
-    std::vector<int> w(F.nsuper), mm(F.nsuper), rowOff(F.nsuper), tOff(F.nsuper);
+    std::vector<int> w(F.nsuper), mm(F.nsuper), rowOff(F.nsuper), tOff(F.nsuper + 1);
     std::vector<long long> valOff(F.nsuper);
     int nt = 0;
     for (int s = 0; s < F.nsuper; ++s) {
@@ -325,16 +357,10 @@ int factorInternal(efem_ctx* c, int* failed)
         tOff[s] = nt;
         nt += mm[s] - w[s];
     }
-    // contributions to row j: t rows of every supernode whose off-diagonal rows hold j
-    std::vector<int> cptr(F.n + 1, 0), cidx(nt);
-    for (int s = 0; s < F.nsuper; ++s)
-        for (int r = w[s]; r < mm[s]; ++r) cptr[F.rows[rowOff[s] + r] + 1]++;
-    for (int j = 0; j < F.n; ++j) cptr[j + 1] += cptr[j];
-    {
-        std::vector<int> pos(cptr.begin(), cptr.end() - 1);
-        for (int s = 0; s < F.nsuper; ++s)
-            for (int r = w[s]; r < mm[s]; ++r) cidx[pos[F.rows[rowOff[s] + r]]++] = tOff[s] + r - w[s];
-    }
+    tOff[F.nsuper] = nt;
+    // forward-sweep extend-add map (children's update rows -> front rows)
+    std::vector<int> fptr, fidx;
+    efem::frontGatherMap(F, tOff, fptr, fidx);
     // work items per level
     std::vector<std::vector<int>> byLevel(F.nlevels);
     for (int s = 0; s < F.nsuper; ++s) byLevel[F.level[s]].push_back(s);
@@ -362,11 +388,21 @@ int factorInternal(efem_ctx* c, int* failed)
     if ((rc = upload(c, &d.snM, mm))) return rc;
     if ((rc = upload(c, &d.snRow, rowOff))) return rc;
     if ((rc = upload(c, &d.snVal, valOff))) return rc;
-    if ((rc = upload(c, &d.snT, tOff))) return rc;
+    if ((rc = upload(c, &d.snT, std::vector<int>(tOff.begin(), tOff.end() - 1)))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
     if ((rc = uploadReal(c, &d.B, F.B, fac32(c)))) return rc;
-    if ((rc = upload(c, &d.cptr, cptr))) return rc;
-    if ((rc = upload(c, &d.cidx, cidx))) return rc;
+    if ((rc = upload(c, &d.fptr, fptr))) return rc;
+    if ((rc = upload(c, &d.fidx, fidx))) return rc;
+    if ((rc = devAlloc(c, &d.in, 3 * static_cast<size_t>(F.n) + 3))) return rc;
+    if ((rc = devAlloc(c, &d.out, 3 * static_cast<size_t>(F.n) + 3))) return rc;
+    if (d.exec) {
+        (void)hipGraphExecDestroy(d.exec);
+        d.exec = nullptr;
+    }
+    if (d.graph) {
+        (void)hipGraphDestroy(d.graph);
+        d.graph = nullptr;
+    }
     if ((rc = upload(c, &d.items, items))) return rc;
     const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
     double* q = nullptr;
@@ -440,6 +476,8 @@ int efem_destroy(efem_ctx* c)
 {
     if (!c) return EFEM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->fac.exec) (void)hipGraphExecDestroy(c->fac.exec);
+    if (c->fac.graph) (void)hipGraphDestroy(c->fac.graph);
     for (void* p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->evsH0) (void)hipEventDestroy(e);
@@ -627,37 +665,60 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         double* sl = c->S + static_cast<size_t>(loc) * n;
         double* yl = c->Y + static_cast<size_t>(loc) * n;
         efem::dev::launchDiff2(sl, yl, c->x, c->xp, c->r, c->rp, n, c->stream);
-        efem::dev::launchDot2(sl, yl, n, c->work, c->counter, sc + kYs + loc, sc + kYY + loc, c->stream);
-        efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
         if (ncorr < mc) ++ncorr;
         ptr = loc + 1;
-        // two-loop recursion: d = -H r
-        efem::dev::launchScale(c->d, c->r, -1.0, n, c->stream);
-        int j = ptr % mc, prev = -1;
-        for (int i = 0; i < ncorr; ++i) {
-            j = (j + mc - 1) % mc;
-            efem::dev::launchTwoLoopStep(c->d, prev >= 0 ? c->Y + static_cast<size_t>(prev) * n : nullptr,
-                                         prev >= 0 ? sc + kAlpha + prev : nullptr, c->S + static_cast<size_t>(j) * n,
-                                         sc + kYs + j, nullptr, sc + kAlpha + j, 0, n, c->work, c->counter, c->stream);
-            prev = j;
-        }
-        if (prev >= 0) efem::dev::launchAxpyDev(c->d, c->Y + static_cast<size_t>(prev) * n, sc + kAlpha + prev, 1, n, c->stream);
-        if (h0) {
-            if ((rc = applyH0Internal(c, c->d, c->tmp))) return rc;
-            std::swap(c->d, c->tmp);
+        const bool compact = c->compactTwoLoop && mc <= efem::dev::kMaxCompact;
+        if (compact) {
+            efem::dev::launchGramUpdate(c->S, c->Y, n, ncorr, loc, sc + kGram, sc + kYs, sc + kYY + loc, c->work,
+                                        c->counter, c->stream);
         } else {
-            efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
+            efem::dev::launchDot2(sl, yl, n, c->work, c->counter, sc + kYs + loc, sc + kYY + loc, c->stream);
         }
-        prev = -1;
-        for (int i = 0; i < ncorr; ++i) {
-            efem::dev::launchTwoLoopStep(c->d, prev >= 0 ? c->S + static_cast<size_t>(prev) * n : nullptr,
-                                         prev >= 0 ? sc + kCoef + prev : nullptr, c->Y + static_cast<size_t>(j) * n,
-                                         sc + kYs + j, sc + kAlpha + j, sc + kCoef + j, 1, n, c->work, c->counter,
-                                         c->stream);
-            prev = j;
-            j = (j + 1) % mc;
+        efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
+        if (compact) {
+            // two-loop from Gram products: 2 multi-dot passes + 2 multi-axpys
+            efem::dev::launchTwoLoopFirst(c->r, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
+                                          c->d, c->work, c->counter, c->stream);
+            if (h0) {
+                if ((rc = applyH0Internal(c, c->d, c->tmp))) return rc;
+                std::swap(c->d, c->tmp);
+            } else {
+                efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
+            }
+            efem::dev::launchTwoLoopSecond(c->d, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
+                                           sc + kCoef, c->d, c->work, c->counter, c->stream);
+        } else {
+            // two-loop recursion: d = -H r (LBFGSMat::apply_Hv)
+            efem::dev::launchScale(c->d, c->r, -1.0, n, c->stream);
+            int j = ptr % mc, prev = -1;
+            for (int i = 0; i < ncorr; ++i) {
+                j = (j + mc - 1) % mc;
+                efem::dev::launchTwoLoopStep(c->d, prev >= 0 ? c->Y + static_cast<size_t>(prev) * n : nullptr,
+                                             prev >= 0 ? sc + kAlpha + prev : nullptr, c->S + static_cast<size_t>(j) * n,
+                                             sc + kYs + j, nullptr, sc + kAlpha + j, 0, n, c->work, c->counter,
+                                             c->stream);
+                prev = j;
+            }
+            if (prev >= 0)
+                efem::dev::launchAxpyDev(c->d, c->Y + static_cast<size_t>(prev) * n, sc + kAlpha + prev, 1, n, c->stream);
+            if (h0) {
+                if ((rc = applyH0Internal(c, c->d, c->tmp))) return rc;
+                std::swap(c->d, c->tmp);
+            } else {
+                efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
+            }
+            prev = -1;
+            for (int i = 0; i < ncorr; ++i) {
+                efem::dev::launchTwoLoopStep(c->d, prev >= 0 ? c->S + static_cast<size_t>(prev) * n : nullptr,
+                                             prev >= 0 ? sc + kCoef + prev : nullptr, c->Y + static_cast<size_t>(j) * n,
+                                             sc + kYs + j, sc + kAlpha + j, sc + kCoef + j, 1, n, c->work, c->counter,
+                                             c->stream);
+                prev = j;
+                j = (j + 1) % mc;
+            }
+            if (prev >= 0)
+                efem::dev::launchAxpyDev(c->d, c->S + static_cast<size_t>(prev) * n, sc + kCoef + prev, 0, n, c->stream);
         }
-        if (prev >= 0) efem::dev::launchAxpyDev(c->d, c->S + static_cast<size_t>(prev) * n, sc + kCoef + prev, 0, n, c->stream);
         HIP_CHECK(c, hipGetLastError());
         ++k;
     }
@@ -709,6 +770,22 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x, l
         return EFEM_ERR_ARG;
     }
     return EFEM_OK;
+}
+
+int efem_set_option(efem_ctx* c, const char* name, double value)
+{
+    if (!c || !name) return EFEM_ERR_ARG;
+    const std::string k(name);
+    if (k == "h0_graph") {
+        c->useGraph = value != 0.0;
+        return EFEM_OK;
+    }
+    if (k == "two_loop_compact") {
+        c->compactTwoLoop = value != 0.0;
+        return EFEM_OK;
+    }
+    c->err = "efem_set_option: unknown option " + k;
+    return EFEM_ERR_ARG;
 }
 
 int efem_trace(const efem_ctx* c, double* out, int cap)

@@ -13,7 +13,9 @@
 //    bitwise reproducible.
 //  * preconditioner: level-scheduled supernodal triangular solves on the
 //    block-inverse form B_s = [L11^-1 ; L21 L11^-1] with three right-hand
-//    sides (the interleaved x, y, z components).
+//    sides (the interleaved x, y, z components); the forward sweep is
+//    multifrontal (each supernode extend-adds only its children's update
+//    vectors), the backward sweep reads the ancestors' solved values.
 #include <hip/hip_runtime.h>
 
 #include "../material_math.hpp"
@@ -428,6 +430,172 @@ __global__ __launch_bounds__(kThreads) void kTwoLoop(double* __restrict__ q, con
     }
 }
 
+// ---- compact two-loop ------------------------------------------------------
+constexpr int KC = kMaxCompact;
+
+// wave64 butterfly then a fixed pass over the block's 4 waves: the sum is
+// written to sh[k * 4 + wave]; caller syncs
+__device__ __forceinline__ double waveSum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// per-thread sums v[0..cnt) -> block partials -> last block sums the
+// partials in block order into red[0..cnt) (shared); returns true in the
+// last block (all threads), after a __syncthreads
+template <int CNT>
+__device__ __forceinline__ bool multiReduce(double (&v)[CNT], int cnt, double* work, unsigned* counter,
+                                            double* red)
+{
+    __shared__ double sh[CNT * 4];
+    __shared__ int amLast;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < CNT; ++k) {
+        if (k >= cnt) break;
+        const double t = waveSum(v[k]);
+        if (lane == 0) sh[k * 4 + wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+        const int k = threadIdx.x;
+        work[k * kReduceBlocks + blockIdx.x] = ((sh[4 * k] + sh[4 * k + 1]) + sh[4 * k + 2]) + sh[4 * k + 3];
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = atomicAdd(counter, 1u);
+        amLast = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!amLast) return false;
+    __threadfence();
+    if (threadIdx.x < cnt) {
+        const int k = threadIdx.x;
+        double acc = 0.0;
+        for (int b = 0; b < static_cast<int>(gridDim.x); ++b) acc += work[k * kReduceBlocks + b];
+        red[k] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
+__global__ __launch_bounds__(kThreads) void kGramUpdate(const double* __restrict__ S, const double* __restrict__ Y,
+                                                        int n, int nc, int loc, double* G, double* ys, double* yy,
+                                                        double* work, unsigned* counter)
+{
+    __shared__ double red[2 * KC + 2];
+    double v[2 * KC + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * KC + 1; ++k) v[k] = 0.0;
+    const double* sl = S + static_cast<size_t>(loc) * n;
+    const double* yl = Y + static_cast<size_t>(loc) * n;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const double si = sl[i], yi = yl[i];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            if (k >= nc) break;
+            v[k] += si * Y[static_cast<size_t>(k) * n + i];       // s_loc . y_k
+            v[KC + k] += S[static_cast<size_t>(k) * n + i] * yi;  // s_k . y_loc
+        }
+        v[2 * KC] += yi * yi;
+    }
+    // compact the live sums: [0,nc) s_loc.y_k, [nc,2nc) s_k.y_loc, 2nc y.y
+    double c[2 * KC + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * KC + 1; ++k) c[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        if (k >= nc) break;
+        c[k] = v[k];
+        c[nc + k] = v[KC + k];
+    }
+    c[2 * nc] = v[2 * KC];
+    if (!multiReduce<2 * KC + 1>(c, 2 * nc + 1, work, counter, red)) return;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < nc; ++k) {
+            G[loc * KC + k] = red[k];
+            G[k * KC + loc] = red[nc + k];
+        }
+        ys[loc] = red[loc];
+        *yy = red[2 * nc];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
+                                                         int n, int nc, int first, int newest, int mcap,
+                                                         const double* G, const double* ys, const double* alphaIn,
+                                                         double* outCoef, double* work, unsigned* counter)
+{
+    // first: dots of R_k (= S) with q (= r), alpha recurrence newest->oldest
+    // else: dots of R_k (= Y) with q (= H0 output), beta recurrence
+    __shared__ double red[KC];
+    double v[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) v[k] = 0.0;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const double qi = q[i];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            if (k >= nc) break;
+            v[k] += R[static_cast<size_t>(k) * n + i] * qi;
+        }
+    }
+    if (!multiReduce<KC>(v, nc, work, counter, red)) return;
+    if (threadIdx.x == 0) {
+        if (first) {
+            // q_0 = -r: alpha_i = (s_i.q_0 - sum_{k newer} alpha_k s_i.y_k) / ys_i
+            int j = newest;
+            for (int t = 0; t < nc; ++t) {
+                double acc = -red[j];
+                int kk = newest;
+                for (int u = 0; u < t; ++u) {
+                    acc -= outCoef[kk] * G[j * KC + kk];
+                    kk = (kk + mcap - 1) % mcap;
+                }
+                outCoef[j] = acc / ys[j];
+                j = (j + mcap - 1) % mcap;
+            }
+        } else {
+            // beta_i = (y_i.h + sum_{k older} coef_k y_i.s_k) / ys_i ; coef = alpha - beta
+            int oldest = (newest + mcap - nc + 1) % mcap;
+            int j = oldest;
+            for (int t = 0; t < nc; ++t) {
+                double acc = red[j];
+                int kk = oldest;
+                for (int u = 0; u < t; ++u) {
+                    acc += outCoef[kk] * G[kk * KC + j];
+                    kk = (kk + 1) % mcap;
+                }
+                outCoef[j] = alphaIn[j] - acc / ys[j];
+                j = (j + 1) % mcap;
+            }
+        }
+    }
+}
+
+// out = base * sgn_b + sum_k coef_k R_k * sgn_c
+__global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict__ base, double sgnB,
+                                                       const double* __restrict__ R, const double* coef,
+                                                       double sgnC, int n, int nc, double* __restrict__ out)
+{
+    __shared__ double c[KC];
+    if (threadIdx.x < nc) c[threadIdx.x] = sgnC * coef[threadIdx.x];
+    __syncthreads();
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    double v = sgnB * base[i];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        if (k >= nc) break;
+        v += c[k] * R[static_cast<size_t>(k) * n + i];
+    }
+    out[i] = v;
+}
+
 __global__ void kStep(double* x, const double* xp, const double* d, double alpha, int n)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -482,17 +650,18 @@ __global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* _
     __shared__ T sb[kMaxW * 3];
     const int2 it = items[blockIdx.x];
     const int s = it.x, r0 = it.y;
-    const int w = a.snW[s], m = a.snM[s], j0 = a.snFirst[s];
-    const T* t = static_cast<const T*>(a.t);
+    const int w = a.snW[s], m = a.snM[s], j0 = a.snFirst[s], fr0 = a.snRow[s];
+    T* u = static_cast<T*>(a.t);
+    // front entries of the supernode's own columns: b plus the children's
+    // update rows (short lists: at most one entry per child)
     for (int k = threadIdx.x; k < w; k += kThreads) {
-        const int j = j0 + k;
-        const int old = a.perm[j];
+        const int old = a.perm[j0 + k];
         double v0 = b[3 * old], v1 = b[3 * old + 1], v2 = b[3 * old + 2];
-        for (int p = a.cptr[j]; p < a.cptr[j + 1]; ++p) {
-            const T* tv = t + 3 * static_cast<size_t>(a.cidx[p]);
-            v0 -= static_cast<double>(tv[0]);
-            v1 -= static_cast<double>(tv[1]);
-            v2 -= static_cast<double>(tv[2]);
+        for (int p = a.fptr[fr0 + k]; p < a.fptr[fr0 + k + 1]; ++p) {
+            const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+            v0 += static_cast<double>(uv[0]);
+            v1 += static_cast<double>(uv[1]);
+            v2 += static_cast<double>(uv[2]);
         }
         sb[3 * k] = static_cast<T>(v0);
         sb[3 * k + 1] = static_cast<T>(v1);
@@ -510,11 +679,25 @@ __global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* _
         acc1 += bk * sb[3 * k + 1];
         acc2 += bk * sb[3 * k + 2];
     }
-    T* out = r < w ? static_cast<T*>(a.y) + 3 * static_cast<size_t>(j0 + r)
-                   : static_cast<T*>(a.t) + 3 * static_cast<size_t>(a.snT[s] + r - w);
-    out[0] = acc0;
-    out[1] = acc1;
-    out[2] = acc2;
+    if (r < w) {
+        T* out = static_cast<T*>(a.y) + 3 * static_cast<size_t>(j0 + r);
+        out[0] = acc0;
+        out[1] = acc1;
+        out[2] = acc2;
+        return;
+    }
+    // update row: children's contributions to this row minus (L21 L11^-1) f
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int p = a.fptr[fr0 + r]; p < a.fptr[fr0 + r + 1]; ++p) {
+        const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+        f0 += static_cast<double>(uv[0]);
+        f1 += static_cast<double>(uv[1]);
+        f2 += static_cast<double>(uv[2]);
+    }
+    T* out = u + 3 * static_cast<size_t>(a.snT[s] + r - w);
+    out[0] = static_cast<T>(f0) - acc0;
+    out[1] = static_cast<T>(f1) - acc1;
+    out[2] = static_cast<T>(f2) - acc2;
 }
 
 template <typename T>
@@ -644,6 +827,30 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
 {
     kTwoLoop<<<reduceGrid(n), kThreads, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, den, aj, cout, mode, n,
                                                   work, counter);
+}
+
+void launchGramUpdate(const double* S, const double* Y, int n, int nc, int loc, double* G, double* ys, double* yy,
+                      double* work, unsigned* counter, hipStream_t s)
+{
+    kGramUpdate<<<reduceGrid(n), kThreads, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
+}
+
+void launchTwoLoopFirst(const double* r, const double* S, const double* Y, int n, int nc, int newest, int mcap,
+                        const double* G, const double* ys, double* alpha, double* out, double* work,
+                        unsigned* counter, hipStream_t s)
+{
+    kTwoLoopDots<<<reduceGrid(n), kThreads, 0, s>>>(r, S, n, nc, 1, newest, mcap, G, ys, nullptr, alpha, work,
+                                                     counter);
+    if (n > 0) kMultiAxpy<<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, nc, out);
+}
+
+void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
+                         const double* G, const double* ys, const double* alpha, double* coef, double* out,
+                         double* work, unsigned* counter, hipStream_t s)
+{
+    kTwoLoopDots<<<reduceGrid(n), kThreads, 0, s>>>(h, Y, n, nc, 0, newest, mcap, G, ys, alpha, coef, work,
+                                                     counter);
+    if (n > 0) kMultiAxpy<<<blocks(n), kThreads, 0, s>>>(h, 1.0, S, coef, 1.0, n, nc, out);
 }
 
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s)

@@ -47,7 +47,8 @@ void launchGather(const GatherArgs& a, hipStream_t s);
 // deterministic reductions: results written to device scalars; the block
 // partials and an arrival counter live in `work` (see kReduceWorkDoubles)
 constexpr int kReduceBlocks = 512;
-constexpr int kReduceWorkDoubles = 4 * kReduceBlocks + 16;
+constexpr int kMaxCompact = 16;  // history size handled by the compact two-loop
+constexpr int kReduceWorkDoubles = (2 * kMaxCompact + 2) * kReduceBlocks + 64;
 
 // out[0] = a.b, out[1] = max|a|, out[2] = max|c|  (c may be null)
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work,
@@ -67,6 +68,22 @@ void launchScale(double* y, const double* x, double a, int n, hipStream_t s);
 void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, const double* w,
                        const double* den, const double* aj, double* cout, int mode, int n,
                        double* work, unsigned* counter, hipStream_t s);
+// compact two-loop (same recursion as LBFGSMat::apply_Hv, evaluated from
+// Gram products).  History slots 0..nc-1 of the ring S, Y ([m][n]), newest
+// slot `newest`; G[i*16+k] = s_i . y_k; rho[i] = 1 / (s_i . y_i).
+// Gram update after a new pair at slot loc: G[loc][k], G[k][loc], ys, yy.
+void launchGramUpdate(const double* S, const double* Y, int n, int nc, int loc, double* G, double* ys,
+                      double* yy, double* work, unsigned* counter, hipStream_t s);
+// loop 1: alpha_i = (s_i . q_i) / ys_i newest -> oldest with q_0 = -r, then
+// out = -r - sum alpha_i y_i
+void launchTwoLoopFirst(const double* r, const double* S, const double* Y, int n, int nc, int newest,
+                        int mcap, const double* G, const double* ys, double* alpha, double* out, double* work,
+                        unsigned* counter, hipStream_t s);
+// loop 2: beta_i = (y_i . r_i) / ys_i oldest -> newest with r_0 = h (the H0
+// output), then out = h + sum (alpha_i - beta_i) s_i
+void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest,
+                         int mcap, const double* G, const double* ys, const double* alpha, double* coef,
+                         double* out, double* work, unsigned* counter, hipStream_t s);
 // q += (*c) * v   (or q -= when neg)
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s);
 // q /= (*theta)
@@ -83,16 +100,18 @@ struct TrsvArgs {
     const int* snM;         // [S]
     const int* snRow;       // [S] offset into rows
     const long long* snVal; // [S] offset into B
-    const int* snT;         // [S] offset into t (rows)
+    const int* snT;         // [S] offset into the update vectors (rows)
     const int* rows;
     const void* B;          // double or float
-    const int* cptr;        // [n+1] contributions to new row j
-    const int* cidx;        // t row positions
+    const int* fptr;        // [sum m_s + 1] per front row (snRow[s] + r): children's
+    const int* fidx;        //   update rows that extend-add into it
     void* y;                // [n][3] work (T)
-    void* t;                // [nt][3] work (T)
+    void* t;                // [nt][3] update vectors u_s (T)
     void* z;                // [n][3] work (T)
 };
-// one level of the forward solve over work items (supernode, first row)
+// one level of the forward solve over work items (supernode, first row):
+// multifrontal form f_s = [b_s; 0] + sum_children u_c (extend-add),
+// y_s = L11^-1 f_s[:w], u_s = f_s[w:] - (L21 L11^-1) f_s[:w]
 void launchTrsvForward(const TrsvArgs& a, const double* b, const int2* items, int nitems,
                        hipStream_t s);
 void launchTrsvBackward(const TrsvArgs& a, const int2* items, int nitems, hipStream_t s);

@@ -477,37 +477,34 @@ void factorSolve(const SupernodalFactor& F, const double* b, double* x)
 void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x)
 {
     const int n = F.n;
-    std::vector<int> tOff(F.nsuper + 1, 0);
+    std::vector<int> uOff(F.nsuper + 1, 0);
     for (int s = 0; s < F.nsuper; ++s) {
         const int w = F.first[s + 1] - F.first[s], m = F.rowPtr[s + 1] - F.rowPtr[s];
-        tOff[s + 1] = tOff[s] + (m - w);
+        uOff[s + 1] = uOff[s] + (m - w);
     }
-    std::vector<double> t(3 * static_cast<size_t>(tOff[F.nsuper])), y(3 * static_cast<size_t>(n)),
-        z(3 * static_cast<size_t>(n)), bp;
-    // contributions to row j from the t rows of earlier supernodes
-    std::vector<std::vector<int>> contrib(n);
-    for (int s = 0; s < F.nsuper; ++s) {
-        const int w = F.first[s + 1] - F.first[s], m = F.rowPtr[s + 1] - F.rowPtr[s];
-        for (int r = w; r < m; ++r) contrib[F.rows[F.rowPtr[s] + r]].push_back(tOff[s] + r - w);
-    }
+    std::vector<int> fptr, fidx;
+    frontGatherMap(F, uOff, fptr, fidx);
+    std::vector<double> u(3 * static_cast<size_t>(uOff[F.nsuper])), y(3 * static_cast<size_t>(n)),
+        z(3 * static_cast<size_t>(n)), f;
     for (int s = 0; s < F.nsuper; ++s) {
         const int j0 = F.first[s], w = F.first[s + 1] - j0, m = F.rowPtr[s + 1] - F.rowPtr[s];
+        const int fr0 = F.rowPtr[s];
         const double* B = &F.B[F.valPtr[s]];
-        bp.assign(3 * static_cast<size_t>(w), 0.0);
-        for (int k = 0; k < w; ++k)
+        f.assign(3 * static_cast<size_t>(m), 0.0);
+        for (int r = 0; r < m; ++r)
             for (int c = 0; c < 3; ++c) {
-                double v = b[3 * static_cast<size_t>(F.perm[j0 + k]) + c];
-                for (int p : contrib[j0 + k]) v -= t[3 * static_cast<size_t>(p) + c];
-                bp[3 * k + c] = v;
+                double v = r < w ? b[3 * static_cast<size_t>(F.perm[j0 + r]) + c] : 0.0;
+                for (int p = fptr[fr0 + r]; p < fptr[fr0 + r + 1]; ++p) v += u[3 * static_cast<size_t>(fidx[p]) + c];
+                f[3 * r + c] = v;
             }
         for (int r = 0; r < m; ++r)
             for (int c = 0; c < 3; ++c) {
                 double acc = 0.0;
-                for (int k = 0; k < w; ++k) acc += B[static_cast<size_t>(k) * m + r] * bp[3 * k + c];
+                for (int k = 0; k < w; ++k) acc += B[static_cast<size_t>(k) * m + r] * f[3 * k + c];
                 if (r < w)
                     y[3 * static_cast<size_t>(j0 + r) + c] = acc;
                 else
-                    t[3 * static_cast<size_t>(tOff[s] + r - w) + c] = acc;
+                    u[3 * static_cast<size_t>(uOff[s] + r - w) + c] = f[3 * r + c] - acc;
             }
     }
     for (int s = F.nsuper - 1; s >= 0; --s) {
@@ -526,6 +523,37 @@ void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x)
     }
     for (int j = 0; j < n; ++j)
         for (int c = 0; c < 3; ++c) x[3 * static_cast<size_t>(F.perm[j]) + c] = z[3 * static_cast<size_t>(j) + c];
+}
+
+void frontGatherMap(const SupernodalFactor& F, const std::vector<int>& uOff, std::vector<int>& fptr,
+                    std::vector<int>& fidx)
+{
+    // front row (rowPtr[s] + r) <- update rows of the children of s, children
+    // in ascending order, each child's update rows in order
+    const int ns = F.nsuper;
+    std::vector<std::vector<int>> children(ns);
+    for (int s = 0; s < ns; ++s)
+        if (F.parent[s] >= 0) children[F.parent[s]].push_back(s);
+    const int total = F.rowPtr[ns];
+    std::vector<int> cnt(total + 1, 0);
+    std::vector<int> loc(F.n, -1);
+    auto forEach = [&](auto&& fn) {
+        for (int s = 0; s < ns; ++s) {
+            const int r0 = F.rowPtr[s], m = F.rowPtr[s + 1] - r0;
+            for (int r = 0; r < m; ++r) loc[F.rows[r0 + r]] = r;
+            for (int c : children[s]) {
+                const int cw = F.first[c + 1] - F.first[c];
+                const int cm = F.rowPtr[c + 1] - F.rowPtr[c];
+                for (int rr = cw; rr < cm; ++rr) fn(r0 + loc[F.rows[F.rowPtr[c] + rr]], uOff[c] + rr - cw);
+            }
+        }
+    };
+    forEach([&](int frow, int) { cnt[frow + 1]++; });
+    for (int i = 0; i < total; ++i) cnt[i + 1] += cnt[i];
+    fptr = cnt;
+    fidx.assign(cnt[total], 0);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    forEach([&](int frow, int upos) { fidx[pos[frow]++] = upos; });
 }
 
 }  // namespace efem

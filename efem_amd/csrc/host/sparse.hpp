@@ -56,8 +56,15 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFa
 void factorSolve(const SupernodalFactor& F, const double* b, double* x);
 
 // the device algorithm run on the host with 3 interleaved right-hand sides:
-// forward y_s = L11^-1 b'_s, t_s = (L21 L11^-1) b'_s with b'_s gathered from
-// the descendants' t; backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R
+// forward (multifrontal) f_s = [b_s; 0] + sum_children u_c,
+// y_s = L11^-1 f_s[:w], u_s = f_s[w:] - (L21 L11^-1) f_s[:w];
+// backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R
 void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x);
+
+// multifrontal extend-add map of the forward sweep: front row
+// (rowPtr[s] + r) of supernode s receives the update rows uOff[c] + i of
+// its children c; CSR over all front rows
+void frontGatherMap(const SupernodalFactor& F, const std::vector<int>& uOff, std::vector<int>& fptr,
+                    std::vector<int>& fidx);
 
 }  // namespace efem

@@ -158,6 +158,15 @@ int efem_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
 int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
                            long long* info5);
 
+/* tuning / debug options:
+ *   "h0_graph"          1 (default): the preconditioner apply replays a
+ *                       captured HIP graph; 0: plain launches
+ *   "two_loop_compact"  1 (default, history m <= 16): the two-loop recursion
+ *                       is evaluated from Gram products s_i.y_k in two
+ *                       multi-dot passes; 0: one reduction per step exactly
+ *                       as LBFGSMat::apply_Hv orders them */
+int efem_set_option(efem_ctx* ctx, const char* name, double value);
+
 /* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */
 int efem_trace(const efem_ctx* ctx, double* out, int cap);
 

@@ -172,3 +172,49 @@ def test_solve_without_preconditioner_and_fallback():
     om = O.Model(m)
     u_or, st_or = om.solve(epsilon=1e-6, max_iterations=50, use_precond=False)
     assert rel(u, u_or) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_apply_h0_graph_replays_match_eager(prec, rng):
+    # the captured graph is replayed several times with eager work between
+    # the replays; every replay equals the eager result bit for bit
+    m = E.Mesh.workload(2, [30, 6, 6])
+    b1, b2 = rng.standard_normal(m.ndof), rng.standard_normal(m.ndof)
+    u = 1e-4 * rng.standard_normal(m.ndof)
+    with E.Context(m, prec) as ctx:
+        assert ctx.factor()
+        ctx.set_option("h0_graph", 0)
+        e1, e2 = ctx.apply_h0(b1), ctx.apply_h0(b2)
+        ctx.set_option("h0_graph", 1)
+        for _ in range(4):
+            g1 = ctx.apply_h0(b1)
+            ctx.residual(u)
+            g2 = ctx.apply_h0(b2)
+            assert np.array_equal(g1, e1) and np.array_equal(g2, e2)
+
+
+def test_compact_two_loop_matches_sequential():
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    out = {}
+    for compact in (0, 1):
+        with E.Context(m, E.FP64) as ctx:
+            ctx.set_option("two_loop_compact", compact)
+            u10, _ = ctx.solve(epsilon=0.0, max_iterations=10)
+            tr = ctx.trace()
+            u, st = ctx.solve(epsilon=1e-8, max_iterations=3000)
+            out[compact] = (u10, tr, u, st)
+    assert rel(out[1][0], out[0][0]) <= 1e-9
+    assert np.allclose(out[1][1][:, 0], out[0][1][:, 0], rtol=1e-7)
+    assert out[0][3]["rinf"] <= 1e-8 and out[1][3]["rinf"] <= 1e-8
+    assert rel(out[1][2], out[0][2]) <= 1e-6
+
+
+def test_compact_two_loop_without_preconditioner():
+    m = E.Mesh.workload(1, [10, 2, 2, 0, 100.0])
+    res = []
+    for compact in (0, 1):
+        with E.Context(m) as ctx:
+            ctx.set_option("two_loop_compact", compact)
+            u, st = ctx.solve(epsilon=1e-6, max_iterations=20, use_precond=False)
+            res.append(u)
+    assert rel(res[1], res[0]) <= 1e-8
