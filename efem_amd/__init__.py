@@ -333,11 +333,12 @@ def host_factor_solve(mesh: Mesh, b):
     pb = mesh.problem(FP64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.empty_like(b)
-    info = (C.c_longlong * 5)()
+    info = (C.c_longlong * 8)()
     rc = L.efem_host_factor_solve(C.byref(pb), _ptr(b, C.c_double), _ptr(x, C.c_double), info)
     if rc != EFEM_OK:
         raise EfemError(f"efem_host_factor_solve failed ({rc}): {L.efem_global_error().decode()}")
-    keys = ("nnz", "supernodes", "levels", "flops", "microseconds")
+    keys = ("nnz", "supernodes", "levels", "flops", "microseconds", "numeric_us", "laplacian_us",
+            "ordering_us")
     return x, dict(zip(keys, (int(v) for v in info)))
 
 

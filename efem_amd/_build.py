@@ -45,7 +45,7 @@ def _compile(src: str, device: bool) -> Path:
     if device:
         cmd += ["-x", "hip", *DEVICE]
     else:
-        cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__"]
+        cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-fopenmp"]
     cmd += ["-c", str(s), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -59,7 +59,7 @@ def build(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda a: _compile(*a), SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fopenmp", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

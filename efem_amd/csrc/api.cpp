@@ -372,13 +372,13 @@ int factorInternal(efem_ctx* c, int* failed)
     for (int l = 0; l < F.nlevels; ++l) {
         d.fwdOff[l] = static_cast<int>(items.size());
         for (int s : byLevel[l])
-            for (int r0 = 0; r0 < mm[s]; r0 += 256) items.push_back(make_int2(s, r0));
+            for (int r0 = 0; r0 < mm[s]; r0 += 64) items.push_back(make_int2(s, r0));
         d.fwdCnt[l] = static_cast<int>(items.size()) - d.fwdOff[l];
     }
     for (int l = 0; l < F.nlevels; ++l) {
         d.bwdOff[l] = static_cast<int>(items.size());
         for (int s : byLevel[l])
-            for (int k0 = 0; k0 < w[s]; k0 += 16) items.push_back(make_int2(s, k0));
+            for (int k0 = 0; k0 < w[s]; k0 += 4) items.push_back(make_int2(s, k0));
         d.bwdCnt[l] = static_cast<int>(items.size()) - d.bwdOff[l];
     }
     int rc;
@@ -390,7 +390,18 @@ int factorInternal(efem_ctx* c, int* failed)
     if ((rc = upload(c, &d.snVal, valOff))) return rc;
     if ((rc = upload(c, &d.snT, std::vector<int>(tOff.begin(), tOff.end() - 1)))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
-    if ((rc = uploadReal(c, &d.B, F.B, fac32(c)))) return rc;
+    if (fac32(c)) {
+        std::vector<float> Bf(static_cast<size_t>(F.Bsize));
+        for (long long i = 0; i < F.Bsize; ++i) Bf[static_cast<size_t>(i)] = static_cast<float>(F.B[i]);
+        float* q = nullptr;
+        if ((rc = upload(c, &q, Bf))) return rc;
+        d.B = q;
+    } else {
+        double* q = nullptr;
+        if ((rc = devAlloc(c, &q, static_cast<size_t>(F.Bsize)))) return rc;
+        HIP_CHECK(c, hipMemcpy(q, F.B.get(), static_cast<size_t>(F.Bsize) * sizeof(double), hipMemcpyHostToDevice));
+        d.B = q;
+    }
     if ((rc = upload(c, &d.fptr, fptr))) return rc;
     if ((rc = upload(c, &d.fidx, fidx))) return rc;
     if ((rc = devAlloc(c, &d.in, 3 * static_cast<size_t>(F.n) + 3))) return rc;
@@ -749,9 +760,12 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x, l
             g_error = "no free DOF";
             return EFEM_ERR_STATE;
         }
+        const auto t0 = std::chrono::steady_clock::now();
         efem::Csc A;
         efem::scalarLaplacian(m, A);
+        const auto t1 = std::chrono::steady_clock::now();
         const std::vector<int> perm = efem::nestedDissection(A, efem::freeCoords(m));
+        const auto t2 = std::chrono::steady_clock::now();
         efem::SupernodalFactor F;
         if (!efem::supernodalCholesky(A, perm, F, msg)) {
             g_error = msg;
@@ -759,11 +773,15 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x, l
         }
         efem::blockInverseSolve3(F, b, x);
         if (info) {
+            auto us = [](auto a, auto b) { return static_cast<long long>(std::chrono::duration<double>(b - a).count() * 1e6); };
             info[0] = F.nnzL;
             info[1] = F.nsuper;
             info[2] = F.nlevels;
             info[3] = static_cast<long long>(F.flops);
             info[4] = static_cast<long long>(F.seconds * 1e6);
+            info[5] = static_cast<long long>(F.numericSeconds * 1e6);
+            info[6] = us(t0, t1);
+            info[7] = us(t1, t2);
         }
     } catch (const std::exception& e) {
         g_error = e.what();

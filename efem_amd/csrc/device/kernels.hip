@@ -640,14 +640,13 @@ __global__ void kTheta(const double* ys, const double* yy, double* th)
 
 // ---- supernodal triangular solves -----------------------------------------
 constexpr int kMaxW = 256;   // widest supernode (host caps it)
-constexpr int kFwdRows = 256;
-constexpr int kBwdCols = 16;
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* __restrict__ b,
                                                      const int2* __restrict__ items)
 {
     __shared__ T sb[kMaxW * 3];
+    __shared__ T red[4][64][3];
     const int2 it = items[blockIdx.x];
     const int s = it.x, r0 = it.y;
     const int w = a.snW[s], m = a.snM[s], j0 = a.snFirst[s], fr0 = a.snRow[s];
@@ -668,17 +667,39 @@ __global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* _
         sb[3 * k + 2] = static_cast<T>(v2);
     }
     __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r >= m || threadIdx.x >= kFwdRows) return;
-    const T* Bc = static_cast<const T*>(a.B) + a.snVal[s] + r;
+    // 64 rows per item; the four waves split the k range, then reduce in order
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = r0 + lane;
+    const int kc = (w + 3) >> 2;
+    const int kb = wave * kc;
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    const int kend = r < w ? r + 1 : w;  // L11^-1 is lower triangular
-    for (int k = 0; k < kend; ++k) {
-        const T bk = Bc[static_cast<size_t>(k) * m];
-        acc0 += bk * sb[3 * k];
-        acc1 += bk * sb[3 * k + 1];
-        acc2 += bk * sb[3 * k + 2];
+    if (r < m) {
+        int ke = min(w, kb + kc);
+        if (r < w) ke = min(ke, r + 1);  // L11^-1 is lower triangular
+        const T* Bc = static_cast<const T*>(a.B) + a.snVal[s] + r;
+        int k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            const T b0 = Bc[static_cast<size_t>(k) * m], b1 = Bc[static_cast<size_t>(k + 1) * m];
+            const T b2 = Bc[static_cast<size_t>(k + 2) * m], b3 = Bc[static_cast<size_t>(k + 3) * m];
+            acc0 += b0 * sb[3 * k] + b1 * sb[3 * k + 3] + b2 * sb[3 * k + 6] + b3 * sb[3 * k + 9];
+            acc1 += b0 * sb[3 * k + 1] + b1 * sb[3 * k + 4] + b2 * sb[3 * k + 7] + b3 * sb[3 * k + 10];
+            acc2 += b0 * sb[3 * k + 2] + b1 * sb[3 * k + 5] + b2 * sb[3 * k + 8] + b3 * sb[3 * k + 11];
+        }
+        for (; k < ke; ++k) {
+            const T bk = Bc[static_cast<size_t>(k) * m];
+            acc0 += bk * sb[3 * k];
+            acc1 += bk * sb[3 * k + 1];
+            acc2 += bk * sb[3 * k + 2];
+        }
     }
+    red[wave][lane][0] = acc0;
+    red[wave][lane][1] = acc1;
+    red[wave][lane][2] = acc2;
+    __syncthreads();
+    if (wave != 0 || r >= m) return;
+    acc0 = ((red[0][lane][0] + red[1][lane][0]) + red[2][lane][0]) + red[3][lane][0];
+    acc1 = ((red[0][lane][1] + red[1][lane][1]) + red[2][lane][1]) + red[3][lane][1];
+    acc2 = ((red[0][lane][2] + red[1][lane][2]) + red[2][lane][2]) + red[3][lane][2];
     if (r < w) {
         T* out = static_cast<T*>(a.y) + 3 * static_cast<size_t>(j0 + r);
         out[0] = acc0;
@@ -703,48 +724,37 @@ __global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* _
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kTrsvBwd(TrsvArgs a, const int2* __restrict__ items)
 {
+    // four columns per item, one per wave; lanes stride the rows
     const int2 it = items[blockIdx.x];
-    const int s = it.x, k0 = it.y;
+    const int s = it.x;
     const int w = a.snW[s], m = a.snM[s], j0 = a.snFirst[s];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = it.y + wave;
+    if (k >= w) return;
     const int* rows = a.rows + a.snRow[s];
     const T* y = static_cast<const T*>(a.y);
     T* z = static_cast<T*>(a.z);
-    const T* Bs = static_cast<const T*>(a.B) + a.snVal[s];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int kEnd = min(w, k0 + kBwdCols);
-    for (int k = k0 + wave; k < kEnd; k += kThreads / 64) {
-        const T* col = Bs + static_cast<size_t>(k) * m;
-        T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-        for (int r = k + lane; r < m; r += 64) {
-            const T bk = col[r];
-            T w0, w1, w2;
-            if (r < w) {
-                const T* yy = y + 3 * static_cast<size_t>(j0 + r);
-                w0 = yy[0];
-                w1 = yy[1];
-                w2 = yy[2];
-            } else {
-                const T* zz = z + 3 * static_cast<size_t>(rows[r]);
-                w0 = -zz[0];
-                w1 = -zz[1];
-                w2 = -zz[2];
-            }
-            acc0 += bk * w0;
-            acc1 += bk * w1;
-            acc2 += bk * w2;
-        }
+    const T* col = static_cast<const T*>(a.B) + a.snVal[s] + static_cast<size_t>(k) * m;
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    for (int r = k + lane; r < m; r += 64) {
+        const T bk = col[r];
+        const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
+        const T sg = r < w ? T(1) : T(-1);
+        acc0 += bk * (sg * src[0]);
+        acc1 += bk * (sg * src[1]);
+        acc2 += bk * (sg * src[2]);
+    }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            acc0 += __shfl_xor(acc0, off, 64);
-            acc1 += __shfl_xor(acc1, off, 64);
-            acc2 += __shfl_xor(acc2, off, 64);
-        }
-        if (lane == 0) {
-            T* zo = z + 3 * static_cast<size_t>(j0 + k);
-            zo[0] = acc0;
-            zo[1] = acc1;
-            zo[2] = acc2;
-        }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+        acc2 += __shfl_xor(acc2, off, 64);
+    }
+    if (lane == 0) {
+        T* zo = z + 3 * static_cast<size_t>(j0 + k);
+        zo[0] = acc0;
+        zo[1] = acc1;
+        zo[2] = acc2;
     }
 }
 

@@ -246,64 +246,77 @@ bool buildModel(const efem_problem& pb, Model& m, std::string& msg)
 // the I3 factor dropped)
 void scalarLaplacian(const Model& m, Csc& L)
 {
+    // column j (a free node) collects, over its incident elements in element
+    // order, V w c grad N_a . grad N_j for the element's free nodes a; each
+    // column is independent, so columns run in parallel and every entry is
+    // still summed in element order
     const int nf = m.nFree, nen = m.nen, E = m.nElems, ng = m.ng;
-    std::vector<int> cnt(nf + 1, 0);
+    std::vector<int> eptr(nf + 1, 0), elist;
     for (int e = 0; e < E; ++e)
-        for (int b = 0; b < nen; ++b) {
-            const int jb = m.connDof[static_cast<size_t>(b) * E + e];
-            if (jb >= nf) continue;
-            for (int a = 0; a < nen; ++a)
-                if (m.connDof[static_cast<size_t>(a) * E + e] < nf) cnt[jb + 1]++;
+        for (int a = 0; a < nen; ++a) {
+            const int j = m.connDof[static_cast<size_t>(a) * E + e];
+            if (j < nf) eptr[j + 1]++;
         }
-    for (int j = 0; j < nf; ++j) cnt[j + 1] += cnt[j];
-    std::vector<int> ri(cnt[nf]);
-    std::vector<double> rx(cnt[nf]);
-    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-    std::vector<double> G(static_cast<size_t>(nen) * 3), Ke(static_cast<size_t>(nen) * nen);
-    for (int e = 0; e < E; ++e) {
-        std::fill(Ke.begin(), Ke.end(), 0.0);
-        const double ks = m.vol[e] * m.coeff[e];
-        for (int q = 0; q < ng; ++q) {
-            double Ji[9];
-            for (int k = 0; k < 9; ++k) Ji[k] = m.ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
-            const double* dN = &m.gdN[static_cast<size_t>(q) * nen * 3];
-            for (int a = 0; a < nen; ++a)
-                for (int i = 0; i < 3; ++i)
-                    G[3 * a + i] = Ji[i] * dN[3 * a] + Ji[3 + i] * dN[3 * a + 1] + Ji[6 + i] * dN[3 * a + 2];
-            const double w = ks * m.gw[q];
-            for (int a = 0; a < nen; ++a)
-                for (int b = 0; b < nen; ++b)
-                    Ke[a * nen + b] += w * (G[3 * a] * G[3 * b] + G[3 * a + 1] * G[3 * b + 1] + G[3 * a + 2] * G[3 * b + 2]);
-        }
-        for (int b = 0; b < nen; ++b) {
-            const int jb = m.connDof[static_cast<size_t>(b) * E + e];
-            if (jb >= nf) continue;
+    for (int j = 0; j < nf; ++j) eptr[j + 1] += eptr[j];
+    elist.resize(eptr[nf]);
+    {
+        std::vector<int> pos(eptr.begin(), eptr.end() - 1);
+        for (int e = 0; e < E; ++e)
             for (int a = 0; a < nen; ++a) {
-                const int ia = m.connDof[static_cast<size_t>(a) * E + e];
-                if (ia >= nf) continue;
-                ri[pos[jb]] = ia;
-                rx[pos[jb]++] = Ke[a * nen + b];
+                const int j = m.connDof[static_cast<size_t>(a) * E + e];
+                if (j < nf) elist[pos[j]++] = e * nen + a;  // element, local slot
+            }
+    }
+    std::vector<std::vector<int>> ci(nf);
+    std::vector<std::vector<double>> cx(nf);
+#pragma omp parallel
+    {
+        std::vector<double> G(static_cast<size_t>(nen) * 3);
+        std::vector<std::pair<int, double>> col;
+        std::vector<double> kcol(nen);
+#pragma omp for schedule(dynamic, 256)
+        for (int j = 0; j < nf; ++j) {
+            col.clear();
+            for (int p = eptr[j]; p < eptr[j + 1]; ++p) {
+                const int e = elist[p] / nen, b = elist[p] % nen;
+                std::fill(kcol.begin(), kcol.end(), 0.0);
+                const double ks = m.vol[e] * m.coeff[e];
+                for (int q = 0; q < ng; ++q) {
+                    double Ji[9];
+                    for (int k = 0; k < 9; ++k) Ji[k] = m.ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
+                    const double* dN = &m.gdN[static_cast<size_t>(q) * nen * 3];
+                    for (int a = 0; a < nen; ++a)
+                        for (int i = 0; i < 3; ++i)
+                            G[3 * a + i] = Ji[i] * dN[3 * a] + Ji[3 + i] * dN[3 * a + 1] + Ji[6 + i] * dN[3 * a + 2];
+                    const double w = ks * m.gw[q];
+                    for (int a = 0; a < nen; ++a)
+                        kcol[a] += w * (G[3 * a] * G[3 * b] + G[3 * a + 1] * G[3 * b + 1] + G[3 * a + 2] * G[3 * b + 2]);
+                }
+                for (int a = 0; a < nen; ++a) {
+                    const int ia = m.connDof[static_cast<size_t>(a) * E + e];
+                    if (ia < nf) col.push_back({ia, kcol[a]});
+                }
+            }
+            std::stable_sort(col.begin(), col.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (size_t k = 0; k < col.size();) {
+                size_t k2 = k;
+                double sum = 0.0;
+                while (k2 < col.size() && col[k2].first == col[k].first) sum += col[k2++].second;
+                ci[j].push_back(col[k].first);
+                cx[j].push_back(sum);
+                k = k2;
             }
         }
     }
     L.n = nf;
     L.p.assign(nf + 1, 0);
-    L.i.clear();
-    L.x.clear();
-    std::vector<std::pair<int, double>> col;
+    for (int j = 0; j < nf; ++j) L.p[j + 1] = L.p[j] + static_cast<int>(ci[j].size());
+    L.i.resize(L.p[nf]);
+    L.x.resize(L.p[nf]);
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < nf; ++j) {
-        col.clear();
-        for (int k = cnt[j]; k < cnt[j + 1]; ++k) col.push_back({ri[k], rx[k]});
-        std::stable_sort(col.begin(), col.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-        for (size_t k = 0; k < col.size();) {
-            size_t k2 = k;
-            double s = 0.0;
-            while (k2 < col.size() && col[k2].first == col[k].first) s += col[k2++].second;
-            L.i.push_back(col[k].first);
-            L.x.push_back(s);
-            k = k2;
-        }
-        L.p[j + 1] = static_cast<int>(L.i.size());
+        std::copy(ci[j].begin(), ci[j].end(), L.i.begin() + L.p[j]);
+        std::copy(cx[j].begin(), cx[j].end(), L.x.begin() + L.p[j]);
     }
 }
 

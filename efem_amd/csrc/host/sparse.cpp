@@ -9,6 +9,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <functional>
 #include <numeric>
 
 namespace efem {
@@ -104,8 +106,10 @@ bool potrf(double* A, int n, int ld)
     return true;
 }
 
-// A21 (m x n at rows [n, n+m)) <- A21 L11^-T, L11 the leading n x n lower block
-void trsmRightLT(double* A, int n, int m, int ld)
+// A21 (m x n at rows [n, n+m)) <- A21 L11^-T, L11 the leading n x n lower
+// block.  Rows are independent: row blocks of 128 run as tasks when `par`.
+__attribute__((target_clones("arch=x86-64-v3", "default")))
+void trsmRows(double* A, int n, int ld, int i0, int i1)
 {
     for (int k = 0; k < n; ++k) {
         double* ck = A + static_cast<size_t>(k) * ld + n;
@@ -113,29 +117,113 @@ void trsmRightLT(double* A, int n, int m, int ld)
             const double s = A[static_cast<size_t>(p) * ld + k];  // L(k,p)
             if (s == 0.0) continue;
             const double* cp = A + static_cast<size_t>(p) * ld + n;
-            for (int i = 0; i < m; ++i) ck[i] -= cp[i] * s;
+            for (int i = i0; i < i1; ++i) ck[i] -= cp[i] * s;
         }
         const double ir = 1.0 / A[static_cast<size_t>(k) * ld + k];
-        for (int i = 0; i < m; ++i) ck[i] *= ir;
+        for (int i = i0; i < i1; ++i) ck[i] *= ir;
     }
 }
 
-// U (m x m lower, column-major, ldu) -= A21 A21^T, A21 = rows [n, n+m) of A
-void syrkLower(const double* A, int n, int m, int ld, double* U, int ldu)
+void trsmRightLT(double* A, int n, int m, int ld, bool par)
 {
-    const int KB = 64;
-    for (int k0 = 0; k0 < n; k0 += KB) {
-        const int k1 = std::min(n, k0 + KB);
-        for (int j = 0; j < m; ++j) {
-            double* uj = U + static_cast<size_t>(j) * ldu;
-            for (int k = k0; k < k1; ++k) {
-                const double* ak = A + static_cast<size_t>(k) * ld + n;
-                const double s = ak[j];
-                if (s == 0.0) continue;
-                for (int i = j; i < m; ++i) uj[i] -= ak[i] * s;
+    if (!par) {
+        trsmRows(A, n, ld, 0, m);
+        return;
+    }
+#pragma omp taskloop grainsize(1)
+    for (int i0 = 0; i0 < m; i0 += 128) trsmRows(A, n, ld, i0, std::min(m, i0 + 128));
+}
+
+// C[i][j] -= sum_k A[i][k] B[j][k] for a 64 x 64 (or smaller) tile, A and B
+// rows of column-major panels with leading dimension ld; 8 x 4 register tiles
+__attribute__((target_clones("arch=x86-64-v3", "default")))
+void gemmNTTile(double* C, int ldc, const double* A, const double* B, int ld, int mi, int nj, int kk,
+                bool lowerDiag)
+{
+    for (int j = 0; j < nj; j += 4) {
+        const int jn = std::min(4, nj - j);
+        int i = lowerDiag ? (j / 8) * 8 : 0;
+        for (; i < mi; i += 8) {
+            const int in = std::min(8, mi - i);
+            double acc[4][8] = {};
+            if (in == 8 && jn == 4) {
+                for (int k = 0; k < kk; ++k) {
+                    const double* a = A + static_cast<size_t>(k) * ld + i;
+                    const double* b = B + static_cast<size_t>(k) * ld + j;
+                    for (int jj = 0; jj < 4; ++jj)
+                        for (int ii = 0; ii < 8; ++ii) acc[jj][ii] += a[ii] * b[jj];
+                }
+            } else {
+                for (int k = 0; k < kk; ++k) {
+                    const double* a = A + static_cast<size_t>(k) * ld + i;
+                    const double* b = B + static_cast<size_t>(k) * ld + j;
+                    for (int jj = 0; jj < jn; ++jj)
+                        for (int ii = 0; ii < in; ++ii) acc[jj][ii] += a[ii] * b[jj];
+                }
             }
+            for (int jj = 0; jj < jn; ++jj)
+                for (int ii = 0; ii < in; ++ii) C[static_cast<size_t>(j + jj) * ldc + i + ii] -= acc[jj][ii];
         }
     }
+}
+
+// U (m x m lower, column-major, ldu) -= A21 A21^T, A21 = rows [n, n+m) of A;
+// 64 x 64 tiles of the lower triangle, as tasks when `par`
+void syrkLower(const double* A, int n, int m, int ld, double* U, int ldu, bool par)
+{
+    const int T = 64;
+    const int nt = (m + T - 1) / T;
+    auto tile = [&](int bj, int bi) {
+        const int j0 = bj * T, i0 = bi * T;
+        const int nj = std::min(T, m - j0), mi = std::min(T, m - i0);
+        // the strict upper part of diagonal tiles is computed and ignored
+        gemmNTTile(U + static_cast<size_t>(j0) * ldu + i0, ldu, A + n + i0, A + n + j0, ld, mi, nj, n, bi == bj);
+    };
+    if (!par) {
+        for (int bj = 0; bj < nt; ++bj)
+            for (int bi = bj; bi < nt; ++bi) tile(bj, bi);
+        return;
+    }
+#pragma omp taskloop grainsize(1) collapse(2)
+    for (int bj = 0; bj < nt; ++bj)
+        for (int bi = 0; bi < nt; ++bi)
+            if (bi >= bj) tile(bj, bi);
+}
+
+// B = [L11^-1 ; L21 L11^-1] from the supernode's column-major L block (m x w)
+__attribute__((target_clones("arch=x86-64-v3", "default")))
+void mRows(const double* L, double* B, int w, int m, int i0, int i1)
+{
+    // M(i, k) = sum_{p >= k} L21(i, p) Linv(p, k)
+    for (int k = 0; k < w; ++k) {
+        double* mk = B + static_cast<size_t>(k) * m;
+        for (int p = k; p < w; ++p) {
+            const double s = B[static_cast<size_t>(k) * m + p];  // Linv(p, k)
+            if (s == 0.0) continue;
+            const double* lp = L + static_cast<size_t>(p) * m;
+            for (int i = i0; i < i1; ++i) mk[i] += lp[i] * s;
+        }
+    }
+}
+
+void blockInverse(const double* L, double* B, int w, int m, bool par)
+{
+    // L11^-1 column by column: solve L11 x = e_k
+    for (int k = 0; k < w; ++k) {
+        double* x = B + static_cast<size_t>(k) * m;
+        x[k] = 1.0 / L[static_cast<size_t>(k) * m + k];
+        for (int j = k + 1; j < w; ++j) {
+            double acc = 0.0;
+            for (int p = k; p < j; ++p) acc += L[static_cast<size_t>(p) * m + j] * x[p];
+            x[j] = -acc / L[static_cast<size_t>(j) * m + j];
+        }
+    }
+    if (!par) {
+        mRows(L, B, w, m, w, m);
+        return;
+    }
+#pragma omp taskloop grainsize(1)
+    for (int i0 = w; i0 < m; i0 += 128) mRows(L, B, w, m, i0, std::min(m, i0 + 128));
 }
 
 }  // namespace
@@ -358,79 +446,92 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         F.flops += static_cast<double>(w) * w * w / 3.0 + static_cast<double>(m - w) * w * w +
                    static_cast<double>(m - w) * (m - w) * w;
     }
-    F.L.assign(F.valPtr[ns], 0.0);
-    // numeric multifrontal factorisation; update matrices kept on a stack
-    std::vector<std::vector<double>> stack;
-    std::vector<int> loc(n, -1);
+    // numeric multifrontal factorisation, tree-parallel (OpenMP tasks over
+    // subtrees; big fronts split their dense kernels into tasks too).  Each
+    // supernode's update matrix lives in upd[s] until its parent absorbs it.
+    // B is filled supernode by supernode inside the tasks (no serial zero-fill)
+    F.B.reset(new double[static_cast<size_t>(F.valPtr[ns])]);
+    F.Bsize = F.valPtr[ns];
+    std::vector<std::vector<double>> upd(ns);
+    std::vector<long long> subtreeCols(ns, 0);
     for (int s = 0; s < ns; ++s) {
+        subtreeCols[s] += first[s + 1] - first[s];
+        if (F.parent[s] >= 0) subtreeCols[F.parent[s]] += subtreeCols[s];
+    }
+    std::atomic<int> failed{0};
+    const auto tn0 = std::chrono::steady_clock::now();
+    auto factorFront = [&](int s) {
+        if (failed.load(std::memory_order_relaxed)) return;
         const int j0 = first[s], w = first[s + 1] - j0;
         const int r0 = F.rowPtr[s], m = F.rowPtr[s + 1] - r0;
-        for (int r = 0; r < m; ++r) loc[F.rows[r0 + r]] = r;
+        const int* rows = &F.rows[r0];
+        // row -> front position by binary search in the sorted row set
+        auto locOf = [&](int gi) { return static_cast<int>(std::lower_bound(rows, rows + m, gi) - rows); };
         std::vector<double> front(static_cast<size_t>(m) * m, 0.0);
         for (int j = j0; j < j0 + w; ++j)
-            for (int p = Lp[j]; p < Lp[j + 1]; ++p)
-                front[static_cast<size_t>(loc[j]) * m + loc[Li[p]]] += Lx[p];
-        // children were processed last-first; their updates are on top
-        for (size_t t = children[s].size(); t-- > 0;) {
-            const int c = children[s][t];
-            std::vector<double> U = std::move(stack.back());
-            stack.pop_back();
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) front[static_cast<size_t>(j - j0) * m + locOf(Li[p])] += Lx[p];
+        for (int c : children[s]) {
+            std::vector<double>& U = upd[c];
             const int cw = first[c + 1] - first[c];
             const int cm = F.rowPtr[c + 1] - F.rowPtr[c] - cw;
             const int* cr = &F.rows[F.rowPtr[c] + cw];
+            std::vector<int> cl(cm);
+            for (int b = 0; b < cm; ++b) cl[b] = locOf(cr[b]);
             for (int b = 0; b < cm; ++b) {
-                double* dst = &front[static_cast<size_t>(loc[cr[b]]) * m];
+                double* dst = &front[static_cast<size_t>(cl[b]) * m];
                 const double* src = &U[static_cast<size_t>(b) * cm];
-                for (int a = b; a < cm; ++a) dst[loc[cr[a]]] += src[a];
+                for (int a = b; a < cm; ++a) dst[cl[a]] += src[a];
             }
+            std::vector<double>().swap(U);
         }
         if (!potrf(front.data(), w, m)) {
-            msg = "approximate Jacobian is not positive definite";
-            return false;
+            failed = 1;
+            return;
         }
         const int nr = m - w;
-        trsmRightLT(front.data(), w, nr, m);
-        if (nr > 0) {
+        const bool big = static_cast<double>(nr) * nr * w > 2e7;
+        trsmRightLT(front.data(), w, nr, m, big);
+        if (nr > 0 && F.parent[s] >= 0) {
             std::vector<double> U(static_cast<size_t>(nr) * nr, 0.0);
             for (int b = 0; b < nr; ++b)
                 for (int a = b; a < nr; ++a)
                     U[static_cast<size_t>(b) * nr + a] = front[static_cast<size_t>(w + b) * m + w + a];
-            syrkLower(front.data(), w, nr, m, U.data(), nr);
-            if (F.parent[s] >= 0) stack.push_back(std::move(U));
+            syrkLower(front.data(), w, nr, m, U.data(), nr, big);
+            upd[s] = std::move(U);
         }
-        double* Ls = &F.L[F.valPtr[s]];
-        for (int k = 0; k < w; ++k) {
-            for (int i = 0; i < k; ++i) Ls[static_cast<size_t>(k) * m + i] = 0.0;
-            for (int i = k; i < m; ++i) Ls[static_cast<size_t>(k) * m + i] = front[static_cast<size_t>(k) * m + i];
-        }
-    }
-    // device blocks B_s = [L11^-1 ; L21 L11^-1]
-    F.B.assign(F.valPtr[ns], 0.0);
-    for (int s = 0; s < ns; ++s) {
-        const int w = first[s + 1] - first[s], m = F.rowPtr[s + 1] - F.rowPtr[s];
-        const double* Ls = &F.L[F.valPtr[s]];
+        // the first w columns of the front now hold [L11; L21] (ld = m)
         double* Bs = &F.B[F.valPtr[s]];
-        // L11^-1, column by column: solve L11 x = e_k
-        for (int k = 0; k < w; ++k) {
-            double* x = Bs + static_cast<size_t>(k) * m;
-            x[k] = 1.0 / Ls[static_cast<size_t>(k) * m + k];
-            for (int j = k + 1; j < w; ++j) {
-                double acc = 0.0;
-                for (int p = k; p < j; ++p) acc += Ls[static_cast<size_t>(p) * m + j] * x[p];
-                x[j] = -acc / Ls[static_cast<size_t>(j) * m + j];
+        std::fill(Bs, Bs + static_cast<size_t>(w) * m, 0.0);
+        blockInverse(front.data(), Bs, w, m, big);
+    };
+    // postorder recursion with tasks for large subtrees
+    std::function<void(int)> process = [&](int s) {
+        for (int c : children[s]) {
+            if (subtreeCols[c] > 512) {
+#pragma omp task firstprivate(c) shared(process)
+                process(c);
+            } else {
+                process(c);
             }
         }
-        // M = L21 L11^-1  (rows w..m-1)
-        for (int k = 0; k < w; ++k) {
-            double* mk = Bs + static_cast<size_t>(k) * m + w;
-            for (int p = k; p < w; ++p) {
-                const double s2 = Bs[static_cast<size_t>(k) * m + p];  // Linv(p,k)
-                if (s2 == 0.0) continue;
-                const double* lp = Ls + static_cast<size_t>(p) * m + w;
-                for (int i = 0; i < m - w; ++i) mk[i] += lp[i] * s2;
+#pragma omp taskwait
+        factorFront(s);
+    };
+#pragma omp parallel
+#pragma omp single
+    {
+        for (int s = 0; s < ns; ++s)
+            if (F.parent[s] < 0) {
+#pragma omp task firstprivate(s) shared(process)
+                process(s);
             }
-        }
+#pragma omp taskwait
     }
+    if (failed) {
+        msg = "approximate Jacobian is not positive definite";
+        return false;
+    }
+    F.numericSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tn0).count();
     F.level.assign(ns, 0);
     F.nlevels = 0;
     for (int s = 0; s < ns; ++s) {
@@ -440,38 +541,6 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     F.perm = perm;
     F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return true;
-}
-
-void factorSolve(const SupernodalFactor& F, const double* b, double* x)
-{
-    const int n = F.n;
-    std::vector<double> y(n);
-    for (int k = 0; k < n; ++k) y[k] = b[F.perm[k]];
-    for (int s = 0; s < F.nsuper; ++s) {
-        const int j0 = F.first[s], w = F.first[s + 1] - j0;
-        const int m = F.rowPtr[s + 1] - F.rowPtr[s];
-        const int* rows = &F.rows[F.rowPtr[s]];
-        const double* Ls = &F.L[F.valPtr[s]];
-        for (int k = 0; k < w; ++k) {
-            const double* c = Ls + static_cast<size_t>(k) * m;
-            const double v = y[j0 + k] / c[k];
-            y[j0 + k] = v;
-            for (int i = k + 1; i < m; ++i) y[rows[i]] -= c[i] * v;
-        }
-    }
-    for (int s = F.nsuper - 1; s >= 0; --s) {
-        const int j0 = F.first[s], w = F.first[s + 1] - j0;
-        const int m = F.rowPtr[s + 1] - F.rowPtr[s];
-        const int* rows = &F.rows[F.rowPtr[s]];
-        const double* Ls = &F.L[F.valPtr[s]];
-        for (int k = w - 1; k >= 0; --k) {
-            const double* c = Ls + static_cast<size_t>(k) * m;
-            double v = y[j0 + k];
-            for (int i = k + 1; i < m; ++i) v -= c[i] * y[rows[i]];
-            y[j0 + k] = v / c[k];
-        }
-    }
-    for (int k = 0; k < n; ++k) x[F.perm[k]] = y[k];
 }
 
 void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x)

@@ -15,6 +15,7 @@
 // t_s = L21 L11^-1 b'_s; backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R).
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -34,14 +35,15 @@ struct SupernodalFactor {
     std::vector<int> rowPtr;     // [nsuper+1] offsets into rows
     std::vector<int> rows;       // row set of each supernode (its columns first)
     std::vector<long long> valPtr;  // [nsuper+1] offsets into blocks
-    std::vector<double> L;       // column-major m_s x ns blocks of L
-    std::vector<double> B;       // column-major m_s x ns blocks [L11^-1 ; L21 L11^-1]
+    std::unique_ptr<double[]> B; // column-major m_s x ns blocks [L11^-1 ; L21 L11^-1]
+    long long Bsize = 0;
     std::vector<int> parent;     // supernodal elimination tree (-1 root)
     std::vector<int> level;      // height from the leaves
     int nlevels = 0;
     long long nnzL = 0;          // scalar nonzeros of L incl. relaxed zeros
     double flops = 0;
     double seconds = 0;
+    double numericSeconds = 0;
 };
 
 // scalar Laplacian from element gradients (see model.cpp)
@@ -51,9 +53,6 @@ std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz);
 // returns false if a pivot is not positive (the reference's fallback case)
 bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F,
                         std::string& msg);
-
-// reference-form solve on the host (testing aid): x = A^-1 b, b/x length n
-void factorSolve(const SupernodalFactor& F, const double* b, double* x);
 
 // the device algorithm run on the host with 3 interleaved right-hand sides:
 // forward (multifrontal) f_s = [b_s; 0] + sum_children u_c,

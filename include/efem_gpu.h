@@ -154,9 +154,10 @@ int efem_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
 /* host-only diagnostic (no GPU): assemble, order and factor the approximate
  * Jacobian on the host, then run the device's block-inverse triangular-solve
  * algorithm on the host: x = K^-1 b (interleaved).  info (may be NULL) gets
- * {nnz(L), supernodes, levels, factor flops, factor microseconds}. */
+ * {nnz(L), supernodes, levels, factor flops, factor us (symbolic+numeric),
+ *  numeric us, Laplacian assembly us, ordering us}. */
 int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
-                           long long* info5);
+                           long long* info8);
 
 /* tuning / debug options:
  *   "h0_graph"          1 (default): the preconditioner apply replays a
