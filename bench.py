@@ -15,7 +15,7 @@ search, m = 6, approximate-Jacobian H0) does not converge -- the residual
 grows without bound on this mesh, and on coarser meshes of the same beam
 even with 20 load increments -- so the time-to-solution is measured at the
 largest follower pressure it solves in one step (--pressure, default
-150 Pa) on the same mesh; throughput does not depend on the load level.
+100 Pa) on the same mesh; throughput does not depend on the load level.
 
 Prints ONE JSON line (see the field list at the end of main()).
 """
@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pressure", type=float, default=150.0)
+    ap.add_argument("--pressure", type=float, default=100.0)
     ap.add_argument("--target", type=float, default=1e-6)
     ap.add_argument("--max-iters", type=int, default=20000)
     ap.add_argument("--cpu-evals", type=int, default=30)

@@ -62,6 +62,23 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+KRYLOV_AUTO, KRYLOV_CG, KRYLOV_BICGSTAB = 0, 1, 2
+
+
+class KrylovParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iterations", C.c_int), ("method", C.c_int),
+                ("use_precond", C.c_int)]
+
+
+class KrylovStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("precond_applies", C.c_int),
+                ("converged", C.c_int), ("rel_residual", C.c_double), ("seconds", C.c_double),
+                ("factor_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def load_library(path: Path | str | None = None) -> C.CDLL:
     p = Path(path) if path else LIB_PATH
     if not p.exists():
@@ -89,6 +106,17 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
         "efem_profile_reset": (C.c_int, [vp]),
         "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
+        "efem_tangent": (C.c_int, [vp, _dp, _dp, _dp, C.c_int]),
+        "efem_tangent_solve": (C.c_int, [vp, _dp, _dp, _dp, C.c_int, C.POINTER(KrylovParams),
+                                         C.POINTER(KrylovStats)]),
+        "efem_adjoint": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, C.POINTER(KrylovParams),
+                                   C.POINTER(KrylovStats)]),
+        "efem_newton": (C.c_int, [vp, _dp, C.c_double, C.c_int, C.POINTER(KrylovParams),
+                                  C.POINTER(SolveStats)]),
+        "efem_energy": (C.c_int, [vp, _dp, _dp]),
+        "efem_set_params": (C.c_int, [vp, _dp]),
+        "efem_set_load": (C.c_int, [vp, C.c_double, _dp]),
+        "efem_material_point": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         "efem_mesh_build": (vp, [C.c_int, _dp, C.c_int]),
         "efem_mesh_free": (None, [vp]),
         "efem_mesh_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
@@ -324,6 +352,93 @@ class Context:
 
     def factor_release(self):
         self._check(self._lib.efem_factor_release(self._h), "efem_factor_release")
+
+    # ---- differentiation -------------------------------------------------
+    def tangent(self, u, v, transpose=False) -> np.ndarray:
+        """K_t(u) v (or K_t^T v): the true tangent d r / d u applied to v."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty(self.ndof)
+        self._check(self._lib.efem_tangent(self._h, _ptr(u, C.c_double), _ptr(v, C.c_double),
+                                           _ptr(out, C.c_double), 1 if transpose else 0), "efem_tangent")
+        return out
+
+    @staticmethod
+    def _kp(tol, max_iterations, method, use_precond):
+        return KrylovParams(float(tol), int(max_iterations), int(method), 1 if use_precond else 0)
+
+    def tangent_solve(self, u, b, transpose=False, tol=1e-10, max_iterations=1000, method=KRYLOV_AUTO,
+                      use_precond=True):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.ndof)
+        prm = self._kp(tol, max_iterations, method, use_precond)
+        st = KrylovStats()
+        self._check(self._lib.efem_tangent_solve(self._h, _ptr(u, C.c_double), _ptr(b, C.c_double),
+                                                 _ptr(x, C.c_double), 1 if transpose else 0, C.byref(prm),
+                                                 C.byref(st)), "efem_tangent_solve")
+        return x, st.as_dict()
+
+    def newton(self, u0=None, epsilon=1e-9, max_iterations=50, tol=1e-8, max_krylov=500,
+               method=KRYLOV_AUTO, use_precond=True):
+        """Newton-Krylov on the true tangent until ||r||_inf <= epsilon."""
+        u = np.zeros(self.ndof) if u0 is None else np.array(u0, dtype=np.float64)
+        prm = self._kp(tol, max_krylov, method, use_precond)
+        st = SolveStats()
+        rc = self._lib.efem_newton(self._h, _ptr(u, C.c_double), float(epsilon), int(max_iterations),
+                                   C.byref(prm), C.byref(st))
+        if rc not in (EFEM_OK, EFEM_ERR_NONFINITE):
+            self._check(rc, "efem_newton")
+        d = st.as_dict()
+        d["status"] = rc
+        return u, d
+
+    def adjoint(self, u, g, tol=1e-10, max_iterations=1000, method=KRYLOV_AUTO, use_precond=True):
+        """Adjoint gradient at an equilibrium u for dJ/du = g.  Returns
+        (lambda, grad_params [E,4] or [4], grad_load [3], stats)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        lam = np.empty(self.ndof)
+        gp = np.empty((self.mesh.n_elems, 4) if self.mesh.params_per_elem else (4,))
+        gl = np.zeros(3)
+        prm = self._kp(tol, max_iterations, method, use_precond)
+        st = KrylovStats()
+        self._check(self._lib.efem_adjoint(self._h, _ptr(u, C.c_double), _ptr(g, C.c_double),
+                                           _ptr(lam, C.c_double), _ptr(gp, C.c_double), _ptr(gl, C.c_double),
+                                           C.byref(prm), C.byref(st)), "efem_adjoint")
+        return lam, gp, gl, st.as_dict()
+
+    def energy(self, u) -> float:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        W = C.c_double()
+        self._check(self._lib.efem_energy(self._h, _ptr(u, C.c_double), C.byref(W)), "efem_energy")
+        return W.value
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        if p.size != self.mesh.params.size:
+            raise ValueError(f"params: expected {self.mesh.params.size} values, got {p.size}")
+        self._check(self._lib.efem_set_params(self._h, _ptr(p, C.c_double)), "efem_set_params")
+
+    def set_load(self, pressure=0.0, pdir=None):
+        d = None if pdir is None else np.ascontiguousarray(pdir, dtype=np.float64)
+        self._check(self._lib.efem_set_load(self._h, float(pressure), _ptr(d, C.c_double)), "efem_set_load")
+
+
+def material_point(law, params, F, dF=None):
+    """Host-side P(F), W(F) and, with dF, the tangent product dP = dP/dF [dF]."""
+    L = lib()
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    Fm = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    P = np.empty(9)
+    W = C.c_double()
+    dFm = None if dF is None else np.ascontiguousarray(dF, dtype=np.float64).reshape(9)
+    dP = None if dF is None else np.empty(9)
+    rc = L.efem_material_point(int(law), _ptr(p, C.c_double), _ptr(Fm, C.c_double), _ptr(P, C.c_double),
+                               C.byref(W), _ptr(dFm, C.c_double), _ptr(dP, C.c_double))
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_material_point failed ({rc})")
+    return P.reshape(3, 3), W.value, (None if dP is None else dP.reshape(3, 3))
 
 
 def host_factor_solve(mesh: Mesh, b):

@@ -13,6 +13,8 @@
 
 #include "../../include/efem_gpu.h"
 #include "device/kernels.hpp"
+#include "limits.hpp"
+#include "material_math.hpp"
 #include "host/model.hpp"
 #include "host/sparse.hpp"
 
@@ -25,16 +27,14 @@ struct DeviceFactor {
     int n = 0, nsuper = 0, nlevels = 0;
     long long nnz = 0;
     int* perm = nullptr;
-    int *snFirst = nullptr, *snW = nullptr, *snM = nullptr, *snRow = nullptr, *snT = nullptr;
-    long long* snVal = nullptr;
     int* rows = nullptr;
     void* B = nullptr;
     int *fptr = nullptr, *fidx = nullptr;
     void *y = nullptr, *t = nullptr, *z = nullptr;
-    int2* items = nullptr;
-    std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level
+    int4* items = nullptr;  // descriptor pairs
+    std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level, in items
     // the whole apply captured as one HIP graph over fixed staging buffers
-    double *in = nullptr, *out = nullptr;
+    double *in = nullptr, *out = nullptr, *bp = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -61,6 +61,7 @@ struct efem_ctx {
     double* fslots = nullptr;
     int* faceDof = nullptr;
     double *fA0 = nullptr, *fWN = nullptr, *fG = nullptr;
+    double* fpress = nullptr;  // per-face pressure of a batch (null: model.pressure)
     int *gptr = nullptr, *gslot = nullptr;
     double* fext = nullptr;
     // solver state
@@ -80,6 +81,9 @@ struct efem_ctx {
     std::vector<hipEvent_t> evs, evsH0;
     int evUsed = 0, evH0Used = 0;
     double* xtarget = nullptr;
+    // tangent / Krylov / adjoint work vectors (n + 3 each, allocated on first use)
+    double *ku = nullptr, *kb = nullptr, *kx = nullptr, *kr = nullptr, *krh = nullptr, *kp = nullptr,
+           *kv = nullptr, *ks = nullptr, *kt = nullptr, *kph = nullptr, *ksh = nullptr;
     bool useGraph = true;    // option "h0_graph"
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
 };
@@ -191,8 +195,7 @@ int uploadModel(efem_ctx* c)
     return EFEM_OK;
 }
 
-// r = f_int(x) - f_ext(x); x has 3 trailing zeros (the fixed-node slot)
-int residualInternal(efem_ctx* c, const double* x, double* r)
+efem::dev::ElemArgs elemArgs(efem_ctx* c, const double* x, bool f32)
 {
     const efem::Model& m = c->model;
     efem::dev::ElemArgs ea;
@@ -201,7 +204,6 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
     ea.ng = m.ng;
     ea.law = m.law;
     ea.paramsPerElem = m.paramsPerElem;
-    const bool f32 = elem32(c) && !c->elemFp64;
     ea.mixed = f32;
     ea.connDof = c->connDof;
     ea.ginv = f32 ? c->ginv : c->ginv64;
@@ -211,6 +213,54 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
     ea.params = c->params;
     ea.x = x;
     ea.slots = c->eslots;
+    return ea;
+}
+
+efem::dev::FaceArgs faceArgs(efem_ctx* c, const double* x, const double* v)
+{
+    const efem::Model& m = c->model;
+    efem::dev::FaceArgs fa;
+    fa.nFaces = m.nFaces;
+    fa.nen = m.nen;
+    fa.fq = m.fq;
+    fa.directed = m.directed;
+    fa.pressure = m.pressure;
+    for (int k = 0; k < 3; ++k) fa.pdir[k] = m.pdir[k];
+    fa.faceDof = c->faceDof;
+    fa.A0 = c->fA0;
+    fa.WN = c->fWN;
+    fa.G = c->fG;
+    fa.x = x;
+    fa.v = v;
+    fa.fpress = c->fpress;
+    fa.slots = c->fslots;
+    return fa;
+}
+
+efem::dev::GatherArgs gatherArgs(efem_ctx* c, double* r, bool f32, double ce, double cf, const double* fext)
+{
+    const efem::Model& m = c->model;
+    efem::dev::GatherArgs ga;
+    ga.nFree = m.nFree;
+    ga.nElemSlots = m.nElems * m.nen;
+    ga.mixed = f32;
+    ga.gptr = c->gptr;
+    ga.gslot = c->gslot;
+    ga.eslots = c->eslots;
+    ga.fslots = c->fslots;
+    ga.fext = fext;
+    ga.ce = ce;
+    ga.cf = cf;
+    ga.r = r;
+    return ga;
+}
+
+// r = f_int(x) - f_ext(x); x has 3 trailing zeros (the fixed-node slot)
+int residualInternal(efem_ctx* c, const double* x, double* r)
+{
+    const efem::Model& m = c->model;
+    const bool f32 = elem32(c) && !c->elemFp64;
+    const efem::dev::ElemArgs ea = elemArgs(c, x, f32);
     const bool prof = c->profiling && c->evUsed + 2 <= static_cast<int>(c->evs.size());
     if (prof) HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed], c->stream));
     efem::dev::launchElementForce(ea, c->stream);
@@ -220,33 +270,27 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
     }
     HIP_CHECK(c, hipGetLastError());
     if (m.nFaces > 0) {
-        efem::dev::FaceArgs fa;
-        fa.nFaces = m.nFaces;
-        fa.nen = m.nen;
-        fa.fq = m.fq;
-        fa.directed = m.directed;
-        fa.pressure = m.pressure;
-        for (int k = 0; k < 3; ++k) fa.pdir[k] = m.pdir[k];
-        fa.faceDof = c->faceDof;
-        fa.A0 = c->fA0;
-        fa.WN = c->fWN;
-        fa.G = c->fG;
-        fa.x = x;
-        fa.slots = c->fslots;
-        efem::dev::launchPressure(fa, c->stream);
+        efem::dev::launchPressure(faceArgs(c, x, nullptr), c->stream);
         HIP_CHECK(c, hipGetLastError());
     }
-    efem::dev::GatherArgs ga;
-    ga.nFree = m.nFree;
-    ga.nElemSlots = m.nElems * m.nen;
-    ga.mixed = f32;
-    ga.gptr = c->gptr;
-    ga.gslot = c->gslot;
-    ga.eslots = c->eslots;
-    ga.fslots = c->fslots;
-    ga.fext = c->fext;
-    ga.r = r;
-    efem::dev::launchGather(ga, c->stream);
+    efem::dev::launchGather(gatherArgs(c, r, f32, 1.0, -1.0, c->fext), c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    return EFEM_OK;
+}
+
+// out = K_t(x) v = d r / d x [v] (transpose: K_t^T v).  The element part is
+// symmetric (hyperelastic); only the follower-pressure part has a transpose.
+int tangentInternal(efem_ctx* c, const double* x, const double* v, double* out, int transpose)
+{
+    const efem::Model& m = c->model;
+    const bool f32 = elem32(c);
+    efem::dev::launchElementTangent(elemArgs(c, x, f32), v, c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    if (m.nFaces > 0) {
+        efem::dev::launchPressureTangent(faceArgs(c, x, v), transpose, c->stream);
+        HIP_CHECK(c, hipGetLastError());
+    }
+    efem::dev::launchGather(gatherArgs(c, out, f32, 1.0, -1.0, nullptr), c->stream);
     HIP_CHECK(c, hipGetLastError());
     return EFEM_OK;
 }
@@ -258,12 +302,6 @@ efem::dev::TrsvArgs trsvArgs(efem_ctx* c)
     a.n = f.n;
     a.mixed = fac32(c);
     a.perm = f.perm;
-    a.snFirst = f.snFirst;
-    a.snW = f.snW;
-    a.snM = f.snM;
-    a.snRow = f.snRow;
-    a.snVal = f.snVal;
-    a.snT = f.snT;
     a.rows = f.rows;
     a.B = f.B;
     a.fptr = f.fptr;
@@ -278,10 +316,11 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out)
 {
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
+    efem::dev::launchPermute(a, v, f.bp, c->stream);
     for (int l = 0; l < f.nlevels; ++l)
-        efem::dev::launchTrsvForward(a, v, f.items + f.fwdOff[l], f.fwdCnt[l], c->stream);
+        efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], c->stream);
     for (int l = f.nlevels - 1; l >= 0; --l)
-        efem::dev::launchTrsvBackward(a, f.items + f.bwdOff[l], f.bwdCnt[l], c->stream);
+        efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], c->stream);
     efem::dev::launchUnpermute(a, out, c->stream);
 }
 
@@ -342,18 +381,15 @@ int factorInternal(efem_ctx* c, int* failed)
     d.nsuper = F.nsuper;
     d.nlevels = F.nlevels;
     d.nnz = F.nnzL;
-    std::vector<int> w(F.nsuper), mm(F.nsuper), rowOff(F.nsuper), tOff(F.nsuper + 1);
-    std::vector<long long> valOff(F.nsuper);
+    std::vector<int> w(F.nsuper), mm(F.nsuper), tOff(F.nsuper + 1);
     int nt = 0;
     for (int s = 0; s < F.nsuper; ++s) {
         w[s] = F.first[s + 1] - F.first[s];
-        if (w[s] > 256) {
+        if (w[s] > efem::kMaxSupernodeWidth) {
             c->err = "internal: supernode wider than the device staging buffer";
             return EFEM_ERR_STATE;
         }
         mm[s] = F.rowPtr[s + 1] - F.rowPtr[s];
-        rowOff[s] = F.rowPtr[s];
-        valOff[s] = F.valPtr[s];
         tOff[s] = nt;
         nt += mm[s] - w[s];
     }
@@ -361,34 +397,38 @@ int factorInternal(efem_ctx* c, int* failed)
     // forward-sweep extend-add map (children's update rows -> front rows)
     std::vector<int> fptr, fidx;
     efem::frontGatherMap(F, tOff, fptr, fidx);
-    // work items per level
+    // work-item descriptors per level
     std::vector<std::vector<int>> byLevel(F.nlevels);
     for (int s = 0; s < F.nsuper; ++s) byLevel[F.level[s]].push_back(s);
-    std::vector<int2> items;
+    std::vector<int4> items;
+    auto desc1 = [&](int a0, int a1, long long val) {
+        return make_int4(a0, a1, static_cast<int>(static_cast<unsigned>(val & 0xffffffffLL)),
+                         static_cast<int>(val >> 32));
+    };
     d.fwdOff.assign(F.nlevels, 0);
     d.fwdCnt.assign(F.nlevels, 0);
     d.bwdOff.assign(F.nlevels, 0);
     d.bwdCnt.assign(F.nlevels, 0);
     for (int l = 0; l < F.nlevels; ++l) {
-        d.fwdOff[l] = static_cast<int>(items.size());
+        d.fwdOff[l] = static_cast<int>(items.size() / 2);
         for (int s : byLevel[l])
-            for (int r0 = 0; r0 < mm[s]; r0 += 64) items.push_back(make_int2(s, r0));
-        d.fwdCnt[l] = static_cast<int>(items.size()) - d.fwdOff[l];
+            for (int r0 = 0; r0 < mm[s]; r0 += 64) {
+                items.push_back(make_int4(r0, w[s], mm[s], F.first[s]));
+                items.push_back(desc1(F.rowPtr[s], tOff[s], F.valPtr[s]));
+            }
+        d.fwdCnt[l] = static_cast<int>(items.size() / 2) - d.fwdOff[l];
     }
     for (int l = 0; l < F.nlevels; ++l) {
-        d.bwdOff[l] = static_cast<int>(items.size());
+        d.bwdOff[l] = static_cast<int>(items.size() / 2);
         for (int s : byLevel[l])
-            for (int k0 = 0; k0 < w[s]; k0 += 4) items.push_back(make_int2(s, k0));
-        d.bwdCnt[l] = static_cast<int>(items.size()) - d.bwdOff[l];
+            for (int k0 = 0; k0 < w[s]; k0 += 4) {
+                items.push_back(make_int4(k0, w[s], mm[s], F.first[s]));
+                items.push_back(desc1(F.rowPtr[s], 0, F.valPtr[s]));
+            }
+        d.bwdCnt[l] = static_cast<int>(items.size() / 2) - d.bwdOff[l];
     }
     int rc;
     if ((rc = upload(c, &d.perm, F.perm))) return rc;
-    if ((rc = upload(c, &d.snFirst, std::vector<int>(F.first.begin(), F.first.end() - 1)))) return rc;
-    if ((rc = upload(c, &d.snW, w))) return rc;
-    if ((rc = upload(c, &d.snM, mm))) return rc;
-    if ((rc = upload(c, &d.snRow, rowOff))) return rc;
-    if ((rc = upload(c, &d.snVal, valOff))) return rc;
-    if ((rc = upload(c, &d.snT, std::vector<int>(tOff.begin(), tOff.end() - 1)))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
     if (fac32(c)) {
         std::vector<float> Bf(static_cast<size_t>(F.Bsize));
@@ -406,6 +446,7 @@ int factorInternal(efem_ctx* c, int* failed)
     if ((rc = upload(c, &d.fidx, fidx))) return rc;
     if ((rc = devAlloc(c, &d.in, 3 * static_cast<size_t>(F.n) + 3))) return rc;
     if ((rc = devAlloc(c, &d.out, 3 * static_cast<size_t>(F.n) + 3))) return rc;
+    if ((rc = devAlloc(c, &d.bp, 3 * static_cast<size_t>(F.n) + 3))) return rc;
     if (d.exec) {
         (void)hipGraphExecDestroy(d.exec);
         d.exec = nullptr;
@@ -434,6 +475,166 @@ int readScalars(efem_ctx* c, int count)
 {
     HIP_CHECK(c, hipMemcpyAsync(c->hostScal, c->scal + kOut, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
+}
+
+int ensureKrylov(efem_ctx* c)
+{
+    if (c->ku) return EFEM_OK;
+    const size_t np = static_cast<size_t>(c->model.ndof()) + 3;
+    int rc;
+    for (double** v : {&c->ku, &c->kb, &c->kx, &c->kr, &c->krh, &c->kp, &c->kv, &c->ks, &c->kt, &c->kph, &c->ksh}) {
+        if ((rc = devAlloc(c, v, np))) return rc;
+        HIP_CHECK(c, hipMemset(*v, 0, np * sizeof(double)));
+    }
+    return EFEM_OK;
+}
+
+// (a . b, b . b) on the host
+int dotHost(efem_ctx* c, const double* a, const double* b, double* ab, double* bb)
+{
+    efem::dev::launchDot2(a, b, c->model.ndof(), c->work, c->counter, c->scal + kOut, c->scal + kOut + 1, c->stream);
+    int rc = readScalars(c, 2);
+    if (rc) return rc;
+    *ab = c->hostScal[0];
+    if (bb) *bb = c->hostScal[1];
+    return EFEM_OK;
+}
+
+int precondApply(efem_ctx* c, bool h0, const double* in, double* out, efem_krylov_stats& st)
+{
+    if (!h0) {
+        HIP_CHECK(c, hipMemcpyAsync(out, in, c->model.ndof() * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        return EFEM_OK;
+    }
+    ++st.precond_applies;
+    return applyH0Internal(c, in, out);
+}
+
+// Solve K_t(u) x = b (transpose: K_t^T x = b) on the device vectors ku, kb ->
+// kx with preconditioned CG (symmetric tangent) or BiCGStab.  M = the
+// approximate-Jacobian Cholesky factor (or I if it failed / is disabled).
+int krylovInternal(efem_ctx* c, int transpose, const efem_krylov_params& prm, efem_krylov_stats& st)
+{
+    const int n = c->model.ndof();
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if (prm.use_precond && !c->fac.ready) {
+        int failed = 0;
+        if ((rc = factorInternal(c, &failed))) return rc;
+        st.factor_seconds = c->factorSeconds;
+    }
+    const bool h0 = prm.use_precond && c->fac.ready;
+    int method = prm.method;
+    if (method == EFEM_KRYLOV_AUTO) method = c->model.nFaces > 0 ? EFEM_KRYLOV_BICGSTAB : EFEM_KRYLOV_CG;
+    HIP_CHECK(c, hipMemsetAsync(c->kx, 0, n * sizeof(double), c->stream));
+    double bb = 0.0;
+    if ((rc = dotHost(c, c->kb, c->kb, &bb, nullptr))) return rc;
+    const double bnorm = std::sqrt(bb);
+    st.rel_residual = 0.0;
+    if (bnorm == 0.0 || n == 0) {
+        st.converged = 1;
+        return EFEM_OK;
+    }
+    const double stop = prm.tol * bnorm;
+    const size_t nb = static_cast<size_t>(n) * sizeof(double);
+    auto A = [&](const double* v, double* out) {
+        ++st.matvecs;
+        return tangentInternal(c, c->ku, v, out, transpose);
+    };
+    HIP_CHECK(c, hipMemcpyAsync(c->kr, c->kb, nb, hipMemcpyDeviceToDevice, c->stream));
+    if (method == EFEM_KRYLOV_CG) {
+        // z = M r, p = z
+        if ((rc = precondApply(c, h0, c->kr, c->ks, st))) return rc;
+        HIP_CHECK(c, hipMemcpyAsync(c->kp, c->ks, nb, hipMemcpyDeviceToDevice, c->stream));
+        double rz = 0.0;
+        if ((rc = dotHost(c, c->kr, c->ks, &rz, nullptr))) return rc;
+        for (int it = 1; it <= prm.max_iterations; ++it) {
+            if ((rc = A(c->kp, c->kv))) return rc;
+            double pq = 0.0;
+            if ((rc = dotHost(c, c->kp, c->kv, &pq, nullptr))) return rc;
+            if (!(pq != 0.0) || !std::isfinite(pq)) break;
+            const double alpha = rz / pq;
+            efem::dev::launchAxpby(c->kx, 1.0, c->kx, alpha, c->kp, n, c->stream);
+            efem::dev::launchAxpby(c->kr, 1.0, c->kr, -alpha, c->kv, n, c->stream);
+            double rr = 0.0;
+            if ((rc = dotHost(c, c->kr, c->kr, &rr, nullptr))) return rc;
+            st.iterations = it;
+            st.rel_residual = std::sqrt(rr) / bnorm;
+            if (std::sqrt(rr) <= stop) {
+                st.converged = 1;
+                break;
+            }
+            if ((rc = precondApply(c, h0, c->kr, c->ks, st))) return rc;
+            double rz1 = 0.0;
+            if ((rc = dotHost(c, c->kr, c->ks, &rz1, nullptr))) return rc;
+            const double beta = rz1 / rz;
+            rz = rz1;
+            efem::dev::launchAxpby(c->kp, 1.0, c->ks, beta, c->kp, n, c->stream);
+        }
+    } else {
+        // BiCGStab (van der Vorst) with M applied to p and s
+        HIP_CHECK(c, hipMemcpyAsync(c->krh, c->kb, nb, hipMemcpyDeviceToDevice, c->stream));
+        double rho = 1.0, alpha = 1.0, omega = 1.0;
+        for (int it = 1; it <= prm.max_iterations; ++it) {
+            double rho1 = 0.0, rr = 0.0;
+            if ((rc = dotHost(c, c->krh, c->kr, &rho1, &rr))) return rc;
+            if (!(rho1 != 0.0) || !std::isfinite(rho1)) break;
+            if (it == 1) {
+                HIP_CHECK(c, hipMemcpyAsync(c->kp, c->kr, nb, hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                const double beta = (rho1 / rho) * (alpha / omega);
+                efem::dev::launchAxpby(c->kp, 1.0, c->kp, -omega, c->kv, n, c->stream);
+                efem::dev::launchAxpby(c->kp, 1.0, c->kr, beta, c->kp, n, c->stream);
+            }
+            if ((rc = precondApply(c, h0, c->kp, c->kph, st))) return rc;
+            if ((rc = A(c->kph, c->kv))) return rc;
+            double rv = 0.0;
+            if ((rc = dotHost(c, c->krh, c->kv, &rv, nullptr))) return rc;
+            if (!(rv != 0.0) || !std::isfinite(rv)) break;
+            alpha = rho1 / rv;
+            efem::dev::launchAxpby(c->ks, 1.0, c->kr, -alpha, c->kv, n, c->stream);
+            double ss = 0.0;
+            if ((rc = dotHost(c, c->ks, c->ks, &ss, nullptr))) return rc;
+            st.iterations = it;
+            if (std::sqrt(ss) <= stop) {
+                efem::dev::launchAxpby(c->kx, 1.0, c->kx, alpha, c->kph, n, c->stream);
+                st.rel_residual = std::sqrt(ss) / bnorm;
+                st.converged = 1;
+                break;
+            }
+            if ((rc = precondApply(c, h0, c->ks, c->ksh, st))) return rc;
+            if ((rc = A(c->ksh, c->kt))) return rc;
+            double ts = 0.0, tt = 0.0;
+            if ((rc = dotHost(c, c->ks, c->kt, &ts, &tt))) return rc;
+            if (!(tt > 0.0) || !std::isfinite(tt)) break;
+            omega = ts / tt;
+            efem::dev::launchAxpby(c->kx, 1.0, c->kx, alpha, c->kph, n, c->stream);
+            efem::dev::launchAxpby(c->kx, 1.0, c->kx, omega, c->ksh, n, c->stream);
+            efem::dev::launchAxpby(c->kr, 1.0, c->ks, -omega, c->kt, n, c->stream);
+            rho = rho1;
+            double r2 = 0.0;
+            if ((rc = dotHost(c, c->kr, c->kr, &r2, nullptr))) return rc;
+            st.rel_residual = std::sqrt(r2) / bnorm;
+            if (std::sqrt(r2) <= stop) {
+                st.converged = 1;
+                break;
+            }
+            if (omega == 0.0) break;
+        }
+    }
+    HIP_CHECK(c, hipGetLastError());
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return EFEM_OK;
+}
+
+int checkKrylovParams(efem_ctx* c, const efem_krylov_params* prm)
+{
+    if (!prm || !(prm->tol > 0.0) || prm->max_iterations <= 0 || prm->method < 0 ||
+        prm->method > EFEM_KRYLOV_BICGSTAB) {
+        c->err = "invalid Krylov parameters";
+        return EFEM_ERR_ARG;
+    }
     return EFEM_OK;
 }
 
@@ -858,6 +1059,235 @@ int efem_profile_read(efem_ctx* c, double* ms, int* launches)
     }
     if (ms) *ms = tot;
     if (launches) *launches = c->evUsed / 2;
+    return EFEM_OK;
+}
+
+int efem_tangent(efem_ctx* c, const double* u, const double* v, double* out, int transpose)
+{
+    if (!c || !u || !v || !out) return EFEM_ERR_ARG;
+    int rc = ensureKrylov(c);
+    if (rc) return rc;
+    const size_t nb = static_cast<size_t>(c->model.ndof()) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(c->ku, u, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->kp, v, nb, hipMemcpyHostToDevice, c->stream));
+    if ((rc = tangentInternal(c, c->ku, c->kp, c->kv, transpose))) return rc;
+    HIP_CHECK(c, hipMemcpyAsync(out, c->kv, nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
+}
+
+int efem_tangent_solve(efem_ctx* c, const double* u, const double* b, double* x, int transpose,
+                       const efem_krylov_params* prm, efem_krylov_stats* st)
+{
+    if (!c || !u || !b || !x) return EFEM_ERR_ARG;
+    int rc = checkKrylovParams(c, prm);
+    if (rc) return rc;
+    efem_krylov_stats local;
+    efem_krylov_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof(s));
+    if ((rc = ensureKrylov(c))) return rc;
+    const size_t nb = static_cast<size_t>(c->model.ndof()) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(c->ku, u, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->kb, b, nb, hipMemcpyHostToDevice, c->stream));
+    if ((rc = krylovInternal(c, transpose, *prm, s))) return rc;
+    HIP_CHECK(c, hipMemcpyAsync(x, c->kx, nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
+}
+
+int efem_newton(efem_ctx* c, double* u, double epsilon, int max_iterations, const efem_krylov_params* kp,
+                efem_solve_stats* st)
+{
+    if (!c || !u || max_iterations <= 0 || !(epsilon >= 0.0)) return EFEM_ERR_ARG;
+    int rc = checkKrylovParams(c, kp);
+    if (rc) return rc;
+    efem_solve_stats local;
+    efem_solve_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof(s));
+    if ((rc = ensureKrylov(c))) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = c->model.ndof();
+    const size_t nb = static_cast<size_t>(n) * sizeof(double);
+    double* sc = c->scal;
+    c->elemFp64 = false;
+    c->trace.clear();
+    HIP_CHECK(c, hipMemcpyAsync(c->x, u, nb, hipMemcpyHostToDevice, c->stream));
+    auto resNorms = [&](const double* x, double* r, double* r2, double* rinf, double* xinf) {
+        int e = residualInternal(c, x, r);
+        if (e) return e;
+        ++s.r_evals;
+        efem::dev::launchDotMax(r, r, x, n, c->work, c->counter, sc + kOut, c->stream);
+        if ((e = readScalars(c, 3))) return e;
+        *r2 = c->hostScal[0];
+        *rinf = c->hostScal[1];
+        *xinf = c->hostScal[2];
+        return EFEM_OK;
+    };
+    double r2 = 0.0, rinf = 0.0, xinf = 0.0;
+    if ((rc = resNorms(c->x, c->r, &r2, &rinf, &xinf))) return rc;
+    for (int k = 1; k <= max_iterations && rinf > epsilon && std::isfinite(rinf); ++k) {
+        // K_t(x) dx = -r
+        HIP_CHECK(c, hipMemcpyAsync(c->ku, c->x, nb, hipMemcpyDeviceToDevice, c->stream));
+        efem::dev::launchScale(c->kb, c->r, -1.0, n, c->stream);
+        efem_krylov_stats ks;
+        std::memset(&ks, 0, sizeof(ks));
+        if ((rc = krylovInternal(c, 0, *kp, ks))) return rc;
+        s.factor_seconds += ks.factor_seconds;
+        s.ls_trials += ks.iterations;  // Krylov iterations, summed
+        // backtracking on ||r||_2
+        double step = 1.0;
+        bool accepted = false;
+        for (int t = 0; t < 12; ++t) {
+            efem::dev::launchStep(c->xp, c->x, c->kx, step, n, c->stream);
+            double r2n = 0.0, rinfn = 0.0, xinfn = 0.0;
+            if ((rc = resNorms(c->xp, c->rp, &r2n, &rinfn, &xinfn))) return rc;
+            if (std::isfinite(r2n) && r2n < r2) {
+                std::swap(c->x, c->xp);
+                std::swap(c->r, c->rp);
+                r2 = r2n;
+                rinf = rinfn;
+                xinf = xinfn;
+                accepted = true;
+                break;
+            }
+            step *= 0.5;
+        }
+        s.iterations = k;
+        c->trace.push_back(rinf);
+        c->trace.push_back(step);
+        c->trace.push_back(ks.iterations);
+        if (!accepted) break;
+    }
+    s.rinf = rinf;
+    s.xinf = xinf;
+    HIP_CHECK(c, hipMemcpyAsync(u, c->x, nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    s.solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!std::isfinite(rinf)) {
+        std::memset(u, 0, nb);
+        c->err = "Newton produced a non-finite residual";
+        return EFEM_ERR_NONFINITE;
+    }
+    return EFEM_OK;
+}
+
+int efem_adjoint(efem_ctx* c, const double* u, const double* g, double* lambda, double* grad_params,
+                 double* grad_load, const efem_krylov_params* prm, efem_krylov_stats* st)
+{
+    if (!c || !u || !g) return EFEM_ERR_ARG;
+    int rc = checkKrylovParams(c, prm);
+    if (rc) return rc;
+    efem_krylov_stats local;
+    efem_krylov_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof(s));
+    if ((rc = ensureKrylov(c))) return rc;
+    const efem::Model& m = c->model;
+    const int n = m.ndof();
+    const size_t nb = static_cast<size_t>(n) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(c->ku, u, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->kb, g, nb, hipMemcpyHostToDevice, c->stream));
+    // K_t^T lambda = dJ/du
+    if ((rc = krylovInternal(c, 1, *prm, s))) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (lambda) HIP_CHECK(c, hipMemcpyAsync(lambda, c->kx, nb, hipMemcpyDeviceToHost, c->stream));
+    // dJ/dtheta = -lambda . dr/dtheta; r = f_int - f_pressure - f_const
+    if (grad_params) {
+        double* raw = static_cast<double*>(c->eslots);  // [E][4] FP64 (slots hold >= 12 E doubles)
+        efem::dev::launchElementParamGrad(elemArgs(c, c->ku, false), c->kx, raw, c->stream);
+        HIP_CHECK(c, hipGetLastError());
+        std::vector<double> h(4 * static_cast<size_t>(m.nElems));
+        HIP_CHECK(c, hipMemcpyAsync(h.data(), raw, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        if (m.paramsPerElem) {
+            for (size_t i = 0; i < h.size(); ++i) grad_params[i] = -h[i];
+        } else {
+            for (int k = 0; k < 4; ++k) {
+                double t = 0.0;
+                for (int e = 0; e < m.nElems; ++e) t += h[4 * static_cast<size_t>(e) + k];
+                grad_params[k] = -t;
+            }
+        }
+    }
+    if (grad_load) {
+        // dr/dp = -f_pressure at unit load: dJ/dp = lambda . f_pressure(p = 1)
+        grad_load[0] = grad_load[1] = grad_load[2] = 0.0;
+        for (int comp = 0; comp < (m.directed ? 3 : 1) && m.nFaces > 0; ++comp) {
+            efem::dev::FaceArgs fa = faceArgs(c, c->ku, nullptr);
+            fa.pressure = 1.0;
+            if (m.directed)
+                for (int k = 0; k < 3; ++k) fa.pdir[k] = k == comp ? 1.0 : 0.0;
+            efem::dev::launchPressure(fa, c->stream);
+            efem::dev::launchGather(gatherArgs(c, c->kt, false, 0.0, 1.0, nullptr), c->stream);
+            HIP_CHECK(c, hipGetLastError());
+            double v = 0.0;
+            if ((rc = dotHost(c, c->kx, c->kt, &v, nullptr))) return rc;
+            grad_load[comp] = v;
+        }
+    }
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    s.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return EFEM_OK;
+}
+
+int efem_energy(efem_ctx* c, const double* u, double* W)
+{
+    if (!c || !u || !W) return EFEM_ERR_ARG;
+    int rc = ensureKrylov(c);
+    if (rc) return rc;
+    const efem::Model& m = c->model;
+    const size_t nb = static_cast<size_t>(m.ndof()) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(c->ku, u, nb, hipMemcpyHostToDevice, c->stream));
+    double* per = static_cast<double*>(c->eslots);
+    efem::dev::launchElementEnergy(elemArgs(c, c->ku, false), per, c->stream);
+    efem::dev::launchSumComp(per, m.nElems, c->work, c->counter, c->scal + kOut, c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    if ((rc = readScalars(c, 1))) return rc;
+    *W = c->hostScal[0];
+    return EFEM_OK;
+}
+
+int efem_set_params(efem_ctx* c, const double* params)
+{
+    if (!c || !params) return EFEM_ERR_ARG;
+    std::string msg;
+    if (!efem::setParams(c->model, params, msg)) {
+        c->err = msg;
+        return EFEM_ERR_ARG;
+    }
+    HIP_CHECK(c, hipMemcpy(c->params, c->model.params.data(), c->model.params.size() * sizeof(double),
+                           hipMemcpyHostToDevice));
+    c->fac.ready = false;  // the approximate Jacobian depends on the parameters
+    return EFEM_OK;
+}
+
+int efem_set_load(efem_ctx* c, double pressure, const double* pdir)
+{
+    if (!c) return EFEM_ERR_ARG;
+    efem::Model& m = c->model;
+    if (m.nFaces == 0) {
+        c->err = "efem_set_load: the problem has no pressure faces";
+        return EFEM_ERR_STATE;
+    }
+    if (m.directed != (pdir ? 1 : 0)) {
+        c->err = "efem_set_load: the load kind (follower / directed) is fixed at efem_create";
+        return EFEM_ERR_ARG;
+    }
+    if (pdir)
+        for (int k = 0; k < 3; ++k) m.pdir[k] = pdir[k];
+    else
+        m.pressure = pressure;
+    return EFEM_OK;
+}
+
+int efem_material_point(int law, const double* params, const double* F, double* P, double* W, const double* dF,
+                        double* dP)
+{
+    if (!params || !F || law < EFEM_NEOHOOKEAN || law > EFEM_YEOH || (dP && !dF)) return EFEM_ERR_ARG;
+    double H[9];
+    for (int k = 0; k < 9; ++k) H[k] = F[k] - ((k % 4) == 0 ? 1.0 : 0.0);
+    if (P) efem::piolaFromHd<double>(law, params, H, P);
+    if (W) *W = efem::energyFromH(law, params, H);
+    if (dP) efem::piolaTangent<double>(law, params, H, dF, dP);
     return EFEM_OK;
 }
 

@@ -1,0 +1,528 @@
+// Device building blocks shared by the grid kernels (kernels.hip) and the
+// persistent batched solver (batch.hip): element stress moments and tiles,
+// follower-pressure faces, the deterministic gather, and the supernodal
+// triangular-solve work items.  All are __device__ functions for blocks of
+// kThreads = 256 lanes (four wave64s).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../material_math.hpp"
+#include "../limits.hpp"
+#include "kernels.hpp"
+
+namespace efem {
+namespace dev {
+namespace {
+
+constexpr int kThreads = 256;
+
+// H = Gu J0^-1 is formed in FP64 in both modes: J - 1 (and with it the
+// near-incompressible volumetric stress (2/D1)(J-1) cof F, 2/D1 = 2e6 here)
+// inherits the rounding of H, so H must carry more than FP32's 24 bits.
+// The deviatoric stress and the force moments then run in T.
+template <typename GT>
+__device__ __forceinline__ void gradToH(const GT* Gu, const double* Jd, double* Hd)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Hd[3 * i + j] = static_cast<double>(Gu[3 * i]) * Jd[j] + static_cast<double>(Gu[3 * i + 1]) * Jd[3 + j] +
+                            static_cast<double>(Gu[3 * i + 2]) * Jd[6 + j];
+}
+
+// MODE_FORCE: Q = w V P(H) J0^-T.  MODE_TANGENT: Q = w V dP J0^-T with
+// dP = (dP/dF)[dH] the exact material tangent applied to the direction's
+// gradient dH (forward-mode dual numbers through the same stress code).
+enum { MODE_FORCE = 0, MODE_TANGENT = 1 };
+
+template <typename T, int MODE>
+__device__ __forceinline__ void stressMoment(int law, const double* p, const double* Gu, const double* Gv,
+                                             const double* Jd, double wv, T* Q)
+{
+    double Hd[9];
+    gradToH(Gu, Jd, Hd);
+    T P[9];
+    if (MODE == MODE_TANGENT) {
+        double dHd[9];
+        gradToH(Gv, Jd, dHd);
+        piolaTangent<T>(law, p, Hd, dHd, P);
+    } else {
+        piolaFromHd<T>(law, p, Hd, P);
+    }
+    T Ji[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ji[k] = static_cast<T>(Jd[k]);
+    const T w = static_cast<T>(wv);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            Q[3 * i + k] = w * (P[3 * i] * Ji[3 * k] + P[3 * i + 1] * Ji[3 * k + 1] + P[3 * i + 2] * Ji[3 * k + 2]);
+}
+
+// ---- linear tetrahedra: one lane per element ------------------------------
+template <int MODE>
+__device__ __forceinline__ void tet4Grad(const double* __restrict__ x, const int* id, double* Gu)
+{
+    // dN = (e1, e2, e3, -1 -1 -1): Gu[:,k] = u_k - u_3
+    double u3[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u3[c] = x[3 * id[3] + c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Gu[3 * i + k] = x[3 * id[k] + i] - u3[i];
+}
+
+template <typename T, int LAW, int MODE>
+__device__ __forceinline__ void tet4Lane(int e, int E, const int* __restrict__ connDof,
+                                         const double* __restrict__ ginv, const double* __restrict__ vol, double w0,
+                                         const double* __restrict__ params, int ppe, const double* __restrict__ x,
+                                         const double* __restrict__ v, T* __restrict__ slots)
+{
+    int id[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) id[a] = connDof[a * E + e];
+    double Gu[9], Gv[9];
+    tet4Grad<MODE>(x, id, Gu);
+    if (MODE == MODE_TANGENT) tet4Grad<MODE>(v, id, Gv);
+    double Jd[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + e];
+    T Q[9];
+    stressMoment<T, MODE>(LAW, params + (ppe ? 4 * e : 0), Gu, Gv, Jd, vol[e] * w0, Q);
+    T* out = slots + 12 * static_cast<size_t>(e);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        out[i] = Q[3 * i];
+        out[3 + i] = Q[3 * i + 1];
+        out[6 + i] = Q[3 * i + 2];
+        out[9 + i] = -(Q[3 * i] + Q[3 * i + 1] + Q[3 * i + 2]);
+    }
+}
+
+
+// ---- general elements: LDS-staged (element, Gauss point) lanes ------------
+template <int NG>
+struct Tile {
+    static constexpr int epb = NG >= 4 ? kThreads / NG : 64;  // elements per block
+};
+
+// LDS of one element tile (EPB elements x NG Gauss points over 256 lanes)
+template <typename T, int NEN, int NG, int MODE>
+struct ElemTileLds {
+    static constexpr int EPB = Tile<NG>::epb;
+    static constexpr int NV = MODE == MODE_TANGENT ? 2 : 1;
+    double sdN[NG * NEN * 3];
+    double su[NV][EPB * NEN * 3];
+    T sQ[EPB * NG * 9];
+};
+
+// One tile of elements [e0, min(e0 + EPB, eEnd)), called by all 256 lanes of
+// a block (contains barriers).  The local-gradient table sdN must be written
+// before the first call (the tile's first barrier publishes it).
+template <typename T, int NEN, int NG, int LAW, int MODE>
+__device__ __forceinline__ void elemTile(int E, int e0, int eEnd, const int* __restrict__ connDof,
+                                         const double* __restrict__ ginv, const double* __restrict__ vol,
+                                         const double* __restrict__ gw, const double* __restrict__ params, int ppe,
+                                         const double* __restrict__ x, const double* __restrict__ v,
+                                         T* __restrict__ slots, ElemTileLds<T, NEN, NG, MODE>& L)
+{
+    constexpr int EPB = Tile<NG>::epb;
+    constexpr int NV = ElemTileLds<T, NEN, NG, MODE>::NV;
+    const int tid = threadIdx.x;
+    // relative displacements u_a - u_0 (and the direction's v_a - v_0)
+    for (int k = tid; k < EPB * NEN; k += kThreads) {
+        const int el = k / NEN, a = k - el * NEN, e = e0 + el;
+        if (e < eEnd) {
+            const int id = connDof[a * E + e], id0 = connDof[e];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) L.su[0][3 * k + c] = x[3 * id + c] - x[3 * id0 + c];
+            if (MODE == MODE_TANGENT)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) L.su[NV - 1][3 * k + c] = v[3 * id + c] - v[3 * id0 + c];
+        }
+    }
+    __syncthreads();
+    {
+        const int el = tid / NG, q = tid - el * NG, e = e0 + el;
+        if (el < EPB && e < eEnd) {
+            double G[NV][9];
+            const double* dq = L.sdN + q * NEN * 3;
+#pragma unroll
+            for (int h = 0; h < NV; ++h) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) G[h][k] = 0.0;
+                const double* ue = L.su[h] + el * NEN * 3;
+                for (int a = 1; a < NEN; ++a) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) G[h][3 * i + k] += ue[3 * a + i] * dq[3 * a + k];
+                }
+            }
+            double Jd[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Jd[k] = ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
+            stressMoment<T, MODE>(LAW, params + (ppe ? 4 * e : 0), G[0], G[NV - 1], Jd, vol[e] * gw[q],
+                                  L.sQ + (el * NG + q) * 9);
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < EPB * NEN; k += kThreads) {
+        const int el = k / NEN, a = k - el * NEN, e = e0 + el;
+        if (e >= eEnd) continue;
+        T f[3] = {T(0), T(0), T(0)};
+        for (int q = 0; q < NG; ++q) {
+            const T* Q = L.sQ + (el * NG + q) * 9;
+            const double* d = L.sdN + (q * NEN + a) * 3;
+            const T d0 = static_cast<T>(d[0]), d1 = static_cast<T>(d[1]), d2 = static_cast<T>(d[2]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) f[i] += Q[3 * i] * d0 + Q[3 * i + 1] * d1 + Q[3 * i + 2] * d2;
+        }
+        T* out = slots + (static_cast<size_t>(e) * NEN + a) * 3;
+        out[0] = f[0];
+        out[1] = f[1];
+        out[2] = f[2];
+    }
+}
+
+
+// ---- follower pressure: one lane per face (FP64) --------------------------
+// f_a = p sum_q w_q N_a(xi_q) J F^-T A0 = p sum_q w_q N_a cof(F_q) A0
+// (ExternalConstraint.cpp:829-836); directed: p_vec sum_q w_q N_a |cof F A0|
+// MODE 0: the force.  MODE 1: its tangent applied to a direction v (the
+// pressure load stiffness; cof F is quadratic in F, so dual numbers give the
+// exact derivative).  MODE 2: the transpose of that tangent applied to v,
+// from g(F) = mu . cof(F) A0 with mu = sum_a w N_a v_a:
+//   dg/dF[:,K] = (F (A0 x e_K)) x mu.
+enum { PMODE_FORCE = 0, PMODE_TANGENT = 1, PMODE_TRANSPOSE = 2 };
+
+template <typename S>
+__device__ __forceinline__ void cofactor(const S* Fm, S* cf)
+{
+    cf[0] = Fm[4] * Fm[8] - Fm[5] * Fm[7];
+    cf[1] = Fm[5] * Fm[6] - Fm[3] * Fm[8];
+    cf[2] = Fm[3] * Fm[7] - Fm[4] * Fm[6];
+    cf[3] = Fm[2] * Fm[7] - Fm[1] * Fm[8];
+    cf[4] = Fm[0] * Fm[8] - Fm[2] * Fm[6];
+    cf[5] = Fm[1] * Fm[6] - Fm[0] * Fm[7];
+    cf[6] = Fm[1] * Fm[5] - Fm[2] * Fm[4];
+    cf[7] = Fm[2] * Fm[3] - Fm[0] * Fm[5];
+    cf[8] = Fm[0] * Fm[4] - Fm[1] * Fm[3];
+}
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <int NEN, int MODE>
+__device__ __forceinline__ void pressureFace(const FaceArgs& a, int f)
+{
+    const int F = a.nFaces;
+    double du[NEN][3];
+    double dv[NEN][3];  // MODE 1: direction relative to node 0; MODE 2: v itself
+    const int id0 = a.faceDof[f];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) {
+        const int id = a.faceDof[n * F + f];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            du[n][c] = a.x[3 * id + c] - a.x[3 * id0 + c];
+            if (MODE == PMODE_TANGENT) dv[n][c] = a.v[3 * id + c] - a.v[3 * id0 + c];
+            if (MODE == PMODE_TRANSPOSE) dv[n][c] = a.v[3 * id + c];
+        }
+    }
+    double out[NEN][3];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) out[n][0] = out[n][1] = out[n][2] = 0.0;
+    const double A0[3] = {a.A0[3 * f], a.A0[3 * f + 1], a.A0[3 * f + 2]};
+    for (int q = 0; q < a.fq; ++q) {
+        const double* G = a.G + (static_cast<size_t>(f) * a.fq + q) * NEN * 3;
+        const double* WN = a.WN + (static_cast<size_t>(f) * a.fq + q) * NEN;
+        double Fm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+        for (int n = 1; n < NEN; ++n)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Fm[3 * i + j] += du[n][i] * G[3 * n + j];
+        if (MODE == PMODE_FORCE) {
+            double cf[9];
+            cofactor(Fm, cf);
+            double v[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[i] = cf[3 * i] * A0[0] + cf[3 * i + 1] * A0[1] + cf[3 * i + 2] * A0[2];
+            if (a.directed) {
+                const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                v[0] = nv * a.pdir[0];
+                v[1] = nv * a.pdir[1];
+                v[2] = nv * a.pdir[2];
+            }
+#pragma unroll
+            for (int n = 0; n < NEN; ++n) {
+                const double wn = WN[n];
+                out[n][0] += wn * v[0];
+                out[n][1] += wn * v[1];
+                out[n][2] += wn * v[2];
+            }
+        } else if (MODE == PMODE_TANGENT) {
+            Dual<double> Fd[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Fd[k] = Dual<double>(Fm[k], 0.0);
+#pragma unroll
+            for (int n = 1; n < NEN; ++n)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) Fd[3 * i + j].d += dv[n][i] * G[3 * n + j];
+            Dual<double> cf[9];
+            cofactor(Fd, cf);
+            Dual<double> v[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[i] = cf[3 * i] * A0[0] + cf[3 * i + 1] * A0[1] + cf[3 * i + 2] * A0[2];
+            double dvq[3] = {v[0].d, v[1].d, v[2].d};
+            if (a.directed) {
+                const Dual<double> nv = sqrt_(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                dvq[0] = nv.d * a.pdir[0];
+                dvq[1] = nv.d * a.pdir[1];
+                dvq[2] = nv.d * a.pdir[2];
+            }
+#pragma unroll
+            for (int n = 0; n < NEN; ++n) {
+                const double wn = WN[n];
+                out[n][0] += wn * dvq[0];
+                out[n][1] += wn * dvq[1];
+                out[n][2] += wn * dvq[2];
+            }
+        } else {
+            double mu[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int n = 0; n < NEN; ++n) {
+                mu[0] += WN[n] * dv[n][0];
+                mu[1] += WN[n] * dv[n][1];
+                mu[2] += WN[n] * dv[n][2];
+            }
+            if (a.directed) {
+                // g = (mu . pdir) |cof F A0|: dg/dF as above with mu' = (mu . pdir) v / |v|
+                double cf[9];
+                cofactor(Fm, cf);
+                double v[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) v[i] = cf[3 * i] * A0[0] + cf[3 * i + 1] * A0[1] + cf[3 * i + 2] * A0[2];
+                const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                const double sc = (mu[0] * a.pdir[0] + mu[1] * a.pdir[1] + mu[2] * a.pdir[2]) / nv;
+                mu[0] = sc * v[0];
+                mu[1] = sc * v[1];
+                mu[2] = sc * v[2];
+            }
+            double dg[3][3];  // dg[K][b] = dg/dF[b][K]
+#pragma unroll
+            for (int K = 0; K < 3; ++K) {
+                double ak[3] = {0.0, 0.0, 0.0}, w[3];
+                // A0 x e_K
+                ak[(K + 1) % 3] = A0[(K + 2) % 3];
+                ak[(K + 2) % 3] = -A0[(K + 1) % 3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) w[i] = Fm[3 * i] * ak[0] + Fm[3 * i + 1] * ak[1] + Fm[3 * i + 2] * ak[2];
+                cross3(w, mu, dg[K]);
+            }
+#pragma unroll
+            for (int n = 1; n < NEN; ++n)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const double t = dg[0][b] * G[3 * n] + dg[1][b] * G[3 * n + 1] + dg[2][b] * G[3 * n + 2];
+                    out[n][b] += t;
+                    out[0][b] -= t;
+                }
+        }
+    }
+    double* dst = a.slots + static_cast<size_t>(f) * NEN * 3;
+    const double pr = a.fpress ? a.fpress[f] : a.pressure;
+#pragma unroll
+    for (int n = 0; n < NEN; ++n)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dst[3 * n + c] = pr * out[n][c];
+}
+
+
+// ---- gather assembly ------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void gatherNode(const GatherArgs& a, int i)
+{
+    const T* es = static_cast<const T*>(a.eslots);
+    double se[3] = {0, 0, 0}, sf[3] = {0, 0, 0};
+    const int p1 = a.gptr[i + 1];
+    for (int p = a.gptr[i]; p < p1; ++p) {
+        const int s = a.gslot[p];
+        if (s < a.nElemSlots) {
+            if (a.ce == 0.0) continue;
+            const T* v = es + 3 * static_cast<size_t>(s);
+            se[0] += static_cast<double>(v[0]);
+            se[1] += static_cast<double>(v[1]);
+            se[2] += static_cast<double>(v[2]);
+        } else {
+            const double* v = a.fslots + 3 * static_cast<size_t>(s - a.nElemSlots);
+            sf[0] += v[0];
+            sf[1] += v[1];
+            sf[2] += v[2];
+        }
+    }
+    // residual: ce = 1, cf = -1 (r = f_int - f_pressure - f_const)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double r = a.ce * se[c] + a.cf * sf[c];
+        if (a.fext) r -= a.fext[3 * i + c];
+        a.r[3 * i + c] = r;
+    }
+}
+
+
+constexpr int kMaxW = kMaxSupernodeWidth;
+
+template <typename T>
+struct TrsvFwdLds {
+    T sb[kMaxW * 3];
+    T red[4][64][3];
+};
+
+__device__ __forceinline__ long long itemVal(const int4& d1)
+{
+    return static_cast<long long>(static_cast<unsigned>(d1.z)) | (static_cast<long long>(d1.w) << 32);
+}
+
+// One forward item -- 64 rows [r0, r0 + 64) of supernode s -- by the 256
+// lanes of a block (contains barriers; every lane must call it).
+// Descriptor d0 = {r0, w, m, j0}, d1 = {front row offset, update-vector
+// offset, B offset lo, hi}.  bp is the right-hand side already permuted
+// (bp[3 j + c] = b[3 perm[j] + c]).  Multifrontal: the front of s is its
+// rows of bp plus its children's update vectors (extend-add through
+// fptr/fidx); y_s = L11^-1 f[:w], u_s = f[w:] - (L21 L11^-1) f[:w].
+template <typename T>
+__device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __restrict__ bp, int4 d0, int4 d1,
+                                            TrsvFwdLds<T>& L)
+{
+    const int r0 = d0.x, w = d0.y, m = d0.z, j0 = d0.w, fr0 = d1.x, tb = d1.y;
+    const long long val = itemVal(d1);
+    T* sb = L.sb;
+    T* u = static_cast<T*>(a.t);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = r0 + lane;
+    // update rows: gather the children's contributions first, so that the
+    // dependent index loads overlap the GEMV below
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    if (wave == 0 && r >= w && r < m) {
+        const int p1 = a.fptr[fr0 + r + 1];
+        for (int p = a.fptr[fr0 + r]; p < p1; ++p) {
+            const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+            f0 += static_cast<double>(uv[0]);
+            f1 += static_cast<double>(uv[1]);
+            f2 += static_cast<double>(uv[2]);
+        }
+    }
+    // front entries of the supernode's own columns (short lists: at most one
+    // entry per child)
+    for (int k = threadIdx.x; k < w; k += kThreads) {
+        const double* bk = bp + 3 * static_cast<size_t>(j0 + k);
+        double v0 = bk[0], v1 = bk[1], v2 = bk[2];
+        const int p1 = a.fptr[fr0 + k + 1];
+        for (int p = a.fptr[fr0 + k]; p < p1; ++p) {
+            const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+            v0 += static_cast<double>(uv[0]);
+            v1 += static_cast<double>(uv[1]);
+            v2 += static_cast<double>(uv[2]);
+        }
+        sb[3 * k] = static_cast<T>(v0);
+        sb[3 * k + 1] = static_cast<T>(v1);
+        sb[3 * k + 2] = static_cast<T>(v2);
+    }
+    __syncthreads();
+    // 64 rows per item; the four waves split the k range, then reduce in order
+    const int kc = (w + 3) >> 2;
+    const int kb = wave * kc;
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    if (r < m) {
+        int ke = min(w, kb + kc);
+        if (r < w) ke = min(ke, r + 1);  // L11^-1 is lower triangular
+        const T* Bc = static_cast<const T*>(a.B) + val + r;
+        int k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            const T b0 = Bc[static_cast<size_t>(k) * m], b1 = Bc[static_cast<size_t>(k + 1) * m];
+            const T b2 = Bc[static_cast<size_t>(k + 2) * m], b3 = Bc[static_cast<size_t>(k + 3) * m];
+            acc0 += b0 * sb[3 * k] + b1 * sb[3 * k + 3] + b2 * sb[3 * k + 6] + b3 * sb[3 * k + 9];
+            acc1 += b0 * sb[3 * k + 1] + b1 * sb[3 * k + 4] + b2 * sb[3 * k + 7] + b3 * sb[3 * k + 10];
+            acc2 += b0 * sb[3 * k + 2] + b1 * sb[3 * k + 5] + b2 * sb[3 * k + 8] + b3 * sb[3 * k + 11];
+        }
+        for (; k < ke; ++k) {
+            const T bk = Bc[static_cast<size_t>(k) * m];
+            acc0 += bk * sb[3 * k];
+            acc1 += bk * sb[3 * k + 1];
+            acc2 += bk * sb[3 * k + 2];
+        }
+    }
+    L.red[wave][lane][0] = acc0;
+    L.red[wave][lane][1] = acc1;
+    L.red[wave][lane][2] = acc2;
+    __syncthreads();
+    if (wave != 0 || r >= m) return;
+    acc0 = ((L.red[0][lane][0] + L.red[1][lane][0]) + L.red[2][lane][0]) + L.red[3][lane][0];
+    acc1 = ((L.red[0][lane][1] + L.red[1][lane][1]) + L.red[2][lane][1]) + L.red[3][lane][1];
+    acc2 = ((L.red[0][lane][2] + L.red[1][lane][2]) + L.red[2][lane][2]) + L.red[3][lane][2];
+    if (r < w) {
+        T* out = static_cast<T*>(a.y) + 3 * static_cast<size_t>(j0 + r);
+        out[0] = acc0;
+        out[1] = acc1;
+        out[2] = acc2;
+        return;
+    }
+    T* out = u + 3 * static_cast<size_t>(tb + r - w);
+    out[0] = static_cast<T>(f0) - acc0;
+    out[1] = static_cast<T>(f1) - acc1;
+    out[2] = static_cast<T>(f2) - acc2;
+}
+
+// One backward item: four columns k0 .. k0 + 3 of supernode s, one per wave
+// (no barrier).  Descriptor d0 = {k0, w, m, j0}, d1 = {row offset, 0, B
+// offset lo, hi}.  z_s = L11^-T (y_s - L21^T z_ancestors) =
+// B11^T y_s - B21^T z[rows].
+template <typename T>
+__device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1)
+{
+    const int w = d0.y, m = d0.z, j0 = d0.w;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = d0.x + wave;
+    if (k >= w) return;
+    const int* rows = a.rows + d1.x;
+    const T* y = static_cast<const T*>(a.y);
+    T* z = static_cast<T*>(a.z);
+    const T* col = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * m;
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    for (int r = k + lane; r < m; r += 64) {
+        const T bk = col[r];
+        const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
+        const T sg = r < w ? T(1) : T(-1);
+        acc0 += bk * (sg * src[0]);
+        acc1 += bk * (sg * src[1]);
+        acc2 += bk * (sg * src[2]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+        acc2 += __shfl_xor(acc2, off, 64);
+    }
+    if (lane == 0) {
+        T* zo = z + 3 * static_cast<size_t>(j0 + k);
+        zo[0] = acc0;
+        zo[1] = acc1;
+        zo[2] = acc2;
+    }
+}
+
+}  // namespace
+}  // namespace dev
+}  // namespace efem

@@ -18,88 +18,27 @@
 //    vectors), the backward sweep reads the ancestors' solved values.
 #include <hip/hip_runtime.h>
 
-#include "../material_math.hpp"
-#include "kernels.hpp"
+#include "device_common.hpp"
 
 namespace efem {
 namespace dev {
 namespace {
 
-constexpr int kThreads = 256;
-
-// H = Gu J0^-1 is formed in FP64 in both modes: J - 1 (and with it the
-// near-incompressible volumetric stress (2/D1)(J-1) cof F, 2/D1 = 2e6 here)
-// inherits the rounding of H, so H must carry more than FP32's 24 bits.
-// The deviatoric stress and the force moments then run in T.
-template <typename T, typename GT>
-__device__ __forceinline__ void stressMoment(int law, const double* p, const GT* Gu, const double* Jd,
-                                             double wv, T* Q)
-{
-    double Hd[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            Hd[3 * i + j] = static_cast<double>(Gu[3 * i]) * Jd[j] + static_cast<double>(Gu[3 * i + 1]) * Jd[3 + j] +
-                            static_cast<double>(Gu[3 * i + 2]) * Jd[6 + j];
-    T P[9];
-    piolaFromHd<T>(law, p, Hd, P);
-    T Ji[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Ji[k] = static_cast<T>(Jd[k]);
-    const T w = static_cast<T>(wv);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            Q[3 * i + k] = w * (P[3 * i] * Ji[3 * k] + P[3 * i + 1] * Ji[3 * k + 1] + P[3 * i + 2] * Ji[3 * k + 2]);
-}
-
-// ---- linear tetrahedra: one lane per element ------------------------------
-template <typename T, int LAW>
+template <typename T, int LAW, int MODE>
 __global__ __launch_bounds__(kThreads) void kElemTet4(int E, const int* __restrict__ connDof,
                                                       const double* __restrict__ ginv,
                                                       const double* __restrict__ vol, double w0,
                                                       const double* __restrict__ params, int ppe,
                                                       const double* __restrict__ x,
+                                                      const double* __restrict__ v,
                                                       T* __restrict__ slots)
 {
     const int e = blockIdx.x * kThreads + threadIdx.x;
     if (e >= E) return;
-    int id[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) id[a] = connDof[a * E + e];
-    // dN = (e1, e2, e3, -1 -1 -1): Gu[:,k] = u_k - u_3
-    double u3[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) u3[c] = x[3 * id[3] + c];
-    double Gu[9];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) Gu[3 * i + k] = x[3 * id[k] + i] - u3[i];
-    double Jd[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + e];
-    T Q[9];
-    stressMoment<T, double>(LAW, params + (ppe ? 4 * e : 0), Gu, Jd, vol[e] * w0, Q);
-    T* out = slots + 12 * static_cast<size_t>(e);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        out[i] = Q[3 * i];
-        out[3 + i] = Q[3 * i + 1];
-        out[6 + i] = Q[3 * i + 2];
-        out[9 + i] = -(Q[3 * i] + Q[3 * i + 1] + Q[3 * i + 2]);
-    }
+    tet4Lane<T, LAW, MODE>(e, E, connDof, ginv, vol, w0, params, ppe, x, v, slots);
 }
 
-// ---- general elements: LDS-staged (element, Gauss point) lanes ------------
-template <int NG>
-struct Tile {
-    static constexpr int epb = NG >= 4 ? kThreads / NG : 64;  // elements per block
-};
-
-template <typename T, int NEN, int NG, int LAW>
+template <typename T, int NEN, int NG, int LAW, int MODE>
 __global__ __launch_bounds__(kThreads) void kElemMulti(int E, const int* __restrict__ connDof,
                                                        const double* __restrict__ ginv,
                                                        const double* __restrict__ vol,
@@ -107,68 +46,19 @@ __global__ __launch_bounds__(kThreads) void kElemMulti(int E, const int* __restr
                                                        const double* __restrict__ gw,
                                                        const double* __restrict__ params, int ppe,
                                                        const double* __restrict__ x,
+                                                       const double* __restrict__ v,
                                                        T* __restrict__ slots)
 {
-    constexpr int EPB = Tile<NG>::epb;
     // the kinematics (relative displacements, local gradients, H) are FP64 in
     // both modes; the stress moments Q and the nodal forces use T
-    __shared__ double sdN[NG * NEN * 3];
-    __shared__ double su[EPB * NEN * 3];
-    __shared__ T sQ[EPB * NG * 9];
-    const int tid = threadIdx.x;
-    const int e0 = blockIdx.x * EPB;
-    for (int k = tid; k < NG * NEN * 3; k += kThreads) sdN[k] = gdN[k];
-    // relative displacements u_a - u_0
-    for (int k = tid; k < EPB * NEN; k += kThreads) {
-        const int el = k / NEN, a = k - el * NEN, e = e0 + el;
-        if (e < E) {
-            const int id = connDof[a * E + e], id0 = connDof[e];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) su[3 * k + c] = x[3 * id + c] - x[3 * id0 + c];
-        }
-    }
-    __syncthreads();
-    {
-        const int el = tid / NG, q = tid - el * NG, e = e0 + el;
-        if (el < EPB && e < E) {
-            double Gu[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) Gu[k] = 0.0;
-            const double* ue = su + el * NEN * 3;
-            const double* dq = sdN + q * NEN * 3;
-            for (int a = 1; a < NEN; ++a) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) Gu[3 * i + k] += ue[3 * a + i] * dq[3 * a + k];
-            }
-            double Jd[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) Jd[k] = ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
-            stressMoment<T, double>(LAW, params + (ppe ? 4 * e : 0), Gu, Jd, vol[e] * gw[q], sQ + (el * NG + q) * 9);
-        }
-    }
-    __syncthreads();
-    for (int k = tid; k < EPB * NEN; k += kThreads) {
-        const int el = k / NEN, a = k - el * NEN, e = e0 + el;
-        if (e >= E) continue;
-        T f[3] = {T(0), T(0), T(0)};
-        for (int q = 0; q < NG; ++q) {
-            const T* Q = sQ + (el * NG + q) * 9;
-            const double* d = sdN + (q * NEN + a) * 3;
-            const T d0 = static_cast<T>(d[0]), d1 = static_cast<T>(d[1]), d2 = static_cast<T>(d[2]);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) f[i] += Q[3 * i] * d0 + Q[3 * i + 1] * d1 + Q[3 * i + 2] * d2;
-        }
-        T* out = slots + (static_cast<size_t>(e) * NEN + a) * 3;
-        out[0] = f[0];
-        out[1] = f[1];
-        out[2] = f[2];
-    }
+    __shared__ ElemTileLds<T, NEN, NG, MODE> L;
+    for (int k = threadIdx.x; k < NG * NEN * 3; k += kThreads) L.sdN[k] = gdN[k];
+    elemTile<T, NEN, NG, LAW, MODE>(E, blockIdx.x * Tile<NG>::epb, E, connDof, ginv, vol, gw, params, ppe, x, v,
+                                    slots, L);
 }
 
-template <typename T, int NEN, int NG>
-void launchMultiLaw(const ElemArgs& a, hipStream_t s)
+template <typename T, int NEN, int NG, int MODE>
+void launchMultiLaw(const ElemArgs& a, const double* v, hipStream_t s)
 {
     const int EPB = Tile<NG>::epb;
     const int grid = (a.nElems + EPB - 1) / EPB;
@@ -177,142 +67,131 @@ void launchMultiLaw(const ElemArgs& a, hipStream_t s)
     T* slots = static_cast<T*>(a.slots);
     switch (a.law) {
         case LAW_NH:
-            kElemMulti<T, NEN, NG, LAW_NH><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, gdN, a.gw,
-                                                                      a.params, a.paramsPerElem, a.x, slots);
+            kElemMulti<T, NEN, NG, LAW_NH, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, gdN,
+                                                                            a.gw, a.params, a.paramsPerElem, a.x, v,
+                                                                            slots);
             break;
         case LAW_MR:
-            kElemMulti<T, NEN, NG, LAW_MR><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, gdN, a.gw,
-                                                                      a.params, a.paramsPerElem, a.x, slots);
+            kElemMulti<T, NEN, NG, LAW_MR, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, gdN,
+                                                                            a.gw, a.params, a.paramsPerElem, a.x, v,
+                                                                            slots);
             break;
         default:
-            kElemMulti<T, NEN, NG, LAW_YEOH><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, gdN, a.gw,
-                                                                        a.params, a.paramsPerElem, a.x, slots);
+            kElemMulti<T, NEN, NG, LAW_YEOH, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, gdN,
+                                                                              a.gw, a.params, a.paramsPerElem, a.x,
+                                                                              v, slots);
     }
 }
 
-template <typename T>
-void launchTet4Law(const ElemArgs& a, hipStream_t s)
+template <typename T, int MODE>
+void launchTet4Law(const ElemArgs& a, const double* v, hipStream_t s)
 {
     const int grid = (a.nElems + kThreads - 1) / kThreads;
     const double* ginv = static_cast<const double*>(a.ginv);
     T* slots = static_cast<T*>(a.slots);
     switch (a.law) {
         case LAW_NH:
-            kElemTet4<T, LAW_NH><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                            a.paramsPerElem, a.x, slots);
+            kElemTet4<T, LAW_NH, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
+                                                                  a.paramsPerElem, a.x, v, slots);
             break;
         case LAW_MR:
-            kElemTet4<T, LAW_MR><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                            a.paramsPerElem, a.x, slots);
+            kElemTet4<T, LAW_MR, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
+                                                                  a.paramsPerElem, a.x, v, slots);
             break;
         default:
-            kElemTet4<T, LAW_YEOH><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                              a.paramsPerElem, a.x, slots);
+            kElemTet4<T, LAW_YEOH, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
+                                                                    a.paramsPerElem, a.x, v, slots);
     }
 }
 
-template <typename T>
-void launchElem(const ElemArgs& a, hipStream_t s)
+template <typename T, int MODE>
+void launchElem(const ElemArgs& a, const double* v, hipStream_t s)
 {
-    if (a.nen == 4) return launchTet4Law<T>(a, s);
-    if (a.nen == 8 && a.ng == 8) return launchMultiLaw<T, 8, 8>(a, s);
-    if (a.nen == 8 && a.ng == 1) return launchMultiLaw<T, 8, 1>(a, s);
-    if (a.nen == 10 && a.ng == 4) return launchMultiLaw<T, 10, 4>(a, s);
-    if (a.nen == 10 && a.ng == 1) return launchMultiLaw<T, 10, 1>(a, s);
-    if (a.nen == 20 && a.ng == 27) return launchMultiLaw<T, 20, 27>(a, s);
-    if (a.nen == 20 && a.ng == 8) return launchMultiLaw<T, 20, 8>(a, s);
+    if (a.nen == 4) return launchTet4Law<T, MODE>(a, v, s);
+    if (a.nen == 8 && a.ng == 8) return launchMultiLaw<T, 8, 8, MODE>(a, v, s);
+    if (a.nen == 8 && a.ng == 1) return launchMultiLaw<T, 8, 1, MODE>(a, v, s);
+    if (a.nen == 10 && a.ng == 4) return launchMultiLaw<T, 10, 4, MODE>(a, v, s);
+    if (a.nen == 10 && a.ng == 1) return launchMultiLaw<T, 10, 1, MODE>(a, v, s);
+    if (a.nen == 20 && a.ng == 27) return launchMultiLaw<T, 20, 27, MODE>(a, v, s);
+    if (a.nen == 20 && a.ng == 8) return launchMultiLaw<T, 20, 8, MODE>(a, v, s);
 }
 
-// ---- follower pressure: one lane per face (FP64) --------------------------
-// f_a = p sum_q w_q N_a(xi_q) J F^-T A0 = p sum_q w_q N_a cof(F_q) A0
-// (ExternalConstraint.cpp:829-836); directed: p_vec sum_q w_q N_a |cof F A0|
-template <int NEN>
+template <int NEN, int MODE>
 __global__ __launch_bounds__(kThreads) void kPressure(FaceArgs a)
 {
     const int f = blockIdx.x * kThreads + threadIdx.x;
-    if (f >= a.nFaces) return;
-    const int F = a.nFaces;
-    double du[NEN][3];
-    const int id0 = a.faceDof[f];
-#pragma unroll
-    for (int n = 0; n < NEN; ++n) {
-        const int id = a.faceDof[n * F + f];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) du[n][c] = a.x[3 * id + c] - a.x[3 * id0 + c];
-    }
-    double out[NEN][3];
-#pragma unroll
-    for (int n = 0; n < NEN; ++n) out[n][0] = out[n][1] = out[n][2] = 0.0;
-    const double A0[3] = {a.A0[3 * f], a.A0[3 * f + 1], a.A0[3 * f + 2]};
-    for (int q = 0; q < a.fq; ++q) {
-        const double* G = a.G + (static_cast<size_t>(f) * a.fq + q) * NEN * 3;
-        double Fm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-#pragma unroll
-        for (int n = 1; n < NEN; ++n)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) Fm[3 * i + j] += du[n][i] * G[3 * n + j];
-        double cf[9];
-        cf[0] = Fm[4] * Fm[8] - Fm[5] * Fm[7];
-        cf[1] = Fm[5] * Fm[6] - Fm[3] * Fm[8];
-        cf[2] = Fm[3] * Fm[7] - Fm[4] * Fm[6];
-        cf[3] = Fm[2] * Fm[7] - Fm[1] * Fm[8];
-        cf[4] = Fm[0] * Fm[8] - Fm[2] * Fm[6];
-        cf[5] = Fm[1] * Fm[6] - Fm[0] * Fm[7];
-        cf[6] = Fm[1] * Fm[5] - Fm[2] * Fm[4];
-        cf[7] = Fm[2] * Fm[3] - Fm[0] * Fm[5];
-        cf[8] = Fm[0] * Fm[4] - Fm[1] * Fm[3];
-        double v[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) v[i] = cf[3 * i] * A0[0] + cf[3 * i + 1] * A0[1] + cf[3 * i + 2] * A0[2];
-        if (a.directed) {
-            const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            v[0] = nv * a.pdir[0];
-            v[1] = nv * a.pdir[1];
-            v[2] = nv * a.pdir[2];
-        }
-        const double* WN = a.WN + (static_cast<size_t>(f) * a.fq + q) * NEN;
-#pragma unroll
-        for (int n = 0; n < NEN; ++n) {
-            const double wn = WN[n];
-            out[n][0] += wn * v[0];
-            out[n][1] += wn * v[1];
-            out[n][2] += wn * v[2];
-        }
-    }
-    double* dst = a.slots + static_cast<size_t>(f) * NEN * 3;
-#pragma unroll
-    for (int n = 0; n < NEN; ++n)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) dst[3 * n + c] = a.pressure * out[n][c];
+    if (f < a.nFaces) pressureFace<NEN, MODE>(a, f);
 }
 
-// ---- gather assembly ------------------------------------------------------
+// ---- per-element scalars (FP64, one lane per element) ---------------------
+// MODE 0: energy  out[e] = sum_q w_q V W(F_q)
+// MODE 1: parameter sensitivities of lam . f_int,
+//         out[4e + k] = sum_q w_q V (dP/dp_k)(F_q) : H_lam,q
+// (H_lam = grad lam at the Gauss point).  Used once per adjoint or energy
+// call, so the simple layout (global reads, no LDS) is enough.
+template <int NEN, int MODE>
+__global__ __launch_bounds__(kThreads) void kElemScalar(ElemArgs a, const double* __restrict__ lam,
+                                                        double* __restrict__ out)
+{
+    const int e = blockIdx.x * kThreads + threadIdx.x;
+    const int E = a.nElems;
+    if (e >= E) return;
+    const double* x = a.x;
+    const double* ginv = static_cast<const double*>(a.ginv);
+    const double* gdN = static_cast<const double*>(a.gdN);
+    const double* p = a.params + (a.paramsPerElem ? 4 * e : 0);
+    int id[NEN];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) id[n] = a.connDof[n * E + e];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < a.ng; ++q) {
+        const double* dq = gdN + static_cast<size_t>(q) * NEN * 3;
+        double Gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Gl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int n = 1; n < NEN; ++n)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double du = x[3 * id[n] + i] - x[3 * id[0] + i];
+                const double dl = MODE == 1 ? lam[3 * id[n] + i] - lam[3 * id[0] + i] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    Gu[3 * i + k] += du * dq[3 * n + k];
+                    if (MODE == 1) Gl[3 * i + k] += dl * dq[3 * n + k];
+                }
+            }
+        double Jd[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Jd[k] = ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
+        double Hd[9];
+        gradToH(Gu, Jd, Hd);
+        const double wv = a.vol[e] * a.gw[q];
+        if (MODE == 0) {
+            acc[0] += wv * energyFromH(a.law, p, Hd);
+        } else {
+            double Hl[9];
+            gradToH(Gl, Jd, Hl);
+            for (int k = 0; k < 4; ++k) {
+                double dP[9];
+                piolaParamDerivative<double>(a.law, p, Hd, k, dP);
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < 9; ++j) t += dP[j] * Hl[j];
+                acc[k] += wv * t;
+            }
+        }
+    }
+    if (MODE == 0) {
+        out[e] = acc[0];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[4 * static_cast<size_t>(e) + k] = acc[k];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= a.nFree) return;
-    const T* es = static_cast<const T*>(a.eslots);
-    double se[3] = {0, 0, 0}, sf[3] = {0, 0, 0};
-    const int p1 = a.gptr[i + 1];
-    for (int p = a.gptr[i]; p < p1; ++p) {
-        const int s = a.gslot[p];
-        if (s < a.nElemSlots) {
-            const T* v = es + 3 * static_cast<size_t>(s);
-            se[0] += static_cast<double>(v[0]);
-            se[1] += static_cast<double>(v[1]);
-            se[2] += static_cast<double>(v[2]);
-        } else {
-            const double* v = a.fslots + 3 * static_cast<size_t>(s - a.nElemSlots);
-            sf[0] += v[0];
-            sf[1] += v[1];
-            sf[2] += v[2];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) a.r[3 * i + c] = se[c] - sf[c] - a.fext[3 * i + c];
+    if (i < a.nFree) gatherNode<T>(a, i);
 }
 
 // ---- reductions -------------------------------------------------------------
@@ -431,10 +310,11 @@ __global__ __launch_bounds__(kThreads) void kTwoLoop(double* __restrict__ q, con
 }
 
 // ---- compact two-loop ------------------------------------------------------
-constexpr int KC = kMaxCompact;
+// Kernels are templated on the history capacity KC (8 covers the reference's
+// m = 6; 16 is the maximum) so that every per-thread array has compile-time
+// indexing (no scratch); live slots are guarded by `k < nc`.
 
-// wave64 butterfly then a fixed pass over the block's 4 waves: the sum is
-// written to sh[k * 4 + wave]; caller syncs
+// wave64 butterfly sum (fixed order)
 __device__ __forceinline__ double waveSum(double v)
 {
 #pragma unroll
@@ -442,26 +322,27 @@ __device__ __forceinline__ double waveSum(double v)
     return v;
 }
 
-// per-thread sums v[0..cnt) -> block partials -> last block sums the
-// partials in block order into red[0..cnt) (shared); returns true in the
-// last block (all threads), after a __syncthreads
-template <int CNT>
-__device__ __forceinline__ bool multiReduce(double (&v)[CNT], int cnt, double* work, unsigned* counter,
-                                            double* red)
+// per-thread sums v[0..N) -> block partials (work[k * kReduceBlocks + b]) ->
+// the last block to arrive reduces the partials of every k with all its
+// lanes (lane t takes blocks t, t + 256, ...; then the fixed wave/block
+// tree) into red[0..N) (shared).  Returns true in the last block (all
+// threads), after a barrier.  Deterministic: the shapes depend on the grid
+// only.
+template <int N>
+__device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsigned* counter, double* red)
 {
-    __shared__ double sh[CNT * 4];
+    __shared__ double sh[N][4];
     __shared__ int amLast;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < CNT; ++k) {
-        if (k >= cnt) break;
+    for (int k = 0; k < N; ++k) {
         const double t = waveSum(v[k]);
-        if (lane == 0) sh[k * 4 + wave] = t;
+        if (lane == 0) sh[k][wave] = t;
     }
     __syncthreads();
-    if (threadIdx.x < cnt) {
+    if (threadIdx.x < N) {
         const int k = threadIdx.x;
-        work[k * kReduceBlocks + blockIdx.x] = ((sh[4 * k] + sh[4 * k + 1]) + sh[4 * k + 2]) + sh[4 * k + 3];
+        work[k * kReduceBlocks + blockIdx.x] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
     }
     __threadfence();
     __syncthreads();
@@ -472,22 +353,37 @@ __device__ __forceinline__ bool multiReduce(double (&v)[CNT], int cnt, double* w
     __syncthreads();
     if (!amLast) return false;
     __threadfence();
-    if (threadIdx.x < cnt) {
+    double p[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k] = 0.0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kThreads)
+#pragma unroll
+        for (int k = 0; k < N; ++k) p[k] += work[k * kReduceBlocks + b];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double t = waveSum(p[k]);
+        if (lane == 0) sh[k][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
         const int k = threadIdx.x;
-        double acc = 0.0;
-        for (int b = 0; b < static_cast<int>(gridDim.x); ++b) acc += work[k * kReduceBlocks + b];
-        red[k] = acc;
+        red[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
     }
     __syncthreads();
     if (threadIdx.x == 0) *counter = 0u;
     return true;
 }
 
+// layout of the Gram matrix in the scalar area: G[i * kMaxCompact + k]
+constexpr int KG = kMaxCompact;
+
+template <int KC>
 __global__ __launch_bounds__(kThreads) void kGramUpdate(const double* __restrict__ S, const double* __restrict__ Y,
                                                         int n, int nc, int loc, double* G, double* ys, double* yy,
                                                         double* work, unsigned* counter)
 {
-    __shared__ double red[2 * KC + 2];
+    // v[k] = s_loc . y_k, v[KC + k] = s_k . y_loc (k < nc), v[2 KC] = y_loc . y_loc
+    __shared__ double red[2 * KC + 1];
     double v[2 * KC + 1];
 #pragma unroll
     for (int k = 0; k < 2 * KC + 1; ++k) v[k] = 0.0;
@@ -497,34 +393,24 @@ __global__ __launch_bounds__(kThreads) void kGramUpdate(const double* __restrict
         const double si = sl[i], yi = yl[i];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
-            if (k >= nc) break;
-            v[k] += si * Y[static_cast<size_t>(k) * n + i];       // s_loc . y_k
-            v[KC + k] += S[static_cast<size_t>(k) * n + i] * yi;  // s_k . y_loc
+            if (k < nc) {
+                v[k] += si * Y[static_cast<size_t>(k) * n + i];
+                v[KC + k] += S[static_cast<size_t>(k) * n + i] * yi;
+            }
         }
         v[2 * KC] += yi * yi;
     }
-    // compact the live sums: [0,nc) s_loc.y_k, [nc,2nc) s_k.y_loc, 2nc y.y
-    double c[2 * KC + 1];
-#pragma unroll
-    for (int k = 0; k < 2 * KC + 1; ++k) c[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-        if (k >= nc) break;
-        c[k] = v[k];
-        c[nc + k] = v[KC + k];
+    if (!multiReduce<2 * KC + 1>(v, work, counter, red)) return;
+    if (threadIdx.x < nc) {
+        const int k = threadIdx.x;
+        G[loc * KG + k] = red[k];
+        G[k * KG + loc] = red[KC + k];
+        if (k == loc) ys[loc] = red[k];
     }
-    c[2 * nc] = v[2 * KC];
-    if (!multiReduce<2 * KC + 1>(c, 2 * nc + 1, work, counter, red)) return;
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < nc; ++k) {
-            G[loc * KC + k] = red[k];
-            G[k * KC + loc] = red[nc + k];
-        }
-        ys[loc] = red[loc];
-        *yy = red[2 * nc];
-    }
+    if (threadIdx.x == 0) *yy = red[2 * KC];
 }
 
+template <int KC>
 __global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
                                                          int n, int nc, int first, int newest, int mcap,
                                                          const double* G, const double* ys, const double* alphaIn,
@@ -533,19 +419,31 @@ __global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restric
     // first: dots of R_k (= S) with q (= r), alpha recurrence newest->oldest
     // else: dots of R_k (= Y) with q (= H0 output), beta recurrence
     __shared__ double red[KC];
+    __shared__ double sG[KC][KC], sys[KC], sa[KC];
     double v[KC];
 #pragma unroll
     for (int k = 0; k < KC; ++k) v[k] = 0.0;
     for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
         const double qi = q[i];
 #pragma unroll
-        for (int k = 0; k < KC; ++k) {
-            if (k >= nc) break;
-            v[k] += R[static_cast<size_t>(k) * n + i] * qi;
-        }
+        for (int k = 0; k < KC; ++k)
+            if (k < nc) v[k] += R[static_cast<size_t>(k) * n + i] * qi;
     }
-    if (!multiReduce<KC>(v, nc, work, counter, red)) return;
+    if (!multiReduce<KC>(v, work, counter, red)) return;
+    // the recurrence reads the Gram block, ys and alpha from LDS
+    for (int t = threadIdx.x; t < KC * KC; t += kThreads) {
+        const int i = t / KC, k = t - i * KC;
+        sG[i][k] = (i < mcap && k < mcap) ? G[i * KG + k] : 0.0;
+    }
+    if (threadIdx.x < KC) {
+        sys[threadIdx.x] = threadIdx.x < mcap ? ys[threadIdx.x] : 1.0;
+        sa[threadIdx.x] = (!first && threadIdx.x < mcap) ? alphaIn[threadIdx.x] : 0.0;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double c[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) c[k] = 0.0;
         if (first) {
             // q_0 = -r: alpha_i = (s_i.q_0 - sum_{k newer} alpha_k s_i.y_k) / ys_i
             int j = newest;
@@ -553,24 +451,26 @@ __global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restric
                 double acc = -red[j];
                 int kk = newest;
                 for (int u = 0; u < t; ++u) {
-                    acc -= outCoef[kk] * G[j * KC + kk];
+                    acc -= c[kk] * sG[j][kk];
                     kk = (kk + mcap - 1) % mcap;
                 }
-                outCoef[j] = acc / ys[j];
+                c[j] = acc / sys[j];
+                outCoef[j] = c[j];
                 j = (j + mcap - 1) % mcap;
             }
         } else {
             // beta_i = (y_i.h + sum_{k older} coef_k y_i.s_k) / ys_i ; coef = alpha - beta
-            int oldest = (newest + mcap - nc + 1) % mcap;
+            const int oldest = (newest + mcap - nc + 1) % mcap;
             int j = oldest;
             for (int t = 0; t < nc; ++t) {
                 double acc = red[j];
                 int kk = oldest;
                 for (int u = 0; u < t; ++u) {
-                    acc += outCoef[kk] * G[kk * KC + j];
+                    acc += c[kk] * sG[kk][j];
                     kk = (kk + 1) % mcap;
                 }
-                outCoef[j] = alphaIn[j] - acc / ys[j];
+                c[j] = sa[j] - acc / sys[j];
+                outCoef[j] = c[j];
                 j = (j + 1) % mcap;
             }
         }
@@ -578,22 +478,84 @@ __global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restric
 }
 
 // out = base * sgn_b + sum_k coef_k R_k * sgn_c
+template <int KC>
 __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict__ base, double sgnB,
                                                        const double* __restrict__ R, const double* coef,
                                                        double sgnC, int n, int nc, double* __restrict__ out)
 {
     __shared__ double c[KC];
-    if (threadIdx.x < nc) c[threadIdx.x] = sgnC * coef[threadIdx.x];
+    if (threadIdx.x < KC) c[threadIdx.x] = threadIdx.x < nc ? sgnC * coef[threadIdx.x] : 0.0;
     __syncthreads();
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     double v = sgnB * base[i];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) {
-        if (k >= nc) break;
-        v += c[k] * R[static_cast<size_t>(k) * n + i];
-    }
+    for (int k = 0; k < KC; ++k)
+        if (k < nc) v += c[k] * R[static_cast<size_t>(k) * n + i];
     out[i] = v;
+}
+
+// ---- compensated sum --------------------------------------------------------
+// Neumaier accumulation per lane, then (sum, compensation) pairs combined in a
+// fixed-shape tree with the exact TwoSum error of every addition; the last
+// block to arrive combines the block pairs in block order.
+__device__ __forceinline__ void neumaier(double& s, double& c, double x)
+{
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ void combinePair(double& s, double& c, double s2, double c2)
+{
+    const double t = s + s2;
+    const double bp = t - s;
+    const double err = (s - (t - bp)) + (s2 - bp);  // TwoSum
+    s = t;
+    c = c + c2 + err;
+}
+
+__global__ __launch_bounds__(kThreads) void kSumComp(const double* __restrict__ a, int n, double* work,
+                                                     unsigned* counter, double* out)
+{
+    __shared__ double ss[kThreads], sc[kThreads];
+    __shared__ int amLast;
+    const int tid = threadIdx.x;
+    double s = 0.0, c = 0.0;
+    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads) neumaier(s, c, a[i]);
+    ss[tid] = s;
+    sc[tid] = c;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            double s1 = ss[tid], c1 = sc[tid];
+            combinePair(s1, c1, ss[tid + w], sc[tid + w]);
+            ss[tid] = s1;
+            sc[tid] = c1;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        work[blockIdx.x] = ss[0];
+        work[kReduceBlocks + blockIdx.x] = sc[0];
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        amLast = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!amLast || tid != 0) return;
+    __threadfence();
+    double S = 0.0, C = 0.0;
+    for (int b = 0; b < static_cast<int>(gridDim.x); ++b) combinePair(S, C, work[b], work[kReduceBlocks + b]);
+    out[0] = S + C;
+    *counter = 0u;
+}
+
+// out = a x + b y (y may be null)
+__global__ void kAxpby(double* out, double a, const double* x, double b, const double* y, int n)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) out[i] = y ? a * x[i] + b * y[i] : a * x[i];
 }
 
 __global__ void kStep(double* x, const double* xp, const double* d, double alpha, int n)
@@ -639,122 +601,29 @@ __global__ void kTheta(const double* ys, const double* yy, double* th)
 }
 
 // ---- supernodal triangular solves -----------------------------------------
-constexpr int kMaxW = 256;   // widest supernode (host caps it)
-
 template <typename T>
-__global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* __restrict__ b,
-                                                     const int2* __restrict__ items)
+__global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* __restrict__ bp,
+                                                     const int4* __restrict__ items)
 {
-    __shared__ T sb[kMaxW * 3];
-    __shared__ T red[4][64][3];
-    const int2 it = items[blockIdx.x];
-    const int s = it.x, r0 = it.y;
-    const int w = a.snW[s], m = a.snM[s], j0 = a.snFirst[s], fr0 = a.snRow[s];
-    T* u = static_cast<T*>(a.t);
-    // front entries of the supernode's own columns: b plus the children's
-    // update rows (short lists: at most one entry per child)
-    for (int k = threadIdx.x; k < w; k += kThreads) {
-        const int old = a.perm[j0 + k];
-        double v0 = b[3 * old], v1 = b[3 * old + 1], v2 = b[3 * old + 2];
-        for (int p = a.fptr[fr0 + k]; p < a.fptr[fr0 + k + 1]; ++p) {
-            const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
-            v0 += static_cast<double>(uv[0]);
-            v1 += static_cast<double>(uv[1]);
-            v2 += static_cast<double>(uv[2]);
-        }
-        sb[3 * k] = static_cast<T>(v0);
-        sb[3 * k + 1] = static_cast<T>(v1);
-        sb[3 * k + 2] = static_cast<T>(v2);
-    }
-    __syncthreads();
-    // 64 rows per item; the four waves split the k range, then reduce in order
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = r0 + lane;
-    const int kc = (w + 3) >> 2;
-    const int kb = wave * kc;
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    if (r < m) {
-        int ke = min(w, kb + kc);
-        if (r < w) ke = min(ke, r + 1);  // L11^-1 is lower triangular
-        const T* Bc = static_cast<const T*>(a.B) + a.snVal[s] + r;
-        int k = kb;
-        for (; k + 4 <= ke; k += 4) {
-            const T b0 = Bc[static_cast<size_t>(k) * m], b1 = Bc[static_cast<size_t>(k + 1) * m];
-            const T b2 = Bc[static_cast<size_t>(k + 2) * m], b3 = Bc[static_cast<size_t>(k + 3) * m];
-            acc0 += b0 * sb[3 * k] + b1 * sb[3 * k + 3] + b2 * sb[3 * k + 6] + b3 * sb[3 * k + 9];
-            acc1 += b0 * sb[3 * k + 1] + b1 * sb[3 * k + 4] + b2 * sb[3 * k + 7] + b3 * sb[3 * k + 10];
-            acc2 += b0 * sb[3 * k + 2] + b1 * sb[3 * k + 5] + b2 * sb[3 * k + 8] + b3 * sb[3 * k + 11];
-        }
-        for (; k < ke; ++k) {
-            const T bk = Bc[static_cast<size_t>(k) * m];
-            acc0 += bk * sb[3 * k];
-            acc1 += bk * sb[3 * k + 1];
-            acc2 += bk * sb[3 * k + 2];
-        }
-    }
-    red[wave][lane][0] = acc0;
-    red[wave][lane][1] = acc1;
-    red[wave][lane][2] = acc2;
-    __syncthreads();
-    if (wave != 0 || r >= m) return;
-    acc0 = ((red[0][lane][0] + red[1][lane][0]) + red[2][lane][0]) + red[3][lane][0];
-    acc1 = ((red[0][lane][1] + red[1][lane][1]) + red[2][lane][1]) + red[3][lane][1];
-    acc2 = ((red[0][lane][2] + red[1][lane][2]) + red[2][lane][2]) + red[3][lane][2];
-    if (r < w) {
-        T* out = static_cast<T*>(a.y) + 3 * static_cast<size_t>(j0 + r);
-        out[0] = acc0;
-        out[1] = acc1;
-        out[2] = acc2;
-        return;
-    }
-    // update row: children's contributions to this row minus (L21 L11^-1) f
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int p = a.fptr[fr0 + r]; p < a.fptr[fr0 + r + 1]; ++p) {
-        const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
-        f0 += static_cast<double>(uv[0]);
-        f1 += static_cast<double>(uv[1]);
-        f2 += static_cast<double>(uv[2]);
-    }
-    T* out = u + 3 * static_cast<size_t>(a.snT[s] + r - w);
-    out[0] = static_cast<T>(f0) - acc0;
-    out[1] = static_cast<T>(f1) - acc1;
-    out[2] = static_cast<T>(f2) - acc2;
+    __shared__ TrsvFwdLds<T> L;
+    trsvFwdItem<T>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void kTrsvBwd(TrsvArgs a, const int2* __restrict__ items)
+__global__ __launch_bounds__(kThreads) void kTrsvBwd(TrsvArgs a, const int4* __restrict__ items)
 {
-    // four columns per item, one per wave; lanes stride the rows
-    const int2 it = items[blockIdx.x];
-    const int s = it.x;
-    const int w = a.snW[s], m = a.snM[s], j0 = a.snFirst[s];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = it.y + wave;
-    if (k >= w) return;
-    const int* rows = a.rows + a.snRow[s];
-    const T* y = static_cast<const T*>(a.y);
-    T* z = static_cast<T*>(a.z);
-    const T* col = static_cast<const T*>(a.B) + a.snVal[s] + static_cast<size_t>(k) * m;
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    for (int r = k + lane; r < m; r += 64) {
-        const T bk = col[r];
-        const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
-        const T sg = r < w ? T(1) : T(-1);
-        acc0 += bk * (sg * src[0]);
-        acc1 += bk * (sg * src[1]);
-        acc2 += bk * (sg * src[2]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc0 += __shfl_xor(acc0, off, 64);
-        acc1 += __shfl_xor(acc1, off, 64);
-        acc2 += __shfl_xor(acc2, off, 64);
-    }
-    if (lane == 0) {
-        T* zo = z + 3 * static_cast<size_t>(j0 + k);
-        zo[0] = acc0;
-        zo[1] = acc1;
-        zo[2] = acc2;
+    trsvBwdItem<T>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1]);
+}
+
+// bp[3 j + c] = b[3 perm[j] + c]
+__global__ void kPermute(const int* __restrict__ perm, const double* __restrict__ b, double* __restrict__ bp, int n)
+{
+    const int j = blockIdx.x * kThreads + threadIdx.x;
+    if (j < n) {
+        const int o = perm[j];
+        bp[3 * j] = b[3 * o];
+        bp[3 * j + 1] = b[3 * o + 1];
+        bp[3 * j + 2] = b[3 * o + 2];
     }
 }
 
@@ -777,21 +646,70 @@ inline int blocks(int n) { return (n + kThreads - 1) / kThreads; }
 void launchElementForce(const ElemArgs& a, hipStream_t s)
 {
     if (a.mixed)
-        launchElem<float>(a, s);
+        launchElem<float, MODE_FORCE>(a, nullptr, s);
     else
-        launchElem<double>(a, s);
+        launchElem<double, MODE_FORCE>(a, nullptr, s);
 }
 
-void launchPressure(const FaceArgs& a, hipStream_t s)
+void launchElementTangent(const ElemArgs& a, const double* v, hipStream_t s)
+{
+    if (a.mixed)
+        launchElem<float, MODE_TANGENT>(a, v, s);
+    else
+        launchElem<double, MODE_TANGENT>(a, v, s);
+}
+
+template <int MODE>
+void launchScalar(const ElemArgs& a, const double* lam, double* out, hipStream_t s)
+{
+    if (a.nElems <= 0) return;
+    const int g = blocks(a.nElems);
+    switch (a.nen) {
+        case 4: kElemScalar<4, MODE><<<g, kThreads, 0, s>>>(a, lam, out); break;
+        case 8: kElemScalar<8, MODE><<<g, kThreads, 0, s>>>(a, lam, out); break;
+        case 10: kElemScalar<10, MODE><<<g, kThreads, 0, s>>>(a, lam, out); break;
+        case 20: kElemScalar<20, MODE><<<g, kThreads, 0, s>>>(a, lam, out); break;
+    }
+}
+
+void launchElementEnergy(const ElemArgs& a, double* out, hipStream_t s) { launchScalar<0>(a, nullptr, out, s); }
+
+void launchElementParamGrad(const ElemArgs& a, const double* lam, double* out, hipStream_t s)
+{
+    launchScalar<1>(a, lam, out, s);
+}
+
+template <int MODE>
+void launchPressureMode(const FaceArgs& a, hipStream_t s)
 {
     if (a.nFaces <= 0) return;
     const int g = blocks(a.nFaces);
     switch (a.nen) {
-        case 4: kPressure<4><<<g, kThreads, 0, s>>>(a); break;
-        case 8: kPressure<8><<<g, kThreads, 0, s>>>(a); break;
-        case 10: kPressure<10><<<g, kThreads, 0, s>>>(a); break;
-        case 20: kPressure<20><<<g, kThreads, 0, s>>>(a); break;
+        case 4: kPressure<4, MODE><<<g, kThreads, 0, s>>>(a); break;
+        case 8: kPressure<8, MODE><<<g, kThreads, 0, s>>>(a); break;
+        case 10: kPressure<10, MODE><<<g, kThreads, 0, s>>>(a); break;
+        case 20: kPressure<20, MODE><<<g, kThreads, 0, s>>>(a); break;
     }
+}
+
+void launchPressure(const FaceArgs& a, hipStream_t s) { launchPressureMode<PMODE_FORCE>(a, s); }
+
+void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s)
+{
+    if (transpose)
+        launchPressureMode<PMODE_TRANSPOSE>(a, s);
+    else
+        launchPressureMode<PMODE_TANGENT>(a, s);
+}
+
+void launchSumComp(const double* a, int n, double* work, unsigned* counter, double* out, hipStream_t s)
+{
+    kSumComp<<<reduceGrid(n), kThreads, 0, s>>>(a, n, work, counter, out);
+}
+
+void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s)
+{
+    if (n > 0) kAxpby<<<blocks(n), kThreads, 0, s>>>(out, a, x, b, y, n);
 }
 
 void launchGather(const GatherArgs& a, hipStream_t s)
@@ -842,25 +760,41 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
 void launchGramUpdate(const double* S, const double* Y, int n, int nc, int loc, double* G, double* ys, double* yy,
                       double* work, unsigned* counter, hipStream_t s)
 {
-    kGramUpdate<<<reduceGrid(n), kThreads, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
+    if (nc <= 8)
+        kGramUpdate<8><<<reduceGrid(n), kThreads, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
+    else
+        kGramUpdate<16><<<reduceGrid(n), kThreads, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
+}
+
+template <int KC>
+void twoLoopPass(const double* q, const double* R, const double* base, double sgnB, const double* axR,
+                 double sgnC, int n, int nc, int first, int newest, int mcap, const double* G, const double* ys,
+                 const double* alphaIn, double* coef, double* out, double* work, unsigned* counter, hipStream_t s)
+{
+    kTwoLoopDots<KC><<<reduceGrid(n), kThreads, 0, s>>>(q, R, n, nc, first, newest, mcap, G, ys, alphaIn, coef,
+                                                         work, counter);
+    if (n > 0) kMultiAxpy<KC><<<blocks(n), kThreads, 0, s>>>(base, sgnB, axR, coef, sgnC, n, nc, out);
 }
 
 void launchTwoLoopFirst(const double* r, const double* S, const double* Y, int n, int nc, int newest, int mcap,
                         const double* G, const double* ys, double* alpha, double* out, double* work,
                         unsigned* counter, hipStream_t s)
 {
-    kTwoLoopDots<<<reduceGrid(n), kThreads, 0, s>>>(r, S, n, nc, 1, newest, mcap, G, ys, nullptr, alpha, work,
-                                                     counter);
-    if (n > 0) kMultiAxpy<<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, nc, out);
+    if (mcap <= 8)
+        twoLoopPass<8>(r, S, r, -1.0, Y, -1.0, n, nc, 1, newest, mcap, G, ys, nullptr, alpha, out, work, counter, s);
+    else
+        twoLoopPass<16>(r, S, r, -1.0, Y, -1.0, n, nc, 1, newest, mcap, G, ys, nullptr, alpha, out, work, counter,
+                        s);
 }
 
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
                          const double* G, const double* ys, const double* alpha, double* coef, double* out,
                          double* work, unsigned* counter, hipStream_t s)
 {
-    kTwoLoopDots<<<reduceGrid(n), kThreads, 0, s>>>(h, Y, n, nc, 0, newest, mcap, G, ys, alpha, coef, work,
-                                                     counter);
-    if (n > 0) kMultiAxpy<<<blocks(n), kThreads, 0, s>>>(h, 1.0, S, coef, 1.0, n, nc, out);
+    if (mcap <= 8)
+        twoLoopPass<8>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s);
+    else
+        twoLoopPass<16>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s);
 }
 
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s)
@@ -878,16 +812,21 @@ void launchTheta(const double* ys, const double* yy, double* theta, hipStream_t 
     kTheta<<<1, 64, 0, s>>>(ys, yy, theta);
 }
 
-void launchTrsvForward(const TrsvArgs& a, const double* b, const int2* items, int nitems, hipStream_t s)
+void launchPermute(const TrsvArgs& a, const double* b, double* bp, hipStream_t s)
+{
+    if (a.n > 0) kPermute<<<blocks(a.n), kThreads, 0, s>>>(a.perm, b, bp, a.n);
+}
+
+void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, hipStream_t s)
 {
     if (nitems <= 0) return;
     if (a.mixed)
-        kTrsvFwd<float><<<nitems, kThreads, 0, s>>>(a, b, items);
+        kTrsvFwd<float><<<nitems, kThreads, 0, s>>>(a, bp, items);
     else
-        kTrsvFwd<double><<<nitems, kThreads, 0, s>>>(a, b, items);
+        kTrsvFwd<double><<<nitems, kThreads, 0, s>>>(a, bp, items);
 }
 
-void launchTrsvBackward(const TrsvArgs& a, const int2* items, int nitems, hipStream_t s)
+void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, hipStream_t s)
 {
     if (nitems <= 0) return;
     if (a.mixed)

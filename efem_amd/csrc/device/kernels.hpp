@@ -19,6 +19,14 @@ struct ElemArgs {
     void* slots;            // [E][nen][3] (double or float)
 };
 void launchElementForce(const ElemArgs& a, hipStream_t s);
+// tangent product of the element forces: slots <- (d f_e / d u)[v] (the
+// material tangent through forward-mode dual numbers)
+void launchElementTangent(const ElemArgs& a, const double* v, hipStream_t s);
+// FP64 per-element scalars (ginv/gdN must be the FP64 copies):
+//   energy     out[e] = sum_q w_q V_e W(F_q)
+//   paramGrad  out[4e + k] = lam . d f_e / d p_k
+void launchElementEnergy(const ElemArgs& a, double* out, hipStream_t s);
+void launchElementParamGrad(const ElemArgs& a, const double* lam, double* out, hipStream_t s);
 
 struct FaceArgs {
     int nFaces, nen, fq, directed;
@@ -28,9 +36,13 @@ struct FaceArgs {
     const double* WN;       // [F][fq][nen]
     const double* G;        // [F][fq][nen][3]
     const double* x;
+    const double* v;        // direction (tangent) or multiplier (transpose)
+    const double* fpress;   // per-face pressure (batches) or null: `pressure`
     double* slots;          // [F][nen][3]
 };
 void launchPressure(const FaceArgs& a, hipStream_t s);
+// load stiffness of the follower pressure applied to a.v, or its transpose
+void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s);
 
 struct GatherArgs {
     int nFree, nElemSlots, mixed;
@@ -38,7 +50,8 @@ struct GatherArgs {
     const int* gslot;
     const void* eslots;     // element slots (double or float)
     const double* fslots;   // face slots
-    const double* fext;     // constant nodal forces
+    const double* fext;     // constant nodal forces (may be null)
+    double ce, cf;          // r = ce sum(element slots) + cf sum(face slots) - fext
     double* r;
 };
 void launchGather(const GatherArgs& a, hipStream_t s);
@@ -56,6 +69,10 @@ void launchDotMax(const double* a, const double* b, const double* c, int n, doub
 // *ab = a.b, *bb = b.b
 void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter,
                 double* ab, double* bb, hipStream_t s);
+// *out = sum a (Neumaier lanes + TwoSum tree, reproducible)
+void launchSumComp(const double* a, int n, double* work, unsigned* counter, double* out, hipStream_t s);
+// out = a x + b y (y may be null)
+void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s);
 // x = xp + alpha d
 void launchStep(double* x, const double* xp, const double* d, double alpha, int n, hipStream_t s);
 // s = x - xp, y = r - rp
@@ -95,26 +112,24 @@ void launchTheta(const double* ys, const double* yy, double* theta, hipStream_t 
 struct TrsvArgs {
     int n, mixed;
     const int* perm;        // [n] new -> old
-    const int* snFirst;     // [S]
-    const int* snW;         // [S]
-    const int* snM;         // [S]
-    const int* snRow;       // [S] offset into rows
-    const long long* snVal; // [S] offset into B
-    const int* snT;         // [S] offset into the update vectors (rows)
-    const int* rows;
-    const void* B;          // double or float
-    const int* fptr;        // [sum m_s + 1] per front row (snRow[s] + r): children's
-    const int* fidx;        //   update rows that extend-add into it
+    const int* rows;        // row indices of every supernode (per-item offset)
+    const void* B;          // double or float, per-supernode [L11^-1 ; L21 L11^-1]
+    const int* fptr;        // [front rows + 1]: children's update rows that
+    const int* fidx;        //   extend-add into each front row
     void* y;                // [n][3] work (T)
     void* t;                // [nt][3] update vectors u_s (T)
     void* z;                // [n][3] work (T)
 };
-// one level of the forward solve over work items (supernode, first row):
-// multifrontal form f_s = [b_s; 0] + sum_children u_c (extend-add),
+// bp[3 j + c] = b[3 perm[j] + c] (once per apply, before the forward sweep)
+void launchPermute(const TrsvArgs& a, const double* b, double* bp, hipStream_t s);
+// one level of the forward solve; items are descriptor pairs (int4 x 2):
+// {r0, w, m, j0}, {front row offset, update offset, B offset lo, hi}.
+// Multifrontal form f_s = [bp_s; 0] + sum_children u_c (extend-add),
 // y_s = L11^-1 f_s[:w], u_s = f_s[w:] - (L21 L11^-1) f_s[:w]
-void launchTrsvForward(const TrsvArgs& a, const double* b, const int2* items, int nitems,
-                       hipStream_t s);
-void launchTrsvBackward(const TrsvArgs& a, const int2* items, int nitems, hipStream_t s);
+void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, hipStream_t s);
+// one level of the backward solve; descriptors {k0, w, m, j0}, {row offset,
+// 0, B offset lo, hi}: four columns per item
+void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, hipStream_t s);
 // out[3 perm[j] + c] = z[j][c]
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s);
 

@@ -43,6 +43,21 @@ void jacobian0(int nen, const double* Xe, const double* dN, double* J0)
 
 }  // namespace
 
+bool setParams(Model& m, const double* params, std::string& msg)
+{
+    const size_t np = m.paramsPerElem ? 4 * static_cast<size_t>(m.nElems) : 4;
+    for (size_t k = 0; k < np; k += 4)
+        if (!(params[k + 3] > 0.0)) {
+            msg = "material parameters: D1 must be positive";
+            return false;
+        }
+    m.params.assign(params, params + np);
+    m.coeff.resize(m.nElems);
+    for (int e = 0; e < m.nElems; ++e)
+        m.coeff[e] = elasticityCoeff(m.law, &m.params[m.paramsPerElem ? 4 * static_cast<size_t>(e) : 0]);
+    return true;
+}
+
 bool buildModel(const efem_problem& pb, Model& m, std::string& msg)
 {
     if (pb.n_nodes <= 0 || !pb.X || pb.n_elems <= 0 || !pb.conn || !pb.params) {
@@ -87,16 +102,7 @@ bool buildModel(const efem_problem& pb, Model& m, std::string& msg)
             m.connDof[static_cast<size_t>(a) * E + e] = f >= 0 ? f : m.nFree;
         }
     // materials
-    const size_t np = m.paramsPerElem ? 4 * static_cast<size_t>(E) : 4;
-    m.params.assign(pb.params, pb.params + np);
-    for (size_t k = 0; k < np; k += 4)
-        if (!(m.params[k + 3] > 0.0)) {
-            msg = "efem_problem: D1 must be positive";
-            return false;
-        }
-    m.coeff.resize(E);
-    for (int e = 0; e < E; ++e)
-        m.coeff[e] = elasticityCoeff(m.law, &m.params[m.paramsPerElem ? 4 * static_cast<size_t>(e) : 0]);
+    if (!setParams(m, pb.params, msg)) return false;
     // Gauss rule and local gradients
     m.gw.resize(ng);
     m.gdN.resize(static_cast<size_t>(ng) * nen * 3);

@@ -46,6 +46,10 @@ struct Model {
 
 bool buildModel(const efem_problem& pb, Model& m, std::string& msg);
 
+// material parameters (layout of efem_problem.params) and the per-element
+// approximate-Jacobian coefficient derived from them
+bool setParams(Model& m, const double* params, std::string& msg);
+
 // scalar approximate Jacobian over free nodes (full symmetric CSC)
 void scalarLaplacian(const Model& m, Csc& L);
 

@@ -4,6 +4,7 @@
 // fundamental + relaxed supernodes, and the multifrontal method of
 // Duff & Reid / Liu.
 #include "sparse.hpp"
+#include "../limits.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -337,7 +338,7 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         if (parent[j] != -1) nchild[parent[j]]++;
     // (no supernode is wider than kMaxWidth: the device stages one
     // supernode's right-hand sides in a kMaxWidth-wide LDS buffer)
-    const int kMaxWidth = 256;
+    const int kMaxWidth = kMaxSupernodeWidth;
     std::vector<int> fstart;
     for (int j = 0; j < n; ++j) {
         const bool cont = j > 0 && parent[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1 &&

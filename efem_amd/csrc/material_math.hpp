@@ -37,29 +37,79 @@ EFEM_HD double elasticityCoeff(int law, const double* p)
     return 2.94001 * p[0] + 3.51194 * p[1] + 3.3154 * p[2] + 2.0 / p[3];
 }
 
-// J - 1 from H without cancellation (exact identity det(I+H) - 1)
-EFEM_HD double detIpHm1(const double* h)
+// ---- forward-mode dual numbers ---------------------------------------------
+// v + d eps with eps^2 = 0: running the stress code on Dual<T> gives the
+// directional derivative dP = (dP/dF)[dF] exactly (the material tangent
+// applied to dF), which the tangent/Hessian-vector kernels use.
+template <typename S>
+struct Dual {
+    S v, d;
+    EFEM_HD Dual() : v(0), d(0) {}
+    EFEM_HD Dual(S a) : v(a), d(0) {}
+    EFEM_HD Dual(S a, S b) : v(a), d(b) {}
+    template <typename S2>
+    EFEM_HD explicit Dual(const Dual<S2>& o) : v(static_cast<S>(o.v)), d(static_cast<S>(o.d)) {}
+    EFEM_HD Dual& operator+=(const Dual& o) { v += o.v; d += o.d; return *this; }
+    EFEM_HD Dual& operator-=(const Dual& o) { v -= o.v; d -= o.d; return *this; }
+};
+template <typename S> EFEM_HD Dual<S> operator-(const Dual<S>& a) { return Dual<S>(-a.v, -a.d); }
+template <typename S> EFEM_HD Dual<S> operator+(const Dual<S>& a, const Dual<S>& b) { return Dual<S>(a.v + b.v, a.d + b.d); }
+template <typename S> EFEM_HD Dual<S> operator-(const Dual<S>& a, const Dual<S>& b) { return Dual<S>(a.v - b.v, a.d - b.d); }
+template <typename S> EFEM_HD Dual<S> operator*(const Dual<S>& a, const Dual<S>& b) { return Dual<S>(a.v * b.v, a.v * b.d + a.d * b.v); }
+template <typename S> EFEM_HD Dual<S> operator/(const Dual<S>& a, const Dual<S>& b)
 {
-    const double tr = h[0] + h[4] + h[8];
-    const double trH2 = h[0] * h[0] + h[4] * h[4] + h[8] * h[8] +
-                        2.0 * (h[1] * h[3] + h[2] * h[6] + h[5] * h[7]);
-    const double det = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
-                       h[2] * (h[3] * h[7] - h[4] * h[6]);
+    const S q = a.v / b.v;
+    return Dual<S>(q, (a.d - q * b.d) / b.v);
+}
+template <typename S> EFEM_HD Dual<S> operator+(const Dual<S>& a, S b) { return Dual<S>(a.v + b, a.d); }
+template <typename S> EFEM_HD Dual<S> operator+(S a, const Dual<S>& b) { return Dual<S>(a + b.v, b.d); }
+template <typename S> EFEM_HD Dual<S> operator-(const Dual<S>& a, S b) { return Dual<S>(a.v - b, a.d); }
+template <typename S> EFEM_HD Dual<S> operator-(S a, const Dual<S>& b) { return Dual<S>(a - b.v, -b.d); }
+template <typename S> EFEM_HD Dual<S> operator*(const Dual<S>& a, S b) { return Dual<S>(a.v * b, a.d * b); }
+template <typename S> EFEM_HD Dual<S> operator*(S a, const Dual<S>& b) { return Dual<S>(a * b.v, a * b.d); }
+template <typename S> EFEM_HD Dual<S> operator/(const Dual<S>& a, S b) { return Dual<S>(a.v / b, a.d / b); }
+template <typename S> EFEM_HD Dual<S> operator/(S a, const Dual<S>& b)
+{
+    const S q = a / b.v;
+    return Dual<S>(q, -q * b.d / b.v);
+}
+
+EFEM_HD double expm1_(double x) { return std::expm1(x); }
+EFEM_HD double log1p_(double x) { return std::log1p(x); }
+EFEM_HD double sqrt_(double x) { return std::sqrt(x); }
+template <typename S> EFEM_HD Dual<S> expm1_(const Dual<S>& x) { return Dual<S>(expm1_(x.v), (expm1_(x.v) + S(1)) * x.d); }
+template <typename S> EFEM_HD Dual<S> log1p_(const Dual<S>& x) { return Dual<S>(log1p_(x.v), x.d / (S(1) + x.v)); }
+template <typename S> EFEM_HD Dual<S> sqrt_(const Dual<S>& x)
+{
+    const S r = sqrt_(x.v);
+    return Dual<S>(r, x.d / (S(2) * r));
+}
+
+// J - 1 from H without cancellation (exact identity det(I+H) - 1)
+template <typename D>
+EFEM_HD D detIpHm1(const D* h)
+{
+    const D tr = h[0] + h[4] + h[8];
+    const D trH2 = h[0] * h[0] + h[4] * h[4] + h[8] * h[8] +
+                   2.0 * (h[1] * h[3] + h[2] * h[6] + h[5] * h[7]);
+    const D det = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                  h[2] * (h[3] * h[7] - h[4] * h[6]);
     return tr + 0.5 * (tr * tr - trH2) + det;
 }
 
-// P(F = I + H) for law with p = {C10, C01|C20, C30, D1}, H given in FP64.
-// T = double or float for the deviatoric terms and the cofactor; J - 1,
-// J^-2/3 - 1 and the volumetric coefficient are FP64 in both cases.
-template <typename T>
-EFEM_HD void piolaFromHd(int law, const double* p, const double* hd, T* P)
+// P(F = I + H) for law with p = {C10, C01|C20, C30, D1}.  D is the type of
+// the FP64-critical quantities (H, J - 1, J^-2/3 - 1, the volumetric
+// coefficient): double, or Dual<double> for tangent products.  T is the type
+// of the deviatoric terms and the cofactor: double or float (or their duals).
+template <typename D, typename T>
+EFEM_HD void piolaGen(int law, const double* p, const D* hd, T* P)
 {
     T H[9];
     for (int k = 0; k < 9; ++k) H[k] = static_cast<T>(hd[k]);
-    const double jm1 = detIpHm1(hd);
-    const double J = 1.0 + jm1;
+    const D jm1 = detIpHm1(hd);
+    const D J = 1.0 + jm1;
     // J^-2/3 - 1 = expm1(-2/3 log1p(J - 1)), accurate for J near 1
-    const double em1 = std::expm1(-2.0 / 3.0 * std::log1p(jm1));
+    const D em1 = expm1_(-2.0 / 3.0 * log1p_(jm1));
     T F[9];
     for (int k = 0; k < 9; ++k) F[k] = H[k];
     F[0] += T(1);
@@ -125,6 +175,12 @@ EFEM_HD void piolaFromHd(int law, const double* p, const double* hd, T* P)
 }
 
 template <typename T>
+EFEM_HD void piolaFromHd(int law, const double* p, const double* hd, T* P)
+{
+    piolaGen<double, T>(law, p, hd, P);
+}
+
+template <typename T>
 EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
 {
     double hd[9];
@@ -132,30 +188,68 @@ EFEM_HD void piolaFromH(int law, const double* p, const T* H, T* P)
     piolaFromHd<T>(law, p, hd, P);
 }
 
-// strain energy density at F = I + H (FP64)
-EFEM_HD double energyFromH(int law, const double* p, const double* H)
+// tangent product dP = (dP/dF)[dH] at F = I + H, FP64-critical part in FP64
+template <typename T>
+EFEM_HD void piolaTangent(int law, const double* p, const double* hd, const double* dhd, T* dP)
 {
-    const double jm1 = detIpHm1(H);
-    const double em1 = std::expm1(-2.0 / 3.0 * std::log1p(jm1));
-    double E[9];
+    Dual<double> h[9];
+    for (int k = 0; k < 9; ++k) h[k] = Dual<double>(hd[k], dhd[k]);
+    Dual<T> P[9];
+    piolaGen<Dual<double>, Dual<T>>(law, p, h, P);
+    for (int k = 0; k < 9; ++k) dP[k] = P[k].d;
+}
+
+// Derivatives of P with respect to the four material parameters, in the
+// same slots as p.  P is linear in C10, C01 (MR), C20, C30 (Yeoh) -- the
+// derivative is P with that parameter 1 and the others (and 1/D1) 0 -- and
+// in 1/D1 through the volumetric term: dP/dD1 = -P_vol / D1.
+template <typename T>
+EFEM_HD void piolaParamDerivative(int law, const double* p, const double* hd, int k, T* dP)
+{
+    const double inf = HUGE_VAL;
+    if (k == 3) {
+        const double q[4] = {0.0, 0.0, 0.0, p[3]};
+        piolaGen<double, T>(law, q, hd, dP);
+        const T s = static_cast<T>(-1.0 / p[3]);
+        for (int i = 0; i < 9; ++i) dP[i] = s * dP[i];
+        return;
+    }
+    double q[4] = {0.0, 0.0, 0.0, inf};
+    q[k] = 1.0;
+    if ((law == LAW_NH && k != 0) || (law == LAW_MR && k == 2)) {
+        for (int i = 0; i < 9; ++i) dP[i] = T(0);
+        return;
+    }
+    piolaGen<double, T>(law, q, hd, dP);
+}
+
+// strain energy density at F = I + H (FP64, or its dual)
+template <typename D>
+EFEM_HD D energyGen(int law, const double* p, const D* H)
+{
+    const D jm1 = detIpHm1(H);
+    const D em1 = expm1_(-2.0 / 3.0 * log1p_(jm1));
+    D E[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             E[3 * i + j] = H[3 * i + j] + H[3 * j + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
-    const double trE = E[0] + E[4] + E[8];
-    const double t = 3.0 * em1 + trE * (1.0 + em1);  // I1bar - 3
-    double W = jm1 * jm1 / p[3];
+    const D trE = E[0] + E[4] + E[8];
+    const D t = 3.0 * em1 + trE * (1.0 + em1);  // I1bar - 3
+    D W = jm1 * jm1 / p[3];
     if (law == LAW_YEOH) return W + p[0] * t + p[1] * t * t + p[2] * t * t * t;
     W += p[0] * t;
     if (law == LAW_MR) {
         // I2 - 3 = 2 tr E + (tr^2 E - tr E^2)/2 ; I2bar - 3 = (I2 - 3) J^-4/3 + 3 (J^-4/3 - 1)
-        double trE2 = 0.0;
+        D trE2 = 0.0;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) trE2 += E[3 * i + j] * E[3 * j + i];
-        const double i2m3 = 2.0 * trE + 0.5 * (trE * trE - trE2);
-        const double j43 = (1.0 + em1) * (1.0 + em1);
+        const D i2m3 = 2.0 * trE + 0.5 * (trE * trE - trE2);
+        const D j43 = (1.0 + em1) * (1.0 + em1);
         W += p[1] * (i2m3 * j43 + 3.0 * (j43 - 1.0));
     }
     return W;
 }
+
+EFEM_HD double energyFromH(int law, const double* p, const double* H) { return energyGen<double>(law, p, H); }
 
 }  // namespace efem

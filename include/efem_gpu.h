@@ -178,6 +178,70 @@ int efem_profile_reset(efem_ctx* ctx);
 int efem_profile_read(efem_ctx* ctx, double* elem_ms_total, int* elem_launches);
 int efem_profile_read_h0(efem_ctx* ctx, double* h0_ms_total, int* h0_applies);
 
+/* ---- differentiation (no counterpart in the reference) -----------------
+ * K_t(u) = d r / d u is the true tangent: the exact material tangent of the
+ * element forces (symmetric) minus the follower-pressure load stiffness
+ * (not symmetric).  The reference only builds the constant approximate
+ * Jacobian (ElementConstraint.cpp:164-210); its never-called pressure
+ * stiffness sketch is at ExternalConstraint.cpp:856-869. */
+#define EFEM_KRYLOV_AUTO 0     /* CG without pressure faces, else BiCGStab */
+#define EFEM_KRYLOV_CG 1
+#define EFEM_KRYLOV_BICGSTAB 2
+
+typedef struct efem_krylov_params {
+    double tol;          /* stop at ||b - K x||_2 <= tol ||b||_2 */
+    int max_iterations;
+    int method;          /* EFEM_KRYLOV_* */
+    int use_precond;     /* 1: preconditioned by the approximate-Jacobian
+                            Cholesky factor (factored here if absent) */
+} efem_krylov_params;
+
+typedef struct efem_krylov_stats {
+    int iterations;
+    int matvecs;         /* tangent products */
+    int precond_applies;
+    int converged;
+    double rel_residual; /* recurrence ||r|| / ||b|| at exit */
+    double seconds;      /* device solve (+ gradient assembly for the adjoint) */
+    double factor_seconds;
+} efem_krylov_stats;
+
+/* out = K_t(u) v, or K_t(u)^T v when transpose != 0 (host pointers) */
+int efem_tangent(efem_ctx* ctx, const double* u, const double* v, double* out,
+                 int transpose);
+/* K_t(u) x = b (transpose: K_t^T x = b) by preconditioned Krylov */
+int efem_tangent_solve(efem_ctx* ctx, const double* u, const double* b,
+                       double* x, int transpose, const efem_krylov_params* prm,
+                       efem_krylov_stats* stats);
+/* Newton-Krylov equilibrium solve from u (in/out): K_t(u) du = -r(u) by
+ * efem_tangent_solve, backtracking on ||r||_2, until ||r||_inf <= epsilon.
+ * stats: iterations = Newton steps, ls_trials = Krylov iterations summed,
+ * r_evals, rinf, xinf, factor/solve seconds. */
+int efem_newton(efem_ctx* ctx, double* u, double epsilon, int max_iterations,
+                const efem_krylov_params* kp, efem_solve_stats* stats);
+/* Adjoint gradient of an objective J(u) at an equilibrium u (r(u) = 0):
+ * solves K_t(u)^T lambda = g with g = dJ/du, then
+ *   grad_params = -lambda . dr/dparams: [n_elems][4] when the problem has
+ *                 per-element parameters, else [4] (same slots as params);
+ *   grad_load   = -lambda . dr/dload: [0] = dJ/dpressure (follower), or
+ *                 [0..2] = dJ/dpdir (directed traction).
+ * lambda, grad_params, grad_load may be NULL. */
+int efem_adjoint(efem_ctx* ctx, const double* u, const double* g, double* lambda,
+                 double* grad_params, double* grad_load,
+                 const efem_krylov_params* prm, efem_krylov_stats* stats);
+/* total strain energy sum_e sum_q w_q V_e W(F_q) (FP64, compensated sum) */
+int efem_energy(efem_ctx* ctx, const double* u, double* W);
+/* replace the material parameters (same layout as efem_problem.params);
+ * drops the factor, which depends on them */
+int efem_set_params(efem_ctx* ctx, const double* params);
+/* replace the follower pressure (pdir NULL) or the directed traction */
+int efem_set_load(efem_ctx* ctx, double pressure, const double* pdir);
+/* one material point, host only: P(F) (Material::getFirstPiola,
+ * src/material/Material.h:18-28), W(F), and dP = (dP/dF)[dF]
+ * (row-major 3x3; any output may be NULL; dP needs dF) */
+int efem_material_point(int law, const double* params, const double* F,
+                        double* P, double* W, const double* dF, double* dP);
+
 /* ---- synthetic workloads (host only, no GPU needed) --------------------
  * which: 1 cantilever 20x4x4 C3D8 NH; 2 beam 200x20x20 Kuhn C3D4 MR;
  *        3 hollow cylinder 48x4x40 C3D20R Yeoh; 4 bar 64x16x12 Kuhn C3D10
