@@ -117,6 +117,10 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_set_params": (C.c_int, [vp, _dp]),
         "efem_set_load": (C.c_int, [vp, C.c_double, _dp]),
         "efem_material_point": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "efem_batch_create": (C.c_int, [C.POINTER(vp), C.POINTER(Problem), C.c_int, C.c_int]),
+        "efem_batch_count": (C.c_int, [vp]),
+        "efem_batch_offsets": (C.c_int, [vp, _ip]),
+        "efem_batch_solve": (C.c_int, [vp, _dp, C.POINTER(LbfgsParams), C.POINTER(SolveStats)]),
         "efem_mesh_build": (vp, [C.c_int, _dp, C.c_int]),
         "efem_mesh_free": (None, [vp]),
         "efem_mesh_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
@@ -423,6 +427,46 @@ class Context:
     def set_load(self, pressure=0.0, pdir=None):
         d = None if pdir is None else np.ascontiguousarray(pdir, dtype=np.float64)
         self._check(self._lib.efem_set_load(self._h, float(pressure), _ptr(d, C.c_double)), "efem_set_load")
+
+
+class Batch(Context):
+    """Independent instances in one context (efem_batch_create); solve()
+    runs every instance's L-BFGS in one persistent launch."""
+
+    def __init__(self, meshes, precision: int = FP64):
+        self._lib = lib()
+        self.meshes = list(meshes)
+        self.mesh = self.meshes[0]
+        pbs = (Problem * len(self.meshes))(*[m.problem(precision) for m in self.meshes])
+        self._pbs = pbs
+        h = C.c_void_p()
+        rc = self._lib.efem_batch_create(C.byref(h), pbs, len(self.meshes), int(precision))
+        if rc != EFEM_OK:
+            raise EfemError(f"efem_batch_create failed ({rc}): {self._lib.efem_global_error().decode()}")
+        self._h = h
+        self.ndof = self._lib.efem_ndof(h)
+        self.count = self._lib.efem_batch_count(h)
+        off = np.empty(self.count + 1, np.int32)
+        self._check(self._lib.efem_batch_offsets(h, _ptr(off, C.c_int)), "efem_batch_offsets")
+        self.offsets = off.astype(np.int64)
+
+    def split(self, u):
+        return [u[self.offsets[i]:self.offsets[i + 1]] for i in range(self.count)]
+
+    def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=10000, max_linesearch=20, m=6,
+              lns_fea=1.0, use_precond=True, fp64_switch=0.0):
+        u = np.zeros(self.ndof) if u0 is None else np.array(u0, dtype=np.float64)
+        prm = LbfgsParams(float(epsilon), float(epsilon if epsilon_rel is None else epsilon_rel),
+                          int(max_iterations), int(max_linesearch), int(m), float(lns_fea),
+                          1 if use_precond else 0, 0, None, 0.0, float(fp64_switch))
+        st = (SolveStats * self.count)()
+        rc = self._lib.efem_batch_solve(self._h, _ptr(u, C.c_double), C.byref(prm), st)
+        if rc not in (EFEM_OK, EFEM_ERR_NONFINITE):
+            self._check(rc, "efem_batch_solve")
+        out = [s.as_dict() for s in st]
+        for d in out:
+            d["status"] = rc
+        return u, out
 
 
 def material_point(law, params, F, dF=None):

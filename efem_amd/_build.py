@@ -30,6 +30,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
 
 SOURCES = [
     ("device/kernels.hip", True),
+    ("device/batch.hip", True),
     ("api.cpp", True),
     ("host/model.cpp", False),
     ("host/element.cpp", False),

@@ -32,6 +32,8 @@ struct DeviceFactor {
     int *fptr = nullptr, *fidx = nullptr;
     void *y = nullptr, *t = nullptr, *z = nullptr;
     int4* items = nullptr;  // descriptor pairs
+    int* instItems = nullptr;  // batches: [instance][level] {fwd off, cnt, bwd off, cnt}
+    int instLevels = 0;
     std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level, in items
     // the whole apply captured as one HIP graph over fixed staging buffers
     double *in = nullptr, *out = nullptr, *bp = nullptr;
@@ -62,6 +64,11 @@ struct efem_ctx {
     int* faceDof = nullptr;
     double *fA0 = nullptr, *fWN = nullptr, *fG = nullptr;
     double* fpress = nullptr;  // per-face pressure of a batch (null: model.pressure)
+    // batched instances: per-instance models (factored one by one) and ranges
+    std::vector<efem::Model> parts;
+    std::vector<efem::PartRange> ranges;
+    efem::dev::BatchRange* dRanges = nullptr;
+    double* batchStats = nullptr;
     int *gptr = nullptr, *gslot = nullptr;
     double* fext = nullptr;
     // solver state
@@ -358,24 +365,12 @@ int applyH0Internal(efem_ctx* c, const double* v, double* out)
     return EFEM_OK;
 }
 
-int factorInternal(efem_ctx* c, int* failed)
+// upload a (possibly merged) supernodal factor: blocks, index arrays, the
+// forward extend-add map and the level-ordered work-item descriptors.  With
+// snOffset (batched instances), also the per-instance item ranges of every
+// level for the persistent solver.
+int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset)
 {
-    const auto t0 = std::chrono::steady_clock::now();
-    const efem::Model& m = c->model;
-    c->fac.ready = false;
-    c->factorFailed = true;
-    if (failed) *failed = 1;
-    if (m.nFree == 0) return EFEM_OK;  // jacobi.nonZeros() == 0 -> fallback
-    efem::Csc A;
-    efem::scalarLaplacian(m, A);
-    const std::vector<int> perm = efem::nestedDissection(A, efem::freeCoords(m));
-    efem::SupernodalFactor F;
-    std::string msg;
-    if (!efem::supernodalCholesky(A, perm, F, msg)) {
-        c->err = msg;
-        c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        return EFEM_OK;  // not SPD: the reference falls back to I/theta
-    }
     DeviceFactor& d = c->fac;
     d.n = F.n;
     d.nsuper = F.nsuper;
@@ -427,6 +422,32 @@ int factorInternal(efem_ctx* c, int* failed)
             }
         d.bwdCnt[l] = static_cast<int>(items.size() / 2) - d.bwdOff[l];
     }
+    if (snOffset) {
+        // items of one instance are contiguous within each level (supernodes
+        // are numbered instance by instance and listed in order)
+        const int np = static_cast<int>(snOffset->size()) - 1;
+        std::vector<int> inst(4 * static_cast<size_t>(np) * F.nlevels, 0);  // fwdOff, fwdCnt, bwdOff, bwdCnt
+        std::vector<int> partOf(F.nsuper);
+        for (int i = 0; i < np; ++i)
+            for (int s = (*snOffset)[i]; s < (*snOffset)[i + 1]; ++s) partOf[s] = i;
+        for (int l = 0; l < F.nlevels; ++l) {
+            int fo = d.fwdOff[l], bo = d.bwdOff[l];
+            for (int s : byLevel[l]) {
+                const int i = partOf[s];
+                int* q = &inst[4 * (static_cast<size_t>(i) * F.nlevels + l)];
+                const int nf = (mm[s] + 63) / 64, nb = (w[s] + 3) / 4;
+                if (q[1] == 0) q[0] = fo;
+                if (q[3] == 0) q[2] = bo;
+                q[1] += nf;
+                q[3] += nb;
+                fo += nf;
+                bo += nb;
+            }
+        }
+        d.instLevels = F.nlevels;
+        int rc0 = upload(c, &d.instItems, inst);
+        if (rc0) return rc0;
+    }
     int rc;
     if ((rc = upload(c, &d.perm, F.perm))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
@@ -465,6 +486,35 @@ int factorInternal(efem_ctx* c, int* failed)
     if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(F.n) * es + 7) / 8))) return rc;
     d.z = q;
     d.ready = true;
+    return EFEM_OK;
+}
+
+int factorInternal(efem_ctx* c, int* failed)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const efem::Model& m = c->model;
+    c->fac.ready = false;
+    c->factorFailed = true;
+    if (failed) *failed = 1;
+    if (m.nFree == 0) return EFEM_OK;  // jacobi.nonZeros() == 0 -> fallback
+    efem::SupernodalFactor F;
+    std::string msg;
+    std::vector<int> snOffset;
+    bool ok;
+    if (!c->parts.empty()) {
+        std::vector<efem::SupernodalFactor> P;
+        ok = efem::factorModels(c->parts, P, msg);
+        if (ok) efem::mergeFactors(P, F, snOffset);
+    } else {
+        ok = efem::factorModel(m, F, msg);
+    }
+    if (!ok) {
+        c->err = msg;
+        c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return EFEM_OK;  // not SPD: the reference falls back to I/theta
+    }
+    int rc = uploadFactor(c, F, c->parts.empty() ? nullptr : &snOffset);
+    if (rc) return rc;
     c->factorFailed = false;
     if (failed) *failed = 0;
     c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -638,6 +688,27 @@ int checkKrylovParams(efem_ctx* c, const efem_krylov_params* prm)
     return EFEM_OK;
 }
 
+// device, stream and model upload of a context whose model is built
+int initContext(std::unique_ptr<efem_ctx>& c)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_error = "efem_create: no HIP device";
+        return EFEM_ERR_HIP;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_error = "efem_create: hipStreamCreate failed";
+        return EFEM_ERR_HIP;
+    }
+    int rc = uploadModel(c.get());
+    if (rc) {
+        g_error = c->err;
+        efem_destroy(c.release());
+        return rc;
+    }
+    return EFEM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -665,21 +736,8 @@ int efem_create(efem_ctx** out, const efem_problem* pb)
         g_error = e.what();
         return EFEM_ERR_ARG;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        g_error = "efem_create: no HIP device";
-        return EFEM_ERR_HIP;
-    }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        g_error = "efem_create: hipStreamCreate failed";
-        return EFEM_ERR_HIP;
-    }
-    int rc = uploadModel(c.get());
-    if (rc) {
-        g_error = c->err;
-        efem_destroy(c.release());
-        return rc;
-    }
+    int rc = initContext(c);
+    if (rc) return rc;
     *out = c.release();
     return EFEM_OK;
 }
@@ -762,6 +820,10 @@ int efem_apply_h0(efem_ctx* c, const double* v, double* out)
 int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_stats* st)
 {
     if (!c || !u || !prm) return EFEM_ERR_ARG;
+    if (!c->parts.empty()) {
+        c->err = "efem_solve: batch context, use efem_batch_solve";
+        return EFEM_ERR_STATE;
+    }
     if (prm->m <= 0 || prm->m > 64 || prm->max_linesearch <= 0 || prm->epsilon < 0 || prm->epsilon_rel < 0 ||
         prm->max_iterations < 0) {
         c->err = "efem_solve: invalid parameters (Param.h:110-127)";
@@ -1288,6 +1350,168 @@ int efem_material_point(int law, const double* params, const double* F, double* 
     if (P) efem::piolaFromHd<double>(law, params, H, P);
     if (W) *W = efem::energyFromH(law, params, H);
     if (dP) efem::piolaTangent<double>(law, params, H, dF, dP);
+    return EFEM_OK;
+}
+
+int efem_batch_create(efem_ctx** out, const efem_problem* problems, int count, int precision)
+{
+    if (!out || !problems || count <= 0) {
+        g_error = "efem_batch_create: null argument or empty batch";
+        return EFEM_ERR_ARG;
+    }
+    *out = nullptr;
+    if (precision < 0 || precision > EFEM_MIXED) {
+        g_error = "efem_batch_create: unknown precision mode";
+        return EFEM_ERR_ARG;
+    }
+    auto c = std::make_unique<efem_ctx>();
+    c->precision = precision;
+    std::string msg;
+    std::vector<double> fpress;
+    try {
+        c->parts.resize(count);
+        for (int i = 0; i < count; ++i)
+            if (!efem::buildModel(problems[i], c->parts[i], msg)) {
+                g_error = "instance " + std::to_string(i) + ": " + msg;
+                return EFEM_ERR_ARG;
+            }
+        if (!efem::concatModels(c->parts, c->model, fpress, c->ranges, msg)) {
+            g_error = msg;
+            return EFEM_ERR_ARG;
+        }
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return EFEM_ERR_ARG;
+    }
+    int rc = initContext(c);
+    if (rc) return rc;
+    efem_ctx* cp = c.get();
+    std::vector<efem::dev::BatchRange> br(count);
+    for (int i = 0; i < count; ++i) {
+        const efem::PartRange& r = c->ranges[i];
+        br[i] = {r.e0, r.e1, r.f0, r.f1, r.n0, r.n1};
+    }
+    if ((rc = upload(cp, &cp->dRanges, br)) || (!fpress.empty() && (rc = upload(cp, &cp->fpress, fpress))) ||
+        (rc = devAlloc(cp, &cp->batchStats, 8 * static_cast<size_t>(count)))) {
+        g_error = cp->err;
+        efem_destroy(c.release());
+        return rc;
+    }
+    *out = c.release();
+    return EFEM_OK;
+}
+
+int efem_batch_count(const efem_ctx* c) { return c ? static_cast<int>(c->parts.size()) : 0; }
+
+int efem_batch_offsets(const efem_ctx* c, int* dof_offsets)
+{
+    if (!c || !dof_offsets || c->parts.empty()) return EFEM_ERR_ARG;
+    for (size_t i = 0; i < c->ranges.size(); ++i) dof_offsets[i] = 3 * c->ranges[i].n0;
+    dof_offsets[c->ranges.size()] = 3 * c->ranges.back().n1;
+    return EFEM_OK;
+}
+
+int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_stats* stats)
+{
+    if (!c || !u || !prm) return EFEM_ERR_ARG;
+    if (c->parts.empty()) {
+        c->err = "efem_batch_solve: not a batch context (efem_batch_create)";
+        return EFEM_ERR_STATE;
+    }
+    if (prm->m <= 0 || prm->m > efem::dev::kMaxCompact || prm->max_linesearch <= 0 || prm->epsilon < 0 ||
+        prm->epsilon_rel < 0 || prm->max_iterations <= 0 || prm->x_target) {
+        c->err = "efem_batch_solve: invalid parameters (history 1..16, max_iterations > 0, no x_target)";
+        return EFEM_ERR_ARG;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const int count = static_cast<int>(c->parts.size());
+    const int n = c->model.ndof();
+    const int mc = prm->m;
+    int rc;
+    if (mc > c->mcap) {
+        if ((rc = devAlloc(c, &c->S, static_cast<size_t>(mc) * std::max(n, 1)))) return rc;
+        if ((rc = devAlloc(c, &c->Y, static_cast<size_t>(mc) * std::max(n, 1)))) return rc;
+        c->mcap = mc;
+    }
+    double factorSeconds = 0.0;
+    if (prm->use_precond && !c->fac.ready) {
+        int failed = 0;
+        if ((rc = factorInternal(c, &failed))) return rc;
+        factorSeconds = c->factorSeconds;
+    }
+    const bool h0 = prm->use_precond && c->fac.ready;
+    if (h0 && !c->fac.bp) {
+        c->err = "internal: factor without staging buffer";
+        return EFEM_ERR_STATE;
+    }
+    const size_t nb = static_cast<size_t>(n) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(c->x, u, nb, hipMemcpyHostToDevice, c->stream));
+    efem::dev::BatchArgs a;
+    a.elem = elemArgs(c, c->x, false);
+    a.face = faceArgs(c, c->x, nullptr);
+    a.gather = gatherArgs(c, c->r, false, 1.0, -1.0, c->fext);
+    a.trsv = h0 ? trsvArgs(c) : efem::dev::TrsvArgs{};
+    a.trsv.mixed = fac32(c);
+    a.items = h0 ? c->fac.items : nullptr;
+    a.instItems = h0 ? c->fac.instItems : nullptr;
+    a.nlev = h0 ? c->fac.instLevels : 0;
+    a.ranges = c->dRanges;
+    a.x = c->x;
+    a.r = c->r;
+    a.xp = c->xp;
+    a.rp = c->rp;
+    a.d = c->d;
+    a.q = c->tmp;
+    a.bp = h0 ? c->fac.bp : nullptr;
+    a.S = c->S;
+    a.Y = c->Y;
+    a.ntot = n;
+    a.m = mc;
+    a.maxIt = prm->max_iterations;
+    a.maxLs = prm->max_linesearch;
+    a.usePrecond = h0 ? 1 : 0;
+    a.elemF32 = elem32(c) ? 1 : 0;
+    a.eps = prm->epsilon;
+    a.epsRel = prm->epsilon_rel;
+    a.lnsFea = prm->lns_fea;
+    a.fp64Switch = elem32(c) ? prm->fp64_switch : 0.0;
+    a.stats = c->batchStats;
+    efem::dev::launchBatchSolve(a, count, c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    std::vector<double> st(8 * static_cast<size_t>(count));
+    HIP_CHECK(c, hipMemcpyAsync(u, c->x, nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(st.data(), c->batchStats, st.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    bool anyNonfinite = false;
+    for (int i = 0; i < count; ++i) {
+        const double* q = &st[8 * static_cast<size_t>(i)];
+        const bool bad = q[5] != 0.0 || !std::isfinite(q[3]) || !std::isfinite(q[4]);
+        if (bad) {
+            anyNonfinite = true;
+            const efem::PartRange& r = c->ranges[i];
+            std::memset(u + 3 * static_cast<size_t>(r.n0), 0, 3 * static_cast<size_t>(r.n1 - r.n0) * sizeof(double));
+        }
+        if (stats) {
+            efem_solve_stats& s = stats[i];
+            std::memset(&s, 0, sizeof(s));
+            s.iterations = static_cast<int>(q[0]);
+            s.r_evals = static_cast<int>(q[1]);
+            s.ls_trials = static_cast<int>(q[2]);
+            s.rinf = q[3];
+            s.xinf = q[4];
+            s.precond_failed = prm->use_precond && !h0;
+            s.fp64_switch_iteration = static_cast<int>(q[6]);
+            s.factor_seconds = factorSeconds;
+            s.solve_seconds = wall;
+            s.factor_nnz = h0 ? c->fac.nnz : 0;
+        }
+    }
+    if (anyNonfinite) {
+        c->err = "batched L-BFGS: non-finite residual or displacement in at least one instance";
+        return EFEM_ERR_NONFINITE;
+    }
     return EFEM_OK;
 }
 

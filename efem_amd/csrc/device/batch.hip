@@ -1,0 +1,326 @@
+// Persistent batched solver: many independent instances (load cases or
+// unevenly sized meshes) in ONE launch, one 256-lane workgroup per instance.
+// Each workgroup runs the reference's L-BFGS loop (LBFGSSolver::solve,
+// LBFGS.h:96-174) with the projection line search (LineSearchFEA.h:37-68)
+// and the two-loop recursion in LBFGSMat::apply_Hv order (LBFGSMat.h:
+// 103-131) on its instance's ranges of the merged model, with no host
+// round trip: the residual (element tiles, pressure faces, gather), the
+// preconditioner (the instance's supernodal factor, level by level), and
+// all dot products are workgroup-local, separated by barriers.  Reductions
+// use fixed wave/block trees, so an instance's result does not depend on
+// the other instances in the batch.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace efem {
+namespace dev {
+namespace {
+
+// K simultaneous block reductions over the 256 lanes; op 0 = sum, 1 = max
+template <int K>
+__device__ __forceinline__ void blockReduceK(double (&v)[K], const int (&op)[K], double (*sh)[4])
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = op[k] ? waveMax(v[k]) : waveSum(v[k]);
+        if (lane == 0) sh[k][wave] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        v[k] = op[k] ? fmax(fmax(fmax(sh[k][0], sh[k][1]), sh[k][2]), sh[k][3])
+                     : ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    __syncthreads();
+}
+
+template <int NEN, int NG, typename TF>
+union BatchLds {
+    ElemTileLds<float, NEN, NG, MODE_FORCE> ef;
+    ElemTileLds<double, NEN, NG, MODE_FORCE> ed;
+    TrsvFwdLds<TF> tf;
+};
+
+template <typename T, int NEN, int NG, int LAW>
+__device__ __forceinline__ void blockElements(const BatchArgs& a, const BatchRange& I,
+                                              ElemTileLds<T, NEN, NG, MODE_FORCE>& L)
+{
+    const ElemArgs& ea = a.elem;
+    T* slots = static_cast<T*>(ea.slots);
+    const double* ginv = static_cast<const double*>(ea.ginv);
+    if (NEN == 4) {
+        for (int e = I.e0 + threadIdx.x; e < I.e1; e += kThreads)
+            tet4Lane<T, LAW, MODE_FORCE>(e, ea.nElems, ea.connDof, ginv, ea.vol, 1.0, ea.params, ea.paramsPerElem,
+                                          a.x, nullptr, slots);
+    } else {
+        const double* gdN = static_cast<const double*>(ea.gdN);
+        for (int k = threadIdx.x; k < NG * NEN * 3; k += kThreads) L.sdN[k] = gdN[k];
+        for (int e0 = I.e0; e0 < I.e1; e0 += Tile<NG>::epb)
+            elemTile<T, NEN, NG, LAW, MODE_FORCE>(ea.nElems, e0, I.e1, ea.connDof, ginv, ea.vol, ea.gw, ea.params,
+                                                   ea.paramsPerElem, a.x, nullptr, slots, L);
+    }
+}
+
+// r = f_int(x) - f_ext(x) on the instance's free nodes (x = a.x, r = a.r)
+template <typename T, int NEN, int NG, typename TF>
+__device__ void blockResidual(const BatchArgs& a, const BatchRange& I, BatchLds<NEN, NG, TF>& U)
+{
+    ElemTileLds<T, NEN, NG, MODE_FORCE>& L =
+        *reinterpret_cast<ElemTileLds<T, NEN, NG, MODE_FORCE>*>(&U);
+    switch (a.elem.law) {
+        case LAW_NH: blockElements<T, NEN, NG, LAW_NH>(a, I, L); break;
+        case LAW_MR: blockElements<T, NEN, NG, LAW_MR>(a, I, L); break;
+        default: blockElements<T, NEN, NG, LAW_YEOH>(a, I, L);
+    }
+    for (int f = I.f0 + threadIdx.x; f < I.f1; f += kThreads) pressureFace<NEN, PMODE_FORCE>(a.face, f);
+    __syncthreads();
+    GatherArgs g = a.gather;
+    g.mixed = sizeof(T) == 4;
+    for (int i = I.n0 + threadIdx.x; i < I.n1; i += kThreads) gatherNode<T>(g, i);
+    __syncthreads();
+}
+
+// out = K^-1 in on the instance (its supernodes of the merged forest)
+template <typename TF, int NEN, int NG>
+__device__ void blockH0(const BatchArgs& a, int inst, const BatchRange& I, const double* in, double* out,
+                        BatchLds<NEN, NG, TF>& U)
+{
+    const TrsvArgs& t = a.trsv;
+    for (int j = I.n0 + threadIdx.x; j < I.n1; j += kThreads) {
+        const int o = t.perm[j];
+        a.bp[3 * j] = in[3 * o];
+        a.bp[3 * j + 1] = in[3 * o + 1];
+        a.bp[3 * j + 2] = in[3 * o + 2];
+    }
+    __syncthreads();
+    const int* li = a.instItems + 4 * static_cast<size_t>(inst) * a.nlev;
+    for (int l = 0; l < a.nlev; ++l) {
+        const int off = li[4 * l], cnt = li[4 * l + 1];
+        for (int it = 0; it < cnt; ++it)
+            trsvFwdItem<TF>(t, a.bp, a.items[2 * (off + it)], a.items[2 * (off + it) + 1], U.tf);
+        __syncthreads();
+    }
+    for (int l = a.nlev - 1; l >= 0; --l) {
+        const int off = li[4 * l + 2], cnt = li[4 * l + 3];
+        for (int it = 0; it < cnt; ++it) trsvBwdItem<TF>(t, a.items[2 * (off + it)], a.items[2 * (off + it) + 1]);
+        __syncthreads();
+    }
+    const TF* z = static_cast<const TF*>(t.z);
+    for (int j = I.n0 + threadIdx.x; j < I.n1; j += kThreads) {
+        const int o = t.perm[j];
+        out[3 * o] = static_cast<double>(z[3 * j]);
+        out[3 * o + 1] = static_cast<double>(z[3 * j + 1]);
+        out[3 * o + 2] = static_cast<double>(z[3 * j + 2]);
+    }
+    __syncthreads();
+}
+
+template <int NEN, int NG, typename TF>
+__device__ __forceinline__ void residualAt(const BatchArgs& a, const BatchRange& I, bool f32,
+                                           BatchLds<NEN, NG, TF>& U)
+{
+    if (f32)
+        blockResidual<float, NEN, NG, TF>(a, I, U);
+    else
+        blockResidual<double, NEN, NG, TF>(a, I, U);
+}
+
+template <int NEN, int NG, typename TF>
+__global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
+{
+    __shared__ BatchLds<NEN, NG, TF> U;
+    __shared__ double red[3][4];
+    __shared__ double ys[kMaxCompact], alpha[kMaxCompact];
+    const int inst = blockIdx.x;
+    const BatchRange I = a.ranges[inst];
+    const int d0 = 3 * I.n0, d1 = 3 * I.n1;
+    const int tid = threadIdx.x;
+    const int m = a.m;
+    const size_t nt = static_cast<size_t>(a.ntot);
+    double* x = a.x;
+    double* r = a.r;
+    double* d = a.d;
+    double* q = a.q;
+    bool f32 = a.elemF32 != 0;
+    int rEvals = 0, lsTrials = 0, iters = 0, switchIt = 0, nonfinite = 0;
+
+    residualAt<NEN, NG, TF>(a, I, f32, U);
+    ++rEvals;
+    double rinf, xinf;
+    {
+        double v[3] = {0.0, 0.0, 0.0};
+        const int op[3] = {0, 1, 1};
+        for (int i = d0 + tid; i < d1; i += kThreads) {
+            v[0] += x[i] * x[i];
+            v[1] = fmax(v[1], fabs(x[i]));
+            v[2] = fmax(v[2], fabs(r[i]));
+        }
+        blockReduceK<3>(v, op, red);
+        rinf = v[2];
+        xinf = v[1];
+        const double xnorm2 = sqrt(v[0]);
+        if (rinf <= a.eps || rinf <= a.epsRel * xnorm2) {
+            iters = 1;
+            goto done;
+        }
+    }
+    {
+        const double rinf0 = rinf;
+        const bool h0 = a.usePrecond != 0;
+        // d = -H0 r
+        if (h0) {
+            blockH0<TF, NEN, NG>(a, inst, I, r, d, U);
+            for (int i = d0 + tid; i < d1; i += kThreads) d[i] = -d[i];
+        } else {
+            for (int i = d0 + tid; i < d1; i += kThreads) d[i] = -r[i];
+        }
+        __syncthreads();
+        int ncorr = 0, ptr = m;
+        double theta = 1.0;
+        for (int k = 1;; ++k) {
+            for (int i = d0 + tid; i < d1; i += kThreads) {
+                a.xp[i] = x[i];
+                a.rp[i] = r[i];
+            }
+            double dgInit;
+            {
+                double v[1] = {0.0};
+                const int op[1] = {0};
+                for (int i = d0 + tid; i < d1; i += kThreads) v[0] += r[i] * d[i];
+                blockReduceK<1>(v, op, red);
+                dgInit = v[0];
+            }
+            double cur = 1.0;
+            int trials = 0;
+            for (int it = 0; it < a.maxLs; ++it) {
+                for (int i = d0 + tid; i < d1; i += kThreads) x[i] = a.xp[i] + cur * d[i];
+                __syncthreads();
+                residualAt<NEN, NG, TF>(a, I, f32, U);
+                ++rEvals;
+                ++trials;
+                double v[3] = {0.0, 0.0, 0.0};
+                const int op[3] = {0, 1, 1};
+                for (int i = d0 + tid; i < d1; i += kThreads) {
+                    v[0] += r[i] * d[i];
+                    v[1] = fmax(v[1], fabs(r[i]));
+                    v[2] = fmax(v[2], fabs(x[i]));
+                }
+                blockReduceK<3>(v, op, red);
+                rinf = v[1];
+                xinf = v[2];
+                if (fabs(v[0]) <= a.lnsFea * fabs(dgInit)) break;
+                cur *= v[0] * dgInit > 0.0 ? 1.5 : 0.5;
+            }
+            lsTrials += trials;
+            iters = k;
+            if (!isfinite(rinf) || !isfinite(xinf)) {
+                nonfinite = 1;
+                break;
+            }
+            if (rinf <= a.eps || rinf <= a.epsRel * xinf) break;
+            if (k >= a.maxIt) break;
+            if (f32 && a.fp64Switch > 0.0 && rinf <= a.fp64Switch * rinf0) {
+                f32 = false;
+                switchIt = k;
+                residualAt<NEN, NG, TF>(a, I, f32, U);
+                ++rEvals;
+            }
+            // add_correction (LBFGSMat.h:81-98)
+            const int loc = ptr % m;
+            double* sl = a.S + loc * nt;
+            double* yl = a.Y + loc * nt;
+            {
+                double v[2] = {0.0, 0.0};
+                const int op[2] = {0, 0};
+                for (int i = d0 + tid; i < d1; i += kThreads) {
+                    const double si = x[i] - a.xp[i], yi = r[i] - a.rp[i];
+                    sl[i] = si;
+                    yl[i] = yi;
+                    v[0] += si * yi;
+                    v[1] += yi * yi;
+                }
+                blockReduceK<2>(v, op, red);
+                if (tid == 0) ys[loc] = v[0];
+                theta = v[1] / v[0];
+            }
+            if (ncorr < m) ++ncorr;
+            ptr = loc + 1;
+            // two-loop recursion d = -H r (LBFGSMat::apply_Hv order)
+            for (int i = d0 + tid; i < d1; i += kThreads) d[i] = -r[i];
+            __syncthreads();
+            int j = ptr % m;
+            for (int c = 0; c < ncorr; ++c) {
+                j = (j + m - 1) % m;
+                const double* sj = a.S + j * nt;
+                const double* yj = a.Y + j * nt;
+                double v[1] = {0.0};
+                const int op[1] = {0};
+                for (int i = d0 + tid; i < d1; i += kThreads) v[0] += sj[i] * d[i];
+                blockReduceK<1>(v, op, red);
+                const double aj = v[0] / ys[j];
+                if (tid == 0) alpha[j] = aj;
+                for (int i = d0 + tid; i < d1; i += kThreads) d[i] -= aj * yj[i];
+                __syncthreads();
+            }
+            if (h0) {
+                blockH0<TF, NEN, NG>(a, inst, I, d, q, U);
+                for (int i = d0 + tid; i < d1; i += kThreads) d[i] = q[i];
+            } else {
+                for (int i = d0 + tid; i < d1; i += kThreads) d[i] = d[i] / theta;
+            }
+            __syncthreads();
+            for (int c = 0; c < ncorr; ++c) {
+                const double* sj = a.S + j * nt;
+                const double* yj = a.Y + j * nt;
+                double v[1] = {0.0};
+                const int op[1] = {0};
+                for (int i = d0 + tid; i < d1; i += kThreads) v[0] += yj[i] * d[i];
+                blockReduceK<1>(v, op, red);
+                const double coef = alpha[j] - v[0] / ys[j];
+                for (int i = d0 + tid; i < d1; i += kThreads) d[i] += coef * sj[i];
+                __syncthreads();
+                j = (j + 1) % m;
+            }
+        }
+    }
+done:
+    if (tid == 0) {
+        double* st = a.stats + 8 * static_cast<size_t>(inst);
+        st[0] = iters;
+        st[1] = rEvals;
+        st[2] = lsTrials;
+        st[3] = rinf;
+        st[4] = xinf;
+        st[5] = nonfinite;
+        st[6] = switchIt;
+        st[7] = a.usePrecond;
+    }
+}
+
+template <typename TF>
+void launchBatchKind(const BatchArgs& a, int count, hipStream_t s)
+{
+    const int nen = a.elem.nen, ng = a.elem.ng;
+    if (nen == 4) kBatchLbfgs<4, 1, TF><<<count, kThreads, 0, s>>>(a);
+    else if (nen == 8 && ng == 8) kBatchLbfgs<8, 8, TF><<<count, kThreads, 0, s>>>(a);
+    else if (nen == 8 && ng == 1) kBatchLbfgs<8, 1, TF><<<count, kThreads, 0, s>>>(a);
+    else if (nen == 10 && ng == 4) kBatchLbfgs<10, 4, TF><<<count, kThreads, 0, s>>>(a);
+    else if (nen == 10 && ng == 1) kBatchLbfgs<10, 1, TF><<<count, kThreads, 0, s>>>(a);
+    else if (nen == 20 && ng == 27) kBatchLbfgs<20, 27, TF><<<count, kThreads, 0, s>>>(a);
+    else if (nen == 20 && ng == 8) kBatchLbfgs<20, 8, TF><<<count, kThreads, 0, s>>>(a);
+}
+
+}  // namespace
+
+void launchBatchSolve(const BatchArgs& a, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    if (a.trsv.mixed)
+        launchBatchKind<float>(a, count, s);
+    else
+        launchBatchKind<double>(a, count, s);
+}
+
+}  // namespace dev
+}  // namespace efem

@@ -16,6 +16,21 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// wave64 butterfly sum (fixed order)
+__device__ __forceinline__ double waveSum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double waveMax(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // H = Gu J0^-1 is formed in FP64 in both modes: J - 1 (and with it the
 // near-incompressible volumetric stress (2/D1)(J-1) cof F, 2/D1 = 2e6 here)
 // inherits the rounding of H, so H must carry more than FP32's 24 bits.

@@ -314,13 +314,6 @@ __global__ __launch_bounds__(kThreads) void kTwoLoop(double* __restrict__ q, con
 // m = 6; 16 is the maximum) so that every per-thread array has compile-time
 // indexing (no scratch); live slots are guarded by `k < nc`.
 
-// wave64 butterfly sum (fixed order)
-__device__ __forceinline__ double waveSum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // per-thread sums v[0..N) -> block partials (work[k * kReduceBlocks + b]) ->
 // the last block to arrive reduces the partials of every k with all its

@@ -133,5 +133,31 @@ void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, hipStr
 // out[3 perm[j] + c] = z[j][c]
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s);
 
+// ---- persistent batched L-BFGS (batch.hip) ---------------------------------
+// One workgroup per instance runs LBFGSSolver::solve (LBFGS.h:96-174) with
+// the projection line search to the end on the instance's ranges of the
+// merged model; every loop is bounded by maxIt / maxLs.
+struct BatchRange {
+    int e0, e1, f0, f1, n0, n1;  // elements, faces, free nodes of the instance
+};
+struct BatchArgs {
+    ElemArgs elem;               // merged model (ginv/gdN FP64; slots sized FP64)
+    FaceArgs face;               // x = the trial-point buffer `x` below
+    GatherArgs gather;           // r = the residual buffer `r` below
+    TrsvArgs trsv;               // merged factor forest (valid when usePrecond)
+    const int4* items;           // descriptor pairs
+    const int* instItems;        // [instance][level] {fwd off, cnt, bwd off, cnt}
+    int nlev;
+    const BatchRange* ranges;
+    double *x, *r, *xp, *rp, *d, *q, *bp;  // merged vectors (+3 dummy zeros)
+    double *S, *Y;               // history [m][ntot]
+    int ntot, m, maxIt, maxLs, usePrecond, elemF32;
+    double eps, epsRel, lnsFea, fp64Switch;
+    double* stats;               // [instance][8]
+};
+// stats per instance: iterations, r_evals, ls_trials, ||r||_inf, ||x||_inf,
+// non-finite flag, FP64-switch iteration, preconditioned flag
+void launchBatchSolve(const BatchArgs& a, int count, hipStream_t s);
+
 }  // namespace dev
 }  // namespace efem

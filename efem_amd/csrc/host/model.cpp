@@ -58,6 +58,109 @@ bool setParams(Model& m, const double* params, std::string& msg)
     return true;
 }
 
+bool concatModels(const std::vector<Model>& parts, Model& out, std::vector<double>& facePressure,
+                  std::vector<PartRange>& ranges, std::string& msg)
+{
+    if (parts.empty()) {
+        msg = "batch: no instances";
+        return false;
+    }
+    const Model& p0 = parts[0];
+    int N = 0, E = 0, F = 0, NF = 0;
+    bool anyFaces = false;
+    for (const Model& p : parts) {
+        if (p.nen != p0.nen || p.ng != p0.ng || p.reduced != p0.reduced || p.law != p0.law) {
+            msg = "batch: instances must share the element kind, integration rule and law";
+            return false;
+        }
+        if (p.nFaces > 0) {
+            if (anyFaces && (p.fq != out.fq || p.directed != out.directed ||
+                             (p.directed && (p.pdir[0] != out.pdir[0] || p.pdir[1] != out.pdir[1] ||
+                                             p.pdir[2] != out.pdir[2])))) {
+                msg = "batch: pressure faces must share the face rule and load kind (directed: the same vector)";
+                return false;
+            }
+            if (!anyFaces) {
+                out.fq = p.fq;
+                out.fnn = p.fnn;
+                out.directed = p.directed;
+                for (int k = 0; k < 3; ++k) out.pdir[k] = p.pdir[k];
+            }
+            anyFaces = true;
+        }
+        N += p.nNodes;
+        E += p.nElems;
+        F += p.nFaces;
+        NF += p.nFree;
+    }
+    const int nen = p0.nen, ng = p0.ng;
+    out.nNodes = N;
+    out.nElems = E;
+    out.nen = nen;
+    out.ng = ng;
+    out.law = p0.law;
+    out.reduced = p0.reduced;
+    out.nFree = NF;
+    out.paramsPerElem = 1;
+    out.gw = p0.gw;
+    out.gdN = p0.gdN;
+    out.nFaces = F;
+    out.pressure = 1.0;
+    out.X.reserve(3 * static_cast<size_t>(N));
+    out.conn.reserve(static_cast<size_t>(E) * nen);
+    out.fid.reserve(N);
+    out.connDof.assign(static_cast<size_t>(nen) * E, 0);
+    out.ginv.assign(static_cast<size_t>(ng) * 9 * E, 0.0);
+    out.params.reserve(4 * static_cast<size_t>(E));
+    facePressure.clear();
+    ranges.clear();
+    int nOff = 0, eOff = 0, fOff = 0, freeOff = 0;
+    out.gptr.assign(1, 0);
+    for (const Model& p : parts) {
+        ranges.push_back({eOff, eOff + p.nElems, fOff, fOff + p.nFaces, freeOff, freeOff + p.nFree});
+        out.X.insert(out.X.end(), p.X.begin(), p.X.end());
+        for (int c : p.conn) out.conn.push_back(c + nOff);
+        for (int f : p.fid) out.fid.push_back(f >= 0 ? f + freeOff : -1);
+        for (int a = 0; a < nen; ++a)
+            for (int e = 0; e < p.nElems; ++e) {
+                const int d = p.connDof[static_cast<size_t>(a) * p.nElems + e];
+                out.connDof[static_cast<size_t>(a) * E + eOff + e] = d == p.nFree ? NF : d + freeOff;
+            }
+        out.vol.insert(out.vol.end(), p.vol.begin(), p.vol.end());
+        for (int q = 0; q < ng; ++q)
+            for (int k = 0; k < 9; ++k)
+                for (int e = 0; e < p.nElems; ++e)
+                    out.ginv[(static_cast<size_t>(q) * 9 + k) * E + eOff + e] =
+                        p.ginv[(static_cast<size_t>(q) * 9 + k) * p.nElems + e];
+        for (int e = 0; e < p.nElems; ++e)
+            for (int k = 0; k < 4; ++k) out.params.push_back(p.params[p.paramsPerElem ? 4 * static_cast<size_t>(e) + k : k]);
+        out.coeff.insert(out.coeff.end(), p.coeff.begin(), p.coeff.end());
+        for (int f = 0; f < p.nFaces; ++f) {
+            out.faceElem.push_back(p.faceElem[f] + eOff);
+            facePressure.push_back(p.pressure);
+        }
+        out.fA0.insert(out.fA0.end(), p.fA0.begin(), p.fA0.end());
+        out.fWN.insert(out.fWN.end(), p.fWN.begin(), p.fWN.end());
+        out.fG.insert(out.fG.end(), p.fG.begin(), p.fG.end());
+        out.fext.insert(out.fext.end(), p.fext.begin(), p.fext.end());
+        // gather CSR: element slots shift by the element offset, face slots
+        // move behind all element slots
+        const int es = p.nElems * nen;
+        for (int i = 0; i < p.nFree; ++i) {
+            for (int q = p.gptr[i]; q < p.gptr[i + 1]; ++q) {
+                const int sl = p.gslot[q];
+                out.gslot.push_back(sl < es ? sl + eOff * nen : E * nen + fOff * nen + (sl - es));
+            }
+            out.gptr.push_back(static_cast<int>(out.gslot.size()));
+        }
+        nOff += p.nNodes;
+        eOff += p.nElems;
+        fOff += p.nFaces;
+        freeOff += p.nFree;
+    }
+    return true;
+}
+
 bool buildModel(const efem_problem& pb, Model& m, std::string& msg)
 {
     if (pb.n_nodes <= 0 || !pb.X || pb.n_elems <= 0 || !pb.conn || !pb.params) {
@@ -333,6 +436,35 @@ std::vector<double> freeCoords(const Model& m)
         if (m.fid[n] >= 0)
             for (int d = 0; d < 3; ++d) xyz[3 * static_cast<size_t>(m.fid[n]) + d] = m.X[3 * static_cast<size_t>(n) + d];
     return xyz;
+}
+
+bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg)
+{
+    if (m.nFree == 0) {
+        msg = "no free DOF";
+        return false;
+    }
+    Csc A;
+    scalarLaplacian(m, A);
+    const std::vector<int> perm = nestedDissection(A, freeCoords(m));
+    return supernodalCholesky(A, perm, F, msg);
+}
+
+bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg)
+{
+    const int np = static_cast<int>(parts.size());
+    F.clear();
+    F.resize(np);
+    std::vector<std::string> msgs(np);
+    std::vector<char> ok(np, 0);
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < np; ++i) ok[i] = factorModel(parts[i], F[i], msgs[i]) ? 1 : 0;
+    for (int i = 0; i < np; ++i)
+        if (!ok[i]) {
+            msg = "instance " + std::to_string(i) + ": " + msgs[i];
+            return false;
+        }
+    return true;
 }
 
 }  // namespace efem

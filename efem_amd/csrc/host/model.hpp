@@ -50,10 +50,28 @@ bool buildModel(const efem_problem& pb, Model& m, std::string& msg);
 // approximate-Jacobian coefficient derived from them
 bool setParams(Model& m, const double* params, std::string& msg);
 
+// Concatenation of independent instances into one model (batched solves):
+// nodes, free DOFs, elements and faces of part i follow those of part i-1;
+// element slots keep their (element, node) order, face slots follow all
+// element slots.  Parts must share the element kind, the integration rule
+// and the law.  Per-face pressures go to facePressure; per-part ranges to
+// ranges[i] = {elem0, elem1, face0, face1, free0, free1}.
+struct PartRange {
+    int e0, e1, f0, f1, n0, n1;
+};
+bool concatModels(const std::vector<Model>& parts, Model& out, std::vector<double>& facePressure,
+                  std::vector<PartRange>& ranges, std::string& msg);
+
 // scalar approximate Jacobian over free nodes (full symmetric CSC)
 void scalarLaplacian(const Model& m, Csc& L);
 
 // free-node coordinates [nFree][3]
 std::vector<double> freeCoords(const Model& m);
+
+// approximate Jacobian -> nested dissection -> supernodal factor (false: not
+// SPD or no free DOF, msg says which)
+bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg);
+// the same for independent parts, in parallel over the parts
+bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg);
 
 }  // namespace efem

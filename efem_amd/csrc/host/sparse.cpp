@@ -242,6 +242,48 @@ std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz)
     return st.out;
 }
 
+void mergeFactors(std::vector<SupernodalFactor>& parts, SupernodalFactor& out, std::vector<int>& snOffset)
+{
+    out = SupernodalFactor();
+    long long bs = 0;
+    for (const auto& P : parts) {
+        out.n += P.n;
+        out.nsuper += P.nsuper;
+        bs += P.Bsize;
+        out.nlevels = std::max(out.nlevels, P.nlevels);
+        out.nnzL += P.nnzL;
+        out.flops += P.flops;
+        out.seconds = std::max(out.seconds, P.seconds);
+        out.numericSeconds = std::max(out.numericSeconds, P.numericSeconds);
+    }
+    out.Bsize = bs;
+    out.B.reset(new double[static_cast<size_t>(std::max(bs, 1LL))]);
+    snOffset.assign(1, 0);
+    int col = 0, sn = 0, row = 0;
+    long long val = 0;
+    for (auto& P : parts) {
+        for (int v : P.perm) out.perm.push_back(v + col);
+        for (int s = 0; s < P.nsuper; ++s) {
+            out.first.push_back(P.first[s] + col);
+            out.rowPtr.push_back(P.rowPtr[s] + row);
+            out.valPtr.push_back(P.valPtr[s] + val);
+            out.parent.push_back(P.parent[s] >= 0 ? P.parent[s] + sn : -1);
+            out.level.push_back(P.level[s]);
+        }
+        for (int r : P.rows) out.rows.push_back(r + col);
+        std::memcpy(out.B.get() + val, P.B.get(), static_cast<size_t>(P.Bsize) * sizeof(double));
+        col += P.n;
+        sn += P.nsuper;
+        row += static_cast<int>(P.rows.size());
+        val += P.Bsize;
+        snOffset.push_back(sn);
+        P.B.reset();  // the merged copy holds the blocks
+    }
+    out.first.push_back(col);
+    out.rowPtr.push_back(row);
+    out.valPtr.push_back(val);
+}
+
 bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, SupernodalFactor& F,
                         std::string& msg)
 {

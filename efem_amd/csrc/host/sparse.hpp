@@ -54,6 +54,11 @@ std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz);
 bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F,
                         std::string& msg);
 
+// Forest of independent factors (batched instances): part i's columns,
+// supernodes, rows and blocks follow those of part i-1; levels stay per part.
+// snOffset[i] = first supernode of part i (size parts + 1).
+void mergeFactors(std::vector<SupernodalFactor>& parts, SupernodalFactor& out, std::vector<int>& snOffset);
+
 // the device algorithm run on the host with 3 interleaved right-hand sides:
 // forward (multifrontal) f_s = [b_s; 0] + sum_children u_c,
 // y_s = L11^-1 f_s[:w], u_s = f_s[w:] - (L21 L11^-1) f_s[:w];

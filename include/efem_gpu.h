@@ -242,10 +242,31 @@ int efem_set_load(efem_ctx* ctx, double pressure, const double* pdir);
 int efem_material_point(int law, const double* params, const double* F,
                         double* P, double* W, const double* dF, double* dP);
 
+/* ---- batches of independent instances (no counterpart in the reference) --
+ * efem_batch_create merges `count` problems (same element kind, integration
+ * rule and law; any sizes, parameters and follower pressures) into one
+ * context.  efem_residual / efem_factor / efem_apply_h0 then act on the
+ * concatenated free-DOF vector (instance i at dof_offsets[i]); each instance
+ * is factored on its own, exactly as in a single-instance context.
+ * efem_batch_solve runs LBFGSSolver::solve (LBFGS.h:96-174) for every
+ * instance in ONE persistent launch, one workgroup per instance, with
+ * per-instance L-BFGS state; prm->m <= 16, prm->max_iterations > 0, no
+ * x_target.  u: concatenated in/out; stats: [count].  Returns
+ * EFEM_ERR_NONFINITE (with those instances' u zeroed) if any instance
+ * produced NaN/Inf.  efem_solve is refused on a batch context. */
+int efem_batch_create(efem_ctx** out, const efem_problem* problems, int count,
+                      int precision);
+int efem_batch_count(const efem_ctx* ctx);
+int efem_batch_offsets(const efem_ctx* ctx, int* dof_offsets /* [count+1] */);
+int efem_batch_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
+                     efem_solve_stats* stats);
+
 /* ---- synthetic workloads (host only, no GPU needed) --------------------
  * which: 1 cantilever 20x4x4 C3D8 NH; 2 beam 200x20x20 Kuhn C3D4 MR;
  *        3 hollow cylinder 48x4x40 C3D20R Yeoh; 4 bar 64x16x12 Kuhn C3D10
- *        NH per-element; 0 custom box (dims from opts).
+ *        NH per-element; 5 one cantilever of the batch (opts = {index,
+ *        seed}: Lx4x4 C3D8, L ~ U{10..40}, C10 ~ U[80k,150k], p ~ U[0.5,5] kPa);
+ *        0 custom box (dims from opts).
  * opts (may be NULL) = {nx, ny, nz, seed, ...} overrides, see efem_mesh_build. */
 typedef struct efem_mesh efem_mesh;
 efem_mesh* efem_mesh_build(int which, const double* opts, int n_opts);

@@ -1,0 +1,124 @@
+"""GPU: batches of independent instances (efem_batch_*).
+
+Pins:
+  * the merged residual equals the oracle's residual of every instance
+    (<= 1e-10, FP64) -- the concatenation adds nothing;
+  * the merged preconditioner apply equals each instance's own context
+    bit for bit (same per-instance factor, same work items);
+  * the persistent batched L-BFGS: an instance's result is bitwise the same
+    whatever else is in the batch (fixed per-workgroup reduction trees);
+    it reaches the converged solution of the grid-path solve of that
+    instance alone (<= 1e-6) with a comparable iteration count; mixed
+    precision reaches the FP64 batch solution.
+"""
+import numpy as np
+import pytest
+
+import efem_amd as E
+import oracle as O
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def beams(idx, seed=0, shorten=True):
+    """Batch instances (workload 5); shortened for the parity tests."""
+    out = []
+    for i in idx:
+        m = E.Mesh.workload(5, [i, seed])
+        if shorten:
+            # keep at most 12 elements along x: cheap oracle solves
+            nx = m.n_elems // 16
+            if nx > 12:
+                keep = m.X[:, 0] <= 12 * 0.005 + 1e-9
+                newid = -np.ones(m.n_nodes, np.int64)
+                newid[keep] = np.arange(keep.sum())
+                ek = np.all(keep[m.conn], axis=1)
+                fk = np.all(keep[m.faces], axis=1)
+                m = E.Mesh(m.X[keep], newid[m.conn[ek]], m.law, m.params, m.fixed[keep], newid[m.faces[fk]],
+                           m.pressure, m.reduced)
+        out.append(m)
+    return out
+
+
+def test_batch_residual_matches_oracle(rng):
+    ms = beams(range(5))
+    with E.Batch(ms) as b:
+        u = 1e-4 * rng.standard_normal(b.ndof)
+        r = b.split(b.residual(u))
+        for i, (m, ui) in enumerate(zip(ms, b.split(u))):
+            assert rel(r[i], O.Model(m).residual(ui)) <= 1e-10, i
+
+
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_batch_apply_h0_matches_single(prec, rng):
+    ms = beams(range(4))
+    with E.Batch(ms, prec) as b:
+        assert b.factor()
+        v = rng.standard_normal(b.ndof)
+        hv = b.split(b.apply_h0(v))
+        for i, (m, vi) in enumerate(zip(ms, b.split(v))):
+            with E.Context(m, prec) as c:
+                assert c.factor()
+                assert np.array_equal(hv[i], c.apply_h0(vi)), i
+
+
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_batch_result_independent_of_batch(prec):
+    ms = beams(range(6))
+    kw = dict(epsilon=1e-7, max_iterations=3000, fp64_switch=1e-3 if prec == E.MIXED else 0.0)
+    with E.Batch(ms, prec) as b:
+        u, st = b.solve(**kw)
+        parts = b.split(u)
+    with E.Batch([ms[3]], prec) as b1:
+        u3, st3 = b1.solve(**kw)
+    assert np.array_equal(u3, parts[3])
+    assert st3[0]["iterations"] == st[3]["iterations"] and st3[0]["r_evals"] == st[3]["r_evals"]
+    with E.Batch([ms[5], ms[3]], prec) as b2:
+        u53, st53 = b2.solve(**kw)
+        assert np.array_equal(b2.split(u53)[1], parts[3])
+
+
+def test_batch_matches_single_grid_solves():
+    ms = beams(range(6))
+    with E.Batch(ms) as b:
+        u, st = b.solve(epsilon=1e-8, max_iterations=5000)
+        parts = b.split(u)
+    for i, m in enumerate(ms):
+        assert st[i]["rinf"] <= 1e-8, (i, st[i])
+        with E.Context(m) as c:
+            ui, si = c.solve(epsilon=1e-8, max_iterations=5000)
+        print(f"instance {i}: batch {st[i]['iterations']} it, grid {si['iterations']} it", flush=True)
+        assert rel(parts[i], ui) <= 1e-6, i
+        assert abs(st[i]["iterations"] - si["iterations"]) <= 0.2 * si["iterations"] + 5
+
+
+def test_batch_matches_oracle_solve():
+    ms = beams([1, 2])
+    with E.Batch(ms) as b:
+        u, st = b.solve(epsilon=1e-8, max_iterations=5000)
+        parts = b.split(u)
+    for i, m in enumerate(ms):
+        uo, so = O.Model(m).solve(epsilon=1e-8, max_iterations=5000)
+        assert rel(parts[i], uo) <= 1e-6, i
+
+
+def test_batch_mixed_reaches_fp64_solution():
+    ms = beams(range(4))
+    with E.Batch(ms) as b:
+        u64, _ = b.solve(epsilon=1e-9, max_iterations=5000)
+    with E.Batch(ms, E.MIXED) as b:
+        u32, st = b.solve(epsilon=1e-9, max_iterations=5000, fp64_switch=1e-3)
+    for a, c in zip(b.split(u32), b.split(u64)):
+        assert rel(a, c) <= 1e-6
+
+
+def test_batch_rejects_bad_input():
+    ms = beams(range(2))
+    with E.Batch(ms) as b:
+        with pytest.raises(E.EfemError):
+            b.solve(max_iterations=0)
+        with pytest.raises(E.EfemError):
+            E.Context.solve(b)
+    with pytest.raises(E.EfemError):
+        E.Batch([ms[0], E.Mesh.workload(2, [4, 2, 2])])  # C3D8 with C3D4
