@@ -70,10 +70,134 @@ def cpu_baseline(mesh, n_evals):
     return n_evals / dt, dt
 
 
+def run_batch(E, args):
+    """64 cantilevers (Lx4x4 C3D8, L ~ U{10..40}, NH, p ~ U[0.5, 5] kPa x scale):
+    all solved in one persistent launch (mixed precision, per-instance
+    L-BFGS), against 64 sequential FP64 grid-path solves.  Both include the
+    factorisation; both stop at the reference criterion |r|inf <= 1e-6."""
+    meshes = [E.Mesh.workload(5, [i, 20240531, args.batch_load_scale]) for i in range(64)]
+    kw = dict(epsilon=1e-6, epsilon_rel=1e-6, max_iterations=20000)
+    ctxs = [E.Context(m, E.FP64) for m in meshes]
+    for c in ctxs[:2]:
+        c.solve(**kw)  # warm-up (code paths, allocations)
+    t = time.perf_counter()
+    seq = []
+    for c in ctxs:
+        c.factor_release()
+        seq.append(c.solve(**kw))
+    t_seq = time.perf_counter() - t
+    b = E.Batch(meshes, E.MIXED)
+    b.solve(fp64_switch=1e-3, **kw)  # warm-up
+    times = []
+    for _ in range(3):
+        b.factor_release()
+        t = time.perf_counter()
+        ub, stb = b.solve(fp64_switch=1e-3, **kw)
+        times.append(time.perf_counter() - t)
+    t_b = min(times)
+    parts = b.split(ub)
+    errs = [float(np.linalg.norm(pb - us) / max(np.linalg.norm(us), 1e-300)) for pb, (us, _) in zip(parts, seq)]
+    its_b = [s["iterations"] for s in stb]
+    its_s = [s["iterations"] for _, s in seq]
+    log(f"batch: {t_b:.3f}s (factor {stb[0]['factor_seconds']:.3f}s) vs sequential FP64 {t_seq:.3f}s")
+    return {
+        "instances": len(meshes),
+        "load_scale": args.batch_load_scale,
+        "elements": int(sum(m.n_elems for m in meshes)),
+        "dof": int(sum(m.ndof for m in meshes)),
+        "batch_mixed_s": round(t_b, 4),
+        "batch_factor_s": round(stb[0]["factor_seconds"], 4),
+        "sequential_fp64_s": round(t_seq, 4),
+        "speedup": round(t_seq / t_b, 2),
+        "iterations_batch_mean": float(np.mean(its_b)),
+        "iterations_sequential_mean": float(np.mean(its_s)),
+        "max_rel_disp_diff_vs_sequential_fp64": max(errs),
+        "converged_batch": int(sum(s["rinf"] <= max(1e-6, 1e-6 * s["xinf"]) for s in stb)),
+        "converged_sequential": int(sum(s["rinf"] <= max(1e-6, 1e-6 * s["xinf"]) for _, s in seq)),
+    }
+
+
+def run_adjoint(E, args):
+    """Bar 0.08x0.02x0.015, 64x16x12 Kuhn C3D10, per-element C10 =
+    114000 exp(0.2 z_e): J = sum over the x = max face of |u - (0, 5 mm, 0)|^2,
+    gradient w.r.t. every log C10_e and log p by the adjoint (one BiCGStab
+    solve on K_t^T), checked by central differences along seeded random
+    directions (step 1e-6) through Newton equilibrium solves."""
+    m = E.Mesh.workload(4, [64, 16, 12, 7, args.adjoint_pressure])
+    xmax = m.X[:, 0].max()
+    S = np.flatnonzero(np.isclose(m.X[:, 0], xmax) & (m.fixed == 0))
+    fid = E.free_ids(m.fixed)
+    idx = (3 * fid[S][:, None] + np.arange(3)).ravel()
+    tgt = np.tile([0.0, 0.005, 0.0], len(S))
+    J = lambda u: float(((u[idx] - tgt) ** 2).sum())
+    C10 = m.params[:, 0].copy()
+    p0 = m.pressure
+    res = {"elements": m.n_elems, "dof": m.ndof, "pressure_pa": p0, "pressure_configured_pa": 20000.0}
+    with E.Context(m, E.FP64) as ctx:
+        fext = np.abs(ctx.residual(np.zeros(m.ndof))).max()
+        t = time.perf_counter()
+        u, st = ctx.solve(epsilon=1e-8 * fext, epsilon_rel=0.0, max_iterations=20000)
+        t_lbfgs = time.perf_counter() - t
+        # Newton to the rounding floor (it stops when |r| no longer decreases)
+        u, sn = ctx.newton(u, epsilon=1e-13 * fext, max_iterations=12, tol=1e-10)
+        t_fwd = time.perf_counter() - t
+        g = np.zeros(m.ndof)
+        g[idx] = 2.0 * (u[idx] - tgt)
+        t = time.perf_counter()
+        _, gp, gl, sa = ctx.adjoint(u, g, tol=1e-10, max_iterations=4000)
+        t_adj = time.perf_counter() - t
+        gz = gp[:, 0] * C10
+        gs = gl[0] * p0
+        rng = np.random.default_rng(20240531)
+        errs = []
+        h = 1e-6
+        for _ in range(args.fd_directions):
+            dz = rng.standard_normal(m.n_elems)
+            ds = rng.standard_normal()
+            vals = []
+            for sg in (1.0, -1.0):
+                P = m.params.copy()
+                P[:, 0] = C10 * np.exp(sg * h * dz)
+                ctx.set_params(P)
+                ctx.set_load(pressure=p0 * np.exp(sg * h * ds))
+                uu, su = ctx.newton(u, epsilon=1e-13 * fext, max_iterations=12, tol=1e-10)
+                vals.append(J(uu))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = float(gz @ dz + gs * ds)
+            errs.append(abs(fd - an) / abs(an))
+        ctx.set_params(m.params)
+        ctx.set_load(pressure=p0)
+    with E.Context(m, E.MIXED) as cm:
+        t = time.perf_counter()
+        _, gp32, gl32, sa32 = cm.adjoint(u, g, tol=1e-8, max_iterations=4000)
+        t_adj32 = time.perf_counter() - t
+    gz32 = gp32[:, 0] * C10
+    log(f"adjoint: forward {t_fwd:.2f}s ({st['iterations']} L-BFGS it + {sn['iterations']} Newton), "
+        f"adjoint {t_adj:.3f}s ({sa['iterations']} BiCGStab it), FD max rel err {max(errs):.2e}")
+    res.update({
+        "J": J(u),
+        "forward_s": round(t_fwd, 3),
+        "forward_lbfgs_s": round(t_lbfgs, 3),
+        "forward_lbfgs_iterations": st["iterations"],
+        "forward_newton_iterations": sn["iterations"],
+        "forward_rinf_over_fext": sn["rinf"] / fext,
+        "factor_s": round(st["factor_seconds"], 3),
+        "adjoint_s": round(t_adj, 4),
+        "adjoint_krylov_iterations": sa["iterations"],
+        "adjoint_cost_in_forward_solves": round(t_adj / t_fwd, 3),
+        "fd_directions": args.fd_directions,
+        "fd_step": h,
+        "fd_max_rel_err": max(errs) if errs else None,
+        "adjoint_mixed_s": round(t_adj32, 4),
+        "mixed_vs_fp64_grad_rel": float(np.linalg.norm(gz32 - gz) / np.linalg.norm(gz)),
+    })
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pressure", type=float, default=100.0)
     ap.add_argument("--target", type=float, default=1e-6)
@@ -83,6 +207,13 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp64-switch", type=float, default=1e-3,
                     help="mixed: element kernels go FP64 once |r|inf <= this x |r0|inf")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-cantilevers section")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
+    ap.add_argument("--batch-load-scale", type=float, default=1.0 / 64.0,
+                    help="scale of the batch pressures (at 1 the longest, softest beams fold; see DESIGN.md)")
+    ap.add_argument("--adjoint-pressure", type=float, default=2000.0,
+                    help="load of the adjoint bar (the configured 20 kPa folds it; see DESIGN.md)")
+    ap.add_argument("--fd-directions", type=int, default=16)
     ap.add_argument("--profile-run", type=int, default=0,
                     help="only run one mixed solve of this many iterations (for rocprofv3)")
     args = ap.parse_args()
@@ -113,21 +244,32 @@ def main():
         ctx["fp64"] = E.Context(mesh, E.FP64)
     c64 = ctx.get("fp64") or E.Context(mesh, E.FP64)
 
-    # u*: FP64, converged far below the 1e-6 displacement target
-    fext = np.abs(c64.residual(np.zeros(mesh.ndof))).max()
-    t0 = time.perf_counter()
-    u_star, st_star = c64.solve(epsilon=1e-10 * fext, epsilon_rel=0.0, max_iterations=args.max_iters)
-    log(f"u*: {st_star['iterations']} it, |r|inf {st_star['rinf']:.3e} "
-        f"(f_ext {fext:.3e}), {time.perf_counter() - t0:.1f}s")
+    # The iteration count of the reference L-BFGS is chaotic in the data
+    # (a 1e-9 relative change of the load can change it several-fold), so
+    # every step k solves at the load p (1 + 1e-6 k) -- the same workload to
+    # six digits -- and both precisions see the same samples.  u*_k: FP64,
+    # the reference L-BFGS to 1e-8 f_ext, then Newton-Krylov on the true
+    # tangent to the rounding floor (not timed).
+    def reference_solution(pk):
+        c64.set_load(pressure=pk)
+        fext = np.abs(c64.residual(np.zeros(mesh.ndof))).max()
+        t0 = time.perf_counter()
+        us, st_l = c64.solve(epsilon=1e-8 * fext, epsilon_rel=0.0, max_iterations=args.max_iters)
+        us, st_n = c64.newton(us, epsilon=1e-13 * fext, max_iterations=12, tol=1e-10)
+        log(f"u* at p = {pk:.6f}: {st_l['iterations']} L-BFGS it + {st_n['iterations']} Newton it, "
+            f"|r|inf/f_ext {st_n['rinf'] / fext:.2e}, {time.perf_counter() - t0:.1f}s")
+        return us, {"lbfgs_iterations": st_l["iterations"], "newton_iterations": st_n["iterations"],
+                    "rinf_over_fext": st_n["rinf"] / fext}
 
-    def one_solve(name, profile=False):
+    def one_solve(name, pk, us, profile=False):
         c = ctx[name]
+        c.set_load(pressure=pk)
         c.factor_release()
         if profile:
             c.profile_reset()
         t = time.perf_counter()
         u, st = c.solve(epsilon=0.0, epsilon_rel=0.0, max_iterations=args.max_iters,
-                        x_target=u_star, target_tol=args.target,
+                        x_target=us, target_tol=args.target,
                         fp64_switch=args.fp64_switch if name == "mixed" else 0.0)
         dt = time.perf_counter() - t
         st["wall"] = dt
@@ -135,15 +277,18 @@ def main():
             log(f"WARNING {name}: target not reached (err {st['target_err']:.2e})")
         return st
 
+    u_star, ustar_info = reference_solution(args.pressure)
     for _ in range(args.warmup):
         for name in ctx:
-            one_solve(name)
+            one_solve(name, args.pressure, u_star)
 
     res = {name: [] for name in ctx}
     prof = {}
     for k in range(args.steps):
+        pk = args.pressure * (1.0 + 1e-6 * (k + 1))
+        u_star, ustar_info = reference_solution(pk)
         for name in ctx:
-            st = one_solve(name, profile=(k == args.steps - 1))
+            st = one_solve(name, pk, u_star, profile=(k == args.steps - 1))
             res[name].append(st)
             log(f"{name}: {st['iterations']} it, {st['r_evals']} evals, err {st['target_err']:.2e}, "
                 f"factor {st['factor_seconds']:.2f}s, total {st['wall']:.3f}s, "
@@ -152,6 +297,15 @@ def main():
                 ems, ek = ctx[name].profile_read()
                 hms, hk = ctx[name].profile_read_h0()
                 prof[name] = (ems, ek, hms, hk, ctx[name].factor_nnz)
+
+    # element kernel alone: 20 residual evaluations at u* per context (the
+    # solve's own launches mix FP32 and, after the switch, FP64 kernels)
+    elem = {}
+    for name in ctx:
+        ctx[name].profile_reset()
+        for _ in range(20):
+            ctx[name].residual(u_star)
+        elem[name] = ctx[name].profile_read()
 
     mix = res["mixed"]
     t_mix = float(np.mean([s["wall"] for s in mix]))
@@ -190,7 +344,9 @@ def main():
         "disp_err_mixed": float(max(s["target_err"] for s in mix)),
         "fp64_switch": args.fp64_switch,
         "fp64_switch_iteration": int(mix[-1]["fp64_switch_iteration"]),
-        "u_star_iterations": st_star["iterations"],
+        "u_star": ustar_info,
+        "load_samples": [round(args.pressure * (1.0 + 1e-6 * (k + 1)), 6) for k in range(args.steps)],
+        "iterations_mixed_per_sample": [s["iterations"] for s in mix],
     }
     if "fp64" in res:
         f64 = res["fp64"]
@@ -200,11 +356,13 @@ def main():
             "speedup_mixed_vs_fp64": round(t64 / t_mix, 3),
             "evals_per_s_fp64": round(sum(s["r_evals"] for s in f64) / sum(s["wall"] for s in f64), 2),
             "iterations_fp64": int(np.mean([s["iterations"] for s in f64])),
+            "iterations_fp64_per_sample": [s["iterations"] for s in f64],
             "disp_err_fp64": float(max(s["target_err"] for s in f64)),
         })
     # roofline of the dominant kernel family: the preconditioner apply
     # (forward + backward supernodal triangular solves), HBM-bound
-    ems, ek, hms, hk, nnz = prof["mixed"]
+    _, _, hms, hk, nnz = prof["mixed"]
+    ems, ek = elem["mixed"]
     h0_ms = hms / max(hk, 1)
     h0_bytes = 2 * nnz * 4 + 48 * mesh.n_free
     out["roofline"] = {
@@ -233,7 +391,8 @@ def main():
         "alg_flops": F,
     }
     if "fp64" in prof:
-        ems64, ek64, hms64, hk64, _ = prof["fp64"]
+        _, _, hms64, hk64, _ = prof["fp64"]
+        ems64, ek64 = elem["fp64"]
         B64, F64 = element_work(mesh, "fp64")
         e64 = 1e3 * ems64 / max(ek64, 1)
         out["roofline_element_fp64"] = {
@@ -242,6 +401,15 @@ def main():
             "frac": round(max(B64 / (HBM_PEAK_GBS * 1e9), F64 / (FP64_PEAK_TFLOPS * 1e12)) / (e64 * 1e-6), 4),
         }
         out["h0_apply_ms_fp64"] = round(hms64 / max(hk64, 1), 4)
+    for key, fn, skip in (("batch_cantilevers", run_batch, args.no_batch),
+                          ("adjoint_bar", run_adjoint, args.no_adjoint)):
+        if skip:
+            continue
+        try:
+            out[key] = fn(E, args)
+        except Exception as e:  # the headline line is printed regardless
+            out[key] = {"error": repr(e)}
+            log(f"{key}: {e!r}")
     if not args.no_cpu:
         rate, dt = cpu_baseline(mesh, args.cpu_evals)
         out["cpu_baseline"] = {

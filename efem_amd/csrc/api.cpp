@@ -90,7 +90,8 @@ struct efem_ctx {
     double* xtarget = nullptr;
     // tangent / Krylov / adjoint work vectors (n + 3 each, allocated on first use)
     double *ku = nullptr, *kb = nullptr, *kx = nullptr, *kr = nullptr, *krh = nullptr, *kp = nullptr,
-           *kv = nullptr, *ks = nullptr, *kt = nullptr, *kph = nullptr, *ksh = nullptr;
+           *kv = nullptr, *ks = nullptr, *kt = nullptr, *kph = nullptr, *ksh = nullptr, *kxo = nullptr,
+           *kbo = nullptr;
     bool useGraph = true;    // option "h0_graph"
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
 };
@@ -287,10 +288,10 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
 
 // out = K_t(x) v = d r / d x [v] (transpose: K_t^T v).  The element part is
 // symmetric (hyperelastic); only the follower-pressure part has a transpose.
-int tangentInternal(efem_ctx* c, const double* x, const double* v, double* out, int transpose)
+int tangentInternal(efem_ctx* c, const double* x, const double* v, double* out, int transpose, bool fp64 = false)
 {
     const efem::Model& m = c->model;
-    const bool f32 = elem32(c);
+    const bool f32 = elem32(c) && !fp64;
     efem::dev::launchElementTangent(elemArgs(c, x, f32), v, c->stream);
     HIP_CHECK(c, hipGetLastError());
     if (m.nFaces > 0) {
@@ -331,32 +332,49 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out)
     efem::dev::launchUnpermute(a, out, c->stream);
 }
 
+// Replay the captured apply: reads the staging buffer f.in, writes f.out
+// (captured on first use).
+int applyH0Graph(efem_ctx* c)
+{
+    DeviceFactor& f = c->fac;
+    const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
+    if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
+    if (!f.exec) {
+        HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launchApplySweeps(c, f.in, f.out);
+        hipError_t e = hipStreamEndCapture(c->stream, &f.graph);
+        if (e != hipSuccess) {
+            c->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+            return EFEM_ERR_HIP;
+        }
+        HIP_CHECK(c, hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
+    }
+    HIP_CHECK(c, hipGraphLaunch(f.exec, c->stream));
+    if (prof) {
+        HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used + 1], c->stream));
+        c->evH0Used += 2;
+    }
+    HIP_CHECK(c, hipGetLastError());
+    return EFEM_OK;
+}
+
 // out = K^-1 v on the device (v, out: interleaved free-DOF vectors).  With
 // the graph option the forward/backward level launches are replayed from one
 // captured HIP graph reading `in` and writing `out` staging buffers.
 int applyH0Internal(efem_ctx* c, const double* v, double* out)
 {
     DeviceFactor& f = c->fac;
-    const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
-    if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
     if (c->useGraph) {
         const size_t nb = 3 * static_cast<size_t>(f.n) * sizeof(double);
-        if (!f.exec) {
-            HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            launchApplySweeps(c, f.in, f.out);
-            hipError_t e = hipStreamEndCapture(c->stream, &f.graph);
-            if (e != hipSuccess) {
-                c->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
-                return EFEM_ERR_HIP;
-            }
-            HIP_CHECK(c, hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
-        }
-        HIP_CHECK(c, hipMemcpyAsync(f.in, v, nb, hipMemcpyDeviceToDevice, c->stream));
-        HIP_CHECK(c, hipGraphLaunch(f.exec, c->stream));
-        HIP_CHECK(c, hipMemcpyAsync(out, f.out, nb, hipMemcpyDeviceToDevice, c->stream));
-    } else {
-        launchApplySweeps(c, v, out);
+        if (v != f.in) HIP_CHECK(c, hipMemcpyAsync(f.in, v, nb, hipMemcpyDeviceToDevice, c->stream));
+        int rc = applyH0Graph(c);
+        if (rc) return rc;
+        if (out != f.out) HIP_CHECK(c, hipMemcpyAsync(out, f.out, nb, hipMemcpyDeviceToDevice, c->stream));
+        return EFEM_OK;
     }
+    const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
+    if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
+    launchApplySweeps(c, v, out);
     if (prof) {
         HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used + 1], c->stream));
         c->evH0Used += 2;
@@ -533,7 +551,8 @@ int ensureKrylov(efem_ctx* c)
     if (c->ku) return EFEM_OK;
     const size_t np = static_cast<size_t>(c->model.ndof()) + 3;
     int rc;
-    for (double** v : {&c->ku, &c->kb, &c->kx, &c->kr, &c->krh, &c->kp, &c->kv, &c->ks, &c->kt, &c->kph, &c->ksh}) {
+    for (double** v : {&c->ku, &c->kb, &c->kx, &c->kr, &c->krh, &c->kp, &c->kv, &c->ks, &c->kt, &c->kph, &c->ksh,
+                       &c->kxo, &c->kbo}) {
         if ((rc = devAlloc(c, v, np))) return rc;
         HIP_CHECK(c, hipMemset(*v, 0, np * sizeof(double)));
     }
@@ -564,7 +583,8 @@ int precondApply(efem_ctx* c, bool h0, const double* in, double* out, efem_krylo
 // Solve K_t(u) x = b (transpose: K_t^T x = b) on the device vectors ku, kb ->
 // kx with preconditioned CG (symmetric tangent) or BiCGStab.  M = the
 // approximate-Jacobian Cholesky factor (or I if it failed / is disabled).
-int krylovInternal(efem_ctx* c, int transpose, const efem_krylov_params& prm, efem_krylov_stats& st)
+// The operator uses the context's element precision unless fp64.
+int krylovCore(efem_ctx* c, int transpose, const efem_krylov_params& prm, efem_krylov_stats& st, bool fp64)
 {
     const int n = c->model.ndof();
     const auto t0 = std::chrono::steady_clock::now();
@@ -590,7 +610,7 @@ int krylovInternal(efem_ctx* c, int transpose, const efem_krylov_params& prm, ef
     const size_t nb = static_cast<size_t>(n) * sizeof(double);
     auto A = [&](const double* v, double* out) {
         ++st.matvecs;
-        return tangentInternal(c, c->ku, v, out, transpose);
+        return tangentInternal(c, c->ku, v, out, transpose, fp64);
     };
     HIP_CHECK(c, hipMemcpyAsync(c->kr, c->kb, nb, hipMemcpyDeviceToDevice, c->stream));
     if (method == EFEM_KRYLOV_CG) {
@@ -674,6 +694,59 @@ int krylovInternal(efem_ctx* c, int transpose, const efem_krylov_params& prm, ef
         }
     }
     HIP_CHECK(c, hipGetLastError());
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return EFEM_OK;
+}
+
+// FP64 element tangent: plain Krylov.  FP32 element tangent (EFEM_F32_ELEMENTS):
+// iterative refinement -- the FP32-tangent Krylov solve is the inner solver,
+// the residual b - K x is recomputed with the FP64 tangent, so the result
+// meets the FP64 tolerance.  The inner tolerance is 1e-2 with at most 200
+// iterations: on smooth vectors the element sums cancel by orders of
+// magnitude, which lifts the FP32 operator's relative noise to ~1e-4, where
+// a tighter inner solve stagnates.
+int krylovInternal(efem_ctx* c, int transpose, const efem_krylov_params& prm, efem_krylov_stats& st)
+{
+    if (!elem32(c)) return krylovCore(c, transpose, prm, st, true);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = c->model.ndof();
+    const size_t nb = static_cast<size_t>(n) * sizeof(double);
+    int rc;
+    double bb = 0.0;
+    if ((rc = dotHost(c, c->kb, c->kb, &bb, nullptr))) return rc;
+    const double bnorm = std::sqrt(bb);
+    HIP_CHECK(c, hipMemcpyAsync(c->kbo, c->kb, nb, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->kxo, 0, nb, c->stream));
+    efem_krylov_params inner = prm;
+    inner.tol = std::max(prm.tol, 1e-2);
+    st.rel_residual = 1.0;
+    for (int outer = 0; outer < 30 && bnorm > 0.0; ++outer) {
+        efem_krylov_stats is;
+        std::memset(&is, 0, sizeof(is));
+        inner.max_iterations = std::min(200, prm.max_iterations - st.iterations);
+        if (inner.max_iterations <= 0) break;
+        if ((rc = krylovCore(c, transpose, inner, is, false))) return rc;
+        st.iterations += is.iterations;
+        st.matvecs += is.matvecs;
+        st.precond_applies += is.precond_applies;
+        st.factor_seconds += is.factor_seconds;
+        efem::dev::launchAxpby(c->kxo, 1.0, c->kxo, 1.0, c->kx, n, c->stream);
+        // b - K x with the FP64 tangent
+        if ((rc = tangentInternal(c, c->ku, c->kxo, c->kt, transpose, true))) return rc;
+        ++st.matvecs;
+        efem::dev::launchAxpby(c->kb, 1.0, c->kbo, -1.0, c->kt, n, c->stream);
+        double rr = 0.0;
+        if ((rc = dotHost(c, c->kb, c->kb, &rr, nullptr))) return rc;
+        st.rel_residual = std::sqrt(rr) / bnorm;
+        if (!std::isfinite(rr)) break;
+        if (st.rel_residual <= prm.tol) {
+            st.converged = 1;
+            break;
+        }
+    }
+    HIP_CHECK(c, hipMemcpyAsync(c->kx, c->kxo, nb, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->kb, c->kbo, nb, hipMemcpyDeviceToDevice, c->stream));
+    if (bnorm == 0.0) st.converged = 1;
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return EFEM_OK;
 }
@@ -951,15 +1024,24 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
         if (compact) {
             // two-loop from Gram products: 2 multi-dot passes + 2 multi-axpys
+            // with the captured preconditioner the first pass writes its
+            // staging input and the second reads its staging output directly
+            const bool staged = h0 && c->useGraph;
+            double* q0 = staged ? c->fac.in : c->d;
             efem::dev::launchTwoLoopFirst(c->r, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
-                                          c->d, c->work, c->counter, c->stream);
-            if (h0) {
+                                          q0, c->work, c->counter, c->stream);
+            const double* h = c->d;
+            if (staged) {
+                if ((rc = applyH0Graph(c))) return rc;
+                h = c->fac.out;
+            } else if (h0) {
                 if ((rc = applyH0Internal(c, c->d, c->tmp))) return rc;
                 std::swap(c->d, c->tmp);
+                h = c->d;
             } else {
                 efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
             }
-            efem::dev::launchTwoLoopSecond(c->d, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
+            efem::dev::launchTwoLoopSecond(h, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
                                            sc + kCoef, c->d, c->work, c->counter, c->stream);
         } else {
             // two-loop recursion: d = -H r (LBFGSMat::apply_Hv)
@@ -1187,13 +1269,19 @@ int efem_newton(efem_ctx* c, double* u, double epsilon, int max_iterations, cons
     };
     double r2 = 0.0, rinf = 0.0, xinf = 0.0;
     if ((rc = resNorms(c->x, c->r, &r2, &rinf, &xinf))) return rc;
+    double r2prev = r2;
     for (int k = 1; k <= max_iterations && rinf > epsilon && std::isfinite(rinf); ++k) {
-        // K_t(x) dx = -r
+        // K_t(x) dx = -r, solved inexactly: forcing term eta_k =
+        // 0.9 (||r_k|| / ||r_k-1||)^2 in [tol, 1e-2] (Eisenstat-Walker choice 2)
+        efem_krylov_params kpk = *kp;
+        const double eta = k == 1 ? 1e-2 : 0.9 * (r2 / r2prev);
+        kpk.tol = std::max(kp->tol, std::min(1e-2, eta));
+        r2prev = r2;
         HIP_CHECK(c, hipMemcpyAsync(c->ku, c->x, nb, hipMemcpyDeviceToDevice, c->stream));
         efem::dev::launchScale(c->kb, c->r, -1.0, n, c->stream);
         efem_krylov_stats ks;
         std::memset(&ks, 0, sizeof(ks));
-        if ((rc = krylovInternal(c, 0, *kp, ks))) return rc;
+        if ((rc = krylovInternal(c, 0, kpk, ks))) return rc;
         s.factor_seconds += ks.factor_seconds;
         s.ls_trials += ks.iterations;  // Krylov iterations, summed
         // backtracking on ||r||_2

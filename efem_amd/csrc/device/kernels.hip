@@ -187,41 +187,83 @@ __global__ __launch_bounds__(kThreads) void kElemScalar(ElemArgs a, const double
     }
 }
 
+// four lanes per free node (they split its slot list, then combine in a
+// fixed butterfly), so four times as many independent loads are in flight
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
 {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i < a.nFree) gatherNode<T>(a, i);
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    const int i = t >> 2, sub = t & 3;
+    const T* es = static_cast<const T*>(a.eslots);
+    double se[3] = {0, 0, 0}, sf[3] = {0, 0, 0};
+    if (i < a.nFree) {
+        const int p1 = a.gptr[i + 1];
+        for (int p = a.gptr[i] + sub; p < p1; p += 4) {
+            const int s = a.gslot[p];
+            if (s < a.nElemSlots) {
+                if (a.ce == 0.0) continue;
+                const T* v = es + 3 * static_cast<size_t>(s);
+                se[0] += static_cast<double>(v[0]);
+                se[1] += static_cast<double>(v[1]);
+                se[2] += static_cast<double>(v[2]);
+            } else {
+                const double* v = a.fslots + 3 * static_cast<size_t>(s - a.nElemSlots);
+                sf[0] += v[0];
+                sf[1] += v[1];
+                sf[2] += v[2];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        se[c] += __shfl_xor(se[c], 1, 64);
+        se[c] += __shfl_xor(se[c], 2, 64);
+        sf[c] += __shfl_xor(sf[c], 1, 64);
+        sf[c] += __shfl_xor(sf[c], 2, 64);
+    }
+    if (i >= a.nFree || sub != 0) return;
+    // residual: ce = 1, cf = -1 (r = f_int - f_pressure - f_const)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double r = a.ce * se[c] + a.cf * sf[c];
+        if (a.fext) r -= a.fext[3 * i + c];
+        a.r[3 * i + c] = r;
+    }
 }
 
 // ---- reductions -------------------------------------------------------------
+// 1024-lane blocks, at most one per CU: on this part the L2 is per XCD, so
+// the device-scope fence each block issues before its arrival count is
+// costly -- few, wide blocks keep the loads in flight without multiplying
+// fences.  Every reduction shape depends on n only (bitwise reproducible).
+constexpr int kRT = 1024;
+constexpr int kRW = kRT / 64;  // waves per reduction block
+
+// K simultaneous block reductions: wave butterflies, then the 16 wave
+// results in wave order.  isMax: components k > 0 are maxima.  All lanes
+// return the block result.
 template <int K>
-__device__ __forceinline__ void blockReduce(double (&v)[K], double* sh, bool isMax)
+__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[kRW], bool isMax)
 {
-    // fixed-shape tree over 256 lanes
-    const int tid = threadIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < K; ++k) sh[k * kThreads + tid] = v[k];
-    __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const double o = sh[k * kThreads + tid + w];
-                double& m = sh[k * kThreads + tid];
-                m = (isMax && k > 0) ? fmax(m, o) : m + o;
-            }
-        }
-        __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        const double t = (isMax && k > 0) ? waveMax(v[k]) : waveSum(v[k]);
+        if (lane == 0) sh[k][wave] = t;
     }
+    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = sh[k * kThreads];
+    for (int k = 0; k < K; ++k) {
+        double t = sh[k][0];
+        for (int w = 1; w < kRW; ++w) t = (isMax && k > 0) ? fmax(t, sh[k][w]) : t + sh[k][w];
+        v[k] = t;
+    }
     __syncthreads();
 }
 
 template <int K>
 __device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsigned* counter,
-                                            double* sh, bool isMax)
+                                            double (*sh)[kRW], bool isMax)
 {
     __shared__ int amLast;
     blockReduce<K>(v, sh, isMax);
@@ -235,10 +277,10 @@ __device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsign
     __syncthreads();
     if (!amLast) return false;
     __threadfence();
-    // final pass over the block partials in block order
+    // final pass over the block partials (one per lane), same tree
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kThreads)
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kRT)
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double o = work[k * kReduceBlocks + b];
@@ -249,15 +291,15 @@ __device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsign
     return true;
 }
 
-int reduceGrid(int n) { return max(1, min(kReduceBlocks, (n + 4 * kThreads - 1) / (4 * kThreads))); }
+int reduceGrid(int n) { return max(1, min(kReduceBlocks, (n + kRT - 1) / kRT)); }
 
-__global__ __launch_bounds__(kThreads) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
-                                                    const double* __restrict__ c, int n, double* work,
-                                                    unsigned* counter, double* out)
+__global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
+                                               const double* __restrict__ c, int n, double* work,
+                                               unsigned* counter, double* out)
 {
-    __shared__ double sh[3 * kThreads];
+    __shared__ double sh[3][kRW];
     double v[3] = {0, 0, 0};
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         const double ai = a[i];
         v[0] += ai * b[i];
         v[1] = fmax(v[1], fabs(ai));
@@ -270,12 +312,12 @@ __global__ __launch_bounds__(kThreads) void kDotMax(const double* __restrict__ a
     }
 }
 
-__global__ __launch_bounds__(kThreads) void kDot2(const double* __restrict__ a, const double* __restrict__ b, int n,
-                                                  double* work, unsigned* counter, double* ab, double* bb)
+__global__ __launch_bounds__(kRT) void kDot2(const double* __restrict__ a, const double* __restrict__ b, int n,
+                                             double* work, unsigned* counter, double* ab, double* bb)
 {
-    __shared__ double sh[2 * kThreads];
+    __shared__ double sh[2][kRW];
     double v[2] = {0, 0};
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         const double bi = b[i];
         v[0] += a[i] * bi;
         v[1] += bi * bi;
@@ -286,16 +328,16 @@ __global__ __launch_bounds__(kThreads) void kDot2(const double* __restrict__ a, 
     }
 }
 
-__global__ __launch_bounds__(kThreads) void kTwoLoop(double* __restrict__ q, const double* __restrict__ vprev,
-                                                     const double* cprev, double sgn,
-                                                     const double* __restrict__ w, const double* den,
-                                                     const double* aj, double* cout, int mode, int n,
-                                                     double* work, unsigned* counter)
+__global__ __launch_bounds__(kRT) void kTwoLoop(double* __restrict__ q, const double* __restrict__ vprev,
+                                                const double* cprev, double sgn,
+                                                const double* __restrict__ w, const double* den,
+                                                const double* aj, double* cout, int mode, int n,
+                                                double* work, unsigned* counter)
 {
-    __shared__ double sh[kThreads];
+    __shared__ double sh[1][kRW];
     double v[1] = {0};
     const double c = vprev ? sgn * (*cprev) : 0.0;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         double qi = q[i];
         if (vprev) {
             qi += c * vprev[i];
@@ -324,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void kTwoLoop(double* __restrict__ q, con
 template <int N>
 __device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsigned* counter, double* red)
 {
-    __shared__ double sh[N][4];
+    __shared__ double sh[N][kRW];
     __shared__ int amLast;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
@@ -335,11 +377,15 @@ __device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsign
     __syncthreads();
     if (threadIdx.x < N) {
         const int k = threadIdx.x;
-        work[k * kReduceBlocks + blockIdx.x] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+        double t = sh[k][0];
+        for (int w = 1; w < kRW; ++w) t += sh[k][w];
+        work[k * kReduceBlocks + blockIdx.x] = t;
     }
-    __threadfence();
+    // the barrier orders the partial stores before lane 0's device-scope
+    // release fence (one fence per block: each is an L2 write-back here)
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();
         const unsigned t = atomicAdd(counter, 1u);
         amLast = (t == gridDim.x - 1) ? 1 : 0;
     }
@@ -349,7 +395,7 @@ __device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsign
     double p[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) p[k] = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kThreads)
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kRT)
 #pragma unroll
         for (int k = 0; k < N; ++k) p[k] += work[k * kReduceBlocks + b];
 #pragma unroll
@@ -360,7 +406,9 @@ __device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsign
     __syncthreads();
     if (threadIdx.x < N) {
         const int k = threadIdx.x;
-        red[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+        double t = sh[k][0];
+        for (int w = 1; w < kRW; ++w) t += sh[k][w];
+        red[k] = t;
     }
     __syncthreads();
     if (threadIdx.x == 0) *counter = 0u;
@@ -371,7 +419,7 @@ __device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsign
 constexpr int KG = kMaxCompact;
 
 template <int KC>
-__global__ __launch_bounds__(kThreads) void kGramUpdate(const double* __restrict__ S, const double* __restrict__ Y,
+__global__ __launch_bounds__(kRT) void kGramUpdate(const double* __restrict__ S, const double* __restrict__ Y,
                                                         int n, int nc, int loc, double* G, double* ys, double* yy,
                                                         double* work, unsigned* counter)
 {
@@ -382,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void kGramUpdate(const double* __restrict
     for (int k = 0; k < 2 * KC + 1; ++k) v[k] = 0.0;
     const double* sl = S + static_cast<size_t>(loc) * n;
     const double* yl = Y + static_cast<size_t>(loc) * n;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         const double si = sl[i], yi = yl[i];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
@@ -404,7 +452,7 @@ __global__ __launch_bounds__(kThreads) void kGramUpdate(const double* __restrict
 }
 
 template <int KC>
-__global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
+__global__ __launch_bounds__(kRT) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
                                                          int n, int nc, int first, int newest, int mcap,
                                                          const double* G, const double* ys, const double* alphaIn,
                                                          double* outCoef, double* work, unsigned* counter)
@@ -416,7 +464,7 @@ __global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restric
     double v[KC];
 #pragma unroll
     for (int k = 0; k < KC; ++k) v[k] = 0.0;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         const double qi = q[i];
 #pragma unroll
         for (int k = 0; k < KC; ++k)
@@ -424,7 +472,7 @@ __global__ __launch_bounds__(kThreads) void kTwoLoopDots(const double* __restric
     }
     if (!multiReduce<KC>(v, work, counter, red)) return;
     // the recurrence reads the Gram block, ys and alpha from LDS
-    for (int t = threadIdx.x; t < KC * KC; t += kThreads) {
+    for (int t = threadIdx.x; t < KC * KC; t += kRT) {
         const int i = t / KC, k = t - i * KC;
         sG[i][k] = (i < mcap && k < mcap) ? G[i * KG + k] : 0.0;
     }
@@ -508,18 +556,18 @@ __device__ __forceinline__ void combinePair(double& s, double& c, double s2, dou
     c = c + c2 + err;
 }
 
-__global__ __launch_bounds__(kThreads) void kSumComp(const double* __restrict__ a, int n, double* work,
-                                                     unsigned* counter, double* out)
+__global__ __launch_bounds__(kRT) void kSumComp(const double* __restrict__ a, int n, double* work,
+                                                unsigned* counter, double* out)
 {
-    __shared__ double ss[kThreads], sc[kThreads];
+    __shared__ double ss[kRT], sc[kRT];
     __shared__ int amLast;
     const int tid = threadIdx.x;
     double s = 0.0, c = 0.0;
-    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads) neumaier(s, c, a[i]);
+    for (int i = blockIdx.x * kRT + tid; i < n; i += gridDim.x * kRT) neumaier(s, c, a[i]);
     ss[tid] = s;
     sc[tid] = c;
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
+    for (int w = kRT / 2; w > 0; w >>= 1) {
         if (tid < w) {
             double s1 = ss[tid], c1 = sc[tid];
             combinePair(s1, c1, ss[tid + w], sc[tid + w]);
@@ -697,7 +745,7 @@ void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s)
 
 void launchSumComp(const double* a, int n, double* work, unsigned* counter, double* out, hipStream_t s)
 {
-    kSumComp<<<reduceGrid(n), kThreads, 0, s>>>(a, n, work, counter, out);
+    kSumComp<<<reduceGrid(n), kRT, 0, s>>>(a, n, work, counter, out);
 }
 
 void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s)
@@ -708,22 +756,23 @@ void launchAxpby(double* out, double a, const double* x, double b, const double*
 void launchGather(const GatherArgs& a, hipStream_t s)
 {
     if (a.nFree <= 0) return;
+    const int g = static_cast<int>((4LL * a.nFree + kThreads - 1) / kThreads);
     if (a.mixed)
-        kGather<float><<<blocks(a.nFree), kThreads, 0, s>>>(a);
+        kGather<float><<<g, kThreads, 0, s>>>(a);
     else
-        kGather<double><<<blocks(a.nFree), kThreads, 0, s>>>(a);
+        kGather<double><<<g, kThreads, 0, s>>>(a);
 }
 
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, unsigned* counter,
                   double* out, hipStream_t s)
 {
-    kDotMax<<<reduceGrid(n), kThreads, 0, s>>>(a, b, c, n, work, counter, out);
+    kDotMax<<<reduceGrid(n), kRT, 0, s>>>(a, b, c, n, work, counter, out);
 }
 
 void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter, double* ab, double* bb,
                 hipStream_t s)
 {
-    kDot2<<<reduceGrid(n), kThreads, 0, s>>>(a, b, n, work, counter, ab, bb);
+    kDot2<<<reduceGrid(n), kRT, 0, s>>>(a, b, n, work, counter, ab, bb);
 }
 
 void launchStep(double* x, const double* xp, const double* d, double alpha, int n, hipStream_t s)
@@ -746,7 +795,7 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
                        const double* aj, double* cout, int mode, int n, double* work, unsigned* counter,
                        hipStream_t s)
 {
-    kTwoLoop<<<reduceGrid(n), kThreads, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, den, aj, cout, mode, n,
+    kTwoLoop<<<reduceGrid(n), kRT, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, den, aj, cout, mode, n,
                                                   work, counter);
 }
 
@@ -754,9 +803,9 @@ void launchGramUpdate(const double* S, const double* Y, int n, int nc, int loc, 
                       double* work, unsigned* counter, hipStream_t s)
 {
     if (nc <= 8)
-        kGramUpdate<8><<<reduceGrid(n), kThreads, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
+        kGramUpdate<8><<<reduceGrid(n), kRT, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
     else
-        kGramUpdate<16><<<reduceGrid(n), kThreads, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
+        kGramUpdate<16><<<reduceGrid(n), kRT, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
 }
 
 template <int KC>
@@ -764,7 +813,7 @@ void twoLoopPass(const double* q, const double* R, const double* base, double sg
                  double sgnC, int n, int nc, int first, int newest, int mcap, const double* G, const double* ys,
                  const double* alphaIn, double* coef, double* out, double* work, unsigned* counter, hipStream_t s)
 {
-    kTwoLoopDots<KC><<<reduceGrid(n), kThreads, 0, s>>>(q, R, n, nc, first, newest, mcap, G, ys, alphaIn, coef,
+    kTwoLoopDots<KC><<<reduceGrid(n), kRT, 0, s>>>(q, R, n, nc, first, newest, mcap, G, ys, alphaIn, coef,
                                                          work, counter);
     if (n > 0) kMultiAxpy<KC><<<blocks(n), kThreads, 0, s>>>(base, sgnB, axR, coef, sgnC, n, nc, out);
 }

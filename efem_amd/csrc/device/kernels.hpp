@@ -59,7 +59,7 @@ void launchGather(const GatherArgs& a, hipStream_t s);
 // ---- vectors ----------------------------------------------------------------
 // deterministic reductions: results written to device scalars; the block
 // partials and an arrival counter live in `work` (see kReduceWorkDoubles)
-constexpr int kReduceBlocks = 512;
+constexpr int kReduceBlocks = 256;
 constexpr int kMaxCompact = 16;  // history size handled by the compact two-loop
 constexpr int kReduceWorkDoubles = (2 * kMaxCompact + 2) * kReduceBlocks + 64;
 

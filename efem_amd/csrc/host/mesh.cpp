@@ -353,9 +353,11 @@ Mesh makeWorkload(int which, const double* opts, int n_opts)
         // one instance of the batch of cantilevers: 4x4 C3D8 cross-section,
         // h = 5 mm, length L ~ U{10..40} elements, NH C10 ~ U[80, 150] kPa,
         // D1 = 1e-6, x = 0 fixed, pressure ~ U[0.5, 5] kPa on z = max.
-        // opts = {instance index, seed}; the draws depend on both only.
+        // opts = {instance index, seed, load scale}; the draws depend on the
+        // index and the seed only.
         const int idx = (opts && n_opts > 0) ? static_cast<int>(opts[0]) : 0;
         const uint64_t seed = static_cast<uint64_t>(opt(1, 20240531));
+        const double scale = opt(2, 1.0);
         SplitMix64 rng(seed + 0x9E3779B97F4A7C15ULL * static_cast<uint64_t>(idx + 1));
         const int len = 10 + std::min(30, static_cast<int>(rng.uniform() * 31.0));
         const double c10 = 80000.0 + 70000.0 * rng.uniform();
@@ -372,7 +374,7 @@ Mesh makeWorkload(int which, const double* opts, int n_opts)
         m.law = EFEM_NEOHOOKEAN;
         m.params = {c10, 0.0, 0.0, 1e-6};
         fixPlane(m, 0, 0.0);
-        pressurePlane(m, 2, s.lz, pr);
+        pressurePlane(m, 2, s.lz, pr * scale);
         return m;
     }
     // custom box: {nx, ny, nz, lx, ly, lz, kind, law, pressure, jitter, seed, reduced}

@@ -214,7 +214,9 @@ int efem_tangent_solve(efem_ctx* ctx, const double* u, const double* b,
                        double* x, int transpose, const efem_krylov_params* prm,
                        efem_krylov_stats* stats);
 /* Newton-Krylov equilibrium solve from u (in/out): K_t(u) du = -r(u) by
- * efem_tangent_solve, backtracking on ||r||_2, until ||r||_inf <= epsilon.
+ * efem_tangent_solve to the relative tolerance max(kp->tol, min(1e-2,
+ * 0.9 (||r_k|| / ||r_k-1||)^2)), backtracking on ||r||_2, until
+ * ||r||_inf <= epsilon.
  * stats: iterations = Newton steps, ls_trials = Krylov iterations summed,
  * r_evals, rinf, xinf, factor/solve seconds. */
 int efem_newton(efem_ctx* ctx, double* u, double epsilon, int max_iterations,
@@ -265,7 +267,8 @@ int efem_batch_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
  * which: 1 cantilever 20x4x4 C3D8 NH; 2 beam 200x20x20 Kuhn C3D4 MR;
  *        3 hollow cylinder 48x4x40 C3D20R Yeoh; 4 bar 64x16x12 Kuhn C3D10
  *        NH per-element; 5 one cantilever of the batch (opts = {index,
- *        seed}: Lx4x4 C3D8, L ~ U{10..40}, C10 ~ U[80k,150k], p ~ U[0.5,5] kPa);
+ *        seed, load scale}: Lx4x4 C3D8, L ~ U{10..40}, C10 ~ U[80k,150k],
+ *        p ~ U[0.5,5] kPa times the scale);
  *        0 custom box (dims from opts).
  * opts (may be NULL) = {nx, ny, nz, seed, ...} overrides, see efem_mesh_build. */
 typedef struct efem_mesh efem_mesh;

@@ -197,8 +197,9 @@ def test_mixed_adjoint_matches_fp64(rng):
         assert st["converged"], st
     err_p = rel(gp32[:, 0], gp64[:, 0])
     err_l = abs(gl32[0] - gl64[0]) / abs(gl64[0])
-    print(f"mixed vs FP64 adjoint: params {err_p:.2e}, load {err_l:.2e}", flush=True)
-    assert err_p <= 1e-3 and err_l <= 1e-3
+    print(f"mixed vs FP64 adjoint: params {err_p:.2e}, load {err_l:.2e}, {st}", flush=True)
+    # FP32-tangent Krylov inside FP64 iterative refinement: FP64 accuracy
+    assert err_p <= 1e-7 and err_l <= 1e-7
 
 
 def test_adjoint_shared_parameters_and_directed_load(rng):
