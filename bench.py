@@ -55,6 +55,19 @@ def element_work(m, prec):
     return B, F
 
 
+def measured_traffic():
+    """HBM bytes per preconditioner apply from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE of one apply, copy-kernel calibrated)."""
+    f = ROOT / "profiles" / "h0_apply_traffic_beam_c3d4_mixed.md"
+    try:
+        for line in f.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_apply:"):
+                return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(mesh, n_evals):
     """The oracle (single-thread C restatement of the reference path) on the
     same mesh: residual evaluations per second on `n_evals` evaluations."""
@@ -115,6 +128,35 @@ def run_batch(E, args):
         "converged_batch": int(sum(s["rinf"] <= max(1e-6, 1e-6 * s["xinf"]) for s in stb)),
         "converged_sequential": int(sum(s["rinf"] <= max(1e-6, 1e-6 * s["xinf"]) for _, s in seq)),
     }
+
+
+def run_configs(E, args):
+    """Residual throughput of every configuration's mesh (FP64 and mixed):
+    device time per evaluation (element + pressure + gather) and the element
+    work's roofline fraction.  Throughput does not depend on the load."""
+    out = {}
+    specs = {"C1 cantilever 20x4x4 C3D8 NH": (1, None, "NH"), "C2 beam 480k C3D4 MR": (2, None, "MR"),
+             "C3 cylinder 7680 C3D20R Yeoh": (3, None, "Yeoh"), "C4 bar 73728 C3D10 NH": (4, None, "NH")}
+    claw = {"NH": 250, "MR": 370, "Yeoh": 260}
+    for name, (which, opts, law) in specs.items():
+        m = E.Mesh.workload(which, opts)
+        rng = np.random.default_rng(1)
+        u = 1e-6 * rng.standard_normal(m.ndof)
+        ng = {4: 1, 8: 8, 10: 4, 20: 8}[m.nen]
+        F = m.n_elems * ng * (72 * m.nen + claw[law])
+        rec = {"elements": m.n_elems, "dof": m.ndof}
+        for prec, label in ((E.FP64, "fp64"), (E.MIXED, "mixed")):
+            with E.Context(m, prec) as c:
+                ms = c.time_residual(u, 50)
+            s_ = 8 if prec == E.FP64 else 4
+            B = 3 * m.n_free * s_ + 3 * m.n_nodes * s_ + 4 * m.nen * m.n_elems + s_ * m.n_elems + 24 * m.n_free
+            peak = FP64_PEAK_TFLOPS if prec == E.FP64 else FP32_PEAK_TFLOPS
+            t_roof = max(B / (HBM_PEAK_GBS * 1e9), F / (peak * 1e12))
+            rec[label] = {"us_per_eval": round(1e3 * ms, 2), "evals_per_s": round(1e3 / ms, 1),
+                          "roofline_frac": round(t_roof / (ms * 1e-3), 4)}
+        out[name] = rec
+        log(f"{name}: fp64 {rec['fp64']['us_per_eval']} us, mixed {rec['mixed']['us_per_eval']} us per residual")
+    return out
 
 
 def run_adjoint(E, args):
@@ -277,6 +319,8 @@ def main():
             log(f"WARNING {name}: target not reached (err {st['target_err']:.2e})")
         return st
 
+    tuned = {name: c.autotune() for name, c in ctx.items()}
+    log(f"autotune: {tuned}")
     u_star, ustar_info = reference_solution(args.pressure)
     for _ in range(args.warmup):
         for name in ctx:
@@ -345,6 +389,8 @@ def main():
         "fp64_switch": args.fp64_switch,
         "fp64_switch_iteration": int(mix[-1]["fp64_switch_iteration"]),
         "u_star": ustar_info,
+        "launch_shapes": {k: {kk: v[kk] for kk in ("gather_lanes", "tet4_threads", "reduce_blocks")}
+                          for k, v in tuned.items()},
         "load_samples": [round(args.pressure * (1.0 + 1e-6 * (k + 1)), 6) for k in range(args.steps)],
         "iterations_mixed_per_sample": [s["iterations"] for s in mix],
     }
@@ -372,7 +418,9 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(h0_bytes / (h0_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(),
+        "traffic_source": "profiles/h0_apply_traffic_beam_c3d4_mixed.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                          "calibrated on a copy kernel)",
         "bytes_per_launch": h0_bytes,
         "avg_ms": round(h0_ms, 4),
         "factor_nnz": nnz,
@@ -401,7 +449,8 @@ def main():
             "frac": round(max(B64 / (HBM_PEAK_GBS * 1e9), F64 / (FP64_PEAK_TFLOPS * 1e12)) / (e64 * 1e-6), 4),
         }
         out["h0_apply_ms_fp64"] = round(hms64 / max(hk64, 1), 4)
-    for key, fn, skip in (("batch_cantilevers", run_batch, args.no_batch),
+    for key, fn, skip in (("residual_throughput", run_configs, False),
+                          ("batch_cantilevers", run_batch, args.no_batch),
                           ("adjoint_bar", run_adjoint, args.no_adjoint)):
         if skip:
             continue

@@ -114,6 +114,8 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_newton": (C.c_int, [vp, _dp, C.c_double, C.c_int, C.POINTER(KrylovParams),
                                   C.POINTER(SolveStats)]),
         "efem_energy": (C.c_int, [vp, _dp, _dp]),
+        "efem_time_residual": (C.c_int, [vp, _dp, C.c_int, _dp]),
+        "efem_autotune": (C.c_int, [vp, _dp]),
         "efem_set_params": (C.c_int, [vp, _dp]),
         "efem_set_load": (C.c_int, [vp, C.c_double, _dp]),
         "efem_material_point": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -350,6 +352,21 @@ class Context:
         k = C.c_int()
         self._check(self._lib.efem_profile_read_h0(self._h, C.byref(ms), C.byref(k)), "efem_profile_read_h0")
         return ms.value, k.value
+
+    def time_residual(self, u, count=50) -> float:
+        """Mean device milliseconds per residual evaluation (count back to back)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        ms = C.c_double()
+        self._check(self._lib.efem_time_residual(self._h, _ptr(u, C.c_double), int(count), C.byref(ms)),
+                    "efem_time_residual")
+        return ms.value
+
+    def autotune(self) -> dict:
+        """Pick the fastest launch shapes for this problem (efem_autotune)."""
+        rep = np.zeros(5)
+        self._check(self._lib.efem_autotune(self._h, _ptr(rep, C.c_double)), "efem_autotune")
+        return {"gather_lanes": int(rep[0]), "tet4_threads": int(rep[1]), "reduce_blocks": int(rep[2]),
+                "residual_ms": rep[3], "dot_ms": rep[4]}
 
     def set_option(self, name: str, value: float):
         self._check(self._lib.efem_set_option(self._h, name.encode(), float(value)), "efem_set_option")

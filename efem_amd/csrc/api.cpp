@@ -92,6 +92,7 @@ struct efem_ctx {
     double *ku = nullptr, *kb = nullptr, *kx = nullptr, *kr = nullptr, *krh = nullptr, *kp = nullptr,
            *kv = nullptr, *ks = nullptr, *kt = nullptr, *kph = nullptr, *ksh = nullptr, *kxo = nullptr,
            *kbo = nullptr;
+    efem::dev::LaunchTuning tuning;  // efem_autotune / options
     bool useGraph = true;    // option "h0_graph"
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
 };
@@ -838,6 +839,7 @@ int efem_precision(const efem_ctx* c) { return c ? c->precision : -1; }
 int efem_residual(efem_ctx* c, const double* u, double* r)
 {
     if (!c || !u || !r) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     const size_t n = static_cast<size_t>(c->model.ndof());
     HIP_CHECK(c, hipMemcpyAsync(c->tmp, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int rc = residualInternal(c, c->tmp, c->r);
@@ -850,6 +852,7 @@ int efem_residual(efem_ctx* c, const double* u, double* r)
 int efem_residual_d(efem_ctx* c, const double* u_d, double* r_d)
 {
     if (!c || !u_d || !r_d) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     const size_t n = static_cast<size_t>(c->model.ndof());
     HIP_CHECK(c, hipMemcpyAsync(c->tmp, u_d, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     return residualInternal(c, c->tmp, r_d);
@@ -875,6 +878,7 @@ int efem_factor_release(efem_ctx* c)
 int efem_apply_h0(efem_ctx* c, const double* v, double* out)
 {
     if (!c || !v || !out) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     if (!c->fac.ready) {
         c->err = "efem_apply_h0: no factor (call efem_factor first)";
         return EFEM_ERR_STATE;
@@ -893,6 +897,7 @@ int efem_apply_h0(efem_ctx* c, const double* v, double* out)
 int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_stats* st)
 {
     if (!c || !u || !prm) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     if (!c->parts.empty()) {
         c->err = "efem_solve: batch context, use efem_batch_solve";
         return EFEM_ERR_STATE;
@@ -1147,7 +1152,20 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         c->compactTwoLoop = value != 0.0;
         return EFEM_OK;
     }
-    c->err = "efem_set_option: unknown option " + k;
+    const int iv = static_cast<int>(value);
+    if (k == "gather_lanes" && (iv == 1 || iv == 2 || iv == 4 || iv == 8)) {
+        c->tuning.gatherLanes = iv;
+        return EFEM_OK;
+    }
+    if (k == "reduce_blocks" && iv >= 1 && iv <= efem::dev::kReduceBlocks) {
+        c->tuning.reduceBlocks = iv;
+        return EFEM_OK;
+    }
+    if (k == "tet4_threads" && (iv == 64 || iv == 128 || iv == 256)) {
+        c->tuning.tet4Threads = iv;
+        return EFEM_OK;
+    }
+    c->err = "efem_set_option: unknown option or value " + k;
     return EFEM_ERR_ARG;
 }
 
@@ -1209,6 +1227,7 @@ int efem_profile_read(efem_ctx* c, double* ms, int* launches)
 int efem_tangent(efem_ctx* c, const double* u, const double* v, double* out, int transpose)
 {
     if (!c || !u || !v || !out) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     int rc = ensureKrylov(c);
     if (rc) return rc;
     const size_t nb = static_cast<size_t>(c->model.ndof()) * sizeof(double);
@@ -1224,6 +1243,7 @@ int efem_tangent_solve(efem_ctx* c, const double* u, const double* b, double* x,
                        const efem_krylov_params* prm, efem_krylov_stats* st)
 {
     if (!c || !u || !b || !x) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     int rc = checkKrylovParams(c, prm);
     if (rc) return rc;
     efem_krylov_stats local;
@@ -1243,6 +1263,7 @@ int efem_newton(efem_ctx* c, double* u, double epsilon, int max_iterations, cons
                 efem_solve_stats* st)
 {
     if (!c || !u || max_iterations <= 0 || !(epsilon >= 0.0)) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     int rc = checkKrylovParams(c, kp);
     if (rc) return rc;
     efem_solve_stats local;
@@ -1325,6 +1346,7 @@ int efem_adjoint(efem_ctx* c, const double* u, const double* g, double* lambda, 
                  double* grad_load, const efem_krylov_params* prm, efem_krylov_stats* st)
 {
     if (!c || !u || !g) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     int rc = checkKrylovParams(c, prm);
     if (rc) return rc;
     efem_krylov_stats local;
@@ -1382,6 +1404,7 @@ int efem_adjoint(efem_ctx* c, const double* u, const double* g, double* lambda, 
 int efem_energy(efem_ctx* c, const double* u, double* W)
 {
     if (!c || !u || !W) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     int rc = ensureKrylov(c);
     if (rc) return rc;
     const efem::Model& m = c->model;
@@ -1502,6 +1525,7 @@ int efem_batch_offsets(const efem_ctx* c, int* dof_offsets)
 int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_stats* stats)
 {
     if (!c || !u || !prm) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
     if (c->parts.empty()) {
         c->err = "efem_batch_solve: not a batch context (efem_batch_create)";
         return EFEM_ERR_STATE;
@@ -1599,6 +1623,103 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
     if (anyNonfinite) {
         c->err = "batched L-BFGS: non-finite residual or displacement in at least one instance";
         return EFEM_ERR_NONFINITE;
+    }
+    return EFEM_OK;
+}
+
+int efem_time_residual(efem_ctx* c, const double* u, int count, double* ms_per_eval)
+{
+    if (!c || !u || count <= 0 || !ms_per_eval) return EFEM_ERR_ARG;
+    efem::dev::setTuning(c->tuning);
+    const size_t nb = static_cast<size_t>(c->model.ndof()) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(c->tmp, u, nb, hipMemcpyHostToDevice, c->stream));
+    int rc = residualInternal(c, c->tmp, c->r);  // warm-up
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_CHECK(c, hipEventCreate(&e0));
+    HIP_CHECK(c, hipEventCreate(&e1));
+    HIP_CHECK(c, hipEventRecord(e0, c->stream));
+    for (int i = 0; i < count && rc == EFEM_OK; ++i) rc = residualInternal(c, c->tmp, c->r);
+    HIP_CHECK(c, hipEventRecord(e1, c->stream));
+    HIP_CHECK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_eval = ms / count;
+    return rc;
+}
+
+int efem_autotune(efem_ctx* c, double* report)
+{
+    if (!c) return EFEM_ERR_ARG;
+    const int n = c->model.ndof();
+    hipEvent_t e0, e1;
+    HIP_CHECK(c, hipEventCreate(&e0));
+    HIP_CHECK(c, hipEventCreate(&e1));
+    int rc = EFEM_OK;
+    // time `reps` repetitions of f() on the context stream (after one warm-up)
+    auto timeIt = [&](auto&& f, int reps, double* ms) -> int {
+        int r = f();
+        if (r) return r;
+        HIP_CHECK(c, hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps && r == EFEM_OK; ++i) r = f();
+        HIP_CHECK(c, hipEventRecord(e1, c->stream));
+        HIP_CHECK(c, hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_CHECK(c, hipEventElapsedTime(&t, e0, e1));
+        *ms = t / reps;
+        return r;
+    };
+    efem::dev::LaunchTuning best = c->tuning;
+    // residual: gather lanes per node x C3D4 block size (at u = 0; the cost
+    // does not depend on the values)
+    HIP_CHECK(c, hipMemsetAsync(c->tmp, 0, static_cast<size_t>(n) * sizeof(double), c->stream));
+    double bestRes = 1e300;
+    const int lanes[4] = {1, 2, 4, 8};
+    const int tpbs[3] = {64, 128, 256};
+    for (int li = 0; li < 4 && rc == EFEM_OK; ++li)
+        for (int ti = 0; ti < (c->model.nen == 4 ? 3 : 1) && rc == EFEM_OK; ++ti) {
+            efem::dev::LaunchTuning t = best;
+            t.gatherLanes = lanes[li];
+            if (c->model.nen == 4) t.tet4Threads = tpbs[ti];
+            efem::dev::setTuning(t);
+            double ms = 0.0;
+            rc = timeIt([&]() { return residualInternal(c, c->tmp, c->r); }, 10, &ms);
+            if (rc == EFEM_OK && ms < bestRes) {
+                bestRes = ms;
+                best.gatherLanes = t.gatherLanes;
+                best.tet4Threads = t.tet4Threads;
+            }
+        }
+    // reductions: the two-vector dot + norm of the line search
+    double bestRed = 1e300;
+    const int grids[4] = {32, 64, 128, 256};
+    for (int gi = 0; gi < 4 && rc == EFEM_OK; ++gi) {
+        efem::dev::LaunchTuning t = best;
+        t.reduceBlocks = grids[gi];
+        efem::dev::setTuning(t);
+        double ms = 0.0;
+        rc = timeIt([&]() {
+            efem::dev::launchDotMax(c->r, c->tmp, c->x, n, c->work, c->counter, c->scal + kOut, c->stream);
+            return EFEM_OK;
+        }, 20, &ms);
+        if (rc == EFEM_OK && ms < bestRed) {
+            bestRed = ms;
+            best.reduceBlocks = grids[gi];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c->tuning = best;
+    efem::dev::setTuning(best);
+    if (report) {
+        report[0] = best.gatherLanes;
+        report[1] = best.tet4Threads;
+        report[2] = best.reduceBlocks;
+        report[3] = bestRes;
+        report[4] = bestRed;
     }
     return EFEM_OK;
 }

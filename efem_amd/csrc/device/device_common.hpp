@@ -16,6 +16,18 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Workgroups are dispatched to the 8 XCDs round-robin by workgroup id, and
+// each XCD has its own L2.  For gather-heavy kernels whose neighbouring
+// work items share data (nodes sharing elements), give each XCD one
+// contiguous range of logical blocks instead of every 8th block.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcdBlock()
+{
+    const int b = blockIdx.x, g = gridDim.x;
+    const int x = b % kXcds, k = b / kXcds, per = g / kXcds, rem = g % kXcds;
+    return x * per + min(x, rem) + k;
+}
+
 // wave64 butterfly sum (fixed order)
 __device__ __forceinline__ double waveSum(double v)
 {

@@ -24,16 +24,16 @@ namespace efem {
 namespace dev {
 namespace {
 
-template <typename T, int LAW, int MODE>
-__global__ __launch_bounds__(kThreads) void kElemTet4(int E, const int* __restrict__ connDof,
-                                                      const double* __restrict__ ginv,
-                                                      const double* __restrict__ vol, double w0,
-                                                      const double* __restrict__ params, int ppe,
-                                                      const double* __restrict__ x,
-                                                      const double* __restrict__ v,
-                                                      T* __restrict__ slots)
+template <typename T, int LAW, int MODE, int TPB>
+__global__ __launch_bounds__(TPB) void kElemTet4(int E, const int* __restrict__ connDof,
+                                                 const double* __restrict__ ginv,
+                                                 const double* __restrict__ vol, double w0,
+                                                 const double* __restrict__ params, int ppe,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ v,
+                                                 T* __restrict__ slots)
 {
-    const int e = blockIdx.x * kThreads + threadIdx.x;
+    const int e = xcdBlock() * TPB + threadIdx.x;
     if (e >= E) return;
     tet4Lane<T, LAW, MODE>(e, E, connDof, ginv, vol, w0, params, ppe, x, v, slots);
 }
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kThreads) void kElemMulti(int E, const int* __restr
     // both modes; the stress moments Q and the nodal forces use T
     __shared__ ElemTileLds<T, NEN, NG, MODE> L;
     for (int k = threadIdx.x; k < NG * NEN * 3; k += kThreads) L.sdN[k] = gdN[k];
-    elemTile<T, NEN, NG, LAW, MODE>(E, blockIdx.x * Tile<NG>::epb, E, connDof, ginv, vol, gw, params, ppe, x, v,
+    elemTile<T, NEN, NG, LAW, MODE>(E, xcdBlock() * Tile<NG>::epb, E, connDof, ginv, vol, gw, params, ppe, x, v,
                                     slots, L);
 }
 
@@ -83,24 +83,34 @@ void launchMultiLaw(const ElemArgs& a, const double* v, hipStream_t s)
     }
 }
 
-template <typename T, int MODE>
-void launchTet4Law(const ElemArgs& a, const double* v, hipStream_t s)
+template <typename T, int MODE, int TPB>
+void launchTet4Tpb(const ElemArgs& a, const double* v, hipStream_t s)
 {
-    const int grid = (a.nElems + kThreads - 1) / kThreads;
+    const int grid = (a.nElems + TPB - 1) / TPB;
     const double* ginv = static_cast<const double*>(a.ginv);
     T* slots = static_cast<T*>(a.slots);
     switch (a.law) {
         case LAW_NH:
-            kElemTet4<T, LAW_NH, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                                  a.paramsPerElem, a.x, v, slots);
+            kElemTet4<T, LAW_NH, MODE, TPB><<<grid, TPB, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
+                                                                 a.paramsPerElem, a.x, v, slots);
             break;
         case LAW_MR:
-            kElemTet4<T, LAW_MR, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                                  a.paramsPerElem, a.x, v, slots);
+            kElemTet4<T, LAW_MR, MODE, TPB><<<grid, TPB, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
+                                                                 a.paramsPerElem, a.x, v, slots);
             break;
         default:
-            kElemTet4<T, LAW_YEOH, MODE><<<grid, kThreads, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                                    a.paramsPerElem, a.x, v, slots);
+            kElemTet4<T, LAW_YEOH, MODE, TPB><<<grid, TPB, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0,
+                                                                   a.params, a.paramsPerElem, a.x, v, slots);
+    }
+}
+
+template <typename T, int MODE>
+void launchTet4Law(const ElemArgs& a, const double* v, hipStream_t s)
+{
+    switch (tuning().tet4Threads) {
+        case 64: launchTet4Tpb<T, MODE, 64>(a, v, s); break;
+        case 128: launchTet4Tpb<T, MODE, 128>(a, v, s); break;
+        default: launchTet4Tpb<T, MODE, 256>(a, v, s);
     }
 }
 
@@ -187,18 +197,18 @@ __global__ __launch_bounds__(kThreads) void kElemScalar(ElemArgs a, const double
     }
 }
 
-// four lanes per free node (they split its slot list, then combine in a
-// fixed butterfly), so four times as many independent loads are in flight
-template <typename T>
+// LPN lanes per free node (they split its slot list, then combine in a fixed
+// butterfly), so LPN times as many independent loads are in flight
+template <typename T, int LPN>
 __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
 {
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    const int i = t >> 2, sub = t & 3;
+    const int t = xcdBlock() * kThreads + threadIdx.x;
+    const int i = t / LPN, sub = t % LPN;
     const T* es = static_cast<const T*>(a.eslots);
     double se[3] = {0, 0, 0}, sf[3] = {0, 0, 0};
     if (i < a.nFree) {
         const int p1 = a.gptr[i + 1];
-        for (int p = a.gptr[i] + sub; p < p1; p += 4) {
+        for (int p = a.gptr[i] + sub; p < p1; p += LPN) {
             const int s = a.gslot[p];
             if (s < a.nElemSlots) {
                 if (a.ce == 0.0) continue;
@@ -215,12 +225,12 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        se[c] += __shfl_xor(se[c], 1, 64);
-        se[c] += __shfl_xor(se[c], 2, 64);
-        sf[c] += __shfl_xor(sf[c], 1, 64);
-        sf[c] += __shfl_xor(sf[c], 2, 64);
-    }
+    for (int off = 1; off < LPN; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            se[c] += __shfl_xor(se[c], off, 64);
+            sf[c] += __shfl_xor(sf[c], off, 64);
+        }
     if (i >= a.nFree || sub != 0) return;
     // residual: ce = 1, cf = -1 (r = f_int - f_pressure - f_const)
 #pragma unroll
@@ -291,7 +301,7 @@ __device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsign
     return true;
 }
 
-int reduceGrid(int n) { return max(1, min(kReduceBlocks, (n + kRT - 1) / kRT)); }
+int reduceGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().reduceBlocks), (n + kRT - 1) / kRT)); }
 
 __global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
                                                const double* __restrict__ c, int n, double* work,
@@ -682,7 +692,12 @@ __global__ void kUnpermute(const int* __restrict__ perm, const T* __restrict__ z
 
 inline int blocks(int n) { return (n + kThreads - 1) / kThreads; }
 
+thread_local LaunchTuning g_tuning;
+
 }  // namespace
+
+void setTuning(const LaunchTuning& t) { g_tuning = t; }
+const LaunchTuning& tuning() { return g_tuning; }
 
 void launchElementForce(const ElemArgs& a, hipStream_t s)
 {
@@ -756,11 +771,14 @@ void launchAxpby(double* out, double a, const double* x, double b, const double*
 void launchGather(const GatherArgs& a, hipStream_t s)
 {
     if (a.nFree <= 0) return;
-    const int g = static_cast<int>((4LL * a.nFree + kThreads - 1) / kThreads);
-    if (a.mixed)
-        kGather<float><<<g, kThreads, 0, s>>>(a);
-    else
-        kGather<double><<<g, kThreads, 0, s>>>(a);
+    const int lpn = tuning().gatherLanes;
+    const int g = static_cast<int>((static_cast<long long>(lpn) * a.nFree + kThreads - 1) / kThreads);
+    switch (lpn) {
+        case 1: a.mixed ? kGather<float, 1><<<g, kThreads, 0, s>>>(a) : kGather<double, 1><<<g, kThreads, 0, s>>>(a); break;
+        case 2: a.mixed ? kGather<float, 2><<<g, kThreads, 0, s>>>(a) : kGather<double, 2><<<g, kThreads, 0, s>>>(a); break;
+        case 8: a.mixed ? kGather<float, 8><<<g, kThreads, 0, s>>>(a) : kGather<double, 8><<<g, kThreads, 0, s>>>(a); break;
+        default: a.mixed ? kGather<float, 4><<<g, kThreads, 0, s>>>(a) : kGather<double, 4><<<g, kThreads, 0, s>>>(a);
+    }
 }
 
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, unsigned* counter,

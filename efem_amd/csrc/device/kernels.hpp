@@ -6,6 +6,16 @@
 namespace efem {
 namespace dev {
 
+// Launch shapes chosen per context by the autotuner (efem_autotune); the API
+// installs the context's choice for the calling thread before launching.
+struct LaunchTuning {
+    int gatherLanes = 4;     // lanes per free node in kGather: 1, 2, 4, 8
+    int reduceBlocks = 256;  // maximum blocks of a reduction: 32 .. kReduceBlocks
+    int tet4Threads = 256;   // block size of kElemTet4: 64, 128, 256
+};
+void setTuning(const LaunchTuning& t);
+const LaunchTuning& tuning();
+
 // element internal force into per-(element, slot) buffers
 struct ElemArgs {
     int nElems, nen, ng, law, paramsPerElem, mixed;

@@ -168,6 +168,16 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
  *                       as LBFGSMat::apply_Hv orders them */
 int efem_set_option(efem_ctx* ctx, const char* name, double value);
 
+/* autotuner (no counterpart in the reference): times the candidate launch
+ * shapes on this context's problem and keeps the fastest -- gather lanes per
+ * node {1,2,4,8} x C3D4 element block size {64,128,256} (residual time), and
+ * the reduction grid {32,64,128,256} blocks (line-search dot time).  The
+ * choice only changes summation orders, never the formulation.  report[5]
+ * (may be NULL) = {lanes, tet4 threads, reduce blocks, residual ms, dot ms}.
+ * The same shapes can be set by efem_set_option("gather_lanes" |
+ * "tet4_threads" | "reduce_blocks", v). */
+int efem_autotune(efem_ctx* ctx, double* report);
+
 /* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */
 int efem_trace(const efem_ctx* ctx, double* out, int cap);
 
@@ -177,6 +187,10 @@ int efem_trace(const efem_ctx* ctx, double* out, int cap);
 int efem_profile_reset(efem_ctx* ctx);
 int efem_profile_read(efem_ctx* ctx, double* elem_ms_total, int* elem_launches);
 int efem_profile_read_h0(efem_ctx* ctx, double* h0_ms_total, int* h0_applies);
+/* timing helper: `count` back-to-back residual evaluations at u (uploaded
+ * once) on the device; mean device time per evaluation (events around all
+ * element, pressure and gather launches) */
+int efem_time_residual(efem_ctx* ctx, const double* u, int count, double* ms_per_eval);
 
 /* ---- differentiation (no counterpart in the reference) -----------------
  * K_t(u) = d r / d u is the true tangent: the exact material tangent of the

@@ -1,0 +1,57 @@
+"""GPU: the autotuner and the launch-shape options change summation orders
+only: residuals agree to rounding, solves reach the same solution, each
+shape is deterministic, and invalid shapes are refused."""
+import numpy as np
+import pytest
+
+import efem_amd as E
+import oracle as O
+from helpers import rel, small, smooth_u
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("which,opts", [(2, [40, 8, 8]), (4, [16, 4, 4])])
+def test_autotune_keeps_results(which, opts, rng):
+    m = E.Mesh.workload(which, opts)
+    om = O.Model(m)
+    u = smooth_u(m, rng, 0.05)
+    with E.Context(m) as c:
+        r0 = c.residual(u)
+        rep = c.autotune()
+        print("autotune", rep, flush=True)
+        assert rep["gather_lanes"] in (1, 2, 4, 8) and rep["reduce_blocks"] in (32, 64, 128, 256)
+        r1 = c.residual(u)
+        assert rel(r1, om.residual(u)) <= 1e-10 and rel(r1, r0) <= 1e-12
+        assert np.array_equal(r1, c.residual(u))
+
+
+@pytest.mark.parametrize("lanes,tpb,blocks", [(1, 64, 32), (2, 128, 64), (8, 256, 256)])
+def test_launch_shapes(lanes, tpb, blocks, rng):
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    u = smooth_u(m, rng, 0.05)
+    with E.Context(m) as c:
+        ref, _ = c.solve(epsilon=1e-9, max_iterations=5000)
+        r0 = c.residual(u)
+        c.set_option("gather_lanes", lanes)
+        c.set_option("tet4_threads", tpb)
+        c.set_option("reduce_blocks", blocks)
+        assert rel(c.residual(u), r0) <= 1e-12
+        uu, st = c.solve(epsilon=1e-9, max_iterations=5000)
+        assert st["rinf"] <= 1e-9 and rel(uu, ref) <= 1e-6
+    m4 = E.Mesh.workload(2, [20, 4, 4])
+    with E.Context(m4) as c:
+        r0 = c.residual(smooth_u(m4, rng, 0.05))
+    u4 = smooth_u(m4, rng, 0.05)
+    with E.Context(m4) as c:
+        a = c.residual(u4)
+        c.set_option("tet4_threads", tpb)
+        assert np.array_equal(c.residual(u4), a)  # block size: same per-element arithmetic
+
+
+def test_invalid_shapes_refused():
+    m = small(8, 0, 0)
+    with E.Context(m) as c:
+        for name, v in (("gather_lanes", 3), ("tet4_threads", 96), ("reduce_blocks", 0), ("reduce_blocks", 1000)):
+            with pytest.raises(E.EfemError):
+                c.set_option(name, v)
