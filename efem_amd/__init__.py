@@ -123,6 +123,12 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_batch_count": (C.c_int, [vp]),
         "efem_batch_offsets": (C.c_int, [vp, _ip]),
         "efem_batch_solve": (C.c_int, [vp, _dp, C.POINTER(LbfgsParams), C.POINTER(SolveStats)]),
+        "efem_input_load": (vp, [C.c_char_p]),
+        "efem_input_error": (C.c_char_p, []),
+        "efem_input_free": (None, [vp]),
+        "efem_input_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "efem_input_problem": (C.c_int, [vp, C.POINTER(Problem), C.c_int]),
+        "efem_input_lbfgs_params": (C.c_int, [vp, C.POINTER(LbfgsParams)]),
         "efem_mesh_build": (vp, [C.c_int, _dp, C.c_int]),
         "efem_mesh_free": (None, [vp]),
         "efem_mesh_sizes": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
@@ -220,6 +226,42 @@ class Mesh:
         finally:
             L.efem_mesh_free(h)
         return Mesh(X, conn, law, params, fixed, faces, pr.value, red)
+
+    @staticmethod
+    def from_input(json_path) -> tuple["Mesh", dict]:
+        """An EFEM run directory (JSON config + mesh / constraint files) in
+        the reference's formats.  Returns the mesh and the solver settings
+        of its analysis block (keyword arguments of Context.solve)."""
+        L = lib()
+        h = L.efem_input_load(str(json_path).encode())
+        if not h:
+            raise EfemError(f"efem_input_load: {L.efem_input_error().decode()}")
+        try:
+            s = (C.c_longlong * 10)()
+            L.efem_input_sizes(h, s)
+            n, E, nen, law, ppe, F, fnn, nfix, nfn, skipped = (int(v) for v in s)
+            pb = Problem()
+            L.efem_input_problem(h, C.byref(pb), FP64)
+            X = np.ctypeslib.as_array(pb.X, (n * 3,)).reshape(n, 3).copy()
+            conn = np.ctypeslib.as_array(pb.conn, (E * nen,)).reshape(E, nen).copy()
+            params = np.ctypeslib.as_array(pb.params, (E * 4 if ppe else 4,)).copy()
+            if ppe:
+                params = params.reshape(E, 4)
+            fixed = np.ctypeslib.as_array(pb.fixed, (n,)).copy()
+            faces = (np.ctypeslib.as_array(pb.faces, (F * fnn,)).reshape(F, fnn).copy() if F else None)
+            pdir = np.ctypeslib.as_array(pb.pdir, (3,)).copy() if pb.pdir else None
+            fnodes = np.ctypeslib.as_array(pb.fnodes, (nfn,)).copy() if nfn else None
+            fvec = np.ctypeslib.as_array(pb.fvec, (3,)).copy() if nfn else None
+            prm = LbfgsParams()
+            L.efem_input_lbfgs_params(h, C.byref(prm))
+        finally:
+            L.efem_input_free(h)
+        mesh = Mesh(X, conn, law, params, fixed, faces, pb.pressure, 0, pdir=pdir, fnodes=fnodes, fvec=fvec)
+        mesh.skipped_elements = skipped
+        settings = {"epsilon": prm.epsilon, "epsilon_rel": prm.epsilon_rel, "max_iterations": prm.max_iterations,
+                    "max_linesearch": prm.max_linesearch, "m": prm.m, "lns_fea": prm.lns_fea,
+                    "use_precond": bool(prm.use_precond)}
+        return mesh, settings
 
     def problem(self, precision: int = FP64) -> Problem:
         pb = Problem()

@@ -133,6 +133,81 @@ __global__ __launch_bounds__(kThreads) void kPressure(FaceArgs a)
     if (f < a.nFaces) pressureFace<NEN, MODE>(a, f);
 }
 
+// Follower-pressure force, tiled: FPB faces per block.  Phase 1, one lane
+// per (face, Gauss point): F from the LDS-staged relative displacements,
+// v = cof(F) A0 (or p_dir |cof F A0|) into LDS; phase 2, one lane per
+// (face, node): f_a = p sum_q w_q N_a v_q.  Same arithmetic per face as
+// pressureFace<NEN, PMODE_FORCE> (which one lane per face would run
+// serially over up to 9 Gauss points and 20 nodes).
+template <int NEN>
+struct PressureTile {
+    static constexpr int kMaxQ = 9;
+    static constexpr int fpb = 28;  // 28 x 9 Gauss points <= 256 lanes
+};
+
+template <int NEN>
+__global__ __launch_bounds__(kThreads) void kPressureTile(FaceArgs a)
+{
+    constexpr int FPB = PressureTile<NEN>::fpb;
+    __shared__ double sdu[FPB][NEN][3];
+    __shared__ double sv[FPB][PressureTile<NEN>::kMaxQ][3];
+    const int F = a.nFaces, fq = a.fq, tid = threadIdx.x;
+    const int f0 = blockIdx.x * FPB;
+    for (int k = tid; k < FPB * NEN; k += kThreads) {
+        const int fl = k / NEN, n = k - fl * NEN, f = f0 + fl;
+        if (f < F) {
+            const int id = a.faceDof[n * F + f], id0 = a.faceDof[f];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sdu[fl][n][c] = a.x[3 * id + c] - a.x[3 * id0 + c];
+        }
+    }
+    __syncthreads();
+    {
+        const int fl = tid / fq, q = tid - fl * fq, f = f0 + fl;
+        if (fl < FPB && f < F) {
+            const double* G = a.G + (static_cast<size_t>(f) * fq + q) * NEN * 3;
+            double Fm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            for (int n = 1; n < NEN; ++n)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) Fm[3 * i + j] += sdu[fl][n][i] * G[3 * n + j];
+            double cf[9];
+            cofactor(Fm, cf);
+            const double A0[3] = {a.A0[3 * f], a.A0[3 * f + 1], a.A0[3 * f + 2]};
+            double v[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[i] = cf[3 * i] * A0[0] + cf[3 * i + 1] * A0[1] + cf[3 * i + 2] * A0[2];
+            if (a.directed) {
+                const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                v[0] = nv * a.pdir[0];
+                v[1] = nv * a.pdir[1];
+                v[2] = nv * a.pdir[2];
+            }
+            sv[fl][q][0] = v[0];
+            sv[fl][q][1] = v[1];
+            sv[fl][q][2] = v[2];
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < FPB * NEN; k += kThreads) {
+        const int fl = k / NEN, n = k - fl * NEN, f = f0 + fl;
+        if (f >= F) continue;
+        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+        for (int q = 0; q < fq; ++q) {
+            const double wn = a.WN[(static_cast<size_t>(f) * fq + q) * NEN + n];
+            o0 += wn * sv[fl][q][0];
+            o1 += wn * sv[fl][q][1];
+            o2 += wn * sv[fl][q][2];
+        }
+        const double pr = a.fpress ? a.fpress[f] : a.pressure;
+        double* dst = a.slots + (static_cast<size_t>(f) * NEN + n) * 3;
+        dst[0] = pr * o0;
+        dst[1] = pr * o1;
+        dst[2] = pr * o2;
+    }
+}
+
 // ---- per-element scalars (FP64, one lane per element) ---------------------
 // MODE 0: energy  out[e] = sum_q w_q V W(F_q)
 // MODE 1: parameter sensitivities of lam . f_int,
@@ -748,7 +823,18 @@ void launchPressureMode(const FaceArgs& a, hipStream_t s)
     }
 }
 
-void launchPressure(const FaceArgs& a, hipStream_t s) { launchPressureMode<PMODE_FORCE>(a, s); }
+void launchPressure(const FaceArgs& a, hipStream_t s)
+{
+    if (a.nFaces <= 0) return;
+    if (a.fq > PressureTile<4>::kMaxQ) return launchPressureMode<PMODE_FORCE>(a, s);
+    const int g = (a.nFaces + PressureTile<4>::fpb - 1) / PressureTile<4>::fpb;
+    switch (a.nen) {
+        case 4: kPressureTile<4><<<g, kThreads, 0, s>>>(a); break;
+        case 8: kPressureTile<8><<<g, kThreads, 0, s>>>(a); break;
+        case 10: kPressureTile<10><<<g, kThreads, 0, s>>>(a); break;
+        case 20: kPressureTile<20><<<g, kThreads, 0, s>>>(a); break;
+    }
+}
 
 void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s)
 {

@@ -297,6 +297,34 @@ int efem_mesh_arrays(const efem_mesh* m, double* X, int* conn, double* params,
 /* fill an efem_problem that points into the mesh (valid while m lives) */
 int efem_mesh_problem(const efem_mesh* m, efem_problem* pb, int precision);
 
+/* ---- reference input formats (host only, no GPU needed) -----------------
+ * efem_input_load reads an EFEM run directory: the JSON config's keys as
+ * Config::Config reads them (src/io/Config.cpp:23-81: analysis.nl_solver,
+ * max_solver_iters, max_line_search, dx_criterion, residual_criterion,
+ * gravity "non" only), model[0].mesh_file (MeshParser::parse,
+ * src/io/MeshParser.h:22-100; element rows by Body::buildElement,
+ * src/model/Body.cpp:49-90), material[] (neo_hookean / mo_riv / yeoh by id,
+ * ElementConstraint.cpp:11-67; elements without a defined material are
+ * skipped), constraint[] fix / force / pressure with constraint_file and
+ * value (ExternalConstraint.cpp:50-247; files read as IO::readFile,
+ * src/io/IO.cpp:67-96).  Paths are relative to the JSON's directory.  One
+ * body, one element kind, one pressure and one force constraint; Neo-Hookean
+ * elements mixed with Mooney-Rivlin (or Yeoh) ones become Mooney-Rivlin
+ * (Yeoh) elements with C01 = 0 (C20 = C30 = 0), which is the same stress and
+ * approximate-Jacobian coefficient.  NULL on error (efem_input_error). */
+typedef struct efem_input efem_input;
+efem_input* efem_input_load(const char* json_path);
+const char* efem_input_error(void);
+void efem_input_free(efem_input* in);
+/* {n_nodes, n_elems, nen, law, params_per_elem, n_faces, fnn, n_fixed,
+ *  n_force_nodes, skipped_elements} */
+int efem_input_sizes(const efem_input* in, long long* sizes10);
+/* a problem pointing into the input (valid while it lives) */
+int efem_input_problem(const efem_input* in, efem_problem* pb, int precision);
+/* solver parameters of the analysis block (Integrator.cpp:6-20: epsilon =
+ * epsilon_rel = residual_criterion; m = 6; projection line search) */
+int efem_input_lbfgs_params(const efem_input* in, efem_lbfgs_params* prm);
+
 #ifdef __cplusplus
 }
 #endif
