@@ -35,6 +35,7 @@ struct DeviceFactor {
     int* instItems = nullptr;  // batches: [instance][level] {fwd off, cnt, bwd off, cnt}
     int instLevels = 0;
     std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level, in items
+    std::vector<int> fwdWide, bwdWide;               // per level: 16-wave items
     // the whole apply captured as one HIP graph over fixed staging buffers
     double *in = nullptr, *out = nullptr, *bp = nullptr;
     hipGraph_t graph = nullptr;
@@ -98,6 +99,12 @@ struct efem_ctx {
 };
 
 namespace {
+
+// supernode shapes that switch a level to 16-wave triangular-solve items
+constexpr int kWideColumns = 128;
+constexpr int kTallRows = 256;
+constexpr int kWideMaxFwdItems = 500;
+constexpr int kWideMaxBwdItems = 1000;
 
 // scalar slots in ctx->scal
 constexpr int kOut = 0;       // [3] reduction outputs
@@ -327,9 +334,9 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out)
     const efem::dev::TrsvArgs a = trsvArgs(c);
     efem::dev::launchPermute(a, v, f.bp, c->stream);
     for (int l = 0; l < f.nlevels; ++l)
-        efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], c->stream);
+        efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], f.fwdWide[l], c->stream);
     for (int l = f.nlevels - 1; l >= 0; --l)
-        efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], c->stream);
+        efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], f.bwdWide[l], c->stream);
     efem::dev::launchUnpermute(a, out, c->stream);
 }
 
@@ -423,6 +430,27 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     d.fwdCnt.assign(F.nlevels, 0);
     d.bwdOff.assign(F.nlevels, 0);
     d.bwdCnt.assign(F.nlevels, 0);
+    // levels of wide (tall) supernodes use 16-wave forward (backward) items:
+    // the per-lane column (row) chains are 4x shorter
+    d.fwdWide.assign(F.nlevels, 0);
+    d.bwdWide.assign(F.nlevels, 0);
+    {
+        // a level goes wide when it has a wide (tall) supernode and too few
+        // items to fill the chip with 4-wave blocks (measured on the beam:
+        // 16-wave items lose on levels of >500 forward / >1000 backward items)
+        std::vector<int> nf(F.nlevels, 0), nb(F.nlevels, 0), maxw(F.nlevels, 0), maxm(F.nlevels, 0);
+        for (int s = 0; s < F.nsuper; ++s) {
+            const int l = F.level[s];
+            nf[l] += (mm[s] + 63) / 64;
+            nb[l] += (w[s] + 3) / 4;
+            maxw[l] = std::max(maxw[l], w[s]);
+            maxm[l] = std::max(maxm[l], mm[s]);
+        }
+        for (int l = 0; l < F.nlevels; ++l) {
+            d.fwdWide[l] = maxw[l] >= kWideColumns && nf[l] <= kWideMaxFwdItems;
+            d.bwdWide[l] = maxm[l] >= kTallRows && nb[l] <= kWideMaxBwdItems;
+        }
+    }
     for (int l = 0; l < F.nlevels; ++l) {
         d.fwdOff[l] = static_cast<int>(items.size() / 2);
         for (int s : byLevel[l])

@@ -39,7 +39,7 @@ template <int NEN, int NG, typename TF>
 union BatchLds {
     ElemTileLds<float, NEN, NG, MODE_FORCE> ef;
     ElemTileLds<double, NEN, NG, MODE_FORCE> ed;
-    TrsvFwdLds<TF> tf;
+    TrsvFwdLds<TF, 4> tf;
 };
 
 template <typename T, int NEN, int NG, int LAW>
@@ -98,12 +98,13 @@ __device__ void blockH0(const BatchArgs& a, int inst, const BatchRange& I, const
     for (int l = 0; l < a.nlev; ++l) {
         const int off = li[4 * l], cnt = li[4 * l + 1];
         for (int it = 0; it < cnt; ++it)
-            trsvFwdItem<TF>(t, a.bp, a.items[2 * (off + it)], a.items[2 * (off + it) + 1], U.tf);
+            trsvFwdItem<TF, 4>(t, a.bp, a.items[2 * (off + it)], a.items[2 * (off + it) + 1], U.tf);
         __syncthreads();
     }
     for (int l = a.nlev - 1; l >= 0; --l) {
         const int off = li[4 * l + 2], cnt = li[4 * l + 3];
-        for (int it = 0; it < cnt; ++it) trsvBwdItem<TF>(t, a.items[2 * (off + it)], a.items[2 * (off + it) + 1]);
+        for (int it = 0; it < cnt; ++it)
+            trsvBwdItem<TF, 4>(t, a.items[2 * (off + it)], a.items[2 * (off + it) + 1]);
         __syncthreads();
     }
     const TF* z = static_cast<const TF*>(t.z);

@@ -411,10 +411,10 @@ __device__ __forceinline__ void gatherNode(const GatherArgs& a, int i)
 
 constexpr int kMaxW = kMaxSupernodeWidth;
 
-template <typename T>
+template <typename T, int WAVES = 4>
 struct TrsvFwdLds {
     T sb[kMaxW * 3];
-    T red[4][64][3];
+    T red[WAVES][64][3];
 };
 
 __device__ __forceinline__ long long itemVal(const int4& d1)
@@ -422,17 +422,19 @@ __device__ __forceinline__ long long itemVal(const int4& d1)
     return static_cast<long long>(static_cast<unsigned>(d1.z)) | (static_cast<long long>(d1.w) << 32);
 }
 
-// One forward item -- 64 rows [r0, r0 + 64) of supernode s -- by the 256
-// lanes of a block (contains barriers; every lane must call it).
+// One forward item -- 64 rows [r0, r0 + 64) of supernode s -- by the
+// 64 WAVES lanes of a block (contains barriers; every lane must call it).
+// The WAVES waves split the column range and are summed in wave order.
 // Descriptor d0 = {r0, w, m, j0}, d1 = {front row offset, update-vector
 // offset, B offset lo, hi}.  bp is the right-hand side already permuted
 // (bp[3 j + c] = b[3 perm[j] + c]).  Multifrontal: the front of s is its
 // rows of bp plus its children's update vectors (extend-add through
 // fptr/fidx); y_s = L11^-1 f[:w], u_s = f[w:] - (L21 L11^-1) f[:w].
-template <typename T>
+template <typename T, int WAVES = 4>
 __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __restrict__ bp, int4 d0, int4 d1,
-                                            TrsvFwdLds<T>& L)
+                                            TrsvFwdLds<T, WAVES>& L)
 {
+    constexpr int NT = 64 * WAVES;
     const int r0 = d0.x, w = d0.y, m = d0.z, j0 = d0.w, fr0 = d1.x, tb = d1.y;
     const long long val = itemVal(d1);
     T* sb = L.sb;
@@ -453,7 +455,7 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     }
     // front entries of the supernode's own columns (short lists: at most one
     // entry per child)
-    for (int k = threadIdx.x; k < w; k += kThreads) {
+    for (int k = threadIdx.x; k < w; k += NT) {
         const double* bk = bp + 3 * static_cast<size_t>(j0 + k);
         double v0 = bk[0], v1 = bk[1], v2 = bk[2];
         const int p1 = a.fptr[fr0 + k + 1];
@@ -468,8 +470,7 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
         sb[3 * k + 2] = static_cast<T>(v2);
     }
     __syncthreads();
-    // 64 rows per item; the four waves split the k range, then reduce in order
-    const int kc = (w + 3) >> 2;
+    const int kc = (w + WAVES - 1) / WAVES;
     const int kb = wave * kc;
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
     if (r < m) {
@@ -496,9 +497,15 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     L.red[wave][lane][2] = acc2;
     __syncthreads();
     if (wave != 0 || r >= m) return;
-    acc0 = ((L.red[0][lane][0] + L.red[1][lane][0]) + L.red[2][lane][0]) + L.red[3][lane][0];
-    acc1 = ((L.red[0][lane][1] + L.red[1][lane][1]) + L.red[2][lane][1]) + L.red[3][lane][1];
-    acc2 = ((L.red[0][lane][2] + L.red[1][lane][2]) + L.red[2][lane][2]) + L.red[3][lane][2];
+    acc0 = L.red[0][lane][0];
+    acc1 = L.red[0][lane][1];
+    acc2 = L.red[0][lane][2];
+#pragma unroll
+    for (int q = 1; q < WAVES; ++q) {
+        acc0 += L.red[q][lane][0];
+        acc1 += L.red[q][lane][1];
+        acc2 += L.red[q][lane][2];
+    }
     if (r < w) {
         T* out = static_cast<T*>(a.y) + 3 * static_cast<size_t>(j0 + r);
         out[0] = acc0;
@@ -512,29 +519,34 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     out[2] = static_cast<T>(f2) - acc2;
 }
 
-// One backward item: four columns k0 .. k0 + 3 of supernode s, one per wave
-// (no barrier).  Descriptor d0 = {k0, w, m, j0}, d1 = {row offset, 0, B
-// offset lo, hi}.  z_s = L11^-T (y_s - L21^T z_ancestors) =
-// B11^T y_s - B21^T z[rows].
-template <typename T>
-__device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1)
+// One backward item: four columns k0 .. k0 + 3 of supernode s.  With
+// WAVES = 4 one wave per column (no barrier); with WAVES = 16 four waves
+// per column split its rows and are summed in order through LDS (contains a
+// barrier; every lane must call it).  Descriptor d0 = {k0, w, m, j0},
+// d1 = {row offset, 0, B offset lo, hi}.
+// z_s = L11^-T (y_s - L21^T z_ancestors) = B11^T y_s - B21^T z[rows].
+template <typename T, int WAVES = 4>
+__device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1, T (*red)[3] = nullptr)
 {
+    constexpr int G = WAVES / 4;  // waves per column
     const int w = d0.y, m = d0.z, j0 = d0.w;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = d0.x + wave;
-    if (k >= w) return;
+    const int col = wave / G, part = wave - col * G;
+    const int k = d0.x + col;
     const int* rows = a.rows + d1.x;
     const T* y = static_cast<const T*>(a.y);
     T* z = static_cast<T*>(a.z);
-    const T* col = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * m;
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    for (int r = k + lane; r < m; r += 64) {
-        const T bk = col[r];
-        const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
-        const T sg = r < w ? T(1) : T(-1);
-        acc0 += bk * (sg * src[0]);
-        acc1 += bk * (sg * src[1]);
-        acc2 += bk * (sg * src[2]);
+    if (k < w) {
+        const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * m;
+        for (int r = k + part * 64 + lane; r < m; r += 64 * G) {
+            const T bk = cl[r];
+            const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
+            const T sg = r < w ? T(1) : T(-1);
+            acc0 += bk * (sg * src[0]);
+            acc1 += bk * (sg * src[1]);
+            acc2 += bk * (sg * src[2]);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -542,11 +554,32 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1)
         acc1 += __shfl_xor(acc1, off, 64);
         acc2 += __shfl_xor(acc2, off, 64);
     }
-    if (lane == 0) {
+    if (G == 1) {
+        if (lane == 0 && k < w) {
+            T* zo = z + 3 * static_cast<size_t>(j0 + k);
+            zo[0] = acc0;
+            zo[1] = acc1;
+            zo[2] = acc2;
+        }
+        return;
+    }
+    if (lane == 0) {  // wave = col * G + part
+        red[wave][0] = acc0;
+        red[wave][1] = acc1;
+        red[wave][2] = acc2;
+    }
+    __syncthreads();
+    if (part == 0 && lane == 0 && k < w) {
+        T s0 = red[col * G][0], s1 = red[col * G][1], s2 = red[col * G][2];
+        for (int q = 1; q < G; ++q) {
+            s0 += red[col * G + q][0];
+            s1 += red[col * G + q][1];
+            s2 += red[col * G + q][2];
+        }
         T* zo = z + 3 * static_cast<size_t>(j0 + k);
-        zo[0] = acc0;
-        zo[1] = acc1;
-        zo[2] = acc2;
+        zo[0] = s0;
+        zo[1] = s1;
+        zo[2] = s2;
     }
 }
 

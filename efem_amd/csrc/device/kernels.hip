@@ -727,18 +727,19 @@ __global__ void kTheta(const double* ys, const double* yy, double* th)
 }
 
 // ---- supernodal triangular solves -----------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kThreads) void kTrsvFwd(TrsvArgs a, const double* __restrict__ bp,
-                                                     const int4* __restrict__ items)
+template <typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void kTrsvFwd(TrsvArgs a, const double* __restrict__ bp,
+                                                       const int4* __restrict__ items)
 {
-    __shared__ TrsvFwdLds<T> L;
-    trsvFwdItem<T>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L);
+    __shared__ TrsvFwdLds<T, WAVES> L;
+    trsvFwdItem<T, WAVES>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void kTrsvBwd(TrsvArgs a, const int4* __restrict__ items)
+template <typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void kTrsvBwd(TrsvArgs a, const int4* __restrict__ items)
 {
-    trsvBwdItem<T>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1]);
+    __shared__ T red[WAVES][3];
+    trsvBwdItem<T, WAVES>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], red);
 }
 
 // bp[3 j + c] = b[3 perm[j] + c]
@@ -963,22 +964,28 @@ void launchPermute(const TrsvArgs& a, const double* b, double* bp, hipStream_t s
     if (a.n > 0) kPermute<<<blocks(a.n), kThreads, 0, s>>>(a.perm, b, bp, a.n);
 }
 
-void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, hipStream_t s)
+void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, int wide, hipStream_t s)
 {
     if (nitems <= 0) return;
-    if (a.mixed)
-        kTrsvFwd<float><<<nitems, kThreads, 0, s>>>(a, bp, items);
-    else
-        kTrsvFwd<double><<<nitems, kThreads, 0, s>>>(a, bp, items);
+    if (a.mixed) {
+        if (wide) kTrsvFwd<float, 16><<<nitems, 1024, 0, s>>>(a, bp, items);
+        else kTrsvFwd<float, 4><<<nitems, 256, 0, s>>>(a, bp, items);
+    } else {
+        if (wide) kTrsvFwd<double, 16><<<nitems, 1024, 0, s>>>(a, bp, items);
+        else kTrsvFwd<double, 4><<<nitems, 256, 0, s>>>(a, bp, items);
+    }
 }
 
-void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, hipStream_t s)
+void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s)
 {
     if (nitems <= 0) return;
-    if (a.mixed)
-        kTrsvBwd<float><<<nitems, kThreads, 0, s>>>(a, items);
-    else
-        kTrsvBwd<double><<<nitems, kThreads, 0, s>>>(a, items);
+    if (a.mixed) {
+        if (wide) kTrsvBwd<float, 16><<<nitems, 1024, 0, s>>>(a, items);
+        else kTrsvBwd<float, 4><<<nitems, 256, 0, s>>>(a, items);
+    } else {
+        if (wide) kTrsvBwd<double, 16><<<nitems, 1024, 0, s>>>(a, items);
+        else kTrsvBwd<double, 4><<<nitems, 256, 0, s>>>(a, items);
+    }
 }
 
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s)

@@ -136,10 +136,14 @@ void launchPermute(const TrsvArgs& a, const double* b, double* bp, hipStream_t s
 // {r0, w, m, j0}, {front row offset, update offset, B offset lo, hi}.
 // Multifrontal form f_s = [bp_s; 0] + sum_children u_c (extend-add),
 // y_s = L11^-1 f_s[:w], u_s = f_s[w:] - (L21 L11^-1) f_s[:w]
-void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, hipStream_t s);
+// wide = 1: 1024-lane blocks whose 16 waves split the supernode's columns
+// (for levels of wide supernodes), else 256 lanes / 4 waves
+void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, int wide,
+                       hipStream_t s);
 // one level of the backward solve; descriptors {k0, w, m, j0}, {row offset,
-// 0, B offset lo, hi}: four columns per item
-void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, hipStream_t s);
+// 0, B offset lo, hi}: four columns per item (wide = 1: four waves per
+// column split its rows, for levels of tall supernodes)
+void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s);
 // out[3 perm[j] + c] = z[j][c]
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s);
 
