@@ -104,6 +104,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_trace": (C.c_int, [vp, _dp, C.c_int]),
         "efem_set_option": (C.c_int, [vp, C.c_char_p, C.c_double]),
         "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
+        "efem_host_dense_kernel": (C.c_char_p, []),
         "efem_profile_reset": (C.c_int, [vp]),
         "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
         "efem_tangent": (C.c_int, [vp, _dp, _dp, _dp, C.c_int]),
@@ -558,6 +559,11 @@ def host_factor_solve(mesh: Mesh, b):
     keys = ("nnz", "supernodes", "levels", "flops", "microseconds", "numeric_us", "laplacian_us",
             "ordering_us")
     return x, dict(zip(keys, (int(v) for v in info)))
+
+
+def host_dense_kernel() -> str:
+    """Name of the host factorisation's dense micro-kernel."""
+    return lib().efem_host_dense_kernel().decode()
 
 
 def free_ids(fixed) -> np.ndarray:

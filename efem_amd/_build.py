@@ -36,6 +36,7 @@ SOURCES = [
     ("host/element.cpp", False),
     ("host/mesh.cpp", False),
     ("host/sparse.cpp", False),
+    ("host/dense.cpp", False),
     ("host/input.cpp", False),
 ]
 

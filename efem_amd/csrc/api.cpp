@@ -17,6 +17,7 @@
 #include "material_math.hpp"
 #include "host/model.hpp"
 #include "host/sparse.hpp"
+#include "host/dense.hpp"
 
 namespace {
 
@@ -1123,6 +1124,8 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     }
     return EFEM_OK;
 }
+
+const char* efem_host_dense_kernel(void) { return efem::dense::kernelName(); }
 
 int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x, long long* info)
 {

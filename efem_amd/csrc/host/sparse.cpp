@@ -5,6 +5,7 @@
 // Duff & Reid / Liu.
 #include "sparse.hpp"
 #include "../limits.hpp"
+#include "dense.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -12,6 +13,7 @@
 #include <cstring>
 #include <atomic>
 #include <functional>
+#include <memory>
 #include <numeric>
 
 namespace efem {
@@ -84,88 +86,19 @@ void ndRecurse(NdState& st, std::vector<int>& nodes)
     st.out.insert(st.out.end(), sepCopy.begin(), sepCopy.end());
 }
 
-// ---- dense kernels on column-major blocks (leading dimension ld) ----------
+// ---- dense front operations (column-major, leading dimension ld) -----------
+// The O(n^3) work of every operation is dense::gemm (host/dense.cpp); big
+// fronts split their rows (or tiles) into OpenMP tasks.
 
-// in-place lower Cholesky of the leading n x n block; false if not SPD
-bool potrf(double* A, int n, int ld)
-{
-    for (int j = 0; j < n; ++j) {
-        double* cj = A + static_cast<size_t>(j) * ld;
-        for (int k = 0; k < j; ++k) {
-            const double* ck = A + static_cast<size_t>(k) * ld;
-            const double s = ck[j];
-            if (s == 0.0) continue;
-            for (int i = j; i < n; ++i) cj[i] -= ck[i] * s;
-        }
-        const double d = cj[j];
-        if (!(d > 0.0)) return false;
-        const double r = std::sqrt(d);
-        cj[j] = r;
-        const double ir = 1.0 / r;
-        for (int i = j + 1; i < n; ++i) cj[i] *= ir;
-    }
-    return true;
-}
-
-// A21 (m x n at rows [n, n+m)) <- A21 L11^-T, L11 the leading n x n lower
-// block.  Rows are independent: row blocks of 128 run as tasks when `par`.
-__attribute__((target_clones("arch=x86-64-v3", "default")))
-void trsmRows(double* A, int n, int ld, int i0, int i1)
-{
-    for (int k = 0; k < n; ++k) {
-        double* ck = A + static_cast<size_t>(k) * ld + n;
-        for (int p = 0; p < k; ++p) {
-            const double s = A[static_cast<size_t>(p) * ld + k];  // L(k,p)
-            if (s == 0.0) continue;
-            const double* cp = A + static_cast<size_t>(p) * ld + n;
-            for (int i = i0; i < i1; ++i) ck[i] -= cp[i] * s;
-        }
-        const double ir = 1.0 / A[static_cast<size_t>(k) * ld + k];
-        for (int i = i0; i < i1; ++i) ck[i] *= ir;
-    }
-}
-
+// A21 (m rows below the n x n pivot block) <- A21 L11^-T
 void trsmRightLT(double* A, int n, int m, int ld, bool par)
 {
     if (!par) {
-        trsmRows(A, n, ld, 0, m);
+        dense::trsmRightLT(A + n, ld, A, ld, n, 0, m);
         return;
     }
 #pragma omp taskloop grainsize(1)
-    for (int i0 = 0; i0 < m; i0 += 128) trsmRows(A, n, ld, i0, std::min(m, i0 + 128));
-}
-
-// C[i][j] -= sum_k A[i][k] B[j][k] for a 64 x 64 (or smaller) tile, A and B
-// rows of column-major panels with leading dimension ld; 8 x 4 register tiles
-__attribute__((target_clones("arch=x86-64-v3", "default")))
-void gemmNTTile(double* C, int ldc, const double* A, const double* B, int ld, int mi, int nj, int kk,
-                bool lowerDiag)
-{
-    for (int j = 0; j < nj; j += 4) {
-        const int jn = std::min(4, nj - j);
-        int i = lowerDiag ? (j / 8) * 8 : 0;
-        for (; i < mi; i += 8) {
-            const int in = std::min(8, mi - i);
-            double acc[4][8] = {};
-            if (in == 8 && jn == 4) {
-                for (int k = 0; k < kk; ++k) {
-                    const double* a = A + static_cast<size_t>(k) * ld + i;
-                    const double* b = B + static_cast<size_t>(k) * ld + j;
-                    for (int jj = 0; jj < 4; ++jj)
-                        for (int ii = 0; ii < 8; ++ii) acc[jj][ii] += a[ii] * b[jj];
-                }
-            } else {
-                for (int k = 0; k < kk; ++k) {
-                    const double* a = A + static_cast<size_t>(k) * ld + i;
-                    const double* b = B + static_cast<size_t>(k) * ld + j;
-                    for (int jj = 0; jj < jn; ++jj)
-                        for (int ii = 0; ii < in; ++ii) acc[jj][ii] += a[ii] * b[jj];
-                }
-            }
-            for (int jj = 0; jj < jn; ++jj)
-                for (int ii = 0; ii < in; ++ii) C[static_cast<size_t>(j + jj) * ldc + i + ii] -= acc[jj][ii];
-        }
-    }
+    for (int i0 = 0; i0 < m; i0 += 128) dense::trsmRightLT(A + n, ld, A, ld, n, i0, std::min(m, i0 + 128));
 }
 
 // U (m x m lower, column-major, ldu) -= A21 A21^T, A21 = rows [n, n+m) of A;
@@ -178,7 +111,8 @@ void syrkLower(const double* A, int n, int m, int ld, double* U, int ldu, bool p
         const int j0 = bj * T, i0 = bi * T;
         const int nj = std::min(T, m - j0), mi = std::min(T, m - i0);
         // the strict upper part of diagonal tiles is computed and ignored
-        gemmNTTile(U + static_cast<size_t>(j0) * ldu + i0, ldu, A + n + i0, A + n + j0, ld, mi, nj, n, bi == bj);
+        dense::gemm(U + static_cast<size_t>(j0) * ldu + i0, ldu, A + n + i0, ld, A + n + j0, ld, true, mi, nj, n,
+                    -1.0);
     };
     if (!par) {
         for (int bj = 0; bj < nt; ++bj)
@@ -191,40 +125,29 @@ void syrkLower(const double* A, int n, int m, int ld, double* U, int ldu, bool p
             if (bi >= bj) tile(bj, bi);
 }
 
-// B = [L11^-1 ; L21 L11^-1] from the supernode's column-major L block (m x w)
-__attribute__((target_clones("arch=x86-64-v3", "default")))
-void mRows(const double* L, double* B, int w, int m, int i0, int i1)
+// in place: the supernode's column-major m x w block P (ld m) holds
+// [L11; L21] on entry and [L11^-1; L21 L11^-1] on return
+void blockInverse(double* P, int w, int m, bool par)
 {
-    // M(i, k) = sum_{p >= k} L21(i, p) Linv(p, k)
-    for (int k = 0; k < w; ++k) {
-        double* mk = B + static_cast<size_t>(k) * m;
-        for (int p = k; p < w; ++p) {
-            const double s = B[static_cast<size_t>(k) * m + p];  // Linv(p, k)
-            if (s == 0.0) continue;
-            const double* lp = L + static_cast<size_t>(p) * m;
-            for (int i = i0; i < i1; ++i) mk[i] += lp[i] * s;
-        }
-    }
-}
-
-void blockInverse(const double* L, double* B, int w, int m, bool par)
-{
-    // L11^-1 column by column: solve L11 x = e_k
-    for (int k = 0; k < w; ++k) {
-        double* x = B + static_cast<size_t>(k) * m;
-        x[k] = 1.0 / L[static_cast<size_t>(k) * m + k];
-        for (int j = k + 1; j < w; ++j) {
-            double acc = 0.0;
-            for (int p = k; p < j; ++p) acc += L[static_cast<size_t>(p) * m + j] * x[p];
-            x[j] = -acc / L[static_cast<size_t>(j) * m + j];
-        }
-    }
-    if (!par) {
-        mRows(L, B, w, m, w, m);
-        return;
-    }
+    std::vector<double> Linv(static_cast<size_t>(w) * w);
+    dense::trtri(P, m, Linv.data(), w, w);
+    auto chunk = [&](int i0, int i1) {
+        const int r = i1 - i0;
+        std::vector<double> t(static_cast<size_t>(r) * w, 0.0);
+        dense::trmmRightLower(t.data(), r, P + i0, m, Linv.data(), w, w, r);
+        for (int k = 0; k < w; ++k)
+            std::memcpy(P + i0 + static_cast<size_t>(k) * m, t.data() + static_cast<size_t>(k) * r, sizeof(double) * r);
+    };
+    if (m > w) {
+        if (par) {
 #pragma omp taskloop grainsize(1)
-    for (int i0 = w; i0 < m; i0 += 128) mRows(L, B, w, m, i0, std::min(m, i0 + 128));
+            for (int i0 = w; i0 < m; i0 += 128) chunk(i0, std::min(m, i0 + 128));
+        } else {
+            chunk(w, m);
+        }
+    }
+    for (int k = 0; k < w; ++k)
+        std::memcpy(P + static_cast<size_t>(k) * m, Linv.data() + static_cast<size_t>(k) * w, sizeof(double) * w);
 }
 
 }  // namespace
@@ -357,22 +280,74 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
                 }
             }
         }
-        std::vector<int> p2(n);
-        for (int k = 0; k < n; ++k) p2[k] = perm[post[k]];
-        perm.swap(p2);
-        buildUpper();
-        etree();
-    }
-    // column counts from the row subtrees (ereach of each row)
-    std::vector<int> cc(n, 1), flag(n, -1);
-    for (int k = 0; k < n; ++k) {
-        flag[k] = k;
-        for (int p = Up[k]; p < Up[k + 1]; ++p) {
-            for (int i = Ui[p]; flag[i] != k; i = parent[i]) {
-                cc[i]++;
-                flag[i] = k;
-            }
+        // relabel: the tree is unchanged, only its node numbers
+        std::vector<int> ipost(n), p2(n), par2(n);
+        for (int k = 0; k < n; ++k) ipost[post[k]] = k;
+        for (int k = 0; k < n; ++k) {
+            p2[k] = perm[post[k]];
+            par2[k] = parent[post[k]] == -1 ? -1 : ipost[parent[post[k]]];
         }
+        perm.swap(p2);
+        parent.swap(par2);
+        for (int k = 0; k < n; ++k) pinv[perm[k]] = k;
+    }
+    // lower pattern + values of P A P^T by columns
+    std::vector<int> Lp(n + 1, 0), Li;
+    std::vector<double> Lx;
+    {
+        for (int j = 0; j < n; ++j)
+            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
+                if (pinv[A.i[p]] >= pinv[j]) Lp[pinv[j] + 1]++;
+        for (int j = 0; j < n; ++j) Lp[j + 1] += Lp[j];
+        Li.resize(Lp[n]);
+        Lx.resize(Lp[n]);
+        std::vector<int> pos(Lp.begin(), Lp.end() - 1);
+        for (int j = 0; j < n; ++j)
+            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
+                if (pinv[A.i[p]] >= pinv[j]) {
+                    Li[pos[pinv[j]]] = pinv[A.i[p]];
+                    Lx[pos[pinv[j]]++] = A.x[p];
+                }
+    }
+    // column counts of L (diagonal included) by the row-subtree skeleton
+    // method of Gilbert, Ng & Peyton (SIAM J. Matrix Anal. Appl. 15(4),
+    // 1994) on the postordered tree: entry A(i, j), i > j, starts a new
+    // branch of row subtree i only when j is a leaf of it (first descendant
+    // of j beyond the last one seen for i); the branch from j joins the
+    // previous leaf's path at their least common ancestor q, found by path
+    // compression.  delta[] holds per-node differences, summed up the tree.
+    std::vector<int> cc(n, 0);
+    {
+        std::vector<int> fdesc(n, -1), maxFirst(n, -1), prevLeaf(n, -1), anc(n);
+        for (int k = 0; k < n; ++k) {
+            cc[k] = fdesc[k] == -1 ? 1 : 0;
+            for (int j = k; j != -1 && fdesc[j] == -1; j = parent[j]) fdesc[j] = k;
+        }
+        std::iota(anc.begin(), anc.end(), 0);
+        for (int j = 0; j < n; ++j) {
+            if (parent[j] != -1) cc[parent[j]]--;
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) {
+                const int i = Li[p];
+                if (i <= j || fdesc[j] <= maxFirst[i]) continue;
+                maxFirst[i] = fdesc[j];
+                const int jprev = prevLeaf[i];
+                prevLeaf[i] = j;
+                cc[j]++;
+                if (jprev != -1) {
+                    int q = jprev;
+                    while (q != anc[q]) q = anc[q];
+                    for (int t = jprev; t != q;) {
+                        const int up = anc[t];
+                        anc[t] = q;
+                        t = up;
+                    }
+                    cc[q]--;
+                }
+            }
+            if (parent[j] != -1) anc[j] = parent[j];
+        }
+        for (int j = 0; j < n; ++j)
+            if (parent[j] != -1) cc[parent[j]] += cc[j];
     }
     // fundamental supernodes, then relaxed merging along etree chains
     std::vector<int> nchild(n, 0);
@@ -433,24 +408,6 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     std::vector<std::vector<int>> children(ns);
     for (int s = 0; s < ns; ++s)
         if (F.parent[s] >= 0) children[F.parent[s]].push_back(s);
-    // lower pattern + values of P A P^T by columns
-    std::vector<int> Lp(n + 1, 0), Li;
-    std::vector<double> Lx;
-    {
-        for (int j = 0; j < n; ++j)
-            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
-                if (pinv[A.i[p]] >= pinv[j]) Lp[pinv[j] + 1]++;
-        for (int j = 0; j < n; ++j) Lp[j + 1] += Lp[j];
-        Li.resize(Lp[n]);
-        Lx.resize(Lp[n]);
-        std::vector<int> pos(Lp.begin(), Lp.end() - 1);
-        for (int j = 0; j < n; ++j)
-            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
-                if (pinv[A.i[p]] >= pinv[j]) {
-                    Li[pos[pinv[j]]] = pinv[A.i[p]];
-                    Lx[pos[pinv[j]]++] = A.x[p];
-                }
-    }
     // symbolic: row sets of the supernodes
     F.rowPtr.assign(ns + 1, 0);
     std::vector<int> mark(n, -1);
@@ -495,7 +452,7 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     // B is filled supernode by supernode inside the tasks (no serial zero-fill)
     F.B.reset(new double[static_cast<size_t>(F.valPtr[ns])]);
     F.Bsize = F.valPtr[ns];
-    std::vector<std::vector<double>> upd(ns);
+    std::vector<std::unique_ptr<double[]>> upd(ns);
     std::vector<long long> subtreeCols(ns, 0);
     for (int s = 0; s < ns; ++s) {
         subtreeCols[s] += first[s + 1] - first[s];
@@ -503,49 +460,66 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     }
     std::atomic<int> failed{0};
     const auto tn0 = std::chrono::steady_clock::now();
+    // One supernode: the front's first w columns [F11; F21] are assembled,
+    // factored and inverted in place in the supernode's slot P of B (m x w,
+    // ld m, the layout B keeps); its trailing (m - w) x (m - w) lower part is
+    // the update matrix U handed to the parent.
     auto factorFront = [&](int s) {
         if (failed.load(std::memory_order_relaxed)) return;
         const int j0 = first[s], w = first[s + 1] - j0;
         const int r0 = F.rowPtr[s], m = F.rowPtr[s + 1] - r0;
+        const int nr = m - w;
         const int* rows = &F.rows[r0];
-        // row -> front position by binary search in the sorted row set
-        auto locOf = [&](int gi) { return static_cast<int>(std::lower_bound(rows, rows + m, gi) - rows); };
-        std::vector<double> front(static_cast<size_t>(m) * m, 0.0);
-        for (int j = j0; j < j0 + w; ++j)
-            for (int p = Lp[j]; p < Lp[j + 1]; ++p) front[static_cast<size_t>(j - j0) * m + locOf(Li[p])] += Lx[p];
+        double* P = &F.B[F.valPtr[s]];
+        // global row -> front position (no OpenMP scheduling point while a
+        // task uses it, so the map is per thread)
+        thread_local std::vector<int> loc;
+        if (static_cast<int>(loc.size()) < n) loc.resize(n);
+        for (int k = 0; k < m; ++k) loc[rows[k]] = k;
+        std::fill(P, P + static_cast<size_t>(m) * w, 0.0);
+        std::unique_ptr<double[]> U;
+        if (nr > 0 && F.parent[s] >= 0) {
+            U.reset(new double[static_cast<size_t>(nr) * nr]);
+            // lower triangle plus the upper part of the 64 x 64 diagonal
+            // tiles the update computes in full
+            for (int b = 0; b < nr; ++b)
+                std::fill(U.get() + static_cast<size_t>(b) * nr + (b / 64) * 64, U.get() + static_cast<size_t>(b + 1) * nr,
+                          0.0);
+        }
+        for (int j = j0; j < j0 + w; ++j) {
+            double* col = P + static_cast<size_t>(j - j0) * m;
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) col[loc[Li[p]]] += Lx[p];
+        }
         for (int c : children[s]) {
-            std::vector<double>& U = upd[c];
+            const double* Uc = upd[c].get();
             const int cw = first[c + 1] - first[c];
             const int cm = F.rowPtr[c + 1] - F.rowPtr[c] - cw;
             const int* cr = &F.rows[F.rowPtr[c] + cw];
-            std::vector<int> cl(cm);
-            for (int b = 0; b < cm; ++b) cl[b] = locOf(cr[b]);
+            // child rows are sorted, so a >= b keeps loc[cr[a]] >= loc[cr[b]]
             for (int b = 0; b < cm; ++b) {
-                double* dst = &front[static_cast<size_t>(cl[b]) * m];
-                const double* src = &U[static_cast<size_t>(b) * cm];
-                for (int a = b; a < cm; ++a) dst[cl[a]] += src[a];
+                const int cb = loc[cr[b]];
+                const double* src = Uc + static_cast<size_t>(b) * cm;
+                if (cb < w) {
+                    double* dst = P + static_cast<size_t>(cb) * m;
+                    for (int a = b; a < cm; ++a) dst[loc[cr[a]]] += src[a];
+                } else {
+                    double* dst = U.get() + static_cast<size_t>(cb - w) * nr;
+                    for (int a = b; a < cm; ++a) dst[loc[cr[a]] - w] += src[a];
+                }
             }
-            std::vector<double>().swap(U);
+            upd[c].reset();
         }
-        if (!potrf(front.data(), w, m)) {
+        if (!dense::potrf(P, w, m)) {
             failed = 1;
             return;
         }
-        const int nr = m - w;
         const bool big = static_cast<double>(nr) * nr * w > 2e7;
-        trsmRightLT(front.data(), w, nr, m, big);
-        if (nr > 0 && F.parent[s] >= 0) {
-            std::vector<double> U(static_cast<size_t>(nr) * nr, 0.0);
-            for (int b = 0; b < nr; ++b)
-                for (int a = b; a < nr; ++a)
-                    U[static_cast<size_t>(b) * nr + a] = front[static_cast<size_t>(w + b) * m + w + a];
-            syrkLower(front.data(), w, nr, m, U.data(), nr, big);
+        trsmRightLT(P, w, nr, m, big);
+        if (U) {
+            syrkLower(P, w, nr, m, U.get(), nr, big);
             upd[s] = std::move(U);
         }
-        // the first w columns of the front now hold [L11; L21] (ld = m)
-        double* Bs = &F.B[F.valPtr[s]];
-        std::fill(Bs, Bs + static_cast<size_t>(w) * m, 0.0);
-        blockInverse(front.data(), Bs, w, m, big);
+        blockInverse(P, w, m, big);
     };
     // postorder recursion with tasks for large subtrees
     std::function<void(int)> process = [&](int s) {

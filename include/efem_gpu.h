@@ -159,6 +159,11 @@ int efem_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
 int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
                            long long* info8);
 
+/* host-only diagnostic: the dense micro-kernel the host factorisation uses
+ * ("avx512 16x8", "avx2 8x6" or "portable 8x6"; the environment variable
+ * EFEM_DENSE_KERNEL=avx2|portable, read once per process, caps the choice) */
+const char* efem_host_dense_kernel(void);
+
 /* tuning / debug options:
  *   "h0_graph"          1 (default): the preconditioner apply replays a
  *                       captured HIP graph; 0: plain launches

@@ -84,6 +84,52 @@ def test_host_factor_and_block_inverse_solve(opts, rng):
     assert info["nnz"] > 0 and info["levels"] >= 1
 
 
+def test_host_factor_wide_fronts_backward_stable(rng):
+    # 35k DOF: the top separators are 17 x 17 nodes, so the dense front
+    # kernels run their blocked paths (several 32-column blocks, partial
+    # 8 x 6 register tiles) and the task-parallel split (nr^2 w > 2e7).
+    # A backward-stable Cholesky solve leaves |Kx - b| ~ eps |K| |x|.
+    m = E.Mesh.workload(2, [40, 16, 16])
+    K = O.Model(m).K(scalar=False).tocsr()
+    b = rng.standard_normal(m.ndof)
+    x, info = E.host_factor_solve(m, b)
+    r = K @ x - b
+    assert np.linalg.norm(r) <= 1e-13 * spla.norm(K, 1) * np.linalg.norm(x)
+    assert np.linalg.norm(r) <= 1e-11 * np.linalg.norm(b)
+
+
+_KERNEL_PROBE = """
+import sys, numpy as np, scipy.sparse.linalg as spla
+import efem_amd as E, oracle as O
+m = E.Mesh.workload(2, [24, 12, 12])
+K = O.Model(m).K(scalar=False).tocsr()
+b = np.random.default_rng(5).standard_normal(m.ndof)
+x, _ = E.host_factor_solve(m, b)
+r = np.linalg.norm(K @ x - b) / (spla.norm(K, 1) * np.linalg.norm(x))
+print(E.host_dense_kernel(), r)
+sys.exit(0 if r < 1e-13 else 1)
+"""
+
+
+@pytest.mark.parametrize("cap", ["avx2", "portable"])
+def test_host_factor_narrower_micro_kernels(cap):
+    # the AVX2 and portable register tiles (chosen on CPUs without AVX-512,
+    # or forced here) give a backward-stable solve too
+    import os
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parents[1]
+    env = dict(os.environ, EFEM_DENSE_KERNEL=cap, PYTHONPATH=str(root))
+    out = subprocess.run([sys.executable, "-c", _KERNEL_PROBE], env=env, cwd=root, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    name = out.stdout.split()[0]
+    if cap == "portable":
+        assert name == "portable"
+    else:
+        assert name in ("avx2", "portable")  # portable only where the CPU lacks AVX2
+
+
 def test_host_factor_large_beam_fill_is_modest():
     # a 60-cell slice of the 265k-DOF beam: nested dissection keeps the
     # scalar factor within ~1.1x the oracle's simplicial fill (7.86M)
