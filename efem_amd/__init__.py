@@ -105,6 +105,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_set_option": (C.c_int, [vp, C.c_char_p, C.c_double]),
         "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
         "efem_host_dense_kernel": (C.c_char_p, []),
+        "efem_device_memory": (C.c_int, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "efem_profile_reset": (C.c_int, [vp]),
         "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
         "efem_tangent": (C.c_int, [vp, _dp, _dp, _dp, C.c_int]),
@@ -559,6 +560,15 @@ def host_factor_solve(mesh: Mesh, b):
     keys = ("nnz", "supernodes", "levels", "flops", "microseconds", "numeric_us", "laplacian_us",
             "ordering_us")
     return x, dict(zip(keys, (int(v) for v in info)))
+
+
+def device_memory():
+    """(free, total) bytes of the current HIP device."""
+    f, t = C.c_longlong(0), C.c_longlong(0)
+    rc = lib().efem_device_memory(C.byref(f), C.byref(t))
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_device_memory failed ({rc}): {lib().efem_global_error().decode()}")
+    return f.value, t.value
 
 
 def host_dense_kernel() -> str:

@@ -41,6 +41,9 @@ struct DeviceFactor {
     double *in = nullptr, *out = nullptr, *bp = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // device buffers owned by this factor: freed when the factor is
+    // replaced or released
+    std::vector<void*> allocs;
 };
 
 }  // namespace
@@ -51,6 +54,10 @@ struct efem_ctx {
     int precision = EFEM_FP64;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
+    std::vector<void*>* allocList = &allocs;  // where devAlloc records
+    // pinned staging of the factor's block-inverse values for the upload
+    void* pinned = nullptr;
+    size_t pinnedBytes = 0;
     // element / face / gather data
     int* connDof = nullptr;
     void* ginv = nullptr;
@@ -132,7 +139,7 @@ int devAlloc(efem_ctx* c, T** p, size_t count)
     void* q = nullptr;
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     HIP_CHECK(c, hipMalloc(&q, bytes));
-    c->allocs.push_back(q);
+    c->allocList->push_back(q);
     *p = static_cast<T*>(q);
     return EFEM_OK;
 }
@@ -396,9 +403,44 @@ int applyH0Internal(efem_ctx* c, const double* v, double* out)
 // forward extend-add map and the level-ordered work-item descriptors.  With
 // snOffset (batched instances), also the per-instance item ranges of every
 // level for the persistent solver.
-int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset)
+// frees the device factor: its buffers and its captured apply graph
+void releaseFactor(efem_ctx* c)
 {
     DeviceFactor& d = c->fac;
+    if (!d.allocs.empty() || d.exec || d.graph) (void)hipStreamSynchronize(c->stream);
+    if (d.exec) (void)hipGraphExecDestroy(d.exec);
+    if (d.graph) (void)hipGraphDestroy(d.graph);
+    for (void* p : d.allocs) (void)hipFree(p);
+    d = DeviceFactor();
+}
+
+// devAlloc records into `list` while this is alive
+struct AllocInto {
+    efem_ctx* c;
+    AllocInto(efem_ctx* ctx, std::vector<void*>* list) : c(ctx) { c->allocList = list; }
+    ~AllocInto() { c->allocList = &c->allocs; }
+};
+
+// the context's pinned staging buffer, grown to at least `bytes` (kept until
+// efem_destroy, so refactorisations reuse it)
+int pinnedStage(efem_ctx* c, size_t bytes, void** p)
+{
+    if (c->pinnedBytes < bytes) {
+        if (c->pinned) (void)hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinnedBytes = 0;
+        HIP_CHECK(c, hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault));
+        c->pinnedBytes = bytes;
+    }
+    *p = c->pinned;
+    return EFEM_OK;
+}
+
+int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset)
+{
+    releaseFactor(c);
+    DeviceFactor& d = c->fac;
+    AllocInto into(c, &d.allocs);
     d.n = F.n;
     d.nsuper = F.nsuper;
     d.nlevels = F.nlevels;
@@ -499,16 +541,29 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     int rc;
     if ((rc = upload(c, &d.perm, F.perm))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
-    if (fac32(c)) {
-        std::vector<float> Bf(static_cast<size_t>(F.Bsize));
-        for (long long i = 0; i < F.Bsize; ++i) Bf[static_cast<size_t>(i)] = static_cast<float>(F.B[i]);
-        float* q = nullptr;
-        if ((rc = upload(c, &q, Bf))) return rc;
-        d.B = q;
-    } else {
+    {
+        // block-inverse values: converted (FP32) or copied by all host
+        // threads into pinned memory, then one DMA
+        const long long nb = F.Bsize;
+        const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
+        void* h = nullptr;
+        if ((rc = pinnedStage(c, static_cast<size_t>(std::max(nb, 1LL)) * es, &h))) return rc;
+        const double* src = F.B.get();
+        if (fac32(c)) {
+            float* hf = static_cast<float*>(h);
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < nb; ++i) hf[i] = static_cast<float>(src[i]);
+        } else {
+            double* hd = static_cast<double*>(h);
+            const long long chunk = 1 << 18;
+#pragma omp parallel for schedule(static)
+            for (long long i0 = 0; i0 < nb; i0 += chunk)
+                std::memcpy(hd + i0, src + i0, static_cast<size_t>(std::min(chunk, nb - i0)) * sizeof(double));
+        }
         double* q = nullptr;
-        if ((rc = devAlloc(c, &q, static_cast<size_t>(F.Bsize)))) return rc;
-        HIP_CHECK(c, hipMemcpy(q, F.B.get(), static_cast<size_t>(F.Bsize) * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = devAlloc(c, &q, (static_cast<size_t>(std::max(nb, 1LL)) * es + 7) / 8))) return rc;
+        HIP_CHECK(c, hipMemcpyAsync(q, h, static_cast<size_t>(nb) * es, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
         d.B = q;
     }
     if ((rc = upload(c, &d.fptr, fptr))) return rc;
@@ -516,14 +571,6 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     if ((rc = devAlloc(c, &d.in, 3 * static_cast<size_t>(F.n) + 3))) return rc;
     if ((rc = devAlloc(c, &d.out, 3 * static_cast<size_t>(F.n) + 3))) return rc;
     if ((rc = devAlloc(c, &d.bp, 3 * static_cast<size_t>(F.n) + 3))) return rc;
-    if (d.exec) {
-        (void)hipGraphExecDestroy(d.exec);
-        d.exec = nullptr;
-    }
-    if (d.graph) {
-        (void)hipGraphDestroy(d.graph);
-        d.graph = nullptr;
-    }
     if ((rc = upload(c, &d.items, items))) return rc;
     const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
     double* q = nullptr;
@@ -541,7 +588,7 @@ int factorInternal(efem_ctx* c, int* failed)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const efem::Model& m = c->model;
-    c->fac.ready = false;
+    releaseFactor(c);
     c->factorFailed = true;
     if (failed) *failed = 1;
     if (m.nFree == 0) return EFEM_OK;  // jacobi.nonZeros() == 0 -> fallback
@@ -849,9 +896,9 @@ int efem_destroy(efem_ctx* c)
 {
     if (!c) return EFEM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->fac.exec) (void)hipGraphExecDestroy(c->fac.exec);
-    if (c->fac.graph) (void)hipGraphDestroy(c->fac.graph);
+    releaseFactor(c);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->evsH0) (void)hipEventDestroy(e);
     if (c->hostScal) (void)hipHostFree(c->hostScal);
@@ -898,9 +945,8 @@ long long efem_factor_nnz(const efem_ctx* c) { return c && c->fac.ready ? c->fac
 int efem_factor_release(efem_ctx* c)
 {
     if (!c) return EFEM_ERR_ARG;
-    // device buffers of the factor stay in c->allocs until efem_destroy;
-    // dropping `ready` makes the next solve factor again
-    c->fac.ready = false;
+    // frees the factor's device buffers; the next solve factors again
+    releaseFactor(c);
     return EFEM_OK;
 }
 
@@ -1126,6 +1172,20 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
 }
 
 const char* efem_host_dense_kernel(void) { return efem::dense::kernelName(); }
+
+int efem_device_memory(long long* free_bytes, long long* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return EFEM_ERR_ARG;
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    if (e != hipSuccess) {
+        g_error = std::string("hipMemGetInfo: ") + hipGetErrorString(e);
+        return EFEM_ERR_HIP;
+    }
+    *free_bytes = static_cast<long long>(f);
+    *total_bytes = static_cast<long long>(t);
+    return EFEM_OK;
+}
 
 int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x, long long* info)
 {

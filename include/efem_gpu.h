@@ -159,6 +159,11 @@ int efem_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
 int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
                            long long* info8);
 
+/* device memory of the current HIP device (hipMemGetInfo): free and total
+ * bytes; lets a caller check that refactorisations replace, not add to, the
+ * factor's device buffers */
+int efem_device_memory(long long* free_bytes, long long* total_bytes);
+
 /* host-only diagnostic: the dense micro-kernel the host factorisation uses
  * ("avx512 16x8", "avx2 8x6" or "portable 8x6"; the environment variable
  * EFEM_DENSE_KERNEL=avx2|portable, read once per process, caps the choice) */

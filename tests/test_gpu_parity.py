@@ -124,6 +124,26 @@ def test_apply_h0_matches_direct_solve(prec, tol, rng):
         assert rel(ctx.apply_h0(b), x_ref) <= tol
 
 
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_refactoring_frees_the_previous_factor(prec, rng):
+    # each factorisation replaces the device factor: repeated factor /
+    # release cycles keep device memory flat and give the same apply
+    m = E.Mesh.workload(2, [60, 12, 12])
+    b = rng.standard_normal(m.ndof)
+    with E.Context(m, prec) as ctx:
+        assert ctx.factor()
+        x0 = ctx.apply_h0(b)
+        free0 = E.device_memory()[0]
+        for k in range(6):
+            if k % 2:
+                ctx.factor_release()
+            assert ctx.factor()
+        free1 = E.device_memory()[0]
+        assert np.array_equal(ctx.apply_h0(b), x0)
+        one_factor = 4 * ctx.factor_nnz  # bytes of the FP32 block values at least
+        assert free0 - free1 < 2 * one_factor, (free0 - free1, one_factor)
+
+
 def test_solve_matches_oracle_small_load():
     # a load the reference algorithm converges on
     m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
