@@ -389,7 +389,7 @@ def main():
         "fp64_switch": args.fp64_switch,
         "fp64_switch_iteration": int(mix[-1]["fp64_switch_iteration"]),
         "u_star": ustar_info,
-        "launch_shapes": {k: {kk: v[kk] for kk in ("gather_lanes", "tet4_threads", "reduce_blocks")}
+        "launch_shapes": {k: {kk: v[kk] for kk in ("gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks")}
                           for k, v in tuned.items()},
         "load_samples": [round(args.pressure * (1.0 + 1e-6 * (k + 1)), 6) for k in range(args.steps)],
         "iterations_mixed_per_sample": [s["iterations"] for s in mix],

@@ -407,10 +407,11 @@ class Context:
 
     def autotune(self) -> dict:
         """Pick the fastest launch shapes for this problem (efem_autotune)."""
-        rep = np.zeros(5)
+        rep = np.zeros(7)
         self._check(self._lib.efem_autotune(self._h, _ptr(rep, C.c_double)), "efem_autotune")
         return {"gather_lanes": int(rep[0]), "tet4_threads": int(rep[1]), "reduce_blocks": int(rep[2]),
-                "residual_ms": rep[3], "dot_ms": rep[4]}
+                "history_blocks": int(rep[5]), "residual_ms": float(rep[3]), "dot_ms": float(rep[4]),
+                "history_ms": float(rep[6])}
 
     def set_option(self, name: str, value: float):
         self._check(self._lib.efem_set_option(self._h, name.encode(), float(value)), "efem_set_option")

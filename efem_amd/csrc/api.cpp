@@ -122,7 +122,8 @@ constexpr int kAlpha = 136;   // [64]
 constexpr int kCoef = 200;    // [64]
 constexpr int kTheta = 264;   // [1]
 constexpr int kGram = 272;    // [16][16] s_i . y_k of the compact two-loop
-constexpr int kNumScal = 528;
+constexpr int kRS = 528;      // [16] s_k . r of the compact two-loop
+constexpr int kNumScal = 544;
 
 #define HIP_CHECK(ctx, expr)                                                          \
     do {                                                                              \
@@ -206,16 +207,19 @@ int uploadModel(efem_ctx* c)
     if ((rc = upload(c, &c->gslot, m.gslot))) return rc;
     if ((rc = upload(c, &c->fext, m.fext))) return rc;
     const size_t np = static_cast<size_t>(m.ndof()) + 3;  // + dummy zero node
+    // zero fills on the context stream: it is a non-blocking stream, so a
+    // null-stream hipMemset would not be ordered before its work
     for (double** v : {&c->x, &c->r, &c->xp, &c->rp, &c->d, &c->tmp}) {
         if ((rc = devAlloc(c, v, np))) return rc;
-        HIP_CHECK(c, hipMemset(*v, 0, np * sizeof(double)));
+        HIP_CHECK(c, hipMemsetAsync(*v, 0, np * sizeof(double), c->stream));
     }
     if ((rc = devAlloc(c, &c->scal, kNumScal))) return rc;
-    HIP_CHECK(c, hipMemset(c->scal, 0, kNumScal * sizeof(double)));
+    HIP_CHECK(c, hipMemsetAsync(c->scal, 0, kNumScal * sizeof(double), c->stream));
     if ((rc = devAlloc(c, &c->work, efem::dev::kReduceWorkDoubles))) return rc;
     if ((rc = devAlloc(c, &c->counter, 4))) return rc;
-    HIP_CHECK(c, hipMemset(c->counter, 0, 4 * sizeof(unsigned)));
+    HIP_CHECK(c, hipMemsetAsync(c->counter, 0, 4 * sizeof(unsigned), c->stream));
     HIP_CHECK(c, hipHostMalloc(reinterpret_cast<void**>(&c->hostScal), 16 * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return EFEM_OK;
 }
 
@@ -446,6 +450,8 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     d.nlevels = F.nlevels;
     d.nnz = F.nnzL;
     std::vector<int> w(F.nsuper), mm(F.nsuper), tOff(F.nsuper + 1);
+    // device block offsets: each m x w block with leading dimension ldOf(m)
+    std::vector<long long> dval(F.nsuper + 1, 0);
     int nt = 0;
     for (int s = 0; s < F.nsuper; ++s) {
         w[s] = F.first[s + 1] - F.first[s];
@@ -456,6 +462,7 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         mm[s] = F.rowPtr[s + 1] - F.rowPtr[s];
         tOff[s] = nt;
         nt += mm[s] - w[s];
+        dval[s + 1] = dval[s] + static_cast<long long>(w[s]) * efem::dev::ldOf(mm[s]);
     }
     tOff[F.nsuper] = nt;
     // forward-sweep extend-add map (children's update rows -> front rows)
@@ -499,7 +506,7 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         for (int s : byLevel[l])
             for (int r0 = 0; r0 < mm[s]; r0 += 64) {
                 items.push_back(make_int4(r0, w[s], mm[s], F.first[s]));
-                items.push_back(desc1(F.rowPtr[s], tOff[s], F.valPtr[s]));
+                items.push_back(desc1(F.rowPtr[s], tOff[s], dval[s]));
             }
         d.fwdCnt[l] = static_cast<int>(items.size() / 2) - d.fwdOff[l];
     }
@@ -508,7 +515,7 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         for (int s : byLevel[l])
             for (int k0 = 0; k0 < w[s]; k0 += 4) {
                 items.push_back(make_int4(k0, w[s], mm[s], F.first[s]));
-                items.push_back(desc1(F.rowPtr[s], 0, F.valPtr[s]));
+                items.push_back(desc1(F.rowPtr[s], 0, dval[s]));
             }
         d.bwdCnt[l] = static_cast<int>(items.size() / 2) - d.bwdOff[l];
     }
@@ -542,24 +549,17 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     if ((rc = upload(c, &d.perm, F.perm))) return rc;
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
     {
-        // block-inverse values: converted (FP32) or copied by all host
-        // threads into pinned memory, then one DMA
-        const long long nb = F.Bsize;
+        // block-inverse values, repacked to the device layout (leading
+        // dimension ldOf(m), zero padding rows) and converted to FP32 if the
+        // preconditioner runs in FP32, by all host threads (packBlocks) into
+        // pinned memory, then one DMA
+        const long long nb = dval[F.nsuper];
         const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
         void* h = nullptr;
         if ((rc = pinnedStage(c, static_cast<size_t>(std::max(nb, 1LL)) * es, &h))) return rc;
-        const double* src = F.B.get();
-        if (fac32(c)) {
-            float* hf = static_cast<float*>(h);
-#pragma omp parallel for schedule(static)
-            for (long long i = 0; i < nb; ++i) hf[i] = static_cast<float>(src[i]);
-        } else {
-            double* hd = static_cast<double*>(h);
-            const long long chunk = 1 << 18;
-#pragma omp parallel for schedule(static)
-            for (long long i0 = 0; i0 < nb; i0 += chunk)
-                std::memcpy(hd + i0, src + i0, static_cast<size_t>(std::min(chunk, nb - i0)) * sizeof(double));
-        }
+        static_assert(efem::dev::ldOf(1) == 4 && efem::dev::ldOf(4) == 4 && efem::dev::ldOf(5) == 8,
+                      "packBlocks pads to multiples of 4");
+        efem::packBlocks(F, dval, 4, fac32(c), h);
         double* q = nullptr;
         if ((rc = devAlloc(c, &q, (static_cast<size_t>(std::max(nb, 1LL)) * es + 7) / 8))) return rc;
         HIP_CHECK(c, hipMemcpyAsync(q, h, static_cast<size_t>(nb) * es, hipMemcpyHostToDevice, c->stream));
@@ -631,7 +631,8 @@ int ensureKrylov(efem_ctx* c)
     for (double** v : {&c->ku, &c->kb, &c->kx, &c->kr, &c->krh, &c->kp, &c->kv, &c->ks, &c->kt, &c->kph, &c->ksh,
                        &c->kxo, &c->kbo}) {
         if ((rc = devAlloc(c, v, np))) return rc;
-        HIP_CHECK(c, hipMemset(*v, 0, np * sizeof(double)));
+        // on the context stream, ordered before the copies that follow
+        HIP_CHECK(c, hipMemsetAsync(*v, 0, np * sizeof(double), c->stream));
     }
     return EFEM_OK;
 }
@@ -1035,13 +1036,19 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     }
     int ncorr = 0, ptr = mc, k = 1;
     const size_t nb = static_cast<size_t>(n) * sizeof(double);
+    // the first line search's slope r . d (later ones come with d)
+    efem::dev::launchDotMax(c->r, c->d, nullptr, n, c->work, c->counter, sc + kOut + 4, c->stream);
     for (;;) {
-        HIP_CHECK(c, hipMemcpyAsync(c->xp, c->x, nb, hipMemcpyDeviceToDevice, c->stream));
-        HIP_CHECK(c, hipMemcpyAsync(c->rp, c->r, nb, hipMemcpyDeviceToDevice, c->stream));
-        efem::dev::launchDotMax(c->r, c->d, nullptr, n, c->work, c->counter, sc + kOut, c->stream);
-        if ((rc = readScalars(c, 1))) return rc;
-        const double dgInit = c->hostScal[0];
-        double cur = 1.0, used = 1.0;
+        // the current iterate and residual become xp / rp by swapping the
+        // buffers (each trial step writes all of x, each residual all of r;
+        // both buffers keep the zero dummy node)
+        std::swap(c->x, c->xp);
+        std::swap(c->r, c->rp);
+        // dgInit = r . d is in kOut + 4 (written where d was formed) and
+        // comes back with the first trial's scalars (the first trial does not
+        // depend on it); with a target the squared distance to it (kOut + 3)
+        // rides along too, so each trial costs one host synchronisation
+        double dgInit = 0.0, cur = 1.0, used = 1.0;
         int trials = 0;
         for (int it = 0; it < prm->max_linesearch; ++it) {
             efem::dev::launchStep(c->x, c->xp, c->d, cur, n, c->stream);
@@ -1049,11 +1056,13 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             ++s.r_evals;
             ++trials;
             used = cur;
-            efem::dev::launchDotMax(c->r, c->d, c->x, n, c->work, c->counter, sc + kOut, c->stream);
-            if ((rc = readScalars(c, 3))) return rc;
+            efem::dev::launchDotMax(c->r, c->d, c->x, n, c->work, c->counter, sc + kOut, c->stream,
+                                    prm->x_target ? c->xtarget : nullptr);
+            if ((rc = readScalars(c, 5))) return rc;
             const double dg = c->hostScal[0];
             rinf = c->hostScal[1];
             xinf = c->hostScal[2];
+            if (it == 0) dgInit = c->hostScal[4];
             if (std::fabs(dg) <= prm->lns_fea * std::fabs(dgInit)) break;
             cur *= dg * dgInit > 0 ? 1.5 : 0.5;
         }
@@ -1066,12 +1075,8 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         s.xinf = xinf;
         if (!std::isfinite(rinf) || !std::isfinite(xinf)) break;
         if (prm->x_target) {
-            // ||x - x_target||: diff into tmp, then its norm
-            efem::dev::launchDiff2(c->tmp, c->tmp, c->x, c->xtarget, c->x, c->xtarget, n, c->stream);
-            efem::dev::launchDot2(c->tmp, c->tmp, n, c->work, c->counter, sc + kOut + 6, sc + kOut + 7, c->stream);
-            HIP_CHECK(c, hipMemcpyAsync(c->hostScal + 6, sc + kOut + 6, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(c, hipStreamSynchronize(c->stream));
-            s.target_err = std::sqrt(c->hostScal[6]) / tnorm;
+            // ||x - x_target|| of the accepted step (read with its scalars)
+            s.target_err = std::sqrt(c->hostScal[3]) / tnorm;
             if (s.target_err <= prm->target_tol) {
                 s.target_reached = 1;
                 break;
@@ -1091,25 +1096,25 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         const int loc = ptr % mc;
         double* sl = c->S + static_cast<size_t>(loc) * n;
         double* yl = c->Y + static_cast<size_t>(loc) * n;
-        efem::dev::launchDiff2(sl, yl, c->x, c->xp, c->r, c->rp, n, c->stream);
         if (ncorr < mc) ++ncorr;
         ptr = loc + 1;
         const bool compact = c->compactTwoLoop && mc <= efem::dev::kMaxCompact;
-        if (compact) {
-            efem::dev::launchGramUpdate(c->S, c->Y, n, ncorr, loc, sc + kGram, sc + kYs, sc + kYY + loc, c->work,
-                                        c->counter, c->stream);
-        } else {
+        if (!compact) {
+            efem::dev::launchDiff2(sl, yl, c->x, c->xp, c->r, c->rp, n, c->stream);
             efem::dev::launchDot2(sl, yl, n, c->work, c->counter, sc + kYs + loc, sc + kYY + loc, c->stream);
+            efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
         }
-        efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
         if (compact) {
-            // two-loop from Gram products: 2 multi-dot passes + 2 multi-axpys
-            // with the captured preconditioner the first pass writes its
-            // staging input and the second reads its staging output directly
+            // two-loop from Gram products: the new pair, its Gram entries and
+            // the first loop in one reduction pass (+ one multi-axpy), then
+            // the second pass after the preconditioner; with the captured
+            // preconditioner the first pass writes its staging input and the
+            // second reads its staging output directly
             const bool staged = h0 && c->useGraph;
             double* q0 = staged ? c->fac.in : c->d;
-            efem::dev::launchTwoLoopFirst(c->r, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
-                                          q0, c->work, c->counter, c->stream);
+            efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, ncorr, loc, mc, sc + kGram,
+                                       sc + kYs, sc + kYY + loc, sc + kTheta, sc + kAlpha, sc + kRS, q0, c->work,
+                                       c->counter, c->stream);
             const double* h = c->d;
             if (staged) {
                 if ((rc = applyH0Graph(c))) return rc;
@@ -1122,7 +1127,8 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
                 efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
             }
             efem::dev::launchTwoLoopSecond(h, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
-                                           sc + kCoef, c->d, c->work, c->counter, c->stream);
+                                           sc + kCoef, c->d, c->work, c->counter, c->stream, c->r, sc + kRS,
+                                           sc + kOut + 4);
         } else {
             // two-loop recursion: d = -H r (LBFGSMat::apply_Hv)
             efem::dev::launchScale(c->d, c->r, -1.0, n, c->stream);
@@ -1154,6 +1160,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             }
             if (prev >= 0)
                 efem::dev::launchAxpyDev(c->d, c->S + static_cast<size_t>(prev) * n, sc + kCoef + prev, 0, n, c->stream);
+            efem::dev::launchDotMax(c->r, c->d, nullptr, n, c->work, c->counter, sc + kOut + 4, c->stream);
         }
         HIP_CHECK(c, hipGetLastError());
         ++k;
@@ -1254,6 +1261,10 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
     }
     if (k == "tet4_threads" && (iv == 64 || iv == 128 || iv == 256)) {
         c->tuning.tet4Threads = iv;
+        return EFEM_OK;
+    }
+    if (k == "history_blocks" && iv >= 1 && iv <= efem::dev::kReduceBlocks) {
+        c->tuning.historyBlocks = iv;
         return EFEM_OK;
     }
     c->err = "efem_set_option: unknown option or value " + k;
@@ -1518,6 +1529,9 @@ int efem_set_params(efem_ctx* c, const double* params)
         c->err = msg;
         return EFEM_ERR_ARG;
     }
+    // kernels still queued on the (non-blocking) context stream may read the
+    // old parameters: drain it before the null-stream copy
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
     HIP_CHECK(c, hipMemcpy(c->params, c->model.params.data(), c->model.params.size() * sizeof(double),
                            hipMemcpyHostToDevice));
     c->fac.ready = false;  // the approximate Jacobian depends on the parameters
@@ -1800,6 +1814,40 @@ int efem_autotune(efem_ctx* c, double* report)
             best.reduceBlocks = grids[gi];
         }
     }
+    // history reductions: the new pair + first loop and the second loop of
+    // the compact two-loop with m = 8 (history buffers grown if needed;
+    // their contents and the scalar slots are scratch outside a solve)
+    double bestHist = 1e300;
+    const int hm = 8;
+    if (rc == EFEM_OK && c->mcap < hm) {
+        if ((rc = devAlloc(c, &c->S, static_cast<size_t>(hm) * std::max(n, 1))) == EFEM_OK &&
+            (rc = devAlloc(c, &c->Y, static_cast<size_t>(hm) * std::max(n, 1))) == EFEM_OK)
+            c->mcap = hm;
+    }
+    if (rc == EFEM_OK) {
+        HIP_CHECK(c, hipMemsetAsync(c->S, 0, static_cast<size_t>(hm) * n * sizeof(double), c->stream));
+        HIP_CHECK(c, hipMemsetAsync(c->Y, 0, static_cast<size_t>(hm) * n * sizeof(double), c->stream));
+    }
+    double* sc = c->scal;
+    for (int gi = 0; gi < 4 && rc == EFEM_OK; ++gi) {
+        efem::dev::LaunchTuning t = best;
+        t.historyBlocks = grids[gi];
+        efem::dev::setTuning(t);
+        double ms = 0.0;
+        rc = timeIt([&]() {
+            efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, hm, hm - 1, hm, sc + kGram, sc + kYs,
+                                       sc + kYY, sc + kTheta, sc + kAlpha, sc + kRS, c->tmp, c->work, c->counter,
+                                       c->stream);
+            efem::dev::launchTwoLoopSecond(c->tmp, c->S, c->Y, n, hm, hm - 1, hm, sc + kGram, sc + kYs, sc + kAlpha,
+                                           sc + kCoef, c->d, c->work, c->counter, c->stream, c->r, sc + kRS,
+                                           sc + kOut + 4);
+            return EFEM_OK;
+        }, 10, &ms);
+        if (rc == EFEM_OK && ms < bestHist) {
+            bestHist = ms;
+            best.historyBlocks = grids[gi];
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
@@ -1811,6 +1859,8 @@ int efem_autotune(efem_ctx* c, double* report)
         report[2] = best.reduceBlocks;
         report[3] = bestRes;
         report[4] = bestRed;
+        report[5] = best.historyBlocks;
+        report[6] = bestHist;
     }
     return EFEM_OK;
 }

@@ -436,6 +436,7 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
 {
     constexpr int NT = 64 * WAVES;
     const int r0 = d0.x, w = d0.y, m = d0.z, j0 = d0.w, fr0 = d1.x, tb = d1.y;
+    const int ld = ldOf(m);
     const long long val = itemVal(d1);
     T* sb = L.sb;
     T* u = static_cast<T*>(a.t);
@@ -479,14 +480,14 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
         const T* Bc = static_cast<const T*>(a.B) + val + r;
         int k = kb;
         for (; k + 4 <= ke; k += 4) {
-            const T b0 = Bc[static_cast<size_t>(k) * m], b1 = Bc[static_cast<size_t>(k + 1) * m];
-            const T b2 = Bc[static_cast<size_t>(k + 2) * m], b3 = Bc[static_cast<size_t>(k + 3) * m];
+            const T b0 = Bc[static_cast<size_t>(k) * ld], b1 = Bc[static_cast<size_t>(k + 1) * ld];
+            const T b2 = Bc[static_cast<size_t>(k + 2) * ld], b3 = Bc[static_cast<size_t>(k + 3) * ld];
             acc0 += b0 * sb[3 * k] + b1 * sb[3 * k + 3] + b2 * sb[3 * k + 6] + b3 * sb[3 * k + 9];
             acc1 += b0 * sb[3 * k + 1] + b1 * sb[3 * k + 4] + b2 * sb[3 * k + 7] + b3 * sb[3 * k + 10];
             acc2 += b0 * sb[3 * k + 2] + b1 * sb[3 * k + 5] + b2 * sb[3 * k + 8] + b3 * sb[3 * k + 11];
         }
         for (; k < ke; ++k) {
-            const T bk = Bc[static_cast<size_t>(k) * m];
+            const T bk = Bc[static_cast<size_t>(k) * ld];
             acc0 += bk * sb[3 * k];
             acc1 += bk * sb[3 * k + 1];
             acc2 += bk * sb[3 * k + 2];
@@ -538,7 +539,7 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
     T* z = static_cast<T*>(a.z);
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
     if (k < w) {
-        const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * m;
+        const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * ldOf(m);
         for (int r = k + part * 64 + lane; r < m; r += 64 * G) {
             const T bk = cl[r];
             const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);

@@ -325,22 +325,24 @@ constexpr int kRT = 1024;
 constexpr int kRW = kRT / 64;  // waves per reduction block
 
 // K simultaneous block reductions: wave butterflies, then the 16 wave
-// results in wave order.  isMax: components k > 0 are maxima.  All lanes
-// return the block result.
+// results in wave order.  Bit k of maxMask: component k is a maximum (else a
+// sum).  All lanes return the block result.
 template <int K>
-__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[kRW], bool isMax)
+__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[kRW], unsigned maxMask)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double t = (isMax && k > 0) ? waveMax(v[k]) : waveSum(v[k]);
+        const bool mx = (maxMask >> k) & 1u;
+        const double t = mx ? waveMax(v[k]) : waveSum(v[k]);
         if (lane == 0) sh[k][wave] = t;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
+        const bool mx = (maxMask >> k) & 1u;
         double t = sh[k][0];
-        for (int w = 1; w < kRW; ++w) t = (isMax && k > 0) ? fmax(t, sh[k][w]) : t + sh[k][w];
+        for (int w = 1; w < kRW; ++w) t = mx ? fmax(t, sh[k][w]) : t + sh[k][w];
         v[k] = t;
     }
     __syncthreads();
@@ -348,10 +350,10 @@ __device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[kRW], b
 
 template <int K>
 __device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsigned* counter,
-                                            double (*sh)[kRW], bool isMax)
+                                            double (*sh)[kRW], unsigned maxMask)
 {
     __shared__ int amLast;
-    blockReduce<K>(v, sh, isMax);
+    blockReduce<K>(v, sh, maxMask);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) work[k * kReduceBlocks + blockIdx.x] = v[k];
@@ -369,31 +371,46 @@ __device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsign
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double o = work[k * kReduceBlocks + b];
-            v[k] = (isMax && k > 0) ? fmax(v[k], o) : v[k] + o;
+            v[k] = ((maxMask >> k) & 1u) ? fmax(v[k], o) : v[k] + o;
         }
-    blockReduce<K>(v, sh, isMax);
+    blockReduce<K>(v, sh, maxMask);
     if (threadIdx.x == 0) *counter = 0u;
     return true;
 }
 
 int reduceGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().reduceBlocks), (n + kRT - 1) / kRT)); }
+// reductions that stream the whole history (2 m + 4 vectors) read several
+// times the bytes of a line-search dot, so their grid is tuned on its own
+int historyGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().historyBlocks), (n + kRT - 1) / kRT)); }
 
+// out = {a.b, max|a|, max|c|} (HASC) and out[3] = |c - xt|^2 (HASXT)
+template <bool HASC, bool HASXT>
 __global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
-                                               const double* __restrict__ c, int n, double* work,
-                                               unsigned* counter, double* out)
+                                               const double* __restrict__ c, const double* __restrict__ xt,
+                                               int n, double* work, unsigned* counter, double* out)
 {
-    __shared__ double sh[3][kRW];
-    double v[3] = {0, 0, 0};
+    constexpr int K = HASXT ? 4 : 3;
+    __shared__ double sh[K][kRW];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+#pragma unroll 4
     for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         const double ai = a[i];
         v[0] += ai * b[i];
         v[1] = fmax(v[1], fabs(ai));
-        if (c) v[2] = fmax(v[2], fabs(c[i]));
+        if (HASC) {
+            const double ci = c[i];
+            v[2] = fmax(v[2], fabs(ci));
+            if (HASXT) {
+                const double e = ci - xt[i];
+                v[K - 1] += e * e;
+            }
+        }
     }
-    if (lastBlockIn<3>(v, work, counter, sh, true) && threadIdx.x == 0) {
-        out[0] = v[0];
-        out[1] = v[1];
-        out[2] = v[2];
+    if (lastBlockIn<K>(v, work, counter, sh, 0x6u) && threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = v[k];
     }
 }
 
@@ -407,7 +424,7 @@ __global__ __launch_bounds__(kRT) void kDot2(const double* __restrict__ a, const
         v[0] += a[i] * bi;
         v[1] += bi * bi;
     }
-    if (lastBlockIn<2>(v, work, counter, sh, false) && threadIdx.x == 0) {
+    if (lastBlockIn<2>(v, work, counter, sh, 0u) && threadIdx.x == 0) {
         *ab = v[0];
         *bb = v[1];
     }
@@ -430,7 +447,7 @@ __global__ __launch_bounds__(kRT) void kTwoLoop(double* __restrict__ q, const do
         }
         v[0] += w[i] * qi;
     }
-    if (lastBlockIn<1>(v, work, counter, sh, false) && threadIdx.x == 0) {
+    if (lastBlockIn<1>(v, work, counter, sh, 0u) && threadIdx.x == 0) {
         const double r = v[0] / (*den);
         *cout = mode ? (*aj - r) : r;
     }
@@ -503,59 +520,121 @@ __device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsign
 // layout of the Gram matrix in the scalar area: G[i * kMaxCompact + k]
 constexpr int KG = kMaxCompact;
 
+// One pass for the new correction pair and the first loop: s_loc = x - xp
+// and y_loc = r - rp are formed (and stored) on the fly, and the same sweep
+// over the history takes the Gram row/column of the new pair, y_loc.y_loc
+// and the first loop's s_k . r.  The last block writes the Gram entries,
+// ys_loc, yy, theta = yy / ys_loc, rs_k = s_k . r, and runs the alpha
+// recurrence newest -> oldest (q_0 = -r): alpha_i = (s_i.q_0 - sum_{k newer}
+// alpha_k s_i.y_k) / ys_i.  Same reduction shapes as every other reduction
+// here.
 template <int KC>
-__global__ __launch_bounds__(kRT) void kGramUpdate(const double* __restrict__ S, const double* __restrict__ Y,
-                                                        int n, int nc, int loc, double* G, double* ys, double* yy,
-                                                        double* work, unsigned* counter)
+__global__ __launch_bounds__(kRT) void kGramFirst(double* __restrict__ S, double* __restrict__ Y,
+                                                       const double* __restrict__ x, const double* __restrict__ xp,
+                                                       const double* __restrict__ r, const double* __restrict__ rp,
+                                                       int n, int nc, int loc, int mcap, double* G, double* ys,
+                                                       double* yy, double* theta, double* alpha, double* rs,
+                                                       double* work, unsigned* counter)
 {
-    // v[k] = s_loc . y_k, v[KC + k] = s_k . y_loc (k < nc), v[2 KC] = y_loc . y_loc
-    __shared__ double red[2 * KC + 1];
-    double v[2 * KC + 1];
+    // v[k] = s_loc . y_k, v[KC + k] = s_k . y_loc, v[2 KC] = y_loc . y_loc,
+    // v[2 KC + 1 + k] = s_k . r   (k < nc; slot loc from registers)
+    constexpr int N = 3 * KC + 1;
+    __shared__ double red[N];
+    __shared__ double sG[KC][KC], sys[KC];
+    double v[N];
 #pragma unroll
-    for (int k = 0; k < 2 * KC + 1; ++k) v[k] = 0.0;
-    const double* sl = S + static_cast<size_t>(loc) * n;
-    const double* yl = Y + static_cast<size_t>(loc) * n;
+    for (int k = 0; k < N; ++k) v[k] = 0.0;
+    double* sl = S + static_cast<size_t>(loc) * n;
+    double* yl = Y + static_cast<size_t>(loc) * n;
+#pragma unroll 1
     for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
-        const double si = sl[i], yi = yl[i];
+        const double ri = r[i];
+        const double si = x[i] - xp[i], yi = ri - rp[i];
+        sl[i] = si;
+        yl[i] = yi;
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
             if (k < nc) {
-                v[k] += si * Y[static_cast<size_t>(k) * n + i];
-                v[KC + k] += S[static_cast<size_t>(k) * n + i] * yi;
+                const double sk = k == loc ? si : S[static_cast<size_t>(k) * n + i];
+                const double yk = k == loc ? yi : Y[static_cast<size_t>(k) * n + i];
+                v[k] += si * yk;
+                v[KC + k] += sk * yi;
+                v[2 * KC + 1 + k] += sk * ri;
             }
         }
         v[2 * KC] += yi * yi;
     }
-    if (!multiReduce<2 * KC + 1>(v, work, counter, red)) return;
+    if (!multiReduce<N>(v, work, counter, red)) return;
+    // Gram block in LDS: stored entries, with the new row / column from red
+    for (int t = threadIdx.x; t < KC * KC; t += kRT) {
+        const int i = t / KC, k = t - i * KC;
+        double g = 0.0;
+        if (i < mcap && k < mcap) {
+            if (i == loc && k < nc) g = red[k];
+            else if (k == loc && i < nc) g = red[KC + i];
+            else g = G[i * KG + k];
+        }
+        sG[i][k] = g;
+    }
+    if (threadIdx.x < KC) {
+        const int k = threadIdx.x;
+        sys[k] = k == loc ? red[loc] : (k < mcap ? ys[k] : 1.0);
+    }
+    __syncthreads();
     if (threadIdx.x < nc) {
         const int k = threadIdx.x;
         G[loc * KG + k] = red[k];
         G[k * KG + loc] = red[KC + k];
-        if (k == loc) ys[loc] = red[k];
+        rs[k] = red[2 * KC + 1 + k];
     }
-    if (threadIdx.x == 0) *yy = red[2 * KC];
+    if (threadIdx.x == 0) {
+        ys[loc] = red[loc];
+        *yy = red[2 * KC];
+        *theta = red[2 * KC] / red[loc];
+        double c[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) c[k] = 0.0;
+        int j = loc;
+        for (int t = 0; t < nc; ++t) {
+            double acc = -red[2 * KC + 1 + j];
+            int kk = loc;
+            for (int u = 0; u < t; ++u) {
+                acc -= c[kk] * sG[j][kk];
+                kk = (kk + mcap - 1) % mcap;
+            }
+            c[j] = acc / sys[j];
+            alpha[j] = c[j];
+            j = (j + mcap - 1) % mcap;
+        }
+    }
 }
 
 template <int KC>
 __global__ __launch_bounds__(kRT) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
                                                          int n, int nc, int first, int newest, int mcap,
                                                          const double* G, const double* ys, const double* alphaIn,
-                                                         double* outCoef, double* work, unsigned* counter)
+                                                         double* outCoef, double* work, unsigned* counter,
+                                                         const double* __restrict__ dw, const double* rs,
+                                                         double* slope)
 {
     // first: dots of R_k (= S) with q (= r), alpha recurrence newest->oldest
-    // else: dots of R_k (= Y) with q (= H0 output), beta recurrence
-    __shared__ double red[KC];
+    // else: dots of R_k (= Y) with q (= H0 output), beta recurrence; with dw
+    // (second loop only) also dw . q, and then the slope of the direction
+    // d = q + sum coef_k s_k along dw: dw . d = dw . q + sum coef_k rs_k
+    // (rs_k = s_k . dw from the first pass)
+    __shared__ double red[KC + 1];
     __shared__ double sG[KC][KC], sys[KC], sa[KC];
-    double v[KC];
+    double v[KC + 1];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) v[k] = 0.0;
+    for (int k = 0; k <= KC; ++k) v[k] = 0.0;
     for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
         const double qi = q[i];
 #pragma unroll
         for (int k = 0; k < KC; ++k)
             if (k < nc) v[k] += R[static_cast<size_t>(k) * n + i] * qi;
+        if (dw) v[KC] += dw[i] * qi;
     }
-    if (!multiReduce<KC>(v, work, counter, red)) return;
+    if (!multiReduce<KC + 1>(v, work, counter, red)) return;
     // the recurrence reads the Gram block, ys and alpha from LDS
     for (int t = threadIdx.x; t < KC * KC; t += kRT) {
         const int i = t / KC, k = t - i * KC;
@@ -598,6 +677,13 @@ __global__ __launch_bounds__(kRT) void kTwoLoopDots(const double* __restrict__ q
                 c[j] = sa[j] - acc / sys[j];
                 outCoef[j] = c[j];
                 j = (j + 1) % mcap;
+            }
+            if (dw) {
+                double sl = red[KC];
+#pragma unroll
+                for (int k = 0; k < KC; ++k)
+                    if (k < nc) sl += c[k] * rs[k];
+                *slope = sl;
             }
         }
     }
@@ -869,9 +955,14 @@ void launchGather(const GatherArgs& a, hipStream_t s)
 }
 
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, unsigned* counter,
-                  double* out, hipStream_t s)
+                  double* out, hipStream_t s, const double* xt)
 {
-    kDotMax<<<reduceGrid(n), kRT, 0, s>>>(a, b, c, n, work, counter, out);
+    if (!c)
+        kDotMax<false, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, nullptr, n, work, counter, out);
+    else if (!xt)
+        kDotMax<true, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, nullptr, n, work, counter, out);
+    else
+        kDotMax<true, true><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, xt, n, work, counter, out);
 }
 
 void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter, double* ab, double* bb,
@@ -904,44 +995,43 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
                                                   work, counter);
 }
 
-void launchGramUpdate(const double* S, const double* Y, int n, int nc, int loc, double* G, double* ys, double* yy,
-                      double* work, unsigned* counter, hipStream_t s)
-{
-    if (nc <= 8)
-        kGramUpdate<8><<<reduceGrid(n), kRT, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
-    else
-        kGramUpdate<16><<<reduceGrid(n), kRT, 0, s>>>(S, Y, n, nc, loc, G, ys, yy, work, counter);
-}
-
 template <int KC>
 void twoLoopPass(const double* q, const double* R, const double* base, double sgnB, const double* axR,
                  double sgnC, int n, int nc, int first, int newest, int mcap, const double* G, const double* ys,
-                 const double* alphaIn, double* coef, double* out, double* work, unsigned* counter, hipStream_t s)
+                 const double* alphaIn, double* coef, double* out, double* work, unsigned* counter, hipStream_t s,
+                 const double* dw = nullptr, const double* rs = nullptr, double* slope = nullptr)
 {
-    kTwoLoopDots<KC><<<reduceGrid(n), kRT, 0, s>>>(q, R, n, nc, first, newest, mcap, G, ys, alphaIn, coef,
-                                                         work, counter);
+    kTwoLoopDots<KC><<<historyGrid(n), kRT, 0, s>>>(q, R, n, nc, first, newest, mcap, G, ys, alphaIn, coef,
+                                                         work, counter, dw, rs, slope);
     if (n > 0) kMultiAxpy<KC><<<blocks(n), kThreads, 0, s>>>(base, sgnB, axR, coef, sgnC, n, nc, out);
 }
 
-void launchTwoLoopFirst(const double* r, const double* S, const double* Y, int n, int nc, int newest, int mcap,
-                        const double* G, const double* ys, double* alpha, double* out, double* work,
-                        unsigned* counter, hipStream_t s)
+void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
+                     int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
+                     double* alpha, double* rs, double* out, double* work, unsigned* counter, hipStream_t s)
 {
-    if (mcap <= 8)
-        twoLoopPass<8>(r, S, r, -1.0, Y, -1.0, n, nc, 1, newest, mcap, G, ys, nullptr, alpha, out, work, counter, s);
-    else
-        twoLoopPass<16>(r, S, r, -1.0, Y, -1.0, n, nc, 1, newest, mcap, G, ys, nullptr, alpha, out, work, counter,
-                        s);
+    if (mcap <= 8) {
+        kGramFirst<8><<<historyGrid(n), kRT, 0, s>>>(S, Y, x, xp, r, rp, n, nc, loc, mcap, G, ys, yy, theta, alpha,
+                                                     rs, work, counter);
+        if (n > 0) kMultiAxpy<8><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, nc, out);
+    } else {
+        kGramFirst<16><<<historyGrid(n), kRT, 0, s>>>(S, Y, x, xp, r, rp, n, nc, loc, mcap, G, ys, yy, theta,
+                                                      alpha, rs, work, counter);
+        if (n > 0) kMultiAxpy<16><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, nc, out);
+    }
 }
 
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
                          const double* G, const double* ys, const double* alpha, double* coef, double* out,
-                         double* work, unsigned* counter, hipStream_t s)
+                         double* work, unsigned* counter, hipStream_t s, const double* dw, const double* rs,
+                         double* slope)
 {
     if (mcap <= 8)
-        twoLoopPass<8>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s);
+        twoLoopPass<8>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s, dw,
+                       rs, slope);
     else
-        twoLoopPass<16>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s);
+        twoLoopPass<16>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s,
+                        dw, rs, slope);
 }
 
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s)

@@ -12,9 +12,19 @@ struct LaunchTuning {
     int gatherLanes = 4;     // lanes per free node in kGather: 1, 2, 4, 8
     int reduceBlocks = 256;  // maximum blocks of a reduction: 32 .. kReduceBlocks
     int tet4Threads = 256;   // block size of kElemTet4: 64, 128, 256
+    int historyBlocks = 256; // maximum blocks of the reductions over the
+                             // L-BFGS history (Gram row + first loop, second
+                             // loop): 32 .. kReduceBlocks
 };
 void setTuning(const LaunchTuning& t);
 const LaunchTuning& tuning();
+
+// Device layout of the factor's block inverse: supernode s holds its m x w
+// block column-major with leading dimension ldOf(m) = m rounded up to 4
+// (zero rows in the padding) at a multiple-of-4 offset, so that every
+// 16-byte vector of 4 floats or 2 doubles starting at a row r = 0 mod 4 is
+// aligned.
+__host__ __device__ constexpr int ldOf(int m) { return (m + 3) & ~3; }
 
 // element internal force into per-(element, slot) buffers
 struct ElemArgs {
@@ -71,11 +81,11 @@ void launchGather(const GatherArgs& a, hipStream_t s);
 // partials and an arrival counter live in `work` (see kReduceWorkDoubles)
 constexpr int kReduceBlocks = 256;
 constexpr int kMaxCompact = 16;  // history size handled by the compact two-loop
-constexpr int kReduceWorkDoubles = (2 * kMaxCompact + 2) * kReduceBlocks + 64;
-
-// out[0] = a.b, out[1] = max|a|, out[2] = max|c|  (c may be null)
+constexpr int kReduceWorkDoubles = (3 * kMaxCompact + 2) * kReduceBlocks + 64;
+// out[0] = a.b, out[1] = max|a|, out[2] = max|c|  (c may be null); with xt
+// (and c), out[3] = |c - xt|^2
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work,
-                  unsigned* counter, double* out, hipStream_t s);
+                  unsigned* counter, double* out, hipStream_t s, const double* xt = nullptr);
 // *ab = a.b, *bb = b.b
 void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter,
                 double* ab, double* bb, hipStream_t s);
@@ -97,26 +107,29 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
                        double* work, unsigned* counter, hipStream_t s);
 // compact two-loop (same recursion as LBFGSMat::apply_Hv, evaluated from
 // Gram products).  History slots 0..nc-1 of the ring S, Y ([m][n]), newest
-// slot `newest`; G[i*16+k] = s_i . y_k; rho[i] = 1 / (s_i . y_i).
-// Gram update after a new pair at slot loc: G[loc][k], G[k][loc], ys, yy.
-void launchGramUpdate(const double* S, const double* Y, int n, int nc, int loc, double* G, double* ys,
-                      double* yy, double* work, unsigned* counter, hipStream_t s);
-// loop 1: alpha_i = (s_i . q_i) / ys_i newest -> oldest with q_0 = -r, then
-// out = -r - sum alpha_i y_i
-void launchTwoLoopFirst(const double* r, const double* S, const double* Y, int n, int nc, int newest,
-                        int mcap, const double* G, const double* ys, double* alpha, double* out, double* work,
-                        unsigned* counter, hipStream_t s);
+// slot `newest`; G[i*16+k] = s_i . y_k; rho[i] = 1 / (s_i . y_i).  The new
+// pair and loop 1 run in launchGramFirst (declared below).
 // loop 2: beta_i = (y_i . r_i) / ys_i oldest -> newest with r_0 = h (the H0
-// output), then out = h + sum (alpha_i - beta_i) s_i
+// output), then out = h + sum (alpha_i - beta_i) s_i; with dw (and rs_k =
+// s_k . dw from launchGramFirst), also *slope = dw . out, formed as
+// dw . h + sum (alpha_k - beta_k) rs_k
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest,
                          int mcap, const double* G, const double* ys, const double* alpha, double* coef,
-                         double* out, double* work, unsigned* counter, hipStream_t s);
+                         double* out, double* work, unsigned* counter, hipStream_t s, const double* dw = nullptr,
+                         const double* rs = nullptr, double* slope = nullptr);
 // q += (*c) * v   (or q -= when neg)
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s);
 // q /= (*theta)
 void launchDivDev(double* q, const double* theta, int n, hipStream_t s);
 // *theta = yy / ys
 void launchTheta(const double* ys, const double* yy, double* theta, hipStream_t s);
+// the new pair and the first loop in one reduction pass (compact two-loop):
+// S[loc] = x - xp, Y[loc] = r - rp; Gram row / column of loc, ys[loc],
+// *yy = y.y, *theta = yy / ys[loc], rs[k] = s_k . r; alpha newest ->
+// oldest; then out = -r - sum alpha_i y_i
+void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
+                     int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
+                     double* alpha, double* rs, double* out, double* work, unsigned* counter, hipStream_t s);
 
 // ---- supernodal triangular solves (3 right-hand sides) ---------------------
 struct TrsvArgs {

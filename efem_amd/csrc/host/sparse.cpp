@@ -560,6 +560,28 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     return true;
 }
 
+void packBlocks(const SupernodalFactor& F, const std::vector<long long>& off, int align, bool f32, void* dst)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < F.nsuper; ++s) {
+        const int w = F.first[s + 1] - F.first[s], m = F.rowPtr[s + 1] - F.rowPtr[s];
+        const int ld = (m + align - 1) / align * align;
+        for (int k = 0; k < w; ++k) {
+            const double* col = F.B.get() + F.valPtr[s] + static_cast<long long>(k) * m;
+            const long long o = off[s] + static_cast<long long>(k) * ld;
+            if (f32) {
+                float* d = static_cast<float*>(dst) + o;
+                for (int r = 0; r < m; ++r) d[r] = static_cast<float>(col[r]);
+                for (int r = m; r < ld; ++r) d[r] = 0.0f;
+            } else {
+                double* d = static_cast<double*>(dst) + o;
+                std::memcpy(d, col, sizeof(double) * m);
+                for (int r = m; r < ld; ++r) d[r] = 0.0;
+            }
+        }
+    }
+}
+
 void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x)
 {
     const int n = F.n;

@@ -65,6 +65,11 @@ void mergeFactors(std::vector<SupernodalFactor>& parts, SupernodalFactor& out, s
 // backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R
 void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x);
 
+// Copy of B in another layout, by all host threads: supernode s's m x w
+// block to dst + off[s] with leading dimension ld = m rounded up to a
+// multiple of `align` (padding rows zero), as float (f32) or double.
+void packBlocks(const SupernodalFactor& F, const std::vector<long long>& off, int align, bool f32, void* dst);
+
 // multifrontal extend-add map of the forward sweep: front row
 // (rowPtr[s] + r) of supernode s receives the update rows uOff[c] + i of
 // its children c; CSR over all front rows

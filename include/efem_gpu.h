@@ -180,12 +180,15 @@ int efem_set_option(efem_ctx* ctx, const char* name, double value);
 
 /* autotuner (no counterpart in the reference): times the candidate launch
  * shapes on this context's problem and keeps the fastest -- gather lanes per
- * node {1,2,4,8} x C3D4 element block size {64,128,256} (residual time), and
- * the reduction grid {32,64,128,256} blocks (line-search dot time).  The
- * choice only changes summation orders, never the formulation.  report[5]
- * (may be NULL) = {lanes, tet4 threads, reduce blocks, residual ms, dot ms}.
- * The same shapes can be set by efem_set_option("gather_lanes" |
- * "tet4_threads" | "reduce_blocks", v). */
+ * node {1,2,4,8} x C3D4 element block size {64,128,256} (residual time), the
+ * reduction grid {32,64,128,256} blocks (line-search dot time), and the grid
+ * of the reductions over the L-BFGS history {32,64,128,256} blocks (one
+ * compact two-loop pass pair with m = 8; grows the context's history buffers
+ * to m = 8 if smaller).  The choice only changes summation orders, never the
+ * formulation.  report[7] (may be NULL) = {lanes, tet4 threads, reduce
+ * blocks, residual ms, dot ms, history blocks, history ms}.  The same shapes
+ * can be set by efem_set_option("gather_lanes" | "tet4_threads" |
+ * "reduce_blocks" | "history_blocks", v). */
 int efem_autotune(efem_ctx* ctx, double* report);
 
 /* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */

@@ -21,13 +21,29 @@ def test_autotune_keeps_results(which, opts, rng):
         rep = c.autotune()
         print("autotune", rep, flush=True)
         assert rep["gather_lanes"] in (1, 2, 4, 8) and rep["reduce_blocks"] in (32, 64, 128, 256)
+        assert rep["history_blocks"] in (32, 64, 128, 256) and rep["history_ms"] > 0
         r1 = c.residual(u)
         assert rel(r1, om.residual(u)) <= 1e-10 and rel(r1, r0) <= 1e-12
         assert np.array_equal(r1, c.residual(u))
 
 
-@pytest.mark.parametrize("lanes,tpb,blocks", [(1, 64, 32), (2, 128, 64), (8, 256, 256)])
-def test_launch_shapes(lanes, tpb, blocks, rng):
+def test_solve_after_autotune():
+    # the tuner's timing passes use the history buffers and scalar slots as
+    # scratch: a solve afterwards starts from an empty history and reaches
+    # the untuned context's solution (a load the reference converges on)
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    with E.Context(m) as c:
+        ref, st0 = c.solve(epsilon=1e-9, max_iterations=5000)
+    assert st0["rinf"] <= 1e-9
+    with E.Context(m) as c:
+        rep = c.autotune()
+        print("autotune", rep, flush=True)
+        u, st = c.solve(epsilon=1e-9, max_iterations=5000)
+    assert st["rinf"] <= 1e-9 and rel(u, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("lanes,tpb,blocks,hist", [(1, 64, 32, 256), (2, 128, 64, 32), (8, 256, 256, 128)])
+def test_launch_shapes(lanes, tpb, blocks, hist, rng):
     m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
     u = smooth_u(m, rng, 0.05)
     with E.Context(m) as c:
@@ -36,6 +52,7 @@ def test_launch_shapes(lanes, tpb, blocks, rng):
         c.set_option("gather_lanes", lanes)
         c.set_option("tet4_threads", tpb)
         c.set_option("reduce_blocks", blocks)
+        c.set_option("history_blocks", hist)
         assert rel(c.residual(u), r0) <= 1e-12
         uu, st = c.solve(epsilon=1e-9, max_iterations=5000)
         assert st["rinf"] <= 1e-9 and rel(uu, ref) <= 1e-6
@@ -52,6 +69,7 @@ def test_launch_shapes(lanes, tpb, blocks, rng):
 def test_invalid_shapes_refused():
     m = small(8, 0, 0)
     with E.Context(m) as c:
-        for name, v in (("gather_lanes", 3), ("tet4_threads", 96), ("reduce_blocks", 0), ("reduce_blocks", 1000)):
+        for name, v in (("gather_lanes", 3), ("tet4_threads", 96), ("reduce_blocks", 0), ("reduce_blocks", 1000),
+                        ("history_blocks", 0), ("history_blocks", 1000)):
             with pytest.raises(E.EfemError):
                 c.set_option(name, v)
