@@ -48,6 +48,10 @@ struct DeviceFactor {
 
 }  // namespace
 
+// levels with at least this many 4-column backward items use 16-column
+// items (default of the context option "bwd_narrow_items")
+constexpr int kNarrowMinBwdItems = 4096;
+
 struct efem_ctx {
     std::string err;
     efem::Model model;
@@ -104,6 +108,7 @@ struct efem_ctx {
     efem::dev::LaunchTuning tuning;  // efem_autotune / options
     bool useGraph = true;    // option "h0_graph"
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
+    int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
 };
 
 namespace {
@@ -499,6 +504,10 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         for (int l = 0; l < F.nlevels; ++l) {
             d.fwdWide[l] = maxw[l] >= kWideColumns && nf[l] <= kWideMaxFwdItems;
             d.bwdWide[l] = maxm[l] >= kTallRows && nb[l] <= kWideMaxBwdItems;
+            // many 4-column items: 16-column items keep the level's waves
+            // within what the chip holds at once (not in batches, whose
+            // persistent kernel walks 4-column items)
+            if (!snOffset && nb[l] >= c->narrowMinItems) d.bwdWide[l] = efem::dev::kBwdNarrow;
         }
     }
     for (int l = 0; l < F.nlevels; ++l) {
@@ -512,8 +521,9 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     }
     for (int l = 0; l < F.nlevels; ++l) {
         d.bwdOff[l] = static_cast<int>(items.size() / 2);
+        const int cols = d.bwdWide[l] == efem::dev::kBwdNarrow ? 16 : 4;
         for (int s : byLevel[l])
-            for (int k0 = 0; k0 < w[s]; k0 += 4) {
+            for (int k0 = 0; k0 < w[s]; k0 += cols) {
                 items.push_back(make_int4(k0, w[s], mm[s], F.first[s]));
                 items.push_back(desc1(F.rowPtr[s], 0, dval[s]));
             }
@@ -1265,6 +1275,13 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
     }
     if (k == "history_blocks" && iv >= 1 && iv <= efem::dev::kReduceBlocks) {
         c->tuning.historyBlocks = iv;
+        return EFEM_OK;
+    }
+    if (k == "bwd_narrow_items" && value >= 1.0) {
+        // levels with at least this many 4-column backward items use
+        // 16-column items; takes effect at the next factorisation
+        c->narrowMinItems = value >= 2147483647.0 ? 2147483647 : iv;
+        releaseFactor(c);
         return EFEM_OK;
     }
     c->err = "efem_set_option: unknown option or value " + k;

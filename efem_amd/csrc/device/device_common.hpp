@@ -540,7 +540,38 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
     if (k < w) {
         const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * ldOf(m);
-        for (int r = k + part * 64 + lane; r < m; r += 64 * G) {
+        constexpr int step = 64 * G;
+        int r = k + part * 64 + lane;
+        // four rows at a time: their B values and row indices are loaded
+        // before the dependent gathers (same summation order as one by one)
+        for (; r + 3 * step < m; r += 4 * step) {
+            int rr[4];
+            T bk[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                rr[u] = r + u * step;
+                bk[u] = cl[rr[u]];
+            }
+            int gi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) gi[u] = rr[u] < w ? -1 : rows[rr[u]];
+            T sv[4][3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T* src = gi[u] < 0 ? y + 3 * static_cast<size_t>(j0 + rr[u]) : z + 3 * static_cast<size_t>(gi[u]);
+                const T sg = gi[u] < 0 ? T(1) : T(-1);
+                sv[u][0] = sg * src[0];
+                sv[u][1] = sg * src[1];
+                sv[u][2] = sg * src[2];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc0 += bk[u] * sv[u][0];
+                acc1 += bk[u] * sv[u][1];
+                acc2 += bk[u] * sv[u][2];
+            }
+        }
+        for (; r < m; r += step) {
             const T bk = cl[r];
             const T* src = r < w ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
             const T sg = r < w ? T(1) : T(-1);
@@ -581,6 +612,80 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
         zo[0] = s0;
         zo[1] = s1;
         zo[2] = s2;
+    }
+}
+
+// Backward item for levels of many narrow supernodes: up to 16 columns
+// [k0, k0 + 16) of supernode s by a 256-lane block.  The block stages the
+// right-hand side v = [y_s; -z[rows]] of 256 rows at a time in LDS; lane
+// t = 16 kk + p accumulates column k0 + kk over the rows p, p + 16, ... of
+// the chunk (16 independent loads in flight), and the 16 lanes of a column
+// -- adjacent lanes of one wave -- meet in a fixed butterfly.  (4-column
+// items give each wave one column, which multiplies the waves of such a
+// level beyond what the chip holds at once.)  Descriptor as trsvBwdItem
+// with k0 = 0 mod 16.  Contains barriers; every lane must call it.
+template <typename T>
+struct TrsvBwdNarrowLds {
+    T v[256 * 3];
+};
+
+template <typename T>
+__device__ __forceinline__ void trsvBwdNarrowItem(const TrsvArgs& a, int4 d0, int4 d1, TrsvBwdNarrowLds<T>& L)
+{
+    const int k0 = d0.x, w = d0.y, m = d0.z, j0 = d0.w;
+    const int ld = ldOf(m);
+    const int t = threadIdx.x, kk = t >> 4, ph = t & 15;
+    const int k = k0 + kk;
+    const int* rows = a.rows + d1.x;
+    const T* y = static_cast<const T*>(a.y);
+    T* z = static_cast<T*>(a.z);
+    const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(min(k, w - 1)) * ld;
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    // rows below k0 meet only the zero upper triangle of L11^-1
+    for (int c0 = k0; c0 < m; c0 += 256) {
+        const int c1 = min(m, c0 + 256);
+        T bk[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = c0 + ph + 16 * i;
+            bk[i] = (k < w && r < c1) ? cl[r] : T(0);
+        }
+        {
+            const int r = c0 + t;
+            T v0 = T(0), v1 = T(0), v2 = T(0);
+            if (r < c1) {
+                const bool own = r < w;
+                const T* src = own ? y + 3 * static_cast<size_t>(j0 + r) : z + 3 * static_cast<size_t>(rows[r]);
+                const T sg = own ? T(1) : T(-1);
+                v0 = sg * src[0];
+                v1 = sg * src[1];
+                v2 = sg * src[2];
+            }
+            L.v[3 * t] = v0;
+            L.v[3 * t + 1] = v1;
+            L.v[3 * t + 2] = v2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int q = 3 * (ph + 16 * i);
+            acc0 += bk[i] * L.v[q];
+            acc1 += bk[i] * L.v[q + 1];
+            acc2 += bk[i] * L.v[q + 2];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+        acc2 += __shfl_xor(acc2, off, 64);
+    }
+    if (ph == 0 && k < w) {
+        T* zo = z + 3 * static_cast<size_t>(j0 + k);
+        zo[0] = acc0;
+        zo[1] = acc1;
+        zo[2] = acc2;
     }
 }
 

@@ -828,6 +828,13 @@ __global__ __launch_bounds__(64 * WAVES) void kTrsvBwd(TrsvArgs a, const int4* _
     trsvBwdItem<T, WAVES>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], red);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void kTrsvBwdNarrow(TrsvArgs a, const int4* __restrict__ items)
+{
+    __shared__ TrsvBwdNarrowLds<T> L;
+    trsvBwdNarrowItem<T>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L);
+}
+
 // bp[3 j + c] = b[3 perm[j] + c]
 __global__ void kPermute(const int* __restrict__ perm, const double* __restrict__ b, double* __restrict__ bp, int n)
 {
@@ -1069,6 +1076,13 @@ void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, i
 void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s)
 {
     if (nitems <= 0) return;
+    if (wide == kBwdNarrow) {
+        if (a.mixed)
+            kTrsvBwdNarrow<float><<<nitems, 256, 0, s>>>(a, items);
+        else
+            kTrsvBwdNarrow<double><<<nitems, 256, 0, s>>>(a, items);
+        return;
+    }
     if (a.mixed) {
         if (wide) kTrsvBwd<float, 16><<<nitems, 1024, 0, s>>>(a, items);
         else kTrsvBwd<float, 4><<<nitems, 256, 0, s>>>(a, items);

@@ -155,7 +155,9 @@ void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, i
                        hipStream_t s);
 // one level of the backward solve; descriptors {k0, w, m, j0}, {row offset,
 // 0, B offset lo, hi}: four columns per item (wide = 1: four waves per
-// column split its rows, for levels of tall supernodes)
+// column split its rows, for levels of tall supernodes; wide = kBwdNarrow:
+// sixteen columns per item, k0 = 0 mod 16, for levels of many short ones)
+constexpr int kBwdNarrow = 2;
 void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s);
 // out[3 perm[j] + c] = z[j][c]
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s);

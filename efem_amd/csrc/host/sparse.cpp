@@ -356,10 +356,27 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     // (no supernode is wider than kMaxWidth: the device stages one
     // supernode's right-hand sides in a kMaxWidth-wide LDS buffer)
     const int kMaxWidth = kMaxSupernodeWidth;
+    // Every maximal subtree of at most kDenseSubtree columns (contiguous in
+    // postorder) becomes one dense supernode: the bottom of the tree loses
+    // several levels -- each level is one launch per sweep of the device
+    // triangular solves -- for a few percent more stored entries (on the
+    // 265k-DOF beam: 17 -> 12 levels, B +7 %, factor flops +2 %).
+    constexpr int kDenseSubtree = 64;
+    static_assert(kDenseSubtree <= kMaxSupernodeWidth, "dense subtrees fit the device staging width");
+    std::vector<int> sz(n, 1), denseRoot(n, -1);
+    for (int j = 0; j < n; ++j)
+        if (parent[j] != -1) sz[parent[j]] += sz[j];
+    for (int j = n - 1; j >= 0; --j)
+        if (denseRoot[j] < 0 && sz[j] <= kDenseSubtree && (parent[j] == -1 || sz[parent[j]] > kDenseSubtree))
+            for (int i = j - sz[j] + 1; i <= j; ++i) denseRoot[i] = j;
     std::vector<int> fstart;
     for (int j = 0; j < n; ++j) {
-        const bool cont = j > 0 && parent[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1 &&
-                          j - fstart.back() < kMaxWidth;
+        bool cont = j > 0 && parent[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1 &&
+                    j - fstart.back() < kMaxWidth;
+        if (denseRoot[j] >= 0)
+            cont = j > 0 && denseRoot[j - 1] == denseRoot[j];
+        else if (j > 0 && denseRoot[j - 1] >= 0)
+            cont = false;
         if (!cont) fstart.push_back(j);
     }
     fstart.push_back(n);

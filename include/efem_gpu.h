@@ -175,7 +175,13 @@ const char* efem_host_dense_kernel(void);
  *   "two_loop_compact"  1 (default, history m <= 16): the two-loop recursion
  *                       is evaluated from Gram products s_i.y_k in two
  *                       multi-dot passes; 0: one reduction per step exactly
- *                       as LBFGSMat::apply_Hv orders them */
+ *                       as LBFGSMat::apply_Hv orders them
+ *   "bwd_narrow_items"  >= 1 (default 4096): levels of the backward solve with
+ *                       at least this many 4-column work items use
+ *                       16-column items instead; releases the factor (the
+ *                       next solve factors again)
+ *   "gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks":
+ *                       launch shapes (see efem_autotune) */
 int efem_set_option(efem_ctx* ctx, const char* name, double value);
 
 /* autotuner (no counterpart in the reference): times the candidate launch

@@ -194,6 +194,25 @@ def test_solve_without_preconditioner_and_fallback():
     assert rel(u, u_or) <= 1e-8
 
 
+@pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
+def test_apply_h0_narrow_backward_items(prec, tol, rng):
+    # every backward level forced onto 16-column items: same solve as the
+    # default 4-column items (to rounding) and as a direct solve
+    m = E.Mesh.workload(2, [30, 6, 6])
+    K = O.Model(m).K(scalar=False)
+    b = rng.standard_normal(m.ndof)
+    x_ref = spla.spsolve(K.tocsc(), b)
+    with E.Context(m, prec) as ctx:
+        assert ctx.factor()
+        x4 = ctx.apply_h0(b)
+        ctx.set_option("bwd_narrow_items", 1)
+        assert ctx.factor()
+        x16 = ctx.apply_h0(b)
+        assert rel(x16, x_ref) <= tol
+        assert rel(x16, x4) <= (1e-12 if prec == E.FP64 else 1e-5)
+        assert np.array_equal(ctx.apply_h0(b), x16)
+
+
 @pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
 def test_apply_h0_graph_replays_match_eager(prec, rng):
     # the captured graph is replayed several times with eager work between
