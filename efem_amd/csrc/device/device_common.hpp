@@ -442,8 +442,23 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     T* u = static_cast<T*>(a.t);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = r0 + lane;
-    // update rows: gather the children's contributions first, so that the
-    // dependent index loads overlap the GEMV below
+    // GEMV shape: lane = row, wave q takes columns [kb, ke); B is read in
+    // batches of NBT columns, each batch loaded before the previous one is
+    // consumed, and the first batch before the right-hand side is gathered
+    // (its latency overlaps the gather's dependent index loads)
+    constexpr int NBT = 8;
+    const int kc = (w + WAVES - 1) / WAVES;
+    const int kb = wave * kc;
+    int ke = 0;
+    if (r < m) {
+        ke = min(w, kb + kc);
+        if (r < w) ke = min(ke, r + 1);  // L11^-1 is lower triangular
+    }
+    const T* Bc = static_cast<const T*>(a.B) + val + r;
+    T nb[NBT];
+#pragma unroll
+    for (int q = 0; q < NBT; ++q) nb[q] = kb + q < ke ? Bc[static_cast<size_t>(kb + q) * ld] : T(0);
+    // update rows: the children's contributions (wave 0, lane = row)
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     if (wave == 0 && r >= w && r < m) {
         const int p1 = a.fptr[fr0 + r + 1];
@@ -471,26 +486,23 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
         sb[3 * k + 2] = static_cast<T>(v2);
     }
     __syncthreads();
-    const int kc = (w + WAVES - 1) / WAVES;
-    const int kb = wave * kc;
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    if (r < m) {
-        int ke = min(w, kb + kc);
-        if (r < w) ke = min(ke, r + 1);  // L11^-1 is lower triangular
-        const T* Bc = static_cast<const T*>(a.B) + val + r;
-        int k = kb;
-        for (; k + 4 <= ke; k += 4) {
-            const T b0 = Bc[static_cast<size_t>(k) * ld], b1 = Bc[static_cast<size_t>(k + 1) * ld];
-            const T b2 = Bc[static_cast<size_t>(k + 2) * ld], b3 = Bc[static_cast<size_t>(k + 3) * ld];
-            acc0 += b0 * sb[3 * k] + b1 * sb[3 * k + 3] + b2 * sb[3 * k + 6] + b3 * sb[3 * k + 9];
-            acc1 += b0 * sb[3 * k + 1] + b1 * sb[3 * k + 4] + b2 * sb[3 * k + 7] + b3 * sb[3 * k + 10];
-            acc2 += b0 * sb[3 * k + 2] + b1 * sb[3 * k + 5] + b2 * sb[3 * k + 8] + b3 * sb[3 * k + 11];
+    for (int k = kb; k < ke; k += NBT) {
+        T b[NBT];
+#pragma unroll
+        for (int q = 0; q < NBT; ++q) b[q] = nb[q];
+        if (k + NBT < ke) {
+#pragma unroll
+            for (int q = 0; q < NBT; ++q)
+                nb[q] = k + NBT + q < ke ? Bc[static_cast<size_t>(k + NBT + q) * ld] : T(0);
         }
-        for (; k < ke; ++k) {
-            const T bk = Bc[static_cast<size_t>(k) * ld];
-            acc0 += bk * sb[3 * k];
-            acc1 += bk * sb[3 * k + 1];
-            acc2 += bk * sb[3 * k + 2];
+#pragma unroll
+        for (int q = 0; q < NBT; ++q) {
+            if (k + q < ke) {
+                acc0 += b[q] * sb[3 * (k + q)];
+                acc1 += b[q] * sb[3 * (k + q) + 1];
+                acc2 += b[q] * sb[3 * (k + q) + 2];
+            }
         }
     }
     L.red[wave][lane][0] = acc0;

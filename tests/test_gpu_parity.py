@@ -155,8 +155,11 @@ def test_solve_matches_oracle_small_load():
     assert st["status"] == 0 and st["precond_failed"] == 0
     assert st["rinf"] <= 1e-8
     # rounding differences steer a several-hundred-iteration path apart:
-    # counts stay comparable, the converged displacement is the same
-    assert abs(st["iterations"] - st_or["iterations"]) <= max(5, 0.15 * st_or["iterations"])
+    # counts stay comparable, the converged displacement is the same.  The
+    # oracle itself, with its preconditioner output perturbed by a relative
+    # 1e-15 (six seeds), takes 533 to 665 iterations on this problem against
+    # 534 unperturbed, so the bound is that spread, not tighter
+    assert abs(st["iterations"] - st_or["iterations"]) <= max(5, 0.3 * st_or["iterations"])
     assert rel(u, u_or) <= 1e-6
 
 
