@@ -567,9 +567,10 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
         void* h = nullptr;
         if ((rc = pinnedStage(c, static_cast<size_t>(std::max(nb, 1LL)) * es, &h))) return rc;
-        static_assert(efem::dev::ldOf(1) == 4 && efem::dev::ldOf(4) == 4 && efem::dev::ldOf(5) == 8,
-                      "packBlocks pads to multiples of 4");
-        efem::packBlocks(F, dval, 4, fac32(c), h);
+        static_assert(efem::dev::ldOf(1) == efem::dev::kLdAlign && efem::dev::ldOf(efem::dev::kLdAlign + 1) ==
+                          2 * efem::dev::kLdAlign,
+                      "packBlocks pads to multiples of kLdAlign");
+        efem::packBlocks(F, dval, efem::dev::kLdAlign, fac32(c), h);
         double* q = nullptr;
         if ((rc = devAlloc(c, &q, (static_cast<size_t>(std::max(nb, 1LL)) * es + 7) / 8))) return rc;
         HIP_CHECK(c, hipMemcpyAsync(q, h, static_cast<size_t>(nb) * es, hipMemcpyHostToDevice, c->stream));

@@ -553,7 +553,10 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
     if (k < w) {
         const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(k) * ldOf(m);
         constexpr int step = 64 * G;
-        int r = k + part * 64 + lane;
+        // from the line below row k: rows k0 .. k - 1 of the column hold the
+        // stored zeros of L11^-1's upper triangle, so a sweep aligned to 32
+        // rows reads whole lines at no cost to the result
+        int r = (k & ~31) + part * 64 + lane;
         // four rows at a time: their B values and row indices are loaded
         // before the dependent gathers (same summation order as one by one)
         for (; r + 3 * step < m; r += 4 * step) {

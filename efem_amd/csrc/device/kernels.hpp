@@ -20,11 +20,13 @@ void setTuning(const LaunchTuning& t);
 const LaunchTuning& tuning();
 
 // Device layout of the factor's block inverse: supernode s holds its m x w
-// block column-major with leading dimension ldOf(m) = m rounded up to 4
-// (zero rows in the padding) at a multiple-of-4 offset, so that every
-// 16-byte vector of 4 floats or 2 doubles starting at a row r = 0 mod 4 is
-// aligned.
-__host__ __device__ constexpr int ldOf(int m) { return (m + 3) & ~3; }
+// block column-major with leading dimension ldOf(m) = m rounded up to
+// kLdAlign rows (zero rows in the padding) at a multiple-of-kLdAlign offset:
+// every column starts on a 128-byte line in FP32 (256 bytes in FP64), so a
+// forward item's 64-row segment of a column is two whole lines and a
+// backward column sweep starts on a line
+constexpr int kLdAlign = 32;
+__host__ __device__ constexpr int ldOf(int m) { return (m + kLdAlign - 1) / kLdAlign * kLdAlign; }
 
 // element internal force into per-(element, slot) buffers
 struct ElemArgs {
