@@ -23,17 +23,24 @@ namespace {
 struct NdState {
     const Csc* A;
     const std::vector<double>* xyz;
-    std::vector<int> side;  // 0 untouched, 1 left, 2 right (per call)
-    std::vector<int> out;
+    // per node: 0, or 4 (base + 1) + {1 left, 2 right, 3 separator} while
+    // the call whose output starts at `base` is splitting it.  Calls that
+    // run concurrently (sibling subtrees) have different bases, so a node's
+    // mark is never mistaken for another call's.
+    std::vector<int> side;
+    std::vector<int> out;  // the ordering; each call fills its own range
 };
 
-void ndRecurse(NdState& st, std::vector<int>& nodes)
+// Orders `nodes` into out[base, base + nodes.size()): left part, right
+// part, separator.  Subproblems above 4096 nodes run as OpenMP tasks.
+void ndRecurse(NdState& st, std::vector<int>& nodes, int base)
 {
     const int n = static_cast<int>(nodes.size());
     if (n <= 64) {
-        st.out.insert(st.out.end(), nodes.begin(), nodes.end());
+        std::copy(nodes.begin(), nodes.end(), st.out.begin() + base);
         return;
     }
+    const int key = 4 * (base + 1);
     const auto& X = *st.xyz;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int v : nodes)
@@ -49,12 +56,12 @@ void ndRecurse(NdState& st, std::vector<int>& nodes)
         const double xa = X[3 * static_cast<size_t>(a) + ax], xb = X[3 * static_cast<size_t>(b) + ax];
         return xa < xb || (xa == xb && a < b);
     });
-    for (int k = 0; k < n; ++k) st.side[nodes[k]] = k < h ? 1 : 2;
+    for (int k = 0; k < n; ++k) st.side[nodes[k]] = key + (k < h ? 1 : 2);
     // one-sided vertex separators from either side; keep the smaller
     std::vector<int> sepR, sepL;
     for (int k = 0; k < n; ++k) {
         const int v = nodes[k];
-        const int other = k < h ? 2 : 1;
+        const int other = key + (k < h ? 2 : 1);
         bool touch = false;
         for (int p = st.A->p[v]; p < st.A->p[v + 1] && !touch; ++p) touch = st.side[st.A->i[p]] == other;
         if (touch) (k < h ? sepL : sepR).push_back(v);
@@ -65,25 +72,34 @@ void ndRecurse(NdState& st, std::vector<int>& nodes)
     left.reserve(h);
     right.reserve(n - h);
     std::vector<char> inSep;
-    for (int v : sep) st.side[v] = 3;
+    for (int v : sep) st.side[v] = key + 3;
     for (int k = 0; k < n; ++k) {
         const int v = nodes[k];
-        if (st.side[v] == 3) continue;
+        if (st.side[v] == key + 3) continue;
         (k < h ? left : right).push_back(v);
     }
     for (int v : nodes) st.side[v] = 0;
     if (left.empty() || right.empty()) {
         std::sort(nodes.begin(), nodes.end());
-        st.out.insert(st.out.end(), nodes.begin(), nodes.end());
+        std::copy(nodes.begin(), nodes.end(), st.out.begin() + base);
         return;
     }
     std::vector<int> sepCopy = sep;
     nodes.clear();
     nodes.shrink_to_fit();
-    ndRecurse(st, left);
-    ndRecurse(st, right);
+    const int nl = static_cast<int>(left.size()), nr = static_cast<int>(right.size());
     std::sort(sepCopy.begin(), sepCopy.end());
-    st.out.insert(st.out.end(), sepCopy.begin(), sepCopy.end());
+    std::copy(sepCopy.begin(), sepCopy.end(), st.out.begin() + base + nl + nr);
+    if (n > 4096) {
+#pragma omp task default(shared)
+        ndRecurse(st, left, base);
+#pragma omp task default(shared)
+        ndRecurse(st, right, base + nl);
+#pragma omp taskwait
+    } else {
+        ndRecurse(st, left, base);
+        ndRecurse(st, right, base + nl);
+    }
 }
 
 // ---- dense front operations (column-major, leading dimension ld) -----------
@@ -158,10 +174,12 @@ std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz)
     st.A = &A;
     st.xyz = &xyz;
     st.side.assign(A.n, 0);
-    st.out.reserve(A.n);
+    st.out.assign(A.n, -1);
     std::vector<int> nodes(A.n);
     std::iota(nodes.begin(), nodes.end(), 0);
-    ndRecurse(st, nodes);
+#pragma omp parallel
+#pragma omp single
+    ndRecurse(st, nodes, 0);
     return st.out;
 }
 
@@ -219,28 +237,18 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         return false;
     }
     std::vector<int> perm = permIn;
-    std::vector<int> pinv(n), parent(n), Up, Ui;
-    auto buildUpper = [&]() {
-        // pattern of the upper triangle of P A P^T, column-wise
-        for (int k = 0; k < n; ++k) pinv[perm[k]] = k;
-        std::vector<int> cnt(n + 1, 0);
-        for (int j = 0; j < n; ++j)
-            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
-                if (pinv[A.i[p]] <= pinv[j]) cnt[pinv[j] + 1]++;
-        for (int j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
-        Up = cnt;
-        Ui.assign(cnt[n], 0);
-        std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-        for (int j = 0; j < n; ++j)
-            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
-                if (pinv[A.i[p]] <= pinv[j]) Ui[pos[pinv[j]]++] = pinv[A.i[p]];
-    };
+    std::vector<int> pinv(n), parent(n);
+    // elimination tree of P A P^T (Liu's algorithm with path compression)
+    // straight from A's full (symmetric) pattern: column k of P A P^T is
+    // column perm[k] of A renumbered by pinv, its upper entries those < k
+    for (int k = 0; k < n; ++k) pinv[perm[k]] = k;
     auto etree = [&]() {
         std::vector<int> anc(n, -1);
         for (int k = 0; k < n; ++k) {
             parent[k] = -1;
-            for (int p = Up[k]; p < Up[k + 1]; ++p) {
-                int i = Ui[p];
+            const int col = perm[k];
+            for (int p = A.p[col]; p < A.p[col + 1]; ++p) {
+                int i = pinv[A.i[p]];
                 while (i != -1 && i < k) {
                     const int next = anc[i];
                     anc[i] = k;
@@ -253,7 +261,6 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
             }
         }
     };
-    buildUpper();
     etree();
     // postorder so that every subtree is a contiguous column range
     {
@@ -425,36 +432,62 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     std::vector<std::vector<int>> children(ns);
     for (int s = 0; s < ns; ++s)
         if (F.parent[s] >= 0) children[F.parent[s]].push_back(s);
-    // symbolic: row sets of the supernodes
-    F.rowPtr.assign(ns + 1, 0);
-    std::vector<int> mark(n, -1);
+    // symbolic: row sets of the supernodes -- the rows of its columns below
+    // the supernode and its children's update rows, sorted without
+    // duplicates -- children before parents, subtrees of more than 512
+    // columns as OpenMP tasks
+    std::vector<long long> subtreeCols(ns, 0);
     for (int s = 0; s < ns; ++s) {
+        subtreeCols[s] += first[s + 1] - first[s];
+        if (F.parent[s] >= 0) subtreeCols[F.parent[s]] += subtreeCols[s];
+    }
+    std::vector<std::vector<int>> offRows(ns);
+    auto rowSet = [&](int s) {
         const int j0 = first[s], j1 = first[s + 1] - 1;
         std::vector<int> off;
         for (int j = j0; j <= j1; ++j)
-            for (int p = Lp[j]; p < Lp[j + 1]; ++p) {
-                const int i = Li[p];
-                if (i > j1 && mark[i] != s) {
-                    mark[i] = s;
-                    off.push_back(i);
-                }
-            }
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p)
+                if (Li[p] > j1) off.push_back(Li[p]);
+        for (int c : children[s])
+            for (int i : offRows[c])
+                if (i > j1) off.push_back(i);
+        std::sort(off.begin(), off.end());
+        off.erase(std::unique(off.begin(), off.end()), off.end());
+        offRows[s] = std::move(off);
+    };
+    std::function<void(int)> rowsBelow = [&](int s) {
         for (int c : children[s]) {
-            const int m0 = F.rowPtr[c], w = first[c + 1] - first[c];
-            const int mc = F.rowPtr[c + 1] - m0;
-            for (int r = w; r < mc; ++r) {
-                const int i = F.rows[m0 + r];
-                if (i > j1 && mark[i] != s) {
-                    mark[i] = s;
-                    off.push_back(i);
-                }
+            if (subtreeCols[c] > 512) {
+#pragma omp task firstprivate(c) shared(rowsBelow)
+                rowsBelow(c);
+            } else {
+                rowsBelow(c);
             }
         }
-        std::sort(off.begin(), off.end());
-        for (int j = j0; j <= j1; ++j) F.rows.push_back(j);
-        F.rows.insert(F.rows.end(), off.begin(), off.end());
-        F.rowPtr[s + 1] = static_cast<int>(F.rows.size());
+#pragma omp taskwait
+        rowSet(s);
+    };
+#pragma omp parallel
+#pragma omp single
+    {
+        for (int s = 0; s < ns; ++s)
+            if (F.parent[s] < 0) {
+#pragma omp task firstprivate(s) shared(rowsBelow)
+                rowsBelow(s);
+            }
+#pragma omp taskwait
     }
+    F.rowPtr.assign(ns + 1, 0);
+    for (int s = 0; s < ns; ++s)
+        F.rowPtr[s + 1] = F.rowPtr[s] + (first[s + 1] - first[s]) + static_cast<int>(offRows[s].size());
+    F.rows.resize(F.rowPtr[ns]);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int s = 0; s < ns; ++s) {
+        int* r = &F.rows[F.rowPtr[s]];
+        for (int j = first[s]; j < first[s + 1]; ++j) *r++ = j;
+        std::copy(offRows[s].begin(), offRows[s].end(), r);
+    }
+    std::vector<std::vector<int>>().swap(offRows);
     F.valPtr.assign(ns + 1, 0);
     for (int s = 0; s < ns; ++s) {
         const long long w = first[s + 1] - first[s], m = F.rowPtr[s + 1] - F.rowPtr[s];
@@ -470,11 +503,6 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     F.B.reset(new double[static_cast<size_t>(F.valPtr[ns])]);
     F.Bsize = F.valPtr[ns];
     std::vector<std::unique_ptr<double[]>> upd(ns);
-    std::vector<long long> subtreeCols(ns, 0);
-    for (int s = 0; s < ns; ++s) {
-        subtreeCols[s] += first[s + 1] - first[s];
-        if (F.parent[s] >= 0) subtreeCols[F.parent[s]] += subtreeCols[s];
-    }
     std::atomic<int> failed{0};
     const auto tn0 = std::chrono::steady_clock::now();
     // One supernode: the front's first w columns [F11; F21] are assembled,
