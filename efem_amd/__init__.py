@@ -37,6 +37,7 @@ class Problem(C.Structure):
         ("params_per_elem", C.c_int), ("fixed", _up), ("n_faces", C.c_int), ("fnn", C.c_int),
         ("faces", _ip), ("pressure", C.c_double), ("pdir", _dp), ("n_fnodes", C.c_int),
         ("fnodes", _ip), ("fvec", _dp), ("volumes", _dp), ("precision", C.c_int),
+        ("gravity", C.c_int), ("consistent_inertia", C.c_int), ("density", _dp),
     ]
 
 
@@ -105,6 +106,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_set_option": (C.c_int, [vp, C.c_char_p, C.c_double]),
         "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
         "efem_host_dense_kernel": (C.c_char_p, []),
+        "efem_host_constant_force": (C.c_int, [C.POINTER(Problem), _dp, C.c_int]),
         "efem_device_memory": (C.c_int, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "efem_profile_reset": (C.c_int, [vp]),
         "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
@@ -163,11 +165,14 @@ class Mesh:
 
     Fields mirror efem_problem: X [n,3], conn [E,nen], reduced, law,
     params [4] or [E,4], fixed [n] (uint8), faces [F,fnn], pressure,
-    optional pdir [3], fnodes / fvec (constant nodal forces), volumes [E].
+    optional pdir [3], fnodes / fvec (constant nodal forces), volumes [E];
+    gravity (0 none, 1..6 = +x, -x, +y, -y, +z, -z), consistent_inertia and
+    density ([1] or [E], indexed as params).
     """
 
     def __init__(self, X, conn, law, params, fixed=None, faces=None, pressure=0.0, reduced=0,
-                 pdir=None, fnodes=None, fvec=None, volumes=None):
+                 pdir=None, fnodes=None, fvec=None, volumes=None, gravity=0, consistent_inertia=0,
+                 density=None):
         self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
         self.conn = np.ascontiguousarray(conn, dtype=np.int32)
         self.nen = int(self.conn.shape[1])
@@ -186,6 +191,9 @@ class Mesh:
         self.fnodes = None if fnodes is None else np.ascontiguousarray(fnodes, dtype=np.int32)
         self.fvec = None if fvec is None else np.ascontiguousarray(fvec, dtype=np.float64)
         self.volumes = None if volumes is None else np.ascontiguousarray(volumes, dtype=np.float64)
+        self.gravity = int(gravity)
+        self.consistent_inertia = int(consistent_inertia)
+        self.density = None if density is None else np.ascontiguousarray(np.atleast_1d(density), dtype=np.float64)
 
     @property
     def n_nodes(self):
@@ -254,11 +262,14 @@ class Mesh:
             pdir = np.ctypeslib.as_array(pb.pdir, (3,)).copy() if pb.pdir else None
             fnodes = np.ctypeslib.as_array(pb.fnodes, (nfn,)).copy() if nfn else None
             fvec = np.ctypeslib.as_array(pb.fvec, (3,)).copy() if nfn else None
+            gravity, consistent = int(pb.gravity), int(pb.consistent_inertia)
+            density = (np.ctypeslib.as_array(pb.density, (E if ppe else 1,)).copy() if pb.density else None)
             prm = LbfgsParams()
             L.efem_input_lbfgs_params(h, C.byref(prm))
         finally:
             L.efem_input_free(h)
-        mesh = Mesh(X, conn, law, params, fixed, faces, pb.pressure, 0, pdir=pdir, fnodes=fnodes, fvec=fvec)
+        mesh = Mesh(X, conn, law, params, fixed, faces, pb.pressure, 0, pdir=pdir, fnodes=fnodes, fvec=fvec,
+                    gravity=gravity, consistent_inertia=consistent, density=density)
         mesh.skipped_elements = skipped
         settings = {"epsilon": prm.epsilon, "epsilon_rel": prm.epsilon_rel, "max_iterations": prm.max_iterations,
                     "max_linesearch": prm.max_linesearch, "m": prm.m, "lns_fea": prm.lns_fea,
@@ -287,6 +298,9 @@ class Mesh:
         pb.fvec = _ptr(self.fvec, C.c_double)
         pb.volumes = _ptr(self.volumes, C.c_double)
         pb.precision = int(precision)
+        pb.gravity = self.gravity
+        pb.consistent_inertia = self.consistent_inertia
+        pb.density = _ptr(self.density, C.c_double)
         return pb
 
 
@@ -561,6 +575,17 @@ def host_factor_solve(mesh: Mesh, b):
     keys = ("nnz", "supernodes", "levels", "flops", "microseconds", "numeric_us", "laplacian_us",
             "ordering_us")
     return x, dict(zip(keys, (int(v) for v in info)))
+
+
+def host_constant_force(mesh: Mesh) -> np.ndarray:
+    """Host-only: the constant external force on the free DOFs (nodal forces
+    plus gravity M g), as the residual subtracts it."""
+    pb = mesh.problem(FP64)
+    out = np.zeros(3 * mesh.n_free)
+    rc = lib().efem_host_constant_force(C.byref(pb), _ptr(out, C.c_double), len(out))
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_host_constant_force failed ({rc}): {lib().efem_global_error().decode()}")
+    return out
 
 
 def device_memory():

@@ -1191,6 +1191,28 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
 
 const char* efem_host_dense_kernel(void) { return efem::dense::kernelName(); }
 
+int efem_host_constant_force(const efem_problem* pb, double* out, int n)
+{
+    if (!pb || !out) return EFEM_ERR_ARG;
+    efem::Model m;
+    std::string msg;
+    try {
+        if (!efem::buildModel(*pb, m, msg)) {
+            g_error = msg;
+            return EFEM_ERR_ARG;
+        }
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return EFEM_ERR_ARG;
+    }
+    if (n != m.ndof()) {
+        g_error = "efem_host_constant_force: n must be 3 x the free nodes (" + std::to_string(m.ndof()) + ")";
+        return EFEM_ERR_ARG;
+    }
+    std::copy(m.fext.begin(), m.fext.end(), out);
+    return EFEM_OK;
+}
+
 int efem_device_memory(long long* free_bytes, long long* total_bytes)
 {
     if (!free_bytes || !total_bytes) return EFEM_ERR_ARG;

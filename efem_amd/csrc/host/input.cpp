@@ -249,6 +249,7 @@ std::vector<std::string> dataLines(const std::string& text)
 struct Material {
     int law = -1;
     double p[4] = {0, 0, 0, 0};
+    double density = 0.0;  // NeoHookean.cpp:27-28 (MooneyRivlin.cpp, Yeoh.cpp alike), default 0
 };
 
 double num(const JVal* v, double d) { return v && v->kind == JVal::Num ? v->num : d; }
@@ -277,10 +278,14 @@ bool loadInput(const std::string& jsonPath, InputDeck& d, std::string& msg)
         d.maxLineSearch = static_cast<int>(num(an->get("max_line_search"), d.maxLineSearch));
         d.dxCriterion = num(an->get("dx_criterion"), d.dxCriterion);
         d.residualCriterion = num(an->get("residual_criterion"), d.residualCriterion);
-        if (const JVal* g = an->get("gravity"); g && g->kind == JVal::Str && g->str != "non") {
-            msg = "analysis.gravity = \"" + g->str + "\": gravity is not supported";
-            return false;
+        // gravity strings of Config.cpp:45-61 (other strings: unchanged)
+        if (const JVal* g = an->get("gravity"); g && g->kind == JVal::Str) {
+            static const char* names[7] = {"non", "x", "-x", "y", "-y", "z", "-z"};
+            for (int k = 0; k < 7; ++k)
+                if (g->str == names[k]) d.gravity = k;
         }
+        if (const JVal* ci = an->get("consistent_inertia"); ci && ci->kind == JVal::Bool)
+            d.consistentInertia = ci->b ? 1 : 0;
     }
     // model: one body
     const JVal* models = root.get("model");
@@ -365,6 +370,7 @@ bool loadInput(const std::string& jsonPath, InputDeck& d, std::string& msg)
                 continue;  // unsupported type: ignored as the reference does
             }
             mm.p[3] = num(m.get("D1"), 0);
+            mm.density = num(m.get("density"), 0);
             mats[id] = mm;
         }
     }
@@ -373,6 +379,7 @@ bool loadInput(const std::string& jsonPath, InputDeck& d, std::string& msg)
     // defined get no element constraint
     std::vector<int> elemLaw;
     std::vector<std::array<double, 4>> elemP;
+    std::vector<double> elemRho;
     for (const auto& row : rawElems) {
         const int n = static_cast<int>(row.size());
         const int nen = n <= 5 ? 4 : (n <= 9 ? 8 : (n <= 11 ? 10 : (n <= 21 ? 20 : 0)));
@@ -401,6 +408,7 @@ bool loadInput(const std::string& jsonPath, InputDeck& d, std::string& msg)
         }
         elemLaw.push_back(it->second.law);
         elemP.push_back({it->second.p[0], it->second.p[1], it->second.p[2], it->second.p[3]});
+        elemRho.push_back(it->second.density);
     }
     const int E = static_cast<int>(elemLaw.size());
     if (E == 0) {
@@ -421,14 +429,20 @@ bool loadInput(const std::string& jsonPath, InputDeck& d, std::string& msg)
     }
     d.law = hasMR ? EFEM_MOONEY_RIVLIN : (hasY ? EFEM_YEOH : EFEM_NEOHOOKEAN);
     (void)hasNH;
+    // parameters and densities share one indexing: shared if every element
+    // has the same ones, else per element
     bool uniform = true;
-    for (int e = 1; e < E; ++e)
+    for (int e = 1; e < E; ++e) {
         for (int k = 0; k < 4; ++k) uniform &= elemP[e][k] == elemP[0][k];
+        uniform &= elemRho[e] == elemRho[0];
+    }
     d.paramsPerElem = uniform ? 0 : 1;
     if (uniform) {
         d.params.assign(elemP[0].begin(), elemP[0].end());
+        d.density.assign(1, elemRho[0]);
     } else {
         for (const auto& p : elemP) d.params.insert(d.params.end(), p.begin(), p.end());
+        d.density = elemRho;
     }
     // constraints
     d.fixed.assign(N, 0);
@@ -606,6 +620,9 @@ extern "C" int efem_input_problem(const efem_input* h, efem_problem* pb, int pre
     pb->fvec = d.fnodes.empty() ? nullptr : d.fvec;
     pb->volumes = nullptr;
     pb->precision = precision;
+    pb->gravity = d.gravity;
+    pb->consistent_inertia = d.consistentInertia;
+    pb->density = d.density.empty() ? nullptr : d.density.data();
     return EFEM_OK;
 }
 

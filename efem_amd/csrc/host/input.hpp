@@ -38,6 +38,12 @@ struct InputDeck {
     int maxLineSearch = 20;
     double dxCriterion = 1e-8;
     double residualCriterion = 1e-6;
+    // gravity (analysis.gravity "non" / "x" / "-x" / ... -> 0 .. 6) and the
+    // mass matrix it uses (analysis.consistent_inertia); material densities
+    // indexed as params
+    int gravity = 0;
+    int consistentInertia = 0;
+    std::vector<double> density;
 };
 
 // false with msg on any error

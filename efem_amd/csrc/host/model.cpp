@@ -320,6 +320,52 @@ bool buildModel(const efem_problem& pb, Model& m, std::string& msg)
             if (m.fid[n] >= 0)
                 for (int d = 0; d < 3; ++d) m.fext[3 * static_cast<size_t>(m.fid[n]) + d] += pb.fvec[d];
         }
+    // gravity: -M g over the free DOFs becomes the constant force M g
+    // (InertiaConstraint::conEvaluteResidual, InertiaConstraint.cpp:34-49;
+    // masses of ElementConstraint::conInitial, ElementConstraint.cpp:85-137:
+    // consistent M_ab = V rho sum_q w_q N_a N_b, lumped max(V rho, 4e-12) /
+    // nen on the diagonal; only free-free pairs carry mass)
+    if (pb.gravity < 0 || pb.gravity > 6) {
+        msg = "efem_problem: gravity must be 0 (none) or 1..6 (+x, -x, +y, -y, +z, -z)";
+        return false;
+    }
+    if (pb.gravity != 0 && pb.density) {
+        double g[3] = {0.0, 0.0, 0.0};
+        g[(pb.gravity - 1) / 2] = pb.gravity % 2 == 1 ? 9.8 : -9.8;
+        std::vector<double> Nq(static_cast<size_t>(ng) * nen);
+        for (int q = 0; q < ng; ++q) {
+            double xi[3];
+            el->gaussPointLocalPosition(q, xi);
+            el->pointShapeFunc(xi, &Nq[static_cast<size_t>(q) * nen]);
+        }
+        for (int e = 0; e < E; ++e) {
+            const double rho = pb.density[pb.params_per_elem ? e : 0];
+            const int* ce = &m.conn[static_cast<size_t>(e) * nen];
+            if (pb.consistent_inertia) {
+                for (int q = 0; q < ng; ++q) {
+                    const double* N = &Nq[static_cast<size_t>(q) * nen];
+                    double nFree = 0.0;  // sum of N_b over the free nodes
+                    for (int b = 0; b < nen; ++b)
+                        if (m.fid[ce[b]] >= 0) nFree += N[b];
+                    const double wq = m.vol[e] * rho * m.gw[q];
+                    for (int a = 0; a < nen; ++a) {
+                        const int f = m.fid[ce[a]];
+                        if (f < 0) continue;
+                        for (int d = 0; d < 3; ++d) m.fext[3 * static_cast<size_t>(f) + d] += wq * N[a] * nFree * g[d];
+                    }
+                }
+            } else {
+                double mass = m.vol[e] * rho;
+                if (mass < 4e-12) mass = 4e-12;
+                mass /= nen;
+                for (int a = 0; a < nen; ++a) {
+                    const int f = m.fid[ce[a]];
+                    if (f < 0) continue;
+                    for (int d = 0; d < 3; ++d) m.fext[3 * static_cast<size_t>(f) + d] += mass * g[d];
+                }
+            }
+        }
+    }
     // gather CSR: free node -> element slots, then face slots, ascending
     m.gptr.assign(m.nFree + 1, 0);
     for (int e = 0; e < E; ++e)

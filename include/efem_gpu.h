@@ -83,6 +83,14 @@ typedef struct efem_problem {
     const double* fvec;         /* [3] */
     const double* volumes;      /* optional [n_elems] element volumes; NULL: computed */
     int precision;              /* EFEM_FP64, EFEM_MIXED or a mask of EFEM_F32_* */
+    /* gravity (InertiaConstraint.cpp:34-84, analysis.gravity of
+     * Config.cpp:45-65): 0 none, 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z, of
+     * magnitude 9.8; the residual gains -M g over the free DOFs, M the
+     * consistent (consistent_inertia = 1) or lumped mass matrix
+     * (ElementConstraint.cpp:85-137) built with the elements' densities */
+    int gravity;
+    int consistent_inertia;
+    const double* density;      /* [1] or [n_elems] (indexed as params); NULL: 0 */
 } efem_problem;
 
 /* L-BFGS parameters (src/simulation/nlsolver/Param.h:92-103 defaults) */
@@ -163,6 +171,11 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
  * bytes; lets a caller check that refactorisations replace, not add to, the
  * factor's device buffers */
 int efem_device_memory(long long* free_bytes, long long* total_bytes);
+
+/* host-only diagnostic (no GPU): the constant part of the external force on
+ * the free DOFs as the residual subtracts it -- nodal forces
+ * (ExternalForceConstraint) plus gravity M g -- into out[3 * n_free] */
+int efem_host_constant_force(const efem_problem* pb, double* out, int n);
 
 /* host-only diagnostic: the dense micro-kernel the host factorisation uses
  * ("avx512 16x8", "avx2 8x6" or "portable 8x6"; the environment variable

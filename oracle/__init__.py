@@ -25,7 +25,8 @@ class OrProblem(C.Structure):
         ("reduced", C.c_int), ("law", C.c_int), ("params", _dp), ("params_per_elem", C.c_int),
         ("fixed", _up), ("n_faces", C.c_int), ("fnn", C.c_int), ("faces", _ip),
         ("pressure", C.c_double), ("pdir", _dp), ("n_fnodes", C.c_int), ("fnodes", _ip),
-        ("fvec", _dp), ("volumes", _dp),
+        ("fvec", _dp), ("volumes", _dp), ("precision", C.c_int), ("gravity", C.c_int),
+        ("consistent_inertia", C.c_int), ("density", _dp),
     ]
 
 
@@ -183,6 +184,12 @@ class Model:
         pb.fnodes = _p(mesh.fnodes, C.c_int)
         pb.fvec = _p(mesh.fvec)
         pb.volumes = _p(mesh.volumes)
+        # gravity (InertiaConstraint): absent on meshes that do not carry it
+        pb.gravity = int(getattr(mesh, "gravity", 0))
+        pb.consistent_inertia = int(getattr(mesh, "consistent_inertia", 0))
+        density = getattr(mesh, "density", None)
+        self._density = None if density is None else np.ascontiguousarray(np.atleast_1d(density), dtype=np.float64)
+        pb.density = _p(self._density)
         self._pb = pb
         err = C.create_string_buffer(256)
         self._h = lib().or_model_create(C.byref(pb), err, 256)

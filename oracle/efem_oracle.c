@@ -401,6 +401,7 @@ struct or_model {
     int n_fnodes;
     int* fnodes;
     double fvec[3];
+    double* grav;     /* [ndof] gravity residual -M g, or NULL */
     or_chol* chol;
     int chol_scalar;
 };
@@ -611,6 +612,54 @@ static int build_face(or_model* m, const int* fn, int fnn, or_face* F, char* err
 /* Constraint::Constraint (Constraint.cpp:22-91), IntegralElement::initial
  * (IntegralElement.cpp:3-19), ElementConstraint::ElementConstraint
  * (ElementConstraint.cpp:69-83) */
+/* Gravity as the reference assembles it: per element a consistent mass
+ * matrix M_e = V sum_q w_q rho N^T N (3x3 identity blocks) and a lumped one
+ * (max(V rho, 4e-12) / nen on the diagonal), ElementConstraint::conInitial
+ * (ElementConstraint.cpp:85-137), kept for free-free pairs only; the
+ * gravity vector g repeated on every free node (InertiaConstraint::
+ * buildGravity, InertiaConstraint.cpp:60-84); residual -M g
+ * (conEvaluteResidual, :34-49). */
+static void build_gravity(or_model* m, const or_problem* pb)
+{
+    const int nen = m->nen, ng = m->ng;
+    double g[3] = { 0, 0, 0 };
+    g[(pb->gravity - 1) / 2] = (pb->gravity % 2 == 1) ? 9.8 : -9.8;
+    m->grav = (double*)calloc((size_t)m->ndof, sizeof(double));
+    double Ng[27][20];
+    for (int q = 0; q < ng; q++) {
+        double xi[3], w, dN[60];
+        or_gauss(nen, pb->reduced, q, xi, &w);
+        or_shape(nen, xi, Ng[q], dN);
+    }
+    double M[20][20];
+    for (int e = 0; e < m->n_elems; e++) {
+        const double rho = pb->density[pb->params_per_elem ? e : 0];
+        const int* c = m->conn + (size_t)e * nen;
+        if (pb->consistent_inertia) {
+            for (int a = 0; a < nen; a++)
+                for (int b = 0; b < nen; b++) {
+                    double s = 0.0;
+                    for (int q = 0; q < ng; q++) s += m->gw[q] * rho * Ng[q][a] * Ng[q][b];
+                    M[a][b] = s * m->vol[e];
+                }
+        } else {
+            double mass = m->vol[e] * rho;
+            if (mass < 4e-12) mass = 4e-12;
+            mass /= nen;
+            for (int a = 0; a < nen; a++)
+                for (int b = 0; b < nen; b++) M[a][b] = a == b ? mass : 0.0;
+        }
+        for (int a = 0; a < nen; a++) {
+            const int ia = m->fid[c[a]];
+            if (ia < 0) continue;
+            for (int b = 0; b < nen; b++) {
+                if (m->fid[c[b]] < 0) continue;
+                for (int d = 0; d < 3; d++) m->grav[3 * (size_t)ia + d] -= M[a][b] * g[d];
+            }
+        }
+    }
+}
+
 or_model* or_model_create(const or_problem* pb, char* err, int errlen)
 {
     int nen = pb->nen;
@@ -704,6 +753,12 @@ or_model* or_model_create(const or_problem* pb, char* err, int errlen)
         memcpy(m->fnodes, pb->fnodes, sizeof(int) * m->n_fnodes);
         memcpy(m->fvec, pb->fvec, 3 * sizeof(double));
     }
+    if (pb->gravity < 0 || pb->gravity > 6) {
+        set_err(err, errlen, "gravity must be 0..6");
+        or_model_destroy(m);
+        return NULL;
+    }
+    if (pb->gravity > 0 && pb->density) build_gravity(m, pb);
     return m;
 }
 
@@ -735,6 +790,7 @@ void or_model_destroy(or_model* m)
     free(m->T);
     free(m->faces);
     free(m->fnodes);
+    free(m->grav);
     chol_free(m->chol);
     free(m);
 }
@@ -844,10 +900,13 @@ void or_external_force(or_model* m, const double* u, double* r)
             for (int d = 0; d < 3; d++)
                 r[3 * (size_t)id + d] -= m->fvec[d];
     }
+    if (m->grav)
+        for (int i = 0; i < m->ndof; i++) r[i] += m->grav[i];
 }
 
 /* SoftObjective::residual -> Constraint::getResidual (Constraint.cpp:115-126):
- * r = f_int(u) - f_ext(u) on the free DOFs (gravity off) */
+ * r = f_int(u) - f_ext(u) on the free DOFs (f_ext with gravity M g when the
+ * problem has it) */
 void or_residual(or_model* m, const double* u, double* r)
 {
     memset(r, 0, sizeof(double) * (size_t)m->ndof);

@@ -53,6 +53,10 @@ typedef struct {
     const int* fnodes;          /* nodes carrying a constant nodal force */
     const double* fvec;         /* [3] */
     const double* volumes;      /* optional per-element volumes (NULL: computed) */
+    int precision;              /* unused here (keeps the layout of efem_problem) */
+    int gravity;                /* 0 none, 1..6: +x -x +y -y +z -z (9.8) */
+    int consistent_inertia;     /* 1: consistent mass matrix, 0: lumped */
+    const double* density;      /* [1] or [n_elems] as params; NULL: 0 */
 } or_problem;
 
 typedef struct or_model or_model;

@@ -146,15 +146,13 @@ def test_material_column_and_promotion(tmp_path):
     assert m2.n_elems == 1 and m2.skipped_elements == 1
 
 
-@pytest.mark.parametrize("what", ["mr_yeoh", "gravity", "missing_mesh", "bad_json", "bad_node"])
+@pytest.mark.parametrize("what", ["mr_yeoh", "missing_mesh", "bad_json", "bad_node"])
 def test_refused_inputs(what, tmp_path):
     d = tmp_path / what
     if what == "mr_yeoh":
         mesh = TWO_TETS.replace("1, 2, 3, 4", "1, 2, 3, 4, 1").replace("2 3 4 5", "2 3 4 5 2")
         jp = write_deck(d, mesh, materials=[{"id": 1, "type": "mo_riv", "C10": 1, "C01": 1, "D1": 1},
                                             {"id": 2, "type": "yeoh", "C10": 1, "C20": 1, "C30": 1, "D1": 1}])
-    elif what == "gravity":
-        jp = write_deck(d, TWO_TETS, analysis={"gravity": "-z"})
     elif what == "missing_mesh":
         jp = write_deck(d, TWO_TETS)
         (d / "mesh.txt").unlink()
