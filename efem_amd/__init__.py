@@ -108,6 +108,10 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_host_dense_kernel": (C.c_char_p, []),
         "efem_host_constant_force": (C.c_int, [C.POINTER(Problem), _dp, C.c_int]),
         "efem_device_memory": (C.c_int, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "efem_commit": (C.c_int, [vp, _dp]),
+        "efem_committed": (C.c_int, [vp, _dp]),
+        "efem_write_vtk": (C.c_int, [C.POINTER(Problem), _dp, C.c_char_p]),
+        "efem_results_path": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
         "efem_profile_reset": (C.c_int, [vp]),
         "efem_profile_read": (C.c_int, [vp, _dp, _ip]),
         "efem_tangent": (C.c_int, [vp, _dp, _dp, _dp, C.c_int]),
@@ -433,6 +437,23 @@ class Context:
     def factor_release(self):
         self._check(self._lib.efem_factor_release(self._h), "efem_factor_release")
 
+    def commit(self, u):
+        """Constraint::updateDx: u becomes the committed displacement the
+        next solve's increment (and its relative criterion) is measured
+        from; None clears it."""
+        if u is None:
+            self._check(self._lib.efem_commit(self._h, None), "efem_commit")
+            return
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != (self.ndof,):
+            raise ValueError(f"u must have {self.ndof} entries")
+        self._check(self._lib.efem_commit(self._h, _ptr(u, C.c_double)), "efem_commit")
+
+    def committed(self) -> np.ndarray:
+        out = np.empty(self.ndof)
+        self._check(self._lib.efem_committed(self._h, _ptr(out, C.c_double)), "efem_committed")
+        return out
+
     # ---- differentiation -------------------------------------------------
     def tangent(self, u, v, transpose=False) -> np.ndarray:
         """K_t(u) v (or K_t^T v): the true tangent d r / d u applied to v."""
@@ -586,6 +607,69 @@ def host_constant_force(mesh: Mesh) -> np.ndarray:
     if rc != EFEM_OK:
         raise EfemError(f"efem_host_constant_force failed ({rc}): {lib().efem_global_error().decode()}")
     return out
+
+
+def write_vtk(mesh: Mesh, u, path) -> None:
+    """Model::saveVTK's grid at X + u (u over the free DOFs, None for the
+    undeformed frame) as an ASCII legacy VTK file."""
+    pb = mesh.problem(FP64)
+    uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+    if uu is not None and uu.shape != (3 * mesh.n_free,):
+        raise ValueError(f"u must have {3 * mesh.n_free} entries")
+    rc = lib().efem_write_vtk(C.byref(pb), _ptr(uu, C.c_double), str(path).encode())
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_write_vtk failed ({rc}): {lib().efem_global_error().decode()}")
+
+
+def results_path(working_dir, name: str) -> Path:
+    """<working_dir>/results/<date>/<date-time-ms>-<name>.vtk (Model.cpp:93-110),
+    directories created."""
+    buf = C.create_string_buffer(4096)
+    rc = lib().efem_results_path(str(working_dir).encode(), name.encode(), buf, len(buf))
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_results_path failed ({rc}): {lib().efem_global_error().decode()}")
+    return Path(buf.value.decode())
+
+
+def simulate(json_path, precision: int = FP64, save_frame=None):
+    """Simulator::initial + Simulator::start (src/simulation/Simulator.cpp:6-40)
+    for one run directory: the undeformed frame, one static solve from
+    dx = 0 (Integrator::integral), the commit of the result
+    (Constraint::updateDx) and the deformed frame.  save_frame defaults to
+    the deck's output.save_frame (true when absent, Config.h:43-44).
+    Returns the displacement, the solve statistics with "converged"
+    (||dx||_inf < dx_criterion, Integrator.cpp:36) and the frames written."""
+    import json as _json
+
+    json_path = Path(json_path)
+    cfg = _json.loads(json_path.read_text())
+    out_cfg = cfg.get("output", {}) if isinstance(cfg.get("output", {}), dict) else {}
+    if save_frame is None:
+        save_frame = out_cfg.get("save_frame", True) if isinstance(out_cfg.get("save_frame"), bool) else True
+    an = cfg.get("analysis", {}) if isinstance(cfg.get("analysis", {}), dict) else {}
+    dx_criterion = an.get("dx_criterion", 1e-8)
+    if isinstance(dx_criterion, bool) or not isinstance(dx_criterion, (int, float)):
+        dx_criterion = 1e-8
+    mesh, settings = Mesh.from_input(json_path)
+    # IO::IO: the working directory is the config path up to its last '/'
+    wd = str(json_path)[: str(json_path).rfind("/") + 1]
+    frames = []
+    if save_frame:
+        p = results_path(wd, "undeformed")
+        write_vtk(mesh, None, p)
+        frames.append(p)
+    with Context(mesh, precision) as ctx:
+        u, st = ctx.solve(None, **settings)
+        if st["status"] != EFEM_OK:
+            # Integrator.cpp:28-34: a non-finite result is dropped
+            u = np.zeros_like(u)
+        ctx.commit(u)
+    st["converged"] = bool(st["status"] == EFEM_OK and st["xinf"] < dx_criterion)
+    if save_frame:
+        p = results_path(wd, "deformed")
+        write_vtk(mesh, u, p)
+        frames.append(p)
+    return u, st, frames
 
 
 def device_memory():

@@ -38,6 +38,7 @@ SOURCES = [
     ("host/sparse.cpp", False),
     ("host/dense.cpp", False),
     ("host/input.cpp", False),
+    ("host/vtk.cpp", False),
 ]
 
 

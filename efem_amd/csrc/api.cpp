@@ -18,6 +18,7 @@
 #include "host/model.hpp"
 #include "host/sparse.hpp"
 #include "host/dense.hpp"
+#include "host/vtk.hpp"
 
 namespace {
 
@@ -101,6 +102,9 @@ struct efem_ctx {
     std::vector<hipEvent_t> evs, evsH0;
     int evUsed = 0, evH0Used = 0;
     double* xtarget = nullptr;
+    // committed configuration (efem_commit): device copy, or null
+    double* base = nullptr;
+    bool hasBase = false;
     // tangent / Krylov / adjoint work vectors (n + 3 each, allocated on first use)
     double *ku = nullptr, *kb = nullptr, *kx = nullptr, *kr = nullptr, *krh = nullptr, *kp = nullptr,
            *kv = nullptr, *ks = nullptr, *kt = nullptr, *kph = nullptr, *ksh = nullptr, *kxo = nullptr,
@@ -962,6 +966,69 @@ int efem_factor_release(efem_ctx* c)
     return EFEM_OK;
 }
 
+// Constraint::updateDx (src/fe/Constraint.cpp:103-113): the solved
+// displacement becomes the configuration the next solve's increment is
+// measured from
+int efem_commit(efem_ctx* c, const double* u)
+{
+    if (!c) return EFEM_ERR_ARG;
+    if (!c->parts.empty()) {
+        c->err = "efem_commit: batch context";
+        return EFEM_ERR_STATE;
+    }
+    if (!u) {
+        c->hasBase = false;
+        return EFEM_OK;
+    }
+    const size_t n = static_cast<size_t>(c->model.ndof());
+    int rc;
+    if (!c->base && (rc = devAlloc(c, &c->base, n + 3))) return rc;
+    HIP_CHECK(c, hipMemcpyAsync(c->base, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->hasBase = true;
+    return EFEM_OK;
+}
+
+int efem_committed(efem_ctx* c, double* u)
+{
+    if (!c || !u) return EFEM_ERR_ARG;
+    const size_t n = static_cast<size_t>(c->model.ndof());
+    if (!c->hasBase) {
+        std::fill(u, u + n, 0.0);
+        return EFEM_OK;
+    }
+    HIP_CHECK(c, hipMemcpyAsync(u, c->base, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
+}
+
+int efem_write_vtk(const efem_problem* pb, const double* u, const char* path)
+{
+    if (!pb || !path) return EFEM_ERR_ARG;
+    std::string msg;
+    if (!efem::writeVtk(*pb, u, path, msg)) {
+        g_error = msg;
+        return EFEM_ERR_ARG;
+    }
+    return EFEM_OK;
+}
+
+int efem_results_path(const char* working_dir, const char* name, char* out, int cap)
+{
+    if (!working_dir || !name || !out || cap <= 0) return EFEM_ERR_ARG;
+    std::string path, msg;
+    if (!efem::resultsPath(working_dir, name, path, msg)) {
+        g_error = msg;
+        return EFEM_ERR_ARG;
+    }
+    if (static_cast<int>(path.size()) >= cap) {
+        g_error = "efem_results_path: buffer too small (" + std::to_string(path.size() + 1) + " bytes needed)";
+        return EFEM_ERR_ARG;
+    }
+    std::memcpy(out, path.c_str(), path.size() + 1);
+    return EFEM_OK;
+}
+
 int efem_apply_h0(efem_ctx* c, const double* v, double* out)
 {
     if (!c || !v || !out) return EFEM_ERR_ARG;
@@ -1017,7 +1084,16 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     HIP_CHECK(c, hipMemcpyAsync(c->x, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = residualInternal(c, c->x, c->r))) return rc;
     s.r_evals = 1;
-    efem::dev::launchDotMax(c->x, c->x, c->r, n, c->work, c->counter, sc + kOut, c->stream);
+    // the relative criterion measures the solution from the committed
+    // configuration: the reference's solver variable is the increment dx
+    // on top of node->dx (Constraint.cpp:93-113)
+    const double* base = c->hasBase ? c->base : nullptr;
+    if (base) {
+        efem::dev::launchAxpby(c->tmp, 1.0, c->x, -1.0, base, n, c->stream);
+        efem::dev::launchDotMax(c->tmp, c->tmp, c->r, n, c->work, c->counter, sc + kOut, c->stream);
+    } else {
+        efem::dev::launchDotMax(c->x, c->x, c->r, n, c->work, c->counter, sc + kOut, c->stream);
+    }
     if ((rc = readScalars(c, 6))) return rc;
     double rinf = c->hostScal[2], xinf = c->hostScal[1];
     const double xnorm2 = std::sqrt(c->hostScal[0]);
@@ -1068,7 +1144,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             ++trials;
             used = cur;
             efem::dev::launchDotMax(c->r, c->d, c->x, n, c->work, c->counter, sc + kOut, c->stream,
-                                    prm->x_target ? c->xtarget : nullptr);
+                                    prm->x_target ? c->xtarget : nullptr, base);
             if ((rc = readScalars(c, 5))) return rc;
             const double dg = c->hostScal[0];
             rinf = c->hostScal[1];

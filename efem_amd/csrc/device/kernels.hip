@@ -383,11 +383,13 @@ int reduceGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().reduceBloc
 // times the bytes of a line-search dot, so their grid is tuned on its own
 int historyGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().historyBlocks), (n + kRT - 1) / kRT)); }
 
-// out = {a.b, max|a|, max|c|} (HASC) and out[3] = |c - xt|^2 (HASXT)
+// out = {a.b, max|a|, max|c - cb|} (HASC; cb may be null) and
+// out[3] = |c - xt|^2 (HASXT)
 template <bool HASC, bool HASXT>
 __global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
-                                               const double* __restrict__ c, const double* __restrict__ xt,
-                                               int n, double* work, unsigned* counter, double* out)
+                                               const double* __restrict__ c, const double* __restrict__ cb,
+                                               const double* __restrict__ xt, int n, double* work,
+                                               unsigned* counter, double* out)
 {
     constexpr int K = HASXT ? 4 : 3;
     __shared__ double sh[K][kRW];
@@ -401,7 +403,7 @@ __global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, con
         v[1] = fmax(v[1], fabs(ai));
         if (HASC) {
             const double ci = c[i];
-            v[2] = fmax(v[2], fabs(ci));
+            v[2] = fmax(v[2], fabs(cb ? ci - cb[i] : ci));
             if (HASXT) {
                 const double e = ci - xt[i];
                 v[K - 1] += e * e;
@@ -962,14 +964,14 @@ void launchGather(const GatherArgs& a, hipStream_t s)
 }
 
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, unsigned* counter,
-                  double* out, hipStream_t s, const double* xt)
+                  double* out, hipStream_t s, const double* xt, const double* cb)
 {
     if (!c)
-        kDotMax<false, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, nullptr, n, work, counter, out);
+        kDotMax<false, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, nullptr, nullptr, n, work, counter, out);
     else if (!xt)
-        kDotMax<true, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, nullptr, n, work, counter, out);
+        kDotMax<true, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, cb, nullptr, n, work, counter, out);
     else
-        kDotMax<true, true><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, xt, n, work, counter, out);
+        kDotMax<true, true><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, cb, xt, n, work, counter, out);
 }
 
 void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter, double* ab, double* bb,

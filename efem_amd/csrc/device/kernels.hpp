@@ -84,10 +84,11 @@ void launchGather(const GatherArgs& a, hipStream_t s);
 constexpr int kReduceBlocks = 256;
 constexpr int kMaxCompact = 16;  // history size handled by the compact two-loop
 constexpr int kReduceWorkDoubles = (3 * kMaxCompact + 2) * kReduceBlocks + 64;
-// out[0] = a.b, out[1] = max|a|, out[2] = max|c|  (c may be null); with xt
-// (and c), out[3] = |c - xt|^2
+// out[0] = a.b, out[1] = max|a|, out[2] = max|c - cb| (c may be null, cb
+// null means 0); with xt (and c), out[3] = |c - xt|^2
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work,
-                  unsigned* counter, double* out, hipStream_t s, const double* xt = nullptr);
+                  unsigned* counter, double* out, hipStream_t s, const double* xt = nullptr,
+                  const double* cb = nullptr);
 // *ab = a.b, *bb = b.b
 void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter,
                 double* ab, double* bb, hipStream_t s);

@@ -20,6 +20,8 @@
  *                   SimplicialCholesky.h:155-181)
  *   efem_solve      LBFGSSolver<SoftObjective>::solve (LBFGS.h:96-174) as
  *                   called by Integrator::integral (Integrator.cpp:22-42)
+ *   efem_commit     Constraint::updateDx (src/fe/Constraint.cpp:103-113)
+ *   efem_write_vtk  Model::saveVTK (src/model/Model.cpp:50-117)
  *
  * Conventions: plain pointers and sizes; host pointers unless the name ends
  * in _d (device pointers, ordered on the context's HIP stream).  Vectors on
@@ -158,6 +160,31 @@ int efem_apply_h0(efem_ctx* ctx, const double* v, double* out);
  * prm->use_precond and no factor is present. */
 int efem_solve(efem_ctx* ctx, double* u, const efem_lbfgs_params* prm,
                efem_solve_stats* stats);
+
+/* Constraint::updateDx (src/fe/Constraint.cpp:103-113): u (3 x free nodes,
+ * host) becomes the committed displacement u_c, the reference's node->dx.
+ * The reference solves for the increment dx from 0 on top of it; efem_solve
+ * iterates on the total displacement u = u_c + dx (pass u = u_c to start
+ * where the reference does) and measures the increment for the relative
+ * criterion: ||r||_inf <= epsilon_rel ||u - u_c||_inf (LBFGS.h:118, 153), and
+ * stats->xinf = ||u - u_c||_inf (Integrator.cpp:28).  u = NULL clears the
+ * commit (u_c = 0, the state after efem_create).  Single-problem contexts;
+ * the residual and the factor do not depend on it. */
+int efem_commit(efem_ctx* ctx, const double* u);
+/* copies u_c out (zeros when nothing is committed) */
+int efem_committed(efem_ctx* ctx, double* u);
+
+/* Model::saveVTK (src/model/Model.cpp:50-117), host only: the grid at
+ * X + u (u over the free DOFs as above, NULL for the undeformed frame) as an
+ * ASCII legacy VTK unstructured grid: every node a FP32 point (vtkPoints'
+ * default type), every element a cell with its node list unchanged and type
+ * VTK_TETRA / VTK_QUADRATIC_TETRA / VTK_HEXAHEDRON / VTK_QUADRATIC_HEXAHEDRON
+ * for nen 4 / 10 / 8 / 20.  Errors through efem_global_error. */
+int efem_write_vtk(const efem_problem* pb, const double* u, const char* path);
+/* the reference's results file name, directories created:
+ * <working_dir>/results/YYYY-MM-DD/YYYY-MM-DD-HH-MM-SS-mmm-<name>.vtk in
+ * local time (Model.cpp:93-110); out gets the NUL-terminated path */
+int efem_results_path(const char* working_dir, const char* name, char* out, int cap);
 
 /* host-only diagnostic (no GPU): assemble, order and factor the approximate
  * Jacobian on the host, then run the device's block-inverse triangular-solve
@@ -333,7 +360,7 @@ int efem_mesh_problem(const efem_mesh* m, efem_problem* pb, int precision);
  * efem_input_load reads an EFEM run directory: the JSON config's keys as
  * Config::Config reads them (src/io/Config.cpp:23-81: analysis.nl_solver,
  * max_solver_iters, max_line_search, dx_criterion, residual_criterion,
- * gravity "non" only), model[0].mesh_file (MeshParser::parse,
+ * gravity, consistent_inertia), model[0].mesh_file (MeshParser::parse,
  * src/io/MeshParser.h:22-100; element rows by Body::buildElement,
  * src/model/Body.cpp:49-90), material[] (neo_hookean / mo_riv / yeoh by id,
  * ElementConstraint.cpp:11-67; elements without a defined material are
