@@ -88,10 +88,11 @@ struct efem_ctx {
     // solver state
     int mcap = 0;
     double *x = nullptr, *r = nullptr, *xp = nullptr, *rp = nullptr, *d = nullptr, *tmp = nullptr;
+    double *x3 = nullptr, *r3 = nullptr, *d2 = nullptr;  // spares of the solve loop
+    hipEvent_t trialEv[2] = {nullptr, nullptr};          // trial scalars copied
     double *S = nullptr, *Y = nullptr;
     double* scal = nullptr;  // device scalars
     double* work = nullptr;
-    unsigned* counter = nullptr;
     double* hostScal = nullptr;  // pinned
     DeviceFactor fac;
     bool factorFailed = false;
@@ -112,7 +113,9 @@ struct efem_ctx {
     efem::dev::LaunchTuning tuning;  // efem_autotune / options
     bool useGraph = true;    // option "h0_graph"
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
+    bool lookahead = true;       // option "lookahead"
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
+    int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
 };
 
 namespace {
@@ -133,6 +136,10 @@ constexpr int kTheta = 264;   // [1]
 constexpr int kGram = 272;    // [16][16] s_i . y_k of the compact two-loop
 constexpr int kRS = 528;      // [16] s_k . r of the compact two-loop
 constexpr int kNumScal = 544;
+// pinned host scalars: [0, 16) readScalars, then two slots of 8 for the
+// solve loop's trial scalars
+constexpr int kTrialSlot = 16;
+constexpr int kHostScal = kTrialSlot + 16;
 
 #define HIP_CHECK(ctx, expr)                                                          \
     do {                                                                              \
@@ -218,16 +225,16 @@ int uploadModel(efem_ctx* c)
     const size_t np = static_cast<size_t>(m.ndof()) + 3;  // + dummy zero node
     // zero fills on the context stream: it is a non-blocking stream, so a
     // null-stream hipMemset would not be ordered before its work
-    for (double** v : {&c->x, &c->r, &c->xp, &c->rp, &c->d, &c->tmp}) {
+    for (double** v : {&c->x, &c->r, &c->xp, &c->rp, &c->d, &c->tmp, &c->x3, &c->r3, &c->d2}) {
         if ((rc = devAlloc(c, v, np))) return rc;
         HIP_CHECK(c, hipMemsetAsync(*v, 0, np * sizeof(double), c->stream));
     }
     if ((rc = devAlloc(c, &c->scal, kNumScal))) return rc;
     HIP_CHECK(c, hipMemsetAsync(c->scal, 0, kNumScal * sizeof(double), c->stream));
     if ((rc = devAlloc(c, &c->work, efem::dev::kReduceWorkDoubles))) return rc;
-    if ((rc = devAlloc(c, &c->counter, 4))) return rc;
-    HIP_CHECK(c, hipMemsetAsync(c->counter, 0, 4 * sizeof(unsigned), c->stream));
-    HIP_CHECK(c, hipHostMalloc(reinterpret_cast<void**>(&c->hostScal), 16 * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(c, hipHostMalloc(reinterpret_cast<void**>(&c->hostScal), kHostScal * sizeof(double),
+                               hipHostMallocDefault));
+    for (hipEvent_t& e : c->trialEv) HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return EFEM_OK;
 }
@@ -613,10 +620,10 @@ int factorInternal(efem_ctx* c, int* failed)
     bool ok;
     if (!c->parts.empty()) {
         std::vector<efem::SupernodalFactor> P;
-        ok = efem::factorModels(c->parts, P, msg);
+        ok = efem::factorModels(c->parts, P, msg, c->denseSubtree);
         if (ok) efem::mergeFactors(P, F, snOffset);
     } else {
-        ok = efem::factorModel(m, F, msg);
+        ok = efem::factorModel(m, F, msg, c->denseSubtree);
     }
     if (!ok) {
         c->err = msg;
@@ -655,7 +662,7 @@ int ensureKrylov(efem_ctx* c)
 // (a . b, b . b) on the host
 int dotHost(efem_ctx* c, const double* a, const double* b, double* ab, double* bb)
 {
-    efem::dev::launchDot2(a, b, c->model.ndof(), c->work, c->counter, c->scal + kOut, c->scal + kOut + 1, c->stream);
+    efem::dev::launchDot2(a, b, c->model.ndof(), c->work, c->scal + kOut, c->scal + kOut + 1, c->stream);
     int rc = readScalars(c, 2);
     if (rc) return rc;
     *ab = c->hostScal[0];
@@ -918,6 +925,8 @@ int efem_destroy(efem_ctx* c)
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->evsH0) (void)hipEventDestroy(e);
     if (c->hostScal) (void)hipHostFree(c->hostScal);
+    for (hipEvent_t e : c->trialEv)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return EFEM_OK;
@@ -1079,7 +1088,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     if (prm->x_target) {
         if (!c->xtarget && (rc = devAlloc(c, &c->xtarget, static_cast<size_t>(n) + 3))) return rc;
         HIP_CHECK(c, hipMemcpyAsync(c->xtarget, prm->x_target, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        efem::dev::launchDot2(c->xtarget, c->xtarget, n, c->work, c->counter, sc + kOut + 4, sc + kOut + 5, c->stream);
+        efem::dev::launchDot2(c->xtarget, c->xtarget, n, c->work, sc + kOut + 4, sc + kOut + 5, c->stream);
     }
     HIP_CHECK(c, hipMemcpyAsync(c->x, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = residualInternal(c, c->x, c->r))) return rc;
@@ -1090,9 +1099,9 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     const double* base = c->hasBase ? c->base : nullptr;
     if (base) {
         efem::dev::launchAxpby(c->tmp, 1.0, c->x, -1.0, base, n, c->stream);
-        efem::dev::launchDotMax(c->tmp, c->tmp, c->r, n, c->work, c->counter, sc + kOut, c->stream);
+        efem::dev::launchDotMax(c->tmp, c->tmp, c->r, n, c->work, sc + kOut, c->stream);
     } else {
-        efem::dev::launchDotMax(c->x, c->x, c->r, n, c->work, c->counter, sc + kOut, c->stream);
+        efem::dev::launchDotMax(c->x, c->x, c->r, n, c->work, sc + kOut, c->stream);
     }
     if ((rc = readScalars(c, 6))) return rc;
     double rinf = c->hostScal[2], xinf = c->hostScal[1];
@@ -1124,35 +1133,169 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     int ncorr = 0, ptr = mc, k = 1;
     const size_t nb = static_cast<size_t>(n) * sizeof(double);
     // the first line search's slope r . d (later ones come with d)
-    efem::dev::launchDotMax(c->r, c->d, nullptr, n, c->work, c->counter, sc + kOut + 4, c->stream);
-    for (;;) {
-        // the current iterate and residual become xp / rp by swapping the
-        // buffers (each trial step writes all of x, each residual all of r;
-        // both buffers keep the zero dummy node)
-        std::swap(c->x, c->xp);
-        std::swap(c->r, c->rp);
-        // dgInit = r . d is in kOut + 4 (written where d was formed) and
-        // comes back with the first trial's scalars (the first trial does not
-        // depend on it); with a target the squared distance to it (kOut + 3)
-        // rides along too, so each trial costs one host synchronisation
-        double dgInit = 0.0, cur = 1.0, used = 1.0;
-        int trials = 0;
-        for (int it = 0; it < prm->max_linesearch; ++it) {
-            efem::dev::launchStep(c->x, c->xp, c->d, cur, n, c->stream);
-            if ((rc = residualInternal(c, c->x, c->r))) return rc;
-            ++s.r_evals;
-            ++trials;
-            used = cur;
-            efem::dev::launchDotMax(c->r, c->d, c->x, n, c->work, c->counter, sc + kOut, c->stream,
-                                    prm->x_target ? c->xtarget : nullptr, base);
-            if ((rc = readScalars(c, 5))) return rc;
-            const double dg = c->hostScal[0];
-            rinf = c->hostScal[1];
-            xinf = c->hostScal[2];
-            if (it == 0) dgInit = c->hostScal[4];
-            if (std::fabs(dg) <= prm->lns_fea * std::fabs(dgInit)) break;
-            cur *= dg * dgInit > 0 ? 1.5 : 0.5;
+    efem::dev::launchDotMax(c->r, c->d, nullptr, n, c->work, sc + kOut + 4, c->stream);
+    const bool compact = c->compactTwoLoop && mc <= efem::dev::kMaxCompact;
+    const bool staged = h0 && c->useGraph;
+
+    // add_correction (LBFGSMat.h:81-98) of the pair (x - xp, r - rp), then
+    // the next direction d2 = -H r (LBFGSMat::apply_Hv) and its slope r . d2
+    // (kOut + 4).  Writes S/Y slot loc, the Gram row / column of loc, the
+    // scalar slots and d2; reads x, xp, r, rp and the rest of the history.
+    auto nextDirection = [&]() -> int {
+        const int loc = ptr % mc;
+        double* sl = c->S + static_cast<size_t>(loc) * n;
+        double* yl = c->Y + static_cast<size_t>(loc) * n;
+        if (ncorr < mc) ++ncorr;
+        ptr = loc + 1;
+        int rc2;
+        if (compact) {
+            // the new pair, its Gram row / column, theta and the first loop
+            // in one pass; with the captured preconditioner the first pass
+            // writes its staging input and the second reads its output
+            double* q0 = staged ? c->fac.in : c->d2;
+            efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, ncorr, loc, mc, sc + kGram,
+                                       sc + kYs, sc + kYY + loc, sc + kTheta, sc + kAlpha, sc + kRS, q0, c->work,
+                                       c->stream);
+            const double* h = c->d2;
+            if (staged) {
+                if ((rc2 = applyH0Graph(c))) return rc2;
+                h = c->fac.out;
+            } else if (h0) {
+                if ((rc2 = applyH0Internal(c, c->d2, c->tmp))) return rc2;
+                h = c->tmp;
+            } else {
+                efem::dev::launchDivDev(c->d2, sc + kTheta, n, c->stream);
+            }
+            efem::dev::launchTwoLoopSecond(h, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
+                                           sc + kCoef, c->d2, c->work, c->stream, c->r, sc + kRS, sc + kOut + 4);
+        } else {
+            efem::dev::launchDiff2(sl, yl, c->x, c->xp, c->r, c->rp, n, c->stream);
+            efem::dev::launchDot2(sl, yl, n, c->work, sc + kYs + loc, sc + kYY + loc, c->stream);
+            efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
+            // two-loop recursion in the reference's order
+            efem::dev::launchScale(c->d2, c->r, -1.0, n, c->stream);
+            int j = ptr % mc, prev = -1;
+            for (int i = 0; i < ncorr; ++i) {
+                j = (j + mc - 1) % mc;
+                efem::dev::launchTwoLoopStep(c->d2, prev >= 0 ? c->Y + static_cast<size_t>(prev) * n : nullptr,
+                                             prev >= 0 ? sc + kAlpha + prev : nullptr,
+                                             c->S + static_cast<size_t>(j) * n, sc + kYs + j, nullptr,
+                                             sc + kAlpha + j, 0, n, c->work, c->stream);
+                prev = j;
+            }
+            if (prev >= 0)
+                efem::dev::launchAxpyDev(c->d2, c->Y + static_cast<size_t>(prev) * n, sc + kAlpha + prev, 1, n,
+                                         c->stream);
+            if (h0) {
+                if ((rc2 = applyH0Internal(c, c->d2, c->tmp))) return rc2;
+                std::swap(c->d2, c->tmp);
+            } else {
+                efem::dev::launchDivDev(c->d2, sc + kTheta, n, c->stream);
+            }
+            prev = -1;
+            for (int i = 0; i < ncorr; ++i) {
+                efem::dev::launchTwoLoopStep(c->d2, prev >= 0 ? c->S + static_cast<size_t>(prev) * n : nullptr,
+                                             prev >= 0 ? sc + kCoef + prev : nullptr,
+                                             c->Y + static_cast<size_t>(j) * n, sc + kYs + j, sc + kAlpha + j,
+                                             sc + kCoef + j, 1, n, c->work, c->stream);
+                prev = j;
+                j = (j + 1) % mc;
+            }
+            if (prev >= 0)
+                efem::dev::launchAxpyDev(c->d2, c->S + static_cast<size_t>(prev) * n, sc + kCoef + prev, 0, n,
+                                         c->stream);
+            efem::dev::launchDotMax(c->r, c->d2, nullptr, n, c->work, sc + kOut + 4, c->stream);
         }
+        HIP_CHECK(c, hipGetLastError());
+        return EFEM_OK;
+    };
+    // the current iterate and residual become xp / rp; the trial writes the
+    // spare buffers and the previous xp / rp become the spares (every trial
+    // step writes all of x, every residual all of r; all keep the zero dummy
+    // node)
+    auto rotate = [&]() {
+        double* t = c->xp;
+        c->xp = c->x;
+        c->x = c->x3;
+        c->x3 = t;
+        t = c->rp;
+        c->rp = c->r;
+        c->r = c->r3;
+        c->r3 = t;
+    };
+    // one trial x = xp + step d, r(x), and its scalars {r . d, ||r||_inf,
+    // ||x - u_c||_inf, ||x - x_target||^2, slope r . d of the iteration}
+    // copied to pinned slot `slot` behind an event
+    auto enqueueTrial = [&](double step, int slot) -> int {
+        efem::dev::launchStep(c->x, c->xp, c->d, step, n, c->stream);
+        int rc2 = residualInternal(c, c->x, c->r);
+        if (rc2) return rc2;
+        efem::dev::launchDotMax(c->r, c->d, c->x, n, c->work, sc + kOut, c->stream,
+                                prm->x_target ? c->xtarget : nullptr, base);
+        HIP_CHECK(c, hipMemcpyAsync(c->hostScal + kTrialSlot + 8 * slot, sc + kOut, 5 * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(c, hipEventRecord(c->trialEv[slot], c->stream));
+        return EFEM_OK;
+    };
+    // Look-ahead: while the host waits for a first trial's scalars, the
+    // device already forms the next direction and runs the next first trial
+    // (step 1).  Nearly every first trial is accepted and the solve goes on;
+    // otherwise (a rejected trial, a stop, the FP64 switch) the host state is
+    // restored and the speculative results, which only touched the spares,
+    // the S/Y slot and Gram row / column about to be replaced and scalar
+    // slots rewritten before their next use, are ignored.  Either way the
+    // arithmetic is that of the plain loop.
+    struct LoopState {
+        double *x, *xp, *x3, *r, *rp, *r3, *d, *d2, *tmp;
+        int ptr, ncorr;
+    };
+    auto save = [&]() { return LoopState{c->x, c->xp, c->x3, c->r, c->rp, c->r3, c->d, c->d2, c->tmp, ptr, ncorr}; };
+    auto restore = [&](const LoopState& q) {
+        c->x = q.x;
+        c->xp = q.xp;
+        c->x3 = q.x3;
+        c->r = q.r;
+        c->rp = q.rp;
+        c->r3 = q.r3;
+        c->d = q.d;
+        c->d2 = q.d2;
+        c->tmp = q.tmp;
+        ptr = q.ptr;
+        ncorr = q.ncorr;
+    };
+    rotate();
+    if ((rc = enqueueTrial(1.0, 0))) return rc;
+    int slot = 0, it = 0, trials = 0;
+    double cur = 1.0, dgInit = 0.0;
+    for (;;) {
+        bool spec = false;
+        LoopState before{};
+        if (c->lookahead && it == 0 && (prm->max_iterations == 0 || k < prm->max_iterations)) {
+            before = save();
+            if ((rc = nextDirection())) return rc;
+            std::swap(c->d, c->d2);
+            rotate();
+            if ((rc = enqueueTrial(1.0, slot ^ 1))) return rc;
+            spec = true;
+        }
+        HIP_CHECK(c, hipEventSynchronize(c->trialEv[slot]));
+        const double* hsc = c->hostScal + kTrialSlot + 8 * slot;
+        const double dg = hsc[0];
+        rinf = hsc[1];
+        xinf = hsc[2];
+        if (it == 0) dgInit = hsc[4];
+        ++s.r_evals;
+        ++trials;
+        const double used = cur;
+        if (!(std::fabs(dg) <= prm->lns_fea * std::fabs(dgInit)) && it + 1 < prm->max_linesearch) {
+            // rejected: the next trial of this iteration
+            if (spec) restore(before);
+            cur *= dg * dgInit > 0 ? 1.5 : 0.5;
+            ++it;
+            if ((rc = enqueueTrial(cur, slot))) return rc;
+            continue;
+        }
+        // accepted (or the last trial allowed: LineSearchFEA.h:37-68)
         s.ls_trials += trials;
         c->trace.push_back(rinf);
         c->trace.push_back(used);
@@ -1160,97 +1303,41 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         s.iterations = k;
         s.rinf = rinf;
         s.xinf = xinf;
-        if (!std::isfinite(rinf) || !std::isfinite(xinf)) break;
-        if (prm->x_target) {
+        bool stop = !std::isfinite(rinf) || !std::isfinite(xinf);
+        if (!stop && prm->x_target) {
             // ||x - x_target|| of the accepted step (read with its scalars)
-            s.target_err = std::sqrt(c->hostScal[3]) / tnorm;
+            s.target_err = std::sqrt(hsc[3]) / tnorm;
             if (s.target_err <= prm->target_tol) {
                 s.target_reached = 1;
-                break;
+                stop = true;
             }
         }
-        if (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xinf) break;
-        if (prm->max_iterations != 0 && k >= prm->max_iterations) break;
-        if (elem32(c) && !c->elemFp64 && prm->fp64_switch > 0.0 &&
-            rinf <= prm->fp64_switch * rinf0) {
+        if (!stop && (rinf <= prm->epsilon || rinf <= prm->epsilon_rel * xinf)) stop = true;
+        if (!stop && prm->max_iterations != 0 && k >= prm->max_iterations) stop = true;
+        const bool toFp64 = !stop && elem32(c) && !c->elemFp64 && prm->fp64_switch > 0.0 &&
+                            rinf <= prm->fp64_switch * rinf0;
+        if (spec && (stop || toFp64)) {
+            restore(before);
+            spec = false;
+        }
+        if (stop) break;
+        if (toFp64) {
             // FP64 finish: re-evaluate r(x) with the FP64 element kernels
             c->elemFp64 = true;
             s.fp64_switch_iteration = k;
             if ((rc = residualInternal(c, c->x, c->r))) return rc;
             ++s.r_evals;
         }
-        // add_correction (LBFGSMat.h:81-98)
-        const int loc = ptr % mc;
-        double* sl = c->S + static_cast<size_t>(loc) * n;
-        double* yl = c->Y + static_cast<size_t>(loc) * n;
-        if (ncorr < mc) ++ncorr;
-        ptr = loc + 1;
-        const bool compact = c->compactTwoLoop && mc <= efem::dev::kMaxCompact;
-        if (!compact) {
-            efem::dev::launchDiff2(sl, yl, c->x, c->xp, c->r, c->rp, n, c->stream);
-            efem::dev::launchDot2(sl, yl, n, c->work, c->counter, sc + kYs + loc, sc + kYY + loc, c->stream);
-            efem::dev::launchTheta(sc + kYs + loc, sc + kYY + loc, sc + kTheta, c->stream);
-        }
-        if (compact) {
-            // two-loop from Gram products: the new pair, its Gram entries and
-            // the first loop in one reduction pass (+ one multi-axpy), then
-            // the second pass after the preconditioner; with the captured
-            // preconditioner the first pass writes its staging input and the
-            // second reads its staging output directly
-            const bool staged = h0 && c->useGraph;
-            double* q0 = staged ? c->fac.in : c->d;
-            efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, ncorr, loc, mc, sc + kGram,
-                                       sc + kYs, sc + kYY + loc, sc + kTheta, sc + kAlpha, sc + kRS, q0, c->work,
-                                       c->counter, c->stream);
-            const double* h = c->d;
-            if (staged) {
-                if ((rc = applyH0Graph(c))) return rc;
-                h = c->fac.out;
-            } else if (h0) {
-                if ((rc = applyH0Internal(c, c->d, c->tmp))) return rc;
-                std::swap(c->d, c->tmp);
-                h = c->d;
-            } else {
-                efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
-            }
-            efem::dev::launchTwoLoopSecond(h, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
-                                           sc + kCoef, c->d, c->work, c->counter, c->stream, c->r, sc + kRS,
-                                           sc + kOut + 4);
-        } else {
-            // two-loop recursion: d = -H r (LBFGSMat::apply_Hv)
-            efem::dev::launchScale(c->d, c->r, -1.0, n, c->stream);
-            int j = ptr % mc, prev = -1;
-            for (int i = 0; i < ncorr; ++i) {
-                j = (j + mc - 1) % mc;
-                efem::dev::launchTwoLoopStep(c->d, prev >= 0 ? c->Y + static_cast<size_t>(prev) * n : nullptr,
-                                             prev >= 0 ? sc + kAlpha + prev : nullptr, c->S + static_cast<size_t>(j) * n,
-                                             sc + kYs + j, nullptr, sc + kAlpha + j, 0, n, c->work, c->counter,
-                                             c->stream);
-                prev = j;
-            }
-            if (prev >= 0)
-                efem::dev::launchAxpyDev(c->d, c->Y + static_cast<size_t>(prev) * n, sc + kAlpha + prev, 1, n, c->stream);
-            if (h0) {
-                if ((rc = applyH0Internal(c, c->d, c->tmp))) return rc;
-                std::swap(c->d, c->tmp);
-            } else {
-                efem::dev::launchDivDev(c->d, sc + kTheta, n, c->stream);
-            }
-            prev = -1;
-            for (int i = 0; i < ncorr; ++i) {
-                efem::dev::launchTwoLoopStep(c->d, prev >= 0 ? c->S + static_cast<size_t>(prev) * n : nullptr,
-                                             prev >= 0 ? sc + kCoef + prev : nullptr, c->Y + static_cast<size_t>(j) * n,
-                                             sc + kYs + j, sc + kAlpha + j, sc + kCoef + j, 1, n, c->work, c->counter,
-                                             c->stream);
-                prev = j;
-                j = (j + 1) % mc;
-            }
-            if (prev >= 0)
-                efem::dev::launchAxpyDev(c->d, c->S + static_cast<size_t>(prev) * n, sc + kCoef + prev, 0, n, c->stream);
-            efem::dev::launchDotMax(c->r, c->d, nullptr, n, c->work, c->counter, sc + kOut + 4, c->stream);
-        }
-        HIP_CHECK(c, hipGetLastError());
         ++k;
+        it = 0;
+        trials = 0;
+        cur = 1.0;
+        slot ^= 1;
+        if (spec) continue;  // the next first trial is already in flight
+        if ((rc = nextDirection())) return rc;
+        std::swap(c->d, c->d2);
+        rotate();
+        if ((rc = enqueueTrial(1.0, slot))) return rc;
     }
     HIP_CHECK(c, hipMemcpyAsync(u, c->x, nb, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -1359,6 +1446,10 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         c->compactTwoLoop = value != 0.0;
         return EFEM_OK;
     }
+    if (k == "lookahead") {
+        c->lookahead = value != 0.0;
+        return EFEM_OK;
+    }
     const int iv = static_cast<int>(value);
     if (k == "gather_lanes" && (iv == 1 || iv == 2 || iv == 4 || iv == 8)) {
         c->tuning.gatherLanes = iv;
@@ -1380,6 +1471,13 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         // levels with at least this many 4-column backward items use
         // 16-column items; takes effect at the next factorisation
         c->narrowMinItems = value >= 2147483647.0 ? 2147483647 : iv;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
+    if (k == "dense_subtree" && iv >= 1 && iv <= efem::kMaxSupernodeWidth) {
+        // elimination subtrees of at most this many columns become one dense
+        // supernode; takes effect at the next factorisation
+        c->denseSubtree = iv;
         releaseFactor(c);
         return EFEM_OK;
     }
@@ -1499,7 +1597,7 @@ int efem_newton(efem_ctx* c, double* u, double epsilon, int max_iterations, cons
         int e = residualInternal(c, x, r);
         if (e) return e;
         ++s.r_evals;
-        efem::dev::launchDotMax(r, r, x, n, c->work, c->counter, sc + kOut, c->stream);
+        efem::dev::launchDotMax(r, r, x, n, c->work, sc + kOut, c->stream);
         if ((e = readScalars(c, 3))) return e;
         *r2 = c->hostScal[0];
         *rinf = c->hostScal[1];
@@ -1630,7 +1728,7 @@ int efem_energy(efem_ctx* c, const double* u, double* W)
     HIP_CHECK(c, hipMemcpyAsync(c->ku, u, nb, hipMemcpyHostToDevice, c->stream));
     double* per = static_cast<double*>(c->eslots);
     efem::dev::launchElementEnergy(elemArgs(c, c->ku, false), per, c->stream);
-    efem::dev::launchSumComp(per, m.nElems, c->work, c->counter, c->scal + kOut, c->stream);
+    efem::dev::launchSumComp(per, m.nElems, c->work, c->scal + kOut, c->stream);
     HIP_CHECK(c, hipGetLastError());
     if ((rc = readScalars(c, 1))) return rc;
     *W = c->hostScal[0];
@@ -1922,7 +2020,7 @@ int efem_autotune(efem_ctx* c, double* report)
         efem::dev::setTuning(t);
         double ms = 0.0;
         rc = timeIt([&]() {
-            efem::dev::launchDotMax(c->r, c->tmp, c->x, n, c->work, c->counter, c->scal + kOut, c->stream);
+            efem::dev::launchDotMax(c->r, c->tmp, c->x, n, c->work, c->scal + kOut, c->stream);
             return EFEM_OK;
         }, 20, &ms);
         if (rc == EFEM_OK && ms < bestRed) {
@@ -1952,10 +2050,10 @@ int efem_autotune(efem_ctx* c, double* report)
         double ms = 0.0;
         rc = timeIt([&]() {
             efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, hm, hm - 1, hm, sc + kGram, sc + kYs,
-                                       sc + kYY, sc + kTheta, sc + kAlpha, sc + kRS, c->tmp, c->work, c->counter,
+                                       sc + kYY, sc + kTheta, sc + kAlpha, sc + kRS, c->tmp, c->work,
                                        c->stream);
             efem::dev::launchTwoLoopSecond(c->tmp, c->S, c->Y, n, hm, hm - 1, hm, sc + kGram, sc + kYs, sc + kAlpha,
-                                           sc + kCoef, c->d, c->work, c->counter, c->stream, c->r, sc + kRS,
+                                           sc + kCoef, c->d, c->work, c->stream, c->r, sc + kRS,
                                            sc + kOut + 4);
             return EFEM_OK;
         }, 10, &ms);

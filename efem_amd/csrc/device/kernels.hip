@@ -8,15 +8,18 @@
 //    buffer.  Linear tets (one Gauss point) use a one-lane-per-element kernel.
 //  * assembly: deterministic gather over a node -> slot CSR, FP64 sums, no
 //    float atomics (replaces the scatter of ElementConstraint.cpp:154-162).
-//  * reductions: fixed-shape block trees + an in-order final pass by the
-//    last block to arrive (integer arrival counter), so every dot product is
-//    bitwise reproducible.
+//  * reductions: two launches, block partials in a fixed tree, then one
+//    block that combines them in block order and runs the epilogue, so
+//    every dot product is bitwise reproducible and no block pays a
+//    device-scope fence.
 //  * preconditioner: level-scheduled supernodal triangular solves on the
 //    block-inverse form B_s = [L11^-1 ; L21 L11^-1] with three right-hand
 //    sides (the interleaved x, y, z components); the forward sweep is
 //    multifrontal (each supernode extend-adds only its children's update
 //    vectors), the backward sweep reads the ancestors' solved values.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "device_common.hpp"
 
@@ -317,87 +320,123 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
 }
 
 // ---- reductions -------------------------------------------------------------
-// 1024-lane blocks, at most one per CU: on this part the L2 is per XCD, so
-// the device-scope fence each block issues before its arrival count is
-// costly -- few, wide blocks keep the loads in flight without multiplying
-// fences.  Every reduction shape depends on n only (bitwise reproducible).
-constexpr int kRT = 1024;
-constexpr int kRW = kRT / 64;  // waves per reduction block
+// Two launches per reduction.  Stage 1 streams the vectors with kRB-lane
+// blocks and writes each block's partials (wave butterflies, then the wave
+// results in wave order) to work[k * kReduceBlocks + block]; stage 2, one
+// kRF-lane block, combines the partials in block order with the same tree
+// and hands the results to the reduction's epilogue.  The kernel boundary
+// orders the partials before stage 2: a single-launch form (each block
+// fences, counts its arrival, the last one finishes) pays a device-scope
+// fence per block, which on this part writes back the XCD's L2 and grows
+// with the grid.  Every reduction shape depends on n and the tuned grid
+// only (bitwise reproducible).
+constexpr int kRB = 256;  // stage-1 block
+constexpr int kRBW = kRB / 64;
+constexpr int kRF = 256;  // stage-2 block (one lane per block partial)
+constexpr int kRFW = kRF / 64;
 
-// K simultaneous block reductions: wave butterflies, then the 16 wave
-// results in wave order.  Bit k of maxMask: component k is a maximum (else a
-// sum).  All lanes return the block result.
-template <int K>
-__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[kRW], unsigned maxMask)
+// K simultaneous block reductions over W waves: wave butterflies, then the
+// W wave results in wave order.  Bit k of maxMask: component k is a maximum
+// (else a sum; components from 32 on are sums).  All lanes return the block
+// result.
+__device__ __forceinline__ bool isMax(unsigned maxMask, int k) { return k < 32 && ((maxMask >> k) & 1u); }
+
+template <int K, int W>
+__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W], unsigned maxMask)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const bool mx = (maxMask >> k) & 1u;
+        const bool mx = isMax(maxMask, k);
         const double t = mx ? waveMax(v[k]) : waveSum(v[k]);
         if (lane == 0) sh[k][wave] = t;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const bool mx = (maxMask >> k) & 1u;
+        const bool mx = isMax(maxMask, k);
         double t = sh[k][0];
-        for (int w = 1; w < kRW; ++w) t = mx ? fmax(t, sh[k][w]) : t + sh[k][w];
+        for (int w = 1; w < W; ++w) t = mx ? fmax(t, sh[k][w]) : t + sh[k][w];
         v[k] = t;
     }
     __syncthreads();
 }
 
+// stage 1: this block's partials of v[0..K) to work
 template <int K>
-__device__ __forceinline__ bool lastBlockIn(double (&v)[K], double* work, unsigned* counter,
-                                            double (*sh)[kRW], unsigned maxMask)
+__device__ __forceinline__ void writePartials(double (&v)[K], double* work, unsigned maxMask)
 {
-    __shared__ int amLast;
-    blockReduce<K>(v, sh, maxMask);
+    __shared__ double sh[K][kRBW];
+    blockReduce<K, kRBW>(v, sh, maxMask);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) work[k * kReduceBlocks + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        amLast = (t == gridDim.x - 1) ? 1 : 0;
     }
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    // final pass over the block partials (one per lane), same tree
+}
+
+// stage 2: the nb block partials of each component combined (lane t takes
+// blocks t, t + kRF, ...; then the block tree); every lane calls epi(v)
+template <int K, typename Epi>
+__global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, unsigned maxMask,
+                                                     Epi epi)
+{
+    __shared__ double sh[K][kRFW];
+    double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kRT)
+    for (int b = threadIdx.x; b < nb; b += kRF)
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double o = work[k * kReduceBlocks + b];
-            v[k] = ((maxMask >> k) & 1u) ? fmax(v[k], o) : v[k] + o;
+            v[k] = isMax(maxMask, k) ? fmax(v[k], o) : v[k] + o;
         }
-    blockReduce<K>(v, sh, maxMask);
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
+    blockReduce<K, kRFW>(v, sh, maxMask);
+    epi(v);
 }
 
-int reduceGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().reduceBlocks), (n + kRT - 1) / kRT)); }
+int reduceGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().reduceBlocks), (n + kRB - 1) / kRB)); }
 // reductions that stream the whole history (2 m + 4 vectors) read several
 // times the bytes of a line-search dot, so their grid is tuned on its own
-int historyGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().historyBlocks), (n + kRT - 1) / kRT)); }
+int historyGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().historyBlocks), (n + kRB - 1) / kRB)); }
+
+// epilogue: out[k] = v[k]
+template <int K>
+struct EpiStore {
+    double* out;
+    __device__ void operator()(const double (&v)[K]) const
+    {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] = v[k];
+        }
+    }
+};
+
+// epilogue: *p0 = v[0], *p1 = v[1]
+struct EpiStore2 {
+    double *p0, *p1;
+    __device__ void operator()(const double (&v)[2]) const
+    {
+        if (threadIdx.x == 0) {
+            *p0 = v[0];
+            *p1 = v[1];
+        }
+    }
+};
 
 // out = {a.b, max|a|, max|c - cb|} (HASC; cb may be null) and
 // out[3] = |c - xt|^2 (HASXT)
 template <bool HASC, bool HASXT>
-__global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
+__global__ __launch_bounds__(kRB) void kDotMax(const double* __restrict__ a, const double* __restrict__ b,
                                                const double* __restrict__ c, const double* __restrict__ cb,
-                                               const double* __restrict__ xt, int n, double* work,
-                                               unsigned* counter, double* out)
+                                               const double* __restrict__ xt, int n, double* work)
 {
     constexpr int K = HASXT ? 4 : 3;
-    __shared__ double sh[K][kRW];
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
 #pragma unroll 4
-    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
         const double ai = a[i];
         v[0] += ai * b[i];
         v[1] = fmax(v[1], fabs(ai));
@@ -410,38 +449,28 @@ __global__ __launch_bounds__(kRT) void kDotMax(const double* __restrict__ a, con
             }
         }
     }
-    if (lastBlockIn<K>(v, work, counter, sh, 0x6u) && threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) out[k] = v[k];
-    }
+    writePartials<K>(v, work, 0x6u);
 }
 
-__global__ __launch_bounds__(kRT) void kDot2(const double* __restrict__ a, const double* __restrict__ b, int n,
-                                             double* work, unsigned* counter, double* ab, double* bb)
+__global__ __launch_bounds__(kRB) void kDot2(const double* __restrict__ a, const double* __restrict__ b, int n,
+                                             double* work)
 {
-    __shared__ double sh[2][kRW];
     double v[2] = {0, 0};
-    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
         const double bi = b[i];
         v[0] += a[i] * bi;
         v[1] += bi * bi;
     }
-    if (lastBlockIn<2>(v, work, counter, sh, 0u) && threadIdx.x == 0) {
-        *ab = v[0];
-        *bb = v[1];
-    }
+    writePartials<2>(v, work, 0u);
 }
 
-__global__ __launch_bounds__(kRT) void kTwoLoop(double* __restrict__ q, const double* __restrict__ vprev,
+__global__ __launch_bounds__(kRB) void kTwoLoop(double* __restrict__ q, const double* __restrict__ vprev,
                                                 const double* cprev, double sgn,
-                                                const double* __restrict__ w, const double* den,
-                                                const double* aj, double* cout, int mode, int n,
-                                                double* work, unsigned* counter)
+                                                const double* __restrict__ w, int n, double* work)
 {
-    __shared__ double sh[1][kRW];
     double v[1] = {0};
     const double c = vprev ? sgn * (*cprev) : 0.0;
-    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
         double qi = q[i];
         if (vprev) {
             qi += c * vprev[i];
@@ -449,75 +478,29 @@ __global__ __launch_bounds__(kRT) void kTwoLoop(double* __restrict__ q, const do
         }
         v[0] += w[i] * qi;
     }
-    if (lastBlockIn<1>(v, work, counter, sh, 0u) && threadIdx.x == 0) {
-        const double r = v[0] / (*den);
-        *cout = mode ? (*aj - r) : r;
-    }
+    writePartials<1>(v, work, 0u);
 }
+
+// epilogue of a two-loop step: r = v / *den; *cout = mode ? *aj - r : r
+struct EpiTwoLoopStep {
+    const double *den, *aj;
+    double* cout;
+    int mode;
+    __device__ void operator()(const double (&v)[1]) const
+    {
+        if (threadIdx.x == 0) {
+            const double r = v[0] / (*den);
+            *cout = mode ? (*aj - r) : r;
+        }
+    }
+};
 
 // ---- compact two-loop ------------------------------------------------------
 // Kernels are templated on the history capacity KC (8 covers the reference's
-// m = 6; 16 is the maximum) so that every per-thread array has compile-time
-// indexing (no scratch); live slots are guarded by `k < nc`.
-
-
-// per-thread sums v[0..N) -> block partials (work[k * kReduceBlocks + b]) ->
-// the last block to arrive reduces the partials of every k with all its
-// lanes (lane t takes blocks t, t + 256, ...; then the fixed wave/block
-// tree) into red[0..N) (shared).  Returns true in the last block (all
-// threads), after a barrier.  Deterministic: the shapes depend on the grid
-// only.
-template <int N>
-__device__ __forceinline__ bool multiReduce(double (&v)[N], double* work, unsigned* counter, double* red)
-{
-    __shared__ double sh[N][kRW];
-    __shared__ int amLast;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const double t = waveSum(v[k]);
-        if (lane == 0) sh[k][wave] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < N) {
-        const int k = threadIdx.x;
-        double t = sh[k][0];
-        for (int w = 1; w < kRW; ++w) t += sh[k][w];
-        work[k * kReduceBlocks + blockIdx.x] = t;
-    }
-    // the barrier orders the partial stores before lane 0's device-scope
-    // release fence (one fence per block: each is an L2 write-back here)
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        amLast = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    double p[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) p[k] = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kRT)
-#pragma unroll
-        for (int k = 0; k < N; ++k) p[k] += work[k * kReduceBlocks + b];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const double t = waveSum(p[k]);
-        if (lane == 0) sh[k][wave] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < N) {
-        const int k = threadIdx.x;
-        double t = sh[k][0];
-        for (int w = 1; w < kRW; ++w) t += sh[k][w];
-        red[k] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
-}
+// m = 6; 16 is the maximum) and on the number of stored pairs NC, so that
+// every per-thread array has compile-time indexing (no scratch) and every
+// load of an element is issued before the first is used (a run-time trip
+// count makes each pair's loads wait for the previous pair's).
 
 // layout of the Gram matrix in the scalar area: G[i * kMaxCompact + k]
 constexpr int KG = kMaxCompact;
@@ -525,187 +508,220 @@ constexpr int KG = kMaxCompact;
 // One pass for the new correction pair and the first loop: s_loc = x - xp
 // and y_loc = r - rp are formed (and stored) on the fly, and the same sweep
 // over the history takes the Gram row/column of the new pair, y_loc.y_loc
-// and the first loop's s_k . r.  The last block writes the Gram entries,
-// ys_loc, yy, theta = yy / ys_loc, rs_k = s_k . r, and runs the alpha
-// recurrence newest -> oldest (q_0 = -r): alpha_i = (s_i.q_0 - sum_{k newer}
-// alpha_k s_i.y_k) / ys_i.  Same reduction shapes as every other reduction
-// here.
-template <int KC>
-__global__ __launch_bounds__(kRT) void kGramFirst(double* __restrict__ S, double* __restrict__ Y,
-                                                       const double* __restrict__ x, const double* __restrict__ xp,
-                                                       const double* __restrict__ r, const double* __restrict__ rp,
-                                                       int n, int nc, int loc, int mcap, double* G, double* ys,
-                                                       double* yy, double* theta, double* alpha, double* rs,
-                                                       double* work, unsigned* counter)
+// and the first loop's s_k . r.  Pairs are taken by age t (slot (loc - t)
+// mod mcap: the ring order of the stored pairs; age 0 is the new pair);
+// the epilogue maps them back to slots.  Partial sums by age:
+// v[t] = s_loc . y_t, v[NC + t] = s_t . y_loc, v[2 NC] = y_loc . y_loc,
+// v[2 NC + 1 + t] = s_t . r.
+template <int NC>
+__global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double* __restrict__ Y,
+                                                      const double* __restrict__ x, const double* __restrict__ xp,
+                                                      const double* __restrict__ r, const double* __restrict__ rp,
+                                                      int n, int loc, int mcap, double* work)
 {
-    // v[k] = s_loc . y_k, v[KC + k] = s_k . y_loc, v[2 KC] = y_loc . y_loc,
-    // v[2 KC + 1 + k] = s_k . r   (k < nc; slot loc from registers)
-    constexpr int N = 3 * KC + 1;
-    __shared__ double red[N];
-    __shared__ double sG[KC][KC], sys[KC];
+    static_assert(NC >= 1 && NC <= kMaxCompact, "pair count");
+    constexpr int N = 3 * NC + 1;
     double v[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) v[k] = 0.0;
+    const double* sAge[NC > 1 ? NC - 1 : 1];
+    const double* yAge[NC > 1 ? NC - 1 : 1];
+#pragma unroll
+    for (int t = 1; t < NC; ++t) {
+        const int k = (loc - t + mcap) % mcap;
+        sAge[t - 1] = S + static_cast<size_t>(k) * n;
+        yAge[t - 1] = Y + static_cast<size_t>(k) * n;
+    }
     double* sl = S + static_cast<size_t>(loc) * n;
     double* yl = Y + static_cast<size_t>(loc) * n;
-#pragma unroll 1
-    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
-        const double ri = r[i];
-        const double si = x[i] - xp[i], yi = ri - rp[i];
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
+        const double ri = r[i], xi = x[i], xpi = xp[i], rpi = rp[i];
+        double sk[NC > 1 ? NC - 1 : 1], yk[NC > 1 ? NC - 1 : 1];
+#pragma unroll
+        for (int t = 1; t < NC; ++t) {
+            sk[t - 1] = sAge[t - 1][i];
+            yk[t - 1] = yAge[t - 1][i];
+        }
+        const double si = xi - xpi, yi = ri - rpi;
+        v[0] += si * yi;
+        v[NC] += si * yi;
+        v[2 * NC] += yi * yi;
+        v[2 * NC + 1] += si * ri;
+#pragma unroll
+        for (int t = 1; t < NC; ++t) {
+            v[t] += si * yk[t - 1];
+            v[NC + t] += sk[t - 1] * yi;
+            v[2 * NC + 1 + t] += sk[t - 1] * ri;
+        }
         sl[i] = si;
         yl[i] = yi;
-#pragma unroll
-        for (int k = 0; k < KC; ++k) {
-            if (k < nc) {
-                const double sk = k == loc ? si : S[static_cast<size_t>(k) * n + i];
-                const double yk = k == loc ? yi : Y[static_cast<size_t>(k) * n + i];
-                v[k] += si * yk;
-                v[KC + k] += sk * yi;
-                v[2 * KC + 1 + k] += sk * ri;
-            }
-        }
-        v[2 * KC] += yi * yi;
     }
-    if (!multiReduce<N>(v, work, counter, red)) return;
-    // Gram block in LDS: stored entries, with the new row / column from red
-    for (int t = threadIdx.x; t < KC * KC; t += kRT) {
-        const int i = t / KC, k = t - i * KC;
-        double g = 0.0;
-        if (i < mcap && k < mcap) {
-            if (i == loc && k < nc) g = red[k];
-            else if (k == loc && i < nc) g = red[KC + i];
-            else g = G[i * KG + k];
-        }
-        sG[i][k] = g;
-    }
-    if (threadIdx.x < KC) {
-        const int k = threadIdx.x;
-        sys[k] = k == loc ? red[loc] : (k < mcap ? ys[k] : 1.0);
-    }
-    __syncthreads();
-    if (threadIdx.x < nc) {
-        const int k = threadIdx.x;
-        G[loc * KG + k] = red[k];
-        G[k * KG + loc] = red[KC + k];
-        rs[k] = red[2 * KC + 1 + k];
-    }
-    if (threadIdx.x == 0) {
-        ys[loc] = red[loc];
-        *yy = red[2 * KC];
-        *theta = red[2 * KC] / red[loc];
-        double c[KC];
-#pragma unroll
-        for (int k = 0; k < KC; ++k) c[k] = 0.0;
-        int j = loc;
-        for (int t = 0; t < nc; ++t) {
-            double acc = -red[2 * KC + 1 + j];
-            int kk = loc;
-            for (int u = 0; u < t; ++u) {
-                acc -= c[kk] * sG[j][kk];
-                kk = (kk + mcap - 1) % mcap;
-            }
-            c[j] = acc / sys[j];
-            alpha[j] = c[j];
-            j = (j + mcap - 1) % mcap;
-        }
-    }
+    writePartials<N>(v, work, 0u);
 }
 
-template <int KC>
-__global__ __launch_bounds__(kRT) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
-                                                         int n, int nc, int first, int newest, int mcap,
-                                                         const double* G, const double* ys, const double* alphaIn,
-                                                         double* outCoef, double* work, unsigned* counter,
-                                                         const double* __restrict__ dw, const double* rs,
-                                                         double* slope)
+// Epilogue of kGramFirst: writes the Gram row / column of loc, ys_loc, yy,
+// theta = yy / ys_loc, rs_k = s_k . r, and runs the alpha recurrence newest
+// -> oldest (q_0 = -r): alpha_i = (s_i.q_0 - sum_{k newer} alpha_k s_i.y_k)
+// / ys_i.
+template <int KC, int NC>
+struct EpiGramFirst {
+    double *G, *ys, *yy, *theta, *alpha, *rs;
+    int loc, mcap;
+    __device__ void operator()(const double (&v)[3 * NC + 1]) const
+    {
+        constexpr int N = 3 * NC + 1;
+        __shared__ double red[N];
+        __shared__ double sG[KC][KC], sys[KC], sA[KC], sB[KC], sC[KC];
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) red[k] = v[k];
+        }
+        __syncthreads();
+        // by slot: sA[k] = s_loc . y_k, sB[k] = s_k . y_loc, sC[k] = s_k . r
+        if (threadIdx.x < mcap) {
+            const int k = threadIdx.x;
+            const int t = (loc - k + mcap) % mcap;
+            sA[k] = t < NC ? red[t] : 0.0;
+            sB[k] = t < NC ? red[NC + t] : 0.0;
+            sC[k] = t < NC ? red[2 * NC + 1 + t] : 0.0;
+        }
+        __syncthreads();
+        // Gram block in LDS: stored entries, with the new row / column
+        for (int t = threadIdx.x; t < KC * KC; t += kRF) {
+            const int i = t / KC, k = t - i * KC;
+            double g = 0.0;
+            if (i < mcap && k < mcap) {
+                if (i == loc && k < NC) g = sA[k];
+                else if (k == loc && i < NC) g = sB[i];
+                else g = G[i * KG + k];
+            }
+            sG[i][k] = g;
+        }
+        if (threadIdx.x < KC) {
+            const int k = threadIdx.x;
+            sys[k] = k == loc ? red[0] : (k < mcap ? ys[k] : 1.0);
+        }
+        __syncthreads();
+        if (threadIdx.x < NC) {
+            const int k = threadIdx.x;
+            G[loc * KG + k] = sA[k];
+            G[k * KG + loc] = sB[k];
+            rs[k] = sC[k];
+        }
+        if (threadIdx.x == 0) {
+            ys[loc] = red[0];
+            *yy = red[2 * NC];
+            *theta = red[2 * NC] / red[0];
+            // by age (compile-time indices; the slots are independent)
+            int slot[NC];
+#pragma unroll
+            for (int t = 0; t < NC; ++t) slot[t] = (loc - t + mcap) % mcap;
+            double a[NC];
+#pragma unroll
+            for (int t = 0; t < NC; ++t) {
+                double acc = -red[2 * NC + 1 + t];
+#pragma unroll
+                for (int u = 0; u < t; ++u) acc -= a[u] * sG[slot[t]][slot[u]];
+                a[t] = acc / sys[slot[t]];
+                alpha[slot[t]] = a[t];
+            }
+        }
+    }
+};
+
+// Second loop of the two-loop recursion: dots of Y_k with q (the H0 output,
+// stored pairs in slots 0..NC-1) and of dw with q in one sweep.
+template <int NC>
+__global__ __launch_bounds__(kRB) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
+                                                        int n, const double* __restrict__ dw, double* work)
 {
-    // first: dots of R_k (= S) with q (= r), alpha recurrence newest->oldest
-    // else: dots of R_k (= Y) with q (= H0 output), beta recurrence; with dw
-    // (second loop only) also dw . q, and then the slope of the direction
-    // d = q + sum coef_k s_k along dw: dw . d = dw . q + sum coef_k rs_k
-    // (rs_k = s_k . dw from the first pass)
-    __shared__ double red[KC + 1];
-    __shared__ double sG[KC][KC], sys[KC], sa[KC];
-    double v[KC + 1];
+    static_assert(NC >= 1 && NC <= kMaxCompact, "pair count");
+    double v[NC + 1];
 #pragma unroll
-    for (int k = 0; k <= KC; ++k) v[k] = 0.0;
-    for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
-        const double qi = q[i];
+    for (int k = 0; k <= NC; ++k) v[k] = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
+        const double qi = q[i], dwi = dw[i];
+        double rk[NC];
 #pragma unroll
-        for (int k = 0; k < KC; ++k)
-            if (k < nc) v[k] += R[static_cast<size_t>(k) * n + i] * qi;
-        if (dw) v[KC] += dw[i] * qi;
-    }
-    if (!multiReduce<KC + 1>(v, work, counter, red)) return;
-    // the recurrence reads the Gram block, ys and alpha from LDS
-    for (int t = threadIdx.x; t < KC * KC; t += kRT) {
-        const int i = t / KC, k = t - i * KC;
-        sG[i][k] = (i < mcap && k < mcap) ? G[i * KG + k] : 0.0;
-    }
-    if (threadIdx.x < KC) {
-        sys[threadIdx.x] = threadIdx.x < mcap ? ys[threadIdx.x] : 1.0;
-        sa[threadIdx.x] = (!first && threadIdx.x < mcap) ? alphaIn[threadIdx.x] : 0.0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double c[KC];
+        for (int k = 0; k < NC; ++k) rk[k] = R[static_cast<size_t>(k) * n + i];
 #pragma unroll
-        for (int k = 0; k < KC; ++k) c[k] = 0.0;
-        if (first) {
-            // q_0 = -r: alpha_i = (s_i.q_0 - sum_{k newer} alpha_k s_i.y_k) / ys_i
-            int j = newest;
-            for (int t = 0; t < nc; ++t) {
-                double acc = -red[j];
-                int kk = newest;
-                for (int u = 0; u < t; ++u) {
-                    acc -= c[kk] * sG[j][kk];
-                    kk = (kk + mcap - 1) % mcap;
-                }
-                c[j] = acc / sys[j];
-                outCoef[j] = c[j];
-                j = (j + mcap - 1) % mcap;
-            }
-        } else {
-            // beta_i = (y_i.h + sum_{k older} coef_k y_i.s_k) / ys_i ; coef = alpha - beta
-            const int oldest = (newest + mcap - nc + 1) % mcap;
-            int j = oldest;
-            for (int t = 0; t < nc; ++t) {
-                double acc = red[j];
-                int kk = oldest;
-                for (int u = 0; u < t; ++u) {
-                    acc += c[kk] * sG[kk][j];
-                    kk = (kk + 1) % mcap;
-                }
-                c[j] = sa[j] - acc / sys[j];
-                outCoef[j] = c[j];
-                j = (j + 1) % mcap;
-            }
-            if (dw) {
-                double sl = red[KC];
-#pragma unroll
-                for (int k = 0; k < KC; ++k)
-                    if (k < nc) sl += c[k] * rs[k];
-                *slope = sl;
-            }
-        }
+        for (int k = 0; k < NC; ++k) v[k] += rk[k] * qi;
+        v[NC] += dwi * qi;
     }
+    writePartials<NC + 1>(v, work, 0u);
 }
 
-// out = base * sgn_b + sum_k coef_k R_k * sgn_c
-template <int KC>
+// Epilogue of kTwoLoopDots: the beta recurrence oldest -> newest, coef =
+// alpha - beta, and the slope of d = q + sum coef_k s_k along dw: dw . d =
+// dw . q + sum coef_k rs_k (rs_k = s_k . dw from kGramFirst).
+template <int KC, int NC>
+struct EpiTwoLoopDots {
+    const double *G, *ys, *alphaIn, *rs;
+    double *outCoef, *slope;
+    int newest, mcap;
+    __device__ void operator()(const double (&v)[NC + 1]) const
+    {
+        __shared__ double red[NC + 1];
+        __shared__ double sG[KC][KC], sys[KC], sa[KC], cs[KC];
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k <= NC; ++k) red[k] = v[k];
+        }
+        // the recurrence reads the Gram block, ys and alpha from LDS
+        for (int t = threadIdx.x; t < KC * KC; t += kRF) {
+            const int i = t / KC, k = t - i * KC;
+            sG[i][k] = (i < mcap && k < mcap) ? G[i * KG + k] : 0.0;
+        }
+        if (threadIdx.x < KC) {
+            sys[threadIdx.x] = threadIdx.x < mcap ? ys[threadIdx.x] : 1.0;
+            sa[threadIdx.x] = threadIdx.x < mcap ? alphaIn[threadIdx.x] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // beta_i = (y_i.h + sum_{k older} coef_k y_i.s_k) / ys_i ; coef =
+            // alpha - beta, oldest -> newest (position t from the oldest;
+            // compile-time indices, the slots are independent)
+            const int oldest = (newest + mcap - NC + 1) % mcap;
+            int slot[NC];
+#pragma unroll
+            for (int t = 0; t < NC; ++t) slot[t] = (oldest + t) % mcap;
+            double b[NC];
+#pragma unroll
+            for (int t = 0; t < NC; ++t) {
+                double acc = red[slot[t]];
+#pragma unroll
+                for (int u = 0; u < t; ++u) acc += b[u] * sG[slot[u]][slot[t]];
+                b[t] = sa[slot[t]] - acc / sys[slot[t]];
+                outCoef[slot[t]] = b[t];
+                cs[slot[t]] = b[t];
+            }
+            // the stored pairs are slots 0..NC-1: summed in slot order
+            double sl = red[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) sl += cs[k] * rs[k];
+            *slope = sl;
+        }
+    }
+};
+
+// out = base * sgn_b + sum_{k < NC} coef_k R_k * sgn_c
+template <int NC>
 __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict__ base, double sgnB,
                                                        const double* __restrict__ R, const double* coef,
-                                                       double sgnC, int n, int nc, double* __restrict__ out)
+                                                       double sgnC, int n, double* __restrict__ out)
 {
-    __shared__ double c[KC];
-    if (threadIdx.x < KC) c[threadIdx.x] = threadIdx.x < nc ? sgnC * coef[threadIdx.x] : 0.0;
+    __shared__ double c[NC];
+    if (threadIdx.x < NC) c[threadIdx.x] = sgnC * coef[threadIdx.x];
     __syncthreads();
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    double v = sgnB * base[i];
+    const double b = base[i];
+    double rk[NC];
 #pragma unroll
-    for (int k = 0; k < KC; ++k)
-        if (k < nc) v += c[k] * R[static_cast<size_t>(k) * n + i];
+    for (int k = 0; k < NC; ++k) rk[k] = R[static_cast<size_t>(k) * n + i];
+    double v = sgnB * b;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v += c[k] * rk[k];
     out[i] = v;
 }
 
@@ -729,18 +745,16 @@ __device__ __forceinline__ void combinePair(double& s, double& c, double s2, dou
     c = c + c2 + err;
 }
 
-__global__ __launch_bounds__(kRT) void kSumComp(const double* __restrict__ a, int n, double* work,
-                                                unsigned* counter, double* out)
+__global__ __launch_bounds__(kRB) void kSumComp(const double* __restrict__ a, int n, double* work)
 {
-    __shared__ double ss[kRT], sc[kRT];
-    __shared__ int amLast;
+    __shared__ double ss[kRB], sc[kRB];
     const int tid = threadIdx.x;
     double s = 0.0, c = 0.0;
-    for (int i = blockIdx.x * kRT + tid; i < n; i += gridDim.x * kRT) neumaier(s, c, a[i]);
+    for (int i = blockIdx.x * kRB + tid; i < n; i += gridDim.x * kRB) neumaier(s, c, a[i]);
     ss[tid] = s;
     sc[tid] = c;
     __syncthreads();
-    for (int w = kRT / 2; w > 0; w >>= 1) {
+    for (int w = kRB / 2; w > 0; w >>= 1) {
         if (tid < w) {
             double s1 = ss[tid], c1 = sc[tid];
             combinePair(s1, c1, ss[tid + w], sc[tid + w]);
@@ -752,17 +766,16 @@ __global__ __launch_bounds__(kRT) void kSumComp(const double* __restrict__ a, in
     if (tid == 0) {
         work[blockIdx.x] = ss[0];
         work[kReduceBlocks + blockIdx.x] = sc[0];
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        amLast = (t == gridDim.x - 1) ? 1 : 0;
     }
-    __syncthreads();
-    if (!amLast || tid != 0) return;
-    __threadfence();
+}
+
+// the block pairs combined in block order
+__global__ void kSumCompFinish(const double* __restrict__ work, int nb, double* out)
+{
+    if (threadIdx.x != 0) return;
     double S = 0.0, C = 0.0;
-    for (int b = 0; b < static_cast<int>(gridDim.x); ++b) combinePair(S, C, work[b], work[kReduceBlocks + b]);
+    for (int b = 0; b < nb; ++b) combinePair(S, C, work[b], work[kReduceBlocks + b]);
     out[0] = S + C;
-    *counter = 0u;
 }
 
 // out = a x + b y (y may be null)
@@ -940,9 +953,11 @@ void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s)
         launchPressureMode<PMODE_TANGENT>(a, s);
 }
 
-void launchSumComp(const double* a, int n, double* work, unsigned* counter, double* out, hipStream_t s)
+void launchSumComp(const double* a, int n, double* work, double* out, hipStream_t s)
 {
-    kSumComp<<<reduceGrid(n), kRT, 0, s>>>(a, n, work, counter, out);
+    const int g = reduceGrid(n);
+    kSumComp<<<g, kRB, 0, s>>>(a, n, work);
+    kSumCompFinish<<<1, 64, 0, s>>>(work, g, out);
 }
 
 void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s)
@@ -963,21 +978,27 @@ void launchGather(const GatherArgs& a, hipStream_t s)
     }
 }
 
-void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, unsigned* counter,
-                  double* out, hipStream_t s, const double* xt, const double* cb)
+void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, double* out,
+                  hipStream_t s, const double* xt, const double* cb)
 {
-    if (!c)
-        kDotMax<false, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, nullptr, nullptr, n, work, counter, out);
-    else if (!xt)
-        kDotMax<true, false><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, cb, nullptr, n, work, counter, out);
-    else
-        kDotMax<true, true><<<reduceGrid(n), kRT, 0, s>>>(a, b, c, cb, xt, n, work, counter, out);
+    const int g = reduceGrid(n);
+    if (!c) {
+        kDotMax<false, false><<<g, kRB, 0, s>>>(a, b, c, nullptr, nullptr, n, work);
+        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{out});
+    } else if (!xt) {
+        kDotMax<true, false><<<g, kRB, 0, s>>>(a, b, c, cb, nullptr, n, work);
+        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{out});
+    } else {
+        kDotMax<true, true><<<g, kRB, 0, s>>>(a, b, c, cb, xt, n, work);
+        kReduceFinish<4><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<4>{out});
+    }
 }
 
-void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter, double* ab, double* bb,
-                hipStream_t s)
+void launchDot2(const double* a, const double* b, int n, double* work, double* ab, double* bb, hipStream_t s)
 {
-    kDot2<<<reduceGrid(n), kRT, 0, s>>>(a, b, n, work, counter, ab, bb);
+    const int g = reduceGrid(n);
+    kDot2<<<g, kRB, 0, s>>>(a, b, n, work);
+    kReduceFinish<2><<<1, kRF, 0, s>>>(work, g, 0u, EpiStore2{ab, bb});
 }
 
 void launchStep(double* x, const double* xp, const double* d, double alpha, int n, hipStream_t s)
@@ -997,50 +1018,77 @@ void launchScale(double* y, const double* x, double a, int n, hipStream_t s)
 }
 
 void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, const double* w, const double* den,
-                       const double* aj, double* cout, int mode, int n, double* work, unsigned* counter,
-                       hipStream_t s)
+                       const double* aj, double* cout, int mode, int n, double* work, hipStream_t s)
 {
-    kTwoLoop<<<reduceGrid(n), kRT, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, den, aj, cout, mode, n,
-                                                  work, counter);
+    const int g = reduceGrid(n);
+    kTwoLoop<<<g, kRB, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, n, work);
+    kReduceFinish<1><<<1, kRF, 0, s>>>(work, g, 0u, EpiTwoLoopStep{den, aj, cout, mode});
 }
 
-template <int KC>
-void twoLoopPass(const double* q, const double* R, const double* base, double sgnB, const double* axR,
-                 double sgnC, int n, int nc, int first, int newest, int mcap, const double* G, const double* ys,
-                 const double* alphaIn, double* coef, double* out, double* work, unsigned* counter, hipStream_t s,
-                 const double* dw = nullptr, const double* rs = nullptr, double* slope = nullptr)
+// calls f(std::integral_constant<int, nc>) for 1 <= nc <= 16
+template <typename F>
+void withPairCount(int nc, F&& f)
 {
-    kTwoLoopDots<KC><<<historyGrid(n), kRT, 0, s>>>(q, R, n, nc, first, newest, mcap, G, ys, alphaIn, coef,
-                                                         work, counter, dw, rs, slope);
-    if (n > 0) kMultiAxpy<KC><<<blocks(n), kThreads, 0, s>>>(base, sgnB, axR, coef, sgnC, n, nc, out);
+    switch (nc) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 9: f(std::integral_constant<int, 9>{}); break;
+    case 10: f(std::integral_constant<int, 10>{}); break;
+    case 11: f(std::integral_constant<int, 11>{}); break;
+    case 12: f(std::integral_constant<int, 12>{}); break;
+    case 13: f(std::integral_constant<int, 13>{}); break;
+    case 14: f(std::integral_constant<int, 14>{}); break;
+    case 15: f(std::integral_constant<int, 15>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    default: break;
+    }
 }
+static_assert(kMaxCompact == 16, "withPairCount covers 1..kMaxCompact");
 
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
-                     double* alpha, double* rs, double* out, double* work, unsigned* counter, hipStream_t s)
+                     double* alpha, double* rs, double* out, double* work, hipStream_t s)
 {
-    if (mcap <= 8) {
-        kGramFirst<8><<<historyGrid(n), kRT, 0, s>>>(S, Y, x, xp, r, rp, n, nc, loc, mcap, G, ys, yy, theta, alpha,
-                                                     rs, work, counter);
-        if (n > 0) kMultiAxpy<8><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, nc, out);
-    } else {
-        kGramFirst<16><<<historyGrid(n), kRT, 0, s>>>(S, Y, x, xp, r, rp, n, nc, loc, mcap, G, ys, yy, theta,
-                                                      alpha, rs, work, counter);
-        if (n > 0) kMultiAxpy<16><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, nc, out);
-    }
+    withPairCount(nc, [&](auto ncc) {
+        constexpr int NC = decltype(ncc)::value;
+        const int g = historyGrid(n);
+        kGramFirst<NC><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
+        if (mcap <= 8) {
+            if constexpr (NC <= 8)
+                kReduceFinish<3 * NC + 1><<<1, kRF, 0, s>>>(
+                    work, g, 0u, EpiGramFirst<8, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
+        } else {
+            kReduceFinish<3 * NC + 1><<<1, kRF, 0, s>>>(
+                work, g, 0u, EpiGramFirst<16, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
+        }
+        if (n > 0) kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, out);
+    });
 }
 
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
                          const double* G, const double* ys, const double* alpha, double* coef, double* out,
-                         double* work, unsigned* counter, hipStream_t s, const double* dw, const double* rs,
-                         double* slope)
+                         double* work, hipStream_t s, const double* dw, const double* rs, double* slope)
 {
-    if (mcap <= 8)
-        twoLoopPass<8>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s, dw,
-                       rs, slope);
-    else
-        twoLoopPass<16>(h, Y, h, 1.0, S, 1.0, n, nc, 0, newest, mcap, G, ys, alpha, coef, out, work, counter, s,
-                        dw, rs, slope);
+    withPairCount(nc, [&](auto ncc) {
+        constexpr int NC = decltype(ncc)::value;
+        const int g = historyGrid(n);
+        kTwoLoopDots<NC><<<g, kRB, 0, s>>>(h, Y, n, dw, work);
+        if (mcap <= 8) {
+            if constexpr (NC <= 8)
+                kReduceFinish<NC + 1><<<1, kRF, 0, s>>>(
+                    work, g, 0u, EpiTwoLoopDots<8, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
+        } else {
+            kReduceFinish<NC + 1><<<1, kRF, 0, s>>>(
+                work, g, 0u, EpiTwoLoopDots<16, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
+        }
+        if (n > 0) kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(h, 1.0, S, coef, 1.0, n, out);
+    });
 }
 
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s)

@@ -80,20 +80,19 @@ void launchGather(const GatherArgs& a, hipStream_t s);
 
 // ---- vectors ----------------------------------------------------------------
 // deterministic reductions: results written to device scalars; the block
-// partials and an arrival counter live in `work` (see kReduceWorkDoubles)
+// partials live in `work` (see kReduceWorkDoubles); two launches each
 constexpr int kReduceBlocks = 256;
 constexpr int kMaxCompact = 16;  // history size handled by the compact two-loop
 constexpr int kReduceWorkDoubles = (3 * kMaxCompact + 2) * kReduceBlocks + 64;
 // out[0] = a.b, out[1] = max|a|, out[2] = max|c - cb| (c may be null, cb
 // null means 0); with xt (and c), out[3] = |c - xt|^2
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work,
-                  unsigned* counter, double* out, hipStream_t s, const double* xt = nullptr,
-                  const double* cb = nullptr);
+                  double* out, hipStream_t s, const double* xt = nullptr, const double* cb = nullptr);
 // *ab = a.b, *bb = b.b
-void launchDot2(const double* a, const double* b, int n, double* work, unsigned* counter,
-                double* ab, double* bb, hipStream_t s);
+void launchDot2(const double* a, const double* b, int n, double* work, double* ab, double* bb,
+                hipStream_t s);
 // *out = sum a (Neumaier lanes + TwoSum tree, reproducible)
-void launchSumComp(const double* a, int n, double* work, unsigned* counter, double* out, hipStream_t s);
+void launchSumComp(const double* a, int n, double* work, double* out, hipStream_t s);
 // out = a x + b y (y may be null)
 void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s);
 // x = xp + alpha d
@@ -107,19 +106,19 @@ void launchScale(double* y, const double* x, double a, int n, hipStream_t s);
 //   *cout = mode ? (*aj - out / *den) : out / *den
 void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, const double* w,
                        const double* den, const double* aj, double* cout, int mode, int n,
-                       double* work, unsigned* counter, hipStream_t s);
+                       double* work, hipStream_t s);
 // compact two-loop (same recursion as LBFGSMat::apply_Hv, evaluated from
 // Gram products).  History slots 0..nc-1 of the ring S, Y ([m][n]), newest
 // slot `newest`; G[i*16+k] = s_i . y_k; rho[i] = 1 / (s_i . y_i).  The new
 // pair and loop 1 run in launchGramFirst (declared below).
 // loop 2: beta_i = (y_i . r_i) / ys_i oldest -> newest with r_0 = h (the H0
-// output), then out = h + sum (alpha_i - beta_i) s_i; with dw (and rs_k =
-// s_k . dw from launchGramFirst), also *slope = dw . out, formed as
-// dw . h + sum (alpha_k - beta_k) rs_k
+// output), then out = h + sum (alpha_i - beta_i) s_i and, with rs_k =
+// s_k . dw from launchGramFirst, *slope = dw . out, formed as
+// dw . h + sum (alpha_k - beta_k) rs_k (dw required; 1 <= nc <= 16)
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest,
                          int mcap, const double* G, const double* ys, const double* alpha, double* coef,
-                         double* out, double* work, unsigned* counter, hipStream_t s, const double* dw = nullptr,
-                         const double* rs = nullptr, double* slope = nullptr);
+                         double* out, double* work, hipStream_t s, const double* dw, const double* rs,
+                         double* slope);
 // q += (*c) * v   (or q -= when neg)
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s);
 // q /= (*theta)
@@ -132,7 +131,7 @@ void launchTheta(const double* ys, const double* yy, double* theta, hipStream_t 
 // oldest; then out = -r - sum alpha_i y_i
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
-                     double* alpha, double* rs, double* out, double* work, unsigned* counter, hipStream_t s);
+                     double* alpha, double* rs, double* out, double* work, hipStream_t s);
 
 // ---- supernodal triangular solves (3 right-hand sides) ---------------------
 struct TrsvArgs {

@@ -484,7 +484,7 @@ std::vector<double> freeCoords(const Model& m)
     return xyz;
 }
 
-bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg)
+bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int denseSubtree)
 {
     if (m.nFree == 0) {
         msg = "no free DOF";
@@ -493,10 +493,11 @@ bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg)
     Csc A;
     scalarLaplacian(m, A);
     const std::vector<int> perm = nestedDissection(A, freeCoords(m));
-    return supernodalCholesky(A, perm, F, msg);
+    return supernodalCholesky(A, perm, F, msg, denseSubtree);
 }
 
-bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg)
+bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,
+                  int denseSubtree)
 {
     const int np = static_cast<int>(parts.size());
     F.clear();
@@ -504,7 +505,7 @@ bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>
     std::vector<std::string> msgs(np);
     std::vector<char> ok(np, 0);
 #pragma omp parallel for schedule(dynamic)
-    for (int i = 0; i < np; ++i) ok[i] = factorModel(parts[i], F[i], msgs[i]) ? 1 : 0;
+    for (int i = 0; i < np; ++i) ok[i] = factorModel(parts[i], F[i], msgs[i], denseSubtree) ? 1 : 0;
     for (int i = 0; i < np; ++i)
         if (!ok[i]) {
             msg = "instance " + std::to_string(i) + ": " + msgs[i];

@@ -70,8 +70,10 @@ std::vector<double> freeCoords(const Model& m);
 
 // approximate Jacobian -> nested dissection -> supernodal factor (false: not
 // SPD or no free DOF, msg says which)
-bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg);
+bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg,
+                 int denseSubtree = kDefaultDenseSubtree);
 // the same for independent parts, in parallel over the parts
-bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg);
+bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,
+                  int denseSubtree = kDefaultDenseSubtree);
 
 }  // namespace efem

@@ -226,7 +226,7 @@ void mergeFactors(std::vector<SupernodalFactor>& parts, SupernodalFactor& out, s
 }
 
 bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, SupernodalFactor& F,
-                        std::string& msg)
+                        std::string& msg, int denseSubtree)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const int n = A.n;
@@ -363,13 +363,12 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     // (no supernode is wider than kMaxWidth: the device stages one
     // supernode's right-hand sides in a kMaxWidth-wide LDS buffer)
     const int kMaxWidth = kMaxSupernodeWidth;
-    // Every maximal subtree of at most kDenseSubtree columns (contiguous in
+    // Every maximal subtree of at most denseSubtree columns (contiguous in
     // postorder) becomes one dense supernode: the bottom of the tree loses
     // several levels -- each level is one launch per sweep of the device
     // triangular solves -- for a few percent more stored entries (on the
-    // 265k-DOF beam: 17 -> 12 levels, B +7 %, factor flops +2 %).
-    constexpr int kDenseSubtree = 64;
-    static_assert(kDenseSubtree <= kMaxSupernodeWidth, "dense subtrees fit the device staging width");
+    // 265k-DOF beam with 64: 17 -> 12 levels, B +7 %, factor flops +2 %).
+    const int kDenseSubtree = std::max(1, std::min(denseSubtree, kMaxSupernodeWidth));
     std::vector<int> sz(n, 1), denseRoot(n, -1);
     for (int j = 0; j < n; ++j)
         if (parent[j] != -1) sz[parent[j]] += sz[j];

@@ -50,9 +50,14 @@ struct SupernodalFactor {
 // order: nested dissection on the node graph using coordinates xyz[n][3]
 std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz);
 
+// Every maximal elimination subtree of at most `denseSubtree` columns
+// becomes one dense supernode (fewer levels for the device triangular
+// solves, a few percent more stored entries); 1..kMaxSupernodeWidth.
+constexpr int kDefaultDenseSubtree = 64;
+
 // returns false if a pivot is not positive (the reference's fallback case)
 bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F,
-                        std::string& msg);
+                        std::string& msg, int denseSubtree = kDefaultDenseSubtree);
 
 // Forest of independent factors (batched instances): part i's columns,
 // supernodes, rows and blocks follow those of part i-1; levels stay per part.

@@ -216,6 +216,15 @@ const char* efem_host_dense_kernel(void);
  *                       is evaluated from Gram products s_i.y_k in two
  *                       multi-dot passes; 0: one reduction per step exactly
  *                       as LBFGSMat::apply_Hv orders them
+ *   "lookahead"         1 (default): while the host reads a first line-search
+ *                       trial's scalars the device already forms the next
+ *                       direction and runs the next first trial, discarded
+ *                       when the trial is rejected or the solve stops; the
+ *                       iterates are those of 0 (no look-ahead) bit for bit
+ *   "dense_subtree"     1..768 (default 64): elimination subtrees of at most
+ *                       this many columns become one dense supernode (fewer
+ *                       triangular-solve levels, more stored entries);
+ *                       releases the factor (the next solve factors again)
  *   "bwd_narrow_items"  >= 1 (default 4096): levels of the backward solve with
  *                       at least this many 4-column work items use
  *                       16-column items instead; releases the factor (the

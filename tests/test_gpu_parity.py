@@ -250,6 +250,28 @@ def test_apply_h0_narrow_backward_items(prec, tol, rng):
         assert np.array_equal(ctx.apply_h0(b), x16)
 
 
+@pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
+def test_apply_h0_dense_subtree_sizes(prec, tol, rng):
+    # elimination subtrees of up to 1, 16, 64 (default) and 256 columns
+    # merged into dense supernodes: the same factor up to its explicit zeros,
+    # so the same solve (to rounding) as a direct solve, with fewer levels
+    # and more stored entries as the size grows
+    m = E.Mesh.workload(2, [30, 6, 6])
+    K = O.Model(m).K(scalar=False)
+    b = rng.standard_normal(m.ndof)
+    x_ref = spla.spsolve(K.tocsc(), b)
+    nnz = []
+    with E.Context(m, prec) as ctx:
+        for ds in (1, 16, 64, 256):
+            ctx.set_option("dense_subtree", ds)
+            assert ctx.factor()
+            nnz.append(ctx.factor_nnz)
+            assert rel(ctx.apply_h0(b), x_ref) <= tol, ds
+        with pytest.raises(E.EfemError):
+            ctx.set_option("dense_subtree", 0)
+    assert nnz == sorted(nnz) and nnz[-1] > nnz[0]
+
+
 @pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
 def test_apply_h0_graph_replays_match_eager(prec, rng):
     # the captured graph is replayed several times with eager work between
