@@ -99,6 +99,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_factor": (C.c_int, [vp, _ip]),
         "efem_factor_nnz": (C.c_longlong, [vp]),
         "efem_factor_release": (C.c_int, [vp]),
+        "efem_factor_phases": (C.c_int, [vp, _dp]),
         "efem_profile_read_h0": (C.c_int, [vp, _dp, _ip]),
         "efem_apply_h0": (C.c_int, [vp, _dp, _dp]),
         "efem_solve": (C.c_int, [vp, _dp, C.POINTER(LbfgsParams), C.POINTER(SolveStats)]),
@@ -436,6 +437,13 @@ class Context:
 
     def factor_release(self):
         self._check(self._lib.efem_factor_release(self._h), "efem_factor_release")
+
+    def factor_phases(self) -> dict:
+        """Seconds of the last factorisation by phase (efem_factor_phases)."""
+        out = np.zeros(8)
+        self._check(self._lib.efem_factor_phases(self._h, _ptr(out, C.c_double)), "efem_factor_phases")
+        keys = ("host_model", "symbolic", "numeric", "descriptors", "repack", "upload", "other_buffers", "total")
+        return dict(zip(keys, out.tolist()))
 
     def commit(self, u):
         """Constraint::updateDx: u becomes the committed displacement the

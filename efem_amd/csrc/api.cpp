@@ -97,6 +97,10 @@ struct efem_ctx {
     DeviceFactor fac;
     bool factorFailed = false;
     double factorSeconds = 0.0;
+    // last factorisation, seconds: {host model (assembly, ordering, symbolic,
+    // numeric), of which symbolic, of which numeric, device descriptors and
+    // maps, block repack, block upload, other device buffers, total}
+    double factorPhases[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<double> trace;
     // profiling of the element kernel and of H0 applies
     bool profiling = false;
@@ -458,6 +462,10 @@ int pinnedStage(efem_ctx* c, size_t bytes, void** p)
 
 int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset)
 {
+    using clock = std::chrono::steady_clock;
+    const auto u0 = clock::now();
+    auto since = [](clock::time_point a) { return std::chrono::duration<double>(clock::now() - a).count(); };
+    double packS = 0.0, copyS = 0.0;
     releaseFactor(c);
     DeviceFactor& d = c->fac;
     AllocInto into(c, &d.allocs);
@@ -577,15 +585,20 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         const long long nb = dval[F.nsuper];
         const size_t es = fac32(c) ? sizeof(float) : sizeof(double);
         void* h = nullptr;
+        c->factorPhases[3] = since(u0);
+        const auto p0 = clock::now();
         if ((rc = pinnedStage(c, static_cast<size_t>(std::max(nb, 1LL)) * es, &h))) return rc;
         static_assert(efem::dev::ldOf(1) == efem::dev::kLdAlign && efem::dev::ldOf(efem::dev::kLdAlign + 1) ==
                           2 * efem::dev::kLdAlign,
                       "packBlocks pads to multiples of kLdAlign");
         efem::packBlocks(F, dval, efem::dev::kLdAlign, fac32(c), h);
+        packS = since(p0);
+        const auto q0 = clock::now();
         double* q = nullptr;
         if ((rc = devAlloc(c, &q, (static_cast<size_t>(std::max(nb, 1LL)) * es + 7) / 8))) return rc;
         HIP_CHECK(c, hipMemcpyAsync(q, h, static_cast<size_t>(nb) * es, hipMemcpyHostToDevice, c->stream));
         HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        copyS = since(q0);
         d.B = q;
     }
     if ((rc = upload(c, &d.fptr, fptr))) return rc;
@@ -603,6 +616,9 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(F.n) * es + 7) / 8))) return rc;
     d.z = q;
     d.ready = true;
+    c->factorPhases[4] = packS;
+    c->factorPhases[5] = copyS;
+    c->factorPhases[6] = since(u0) - c->factorPhases[3] - packS - copyS;
     return EFEM_OK;
 }
 
@@ -630,11 +646,15 @@ int factorInternal(efem_ctx* c, int* failed)
         c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return EFEM_OK;  // not SPD: the reference falls back to I/theta
     }
+    c->factorPhases[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->factorPhases[1] = F.seconds - F.numericSeconds;
+    c->factorPhases[2] = F.numericSeconds;
     int rc = uploadFactor(c, F, c->parts.empty() ? nullptr : &snOffset);
     if (rc) return rc;
     c->factorFailed = false;
     if (failed) *failed = 0;
     c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->factorPhases[7] = c->factorSeconds;
     return EFEM_OK;
 }
 
@@ -966,6 +986,13 @@ int efem_factor(efem_ctx* c, int* failed)
 }
 
 long long efem_factor_nnz(const efem_ctx* c) { return c && c->fac.ready ? c->fac.nnz : 0; }
+
+int efem_factor_phases(const efem_ctx* c, double* out8)
+{
+    if (!c || !out8) return EFEM_ERR_ARG;
+    std::copy(c->factorPhases, c->factorPhases + 8, out8);
+    return EFEM_OK;
+}
 
 int efem_factor_release(efem_ctx* c)
 {

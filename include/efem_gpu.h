@@ -123,7 +123,7 @@ typedef struct efem_solve_stats {
     int ls_trials;
     int precond_failed;  /* factorisation failed -> H0 = I/theta (LBFGS.h:76-80) */
     double rinf;         /* final ||r||_inf */
-    double xinf;         /* final ||x||_inf */
+    double xinf;         /* final ||x - u_c||_inf (u_c: efem_commit, else 0) */
     double factor_seconds;
     double solve_seconds;
     long long factor_nnz; /* nnz of the (scalar) Cholesky factor */
@@ -153,6 +153,11 @@ int efem_factor(efem_ctx* ctx, int* failed);
 long long efem_factor_nnz(const efem_ctx* ctx);
 /* drop the factor (the next efem_solve with use_precond factors again) */
 int efem_factor_release(efem_ctx* ctx);
+/* seconds spent by the last factorisation (diagnostic, no counterpart in
+ * the reference): {host model: Laplacian assembly + ordering + symbolic +
+ * numeric, of which symbolic, of which numeric, device descriptors and maps,
+ * block repack, block upload, other device buffers, total} */
+int efem_factor_phases(const efem_ctx* ctx, double* out8);
 /* out = K^-1 v (host pointers); requires a successful efem_factor */
 int efem_apply_h0(efem_ctx* ctx, const double* v, double* out);
 
