@@ -38,6 +38,10 @@ struct DeviceFactor {
     int instLevels = 0;
     std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level, in items
     std::vector<int> fwdWide, bwdWide;               // per level: 16-wave items
+    // per level: forward extend-add table (K = 0: none) at fwdTab + fwdTabOff
+    int* fwdTab = nullptr;
+    std::vector<long long> fwdTabOff;
+    std::vector<int> fwdK, fwdW;
     // the whole apply captured as one HIP graph over fixed staging buffers
     double *in = nullptr, *out = nullptr, *bp = nullptr;
     hipGraph_t graph = nullptr;
@@ -118,6 +122,7 @@ struct efem_ctx {
     bool useGraph = true;    // option "h0_graph"
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
     bool lookahead = true;       // option "lookahead"
+    bool fwdTables = true;       // option "fwd_tables"
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
 };
@@ -366,7 +371,8 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out)
     const efem::dev::TrsvArgs a = trsvArgs(c);
     efem::dev::launchPermute(a, v, f.bp, c->stream);
     for (int l = 0; l < f.nlevels; ++l)
-        efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], f.fwdWide[l], c->stream);
+        efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], f.fwdWide[l], c->stream,
+                                     f.fwdK[l] ? f.fwdTab + f.fwdTabOff[l] : nullptr, f.fwdK[l], f.fwdW[l]);
     for (int l = f.nlevels - 1; l >= 0; --l)
         efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], f.bwdWide[l], c->stream);
     efem::dev::launchUnpermute(a, out, c->stream);
@@ -538,6 +544,35 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
             }
         d.fwdCnt[l] = static_cast<int>(items.size() / 2) - d.fwdOff[l];
     }
+    // forward extend-add tables: per level with at most kFwdTableMaxK
+    // contributions per front row (not for batches, whose persistent kernel
+    // walks fptr / fidx); absent entries name the zero slot nt
+    d.fwdTabOff.assign(F.nlevels, 0);
+    d.fwdK.assign(F.nlevels, 0);
+    d.fwdW.assign(F.nlevels, 0);
+    std::vector<int> tab;
+    if (!snOffset && c->fwdTables) {
+        std::vector<efem::FwdTableItem> titems;
+        long long size = 0;
+        for (int l = 0; l < F.nlevels; ++l) {
+            int K = 0, W = 0;
+            for (int s : byLevel[l]) {
+                W = std::max(W, w[s]);
+                for (int r = 0; r < mm[s]; ++r)
+                    K = std::max(K, fptr[F.rowPtr[s] + r + 1] - fptr[F.rowPtr[s] + r]);
+            }
+            if (K < 1 || K > efem::dev::kFwdTableMaxK) continue;
+            d.fwdK[l] = K;
+            d.fwdW[l] = W;
+            d.fwdTabOff[l] = size;
+            for (int s : byLevel[l])
+                for (int r0 = 0; r0 < mm[s]; r0 += 64) {
+                    titems.push_back({s, r0, K, W, size});
+                    size += static_cast<long long>(K) * (64 + W);
+                }
+        }
+        efem::fillForwardTables(F, fptr, fidx, titems, size, nt, tab);
+    }
     for (int l = 0; l < F.nlevels; ++l) {
         d.bwdOff[l] = static_cast<int>(items.size() / 2);
         const int cols = d.bwdWide[l] == efem::dev::kBwdNarrow ? 16 : 4;
@@ -603,6 +638,7 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     }
     if ((rc = upload(c, &d.fptr, fptr))) return rc;
     if ((rc = upload(c, &d.fidx, fidx))) return rc;
+    if (!tab.empty() && (rc = upload(c, &d.fwdTab, tab))) return rc;
     if ((rc = devAlloc(c, &d.in, 3 * static_cast<size_t>(F.n) + 3))) return rc;
     if ((rc = devAlloc(c, &d.out, 3 * static_cast<size_t>(F.n) + 3))) return rc;
     if ((rc = devAlloc(c, &d.bp, 3 * static_cast<size_t>(F.n) + 3))) return rc;
@@ -611,7 +647,9 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     double* q = nullptr;
     if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(F.n) * es + 7) / 8))) return rc;
     d.y = q;
-    if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(std::max(nt, 1)) * es + 7) / 8))) return rc;
+    // update vectors + the zero slot nt that absent table entries name
+    if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(nt + 1) * es + 7) / 8))) return rc;
+    HIP_CHECK(c, hipMemsetAsync(q, 0, (3 * static_cast<size_t>(nt + 1) * es + 7) / 8 * 8, c->stream));
     d.t = q;
     if ((rc = devAlloc(c, &q, (3 * static_cast<size_t>(F.n) * es + 7) / 8))) return rc;
     d.z = q;
@@ -1475,6 +1513,13 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
     }
     if (k == "lookahead") {
         c->lookahead = value != 0.0;
+        return EFEM_OK;
+    }
+    if (k == "fwd_tables") {
+        // per-item extend-add tables for the forward sweep; takes effect at
+        // the next factorisation
+        c->fwdTables = value != 0.0;
+        releaseFactor(c);
         return EFEM_OK;
     }
     const int iv = static_cast<int>(value);

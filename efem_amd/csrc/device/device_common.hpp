@@ -430,9 +430,19 @@ __device__ __forceinline__ long long itemVal(const int4& d1)
 // (bp[3 j + c] = b[3 perm[j] + c]).  Multifrontal: the front of s is its
 // rows of bp plus its children's update vectors (extend-add through
 // fptr/fidx); y_s = L11^-1 f[:w], u_s = f[w:] - (L21 L11^-1) f[:w].
-template <typename T, int WAVES = 4>
+// Extend-add table of one forward item (KT > 0): KT x 64 update-vector
+// positions for the item's 64 rows, then KT x W for the supernode's own
+// columns (k-major); absent entries point to a zero slot.  The table's
+// address depends on the item index only, so its loads do not wait for the
+// descriptor.
+struct FwdTable {
+    const int* tab;  // this item's entries
+    int W;           // own-column stride (the level's widest supernode)
+};
+
+template <typename T, int WAVES = 4, int KT = 0>
 __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __restrict__ bp, int4 d0, int4 d1,
-                                            TrsvFwdLds<T, WAVES>& L)
+                                            TrsvFwdLds<T, WAVES>& L, FwdTable ft = FwdTable{nullptr, 0})
 {
     constexpr int NT = 64 * WAVES;
     const int r0 = d0.x, w = d0.y, m = d0.z, j0 = d0.w, fr0 = d1.x, tb = d1.y;
@@ -460,30 +470,80 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     for (int q = 0; q < NBT; ++q) nb[q] = kb + q < ke ? Bc[static_cast<size_t>(kb + q) * ld] : T(0);
     // update rows: the children's contributions (wave 0, lane = row)
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    if (wave == 0 && r >= w && r < m) {
-        const int p1 = a.fptr[fr0 + r + 1];
-        for (int p = a.fptr[fr0 + r]; p < p1; ++p) {
-            const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
-            f0 += static_cast<double>(uv[0]);
-            f1 += static_cast<double>(uv[1]);
-            f2 += static_cast<double>(uv[2]);
+    if constexpr (KT > 0) {
+        // every position loaded, then every value: two dependent loads, none
+        // waiting for the descriptor (pivot rows and absent entries read
+        // the zero slot; sums in the order of the row lists)
+        if (wave == 0) {
+            int q[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) q[k] = ft.tab[k * 64 + lane];
+            T uv[KT][3];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                const T* up = u + 3 * static_cast<size_t>(q[k]);
+                uv[k][0] = up[0];
+                uv[k][1] = up[1];
+                uv[k][2] = up[2];
+            }
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                f0 += static_cast<double>(uv[k][0]);
+                f1 += static_cast<double>(uv[k][1]);
+                f2 += static_cast<double>(uv[k][2]);
+            }
         }
-    }
-    // front entries of the supernode's own columns (short lists: at most one
-    // entry per child)
-    for (int k = threadIdx.x; k < w; k += NT) {
-        const double* bk = bp + 3 * static_cast<size_t>(j0 + k);
-        double v0 = bk[0], v1 = bk[1], v2 = bk[2];
-        const int p1 = a.fptr[fr0 + k + 1];
-        for (int p = a.fptr[fr0 + k]; p < p1; ++p) {
-            const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
-            v0 += static_cast<double>(uv[0]);
-            v1 += static_cast<double>(uv[1]);
-            v2 += static_cast<double>(uv[2]);
+        const int* own = ft.tab + KT * 64;
+        for (int k = threadIdx.x; k < w; k += NT) {
+            int q[KT];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) q[j] = own[j * ft.W + k];
+            const double* bk = bp + 3 * static_cast<size_t>(j0 + k);
+            double v0 = bk[0], v1 = bk[1], v2 = bk[2];
+            T uv[KT][3];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) {
+                const T* up = u + 3 * static_cast<size_t>(q[j]);
+                uv[j][0] = up[0];
+                uv[j][1] = up[1];
+                uv[j][2] = up[2];
+            }
+#pragma unroll
+            for (int j = 0; j < KT; ++j) {
+                v0 += static_cast<double>(uv[j][0]);
+                v1 += static_cast<double>(uv[j][1]);
+                v2 += static_cast<double>(uv[j][2]);
+            }
+            sb[3 * k] = static_cast<T>(v0);
+            sb[3 * k + 1] = static_cast<T>(v1);
+            sb[3 * k + 2] = static_cast<T>(v2);
         }
-        sb[3 * k] = static_cast<T>(v0);
-        sb[3 * k + 1] = static_cast<T>(v1);
-        sb[3 * k + 2] = static_cast<T>(v2);
+    } else {
+        if (wave == 0 && r >= w && r < m) {
+            const int p1 = a.fptr[fr0 + r + 1];
+            for (int p = a.fptr[fr0 + r]; p < p1; ++p) {
+                const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+                f0 += static_cast<double>(uv[0]);
+                f1 += static_cast<double>(uv[1]);
+                f2 += static_cast<double>(uv[2]);
+            }
+        }
+        // front entries of the supernode's own columns (short lists: at most
+        // one entry per child)
+        for (int k = threadIdx.x; k < w; k += NT) {
+            const double* bk = bp + 3 * static_cast<size_t>(j0 + k);
+            double v0 = bk[0], v1 = bk[1], v2 = bk[2];
+            const int p1 = a.fptr[fr0 + k + 1];
+            for (int p = a.fptr[fr0 + k]; p < p1; ++p) {
+                const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+                v0 += static_cast<double>(uv[0]);
+                v1 += static_cast<double>(uv[1]);
+                v2 += static_cast<double>(uv[2]);
+            }
+            sb[3 * k] = static_cast<T>(v0);
+            sb[3 * k + 1] = static_cast<T>(v1);
+            sb[3 * k + 2] = static_cast<T>(v2);
+        }
     }
     __syncthreads();
     T acc0 = T(0), acc1 = T(0), acc2 = T(0);

@@ -836,6 +836,17 @@ __global__ __launch_bounds__(64 * WAVES) void kTrsvFwd(TrsvArgs a, const double*
     trsvFwdItem<T, WAVES>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L);
 }
 
+// the same with the level's extend-add table (KT entries per front row)
+template <typename T, int WAVES, int KT>
+__global__ __launch_bounds__(64 * WAVES) void kTrsvFwdTab(TrsvArgs a, const double* __restrict__ bp,
+                                                          const int4* __restrict__ items,
+                                                          const int* __restrict__ tab, int W)
+{
+    __shared__ TrsvFwdLds<T, WAVES> L;
+    const FwdTable ft{tab + static_cast<size_t>(blockIdx.x) * KT * (64 + W), W};
+    trsvFwdItem<T, WAVES, KT>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L, ft);
+}
+
 template <typename T, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void kTrsvBwd(TrsvArgs a, const int4* __restrict__ items)
 {
@@ -1111,9 +1122,33 @@ void launchPermute(const TrsvArgs& a, const double* b, double* bp, hipStream_t s
     if (a.n > 0) kPermute<<<blocks(a.n), kThreads, 0, s>>>(a.perm, b, bp, a.n);
 }
 
-void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, int wide, hipStream_t s)
+template <typename T, int WAVES>
+void fwdTabLaunch(const TrsvArgs& a, const double* bp, const int4* items, int nitems, const int* tab, int K, int W,
+                  hipStream_t s)
+{
+    constexpr int NT = 64 * WAVES;
+    switch (K) {
+        case 1: kTrsvFwdTab<T, WAVES, 1><<<nitems, NT, 0, s>>>(a, bp, items, tab, W); break;
+        case 2: kTrsvFwdTab<T, WAVES, 2><<<nitems, NT, 0, s>>>(a, bp, items, tab, W); break;
+        case 3: kTrsvFwdTab<T, WAVES, 3><<<nitems, NT, 0, s>>>(a, bp, items, tab, W); break;
+        default: kTrsvFwdTab<T, WAVES, 4><<<nitems, NT, 0, s>>>(a, bp, items, tab, W); break;
+    }
+}
+
+void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, int wide, hipStream_t s,
+                       const int* tab, int K, int W)
 {
     if (nitems <= 0) return;
+    if (tab && K >= 1 && K <= kFwdTableMaxK) {
+        if (a.mixed) {
+            if (wide) fwdTabLaunch<float, 16>(a, bp, items, nitems, tab, K, W, s);
+            else fwdTabLaunch<float, 4>(a, bp, items, nitems, tab, K, W, s);
+        } else {
+            if (wide) fwdTabLaunch<double, 16>(a, bp, items, nitems, tab, K, W, s);
+            else fwdTabLaunch<double, 4>(a, bp, items, nitems, tab, K, W, s);
+        }
+        return;
+    }
     if (a.mixed) {
         if (wide) kTrsvFwd<float, 16><<<nitems, 1024, 0, s>>>(a, bp, items);
         else kTrsvFwd<float, 4><<<nitems, 256, 0, s>>>(a, bp, items);

@@ -153,8 +153,14 @@ void launchPermute(const TrsvArgs& a, const double* b, double* bp, hipStream_t s
 // y_s = L11^-1 f_s[:w], u_s = f_s[w:] - (L21 L11^-1) f_s[:w]
 // wide = 1: 1024-lane blocks whose 16 waves split the supernode's columns
 // (for levels of wide supernodes), else 256 lanes / 4 waves
+// tab (may be null): the level's extend-add table, K (1..kFwdTableMaxK)
+// update-vector positions per front row: for item i at tab + i K (64 + W),
+// K x 64 for its 64 rows, then K x W for the supernode's own columns
+// (k-major; absent entries name the zero slot after the last update row);
+// the sums are those of fptr / fidx, in the same order
+constexpr int kFwdTableMaxK = 4;
 void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, int nitems, int wide,
-                       hipStream_t s);
+                       hipStream_t s, const int* tab = nullptr, int K = 0, int W = 0);
 // one level of the backward solve; descriptors {k0, w, m, j0}, {row offset,
 // 0, B offset lo, hi}: four columns per item (wide = 1: four waves per
 // column split its rows, for levels of tall supernodes; wide = kBwdNarrow:

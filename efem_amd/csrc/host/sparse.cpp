@@ -677,6 +677,32 @@ void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x)
         for (int c = 0; c < 3; ++c) x[3 * static_cast<size_t>(F.perm[j]) + c] = z[3 * static_cast<size_t>(j) + c];
 }
 
+void fillForwardTables(const SupernodalFactor& F, const std::vector<int>& fptr, const std::vector<int>& fidx,
+                       const std::vector<FwdTableItem>& items, long long size, int zeroSlot,
+                       std::vector<int>& tab)
+{
+    tab.resize(static_cast<size_t>(size));
+    const int ni = static_cast<int>(items.size());
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ni; ++i) {
+        const FwdTableItem& t = items[i];
+        const int fr0 = F.rowPtr[t.s], m = F.rowPtr[t.s + 1] - fr0, w = F.first[t.s + 1] - F.first[t.s];
+        int* q = tab.data() + t.base;
+        std::fill(q, q + static_cast<size_t>(t.K) * (64 + t.W), zeroSlot);
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r = t.r0 + lane;
+            if (r < w || r >= m) continue;
+            const int p0 = fptr[fr0 + r], cnt = fptr[fr0 + r + 1] - p0;
+            for (int k = 0; k < cnt && k < t.K; ++k) q[static_cast<size_t>(k) * 64 + lane] = fidx[p0 + k];
+        }
+        int* own = q + static_cast<size_t>(t.K) * 64;
+        for (int c = 0; c < w; ++c) {
+            const int p0 = fptr[fr0 + c], cnt = fptr[fr0 + c + 1] - p0;
+            for (int k = 0; k < cnt && k < t.K; ++k) own[static_cast<size_t>(k) * t.W + c] = fidx[p0 + k];
+        }
+    }
+}
+
 void frontGatherMap(const SupernodalFactor& F, const std::vector<int>& uOff, std::vector<int>& fptr,
                     std::vector<int>& fidx)
 {

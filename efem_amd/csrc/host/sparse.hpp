@@ -70,6 +70,20 @@ void mergeFactors(std::vector<SupernodalFactor>& parts, SupernodalFactor& out, s
 // backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R
 void blockInverseSolve3(const SupernodalFactor& F, const double* b, double* x);
 
+// One forward work item's extend-add table (64 rows from r0 of supernode s,
+// K entries per front row, own-column stride W) at tab[base, base + K (64 + W))
+struct FwdTableItem {
+    int s, r0, K, W;
+    long long base;
+};
+// Fills tab (resized to size) for the items, by all host threads: entry
+// [k][lane] is the k-th update position (fidx order) of front row r0 + lane
+// when that row is below the supernode's columns, entry K 64 + [k][c] that of
+// the supernode's own column c; every other entry is `zeroSlot`.
+void fillForwardTables(const SupernodalFactor& F, const std::vector<int>& fptr, const std::vector<int>& fidx,
+                       const std::vector<FwdTableItem>& items, long long size, int zeroSlot,
+                       std::vector<int>& tab);
+
 // Copy of B in another layout, by all host threads: supernode s's m x w
 // block to dst + off[s] with leading dimension ld = m rounded up to a
 // multiple of `align` (padding rows zero), as float (f32) or double.

@@ -143,6 +143,12 @@ def test_refactoring_frees_the_previous_factor(prec, rng):
         assert np.array_equal(ctx.apply_h0(b), x0)
         one_factor = 4 * ctx.factor_nnz  # bytes of the FP32 block values at least
         assert free0 - free1 < 2 * one_factor, (free0 - free1, one_factor)
+        # the phase timings of the last factorisation add up
+        ph = ctx.factor_phases()
+        assert all(v >= 0 for v in ph.values()) and ph["numeric"] > 0
+        assert ph["symbolic"] + ph["numeric"] <= ph["host_model"] <= ph["total"]
+        parts = ph["host_model"] + ph["descriptors"] + ph["repack"] + ph["upload"] + ph["other_buffers"]
+        assert parts <= ph["total"] * (1 + 1e-9) + 1e-4
 
 
 @pytest.mark.parametrize("consistent", [1, 0])
@@ -248,6 +254,28 @@ def test_apply_h0_narrow_backward_items(prec, tol, rng):
         assert rel(x16, x_ref) <= tol
         assert rel(x16, x4) <= (1e-12 if prec == E.FP64 else 1e-5)
         assert np.array_equal(ctx.apply_h0(b), x16)
+
+
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+@pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8])])
+def test_apply_h0_forward_tables_match_lists(prec, which, opts, rng):
+    # the forward sweep's per-item extend-add tables sum the same update
+    # values in the same order as the row lists: identical applies, bit for
+    # bit, with the graph and with plain launches
+    m = E.Mesh.workload(which, opts)
+    b = rng.standard_normal(m.ndof)
+    out = {}
+    with E.Context(m, prec) as ctx:
+        for tabs in (0, 1):
+            ctx.set_option("fwd_tables", tabs)
+            assert ctx.factor()
+            for graph in (0, 1):
+                ctx.set_option("h0_graph", graph)
+                out[tabs, graph] = ctx.apply_h0(b)
+    ref = out[0, 0]
+    assert np.all(np.isfinite(ref))
+    for key, x in out.items():
+        assert np.array_equal(x, ref), key
 
 
 @pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
