@@ -226,6 +226,11 @@ const char* efem_host_dense_kernel(void);
  *                       direction and runs the next first trial, discarded
  *                       when the trial is rejected or the solve stops; the
  *                       iterates are those of 0 (no look-ahead) bit for bit
+ *   "fwd_tables"        1 (default): forward triangular-solve levels with at
+ *                       most 4 contributions per front row read their
+ *                       children's update values through per-item tables
+ *                       (same sums, same order); 0: through the row lists;
+ *                       releases the factor
  *   "dense_subtree"     1..768 (default 64): elimination subtrees of at most
  *                       this many columns become one dense supernode (fewer
  *                       triangular-solve levels, more stored entries);
