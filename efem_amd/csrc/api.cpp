@@ -119,7 +119,10 @@ struct efem_ctx {
            *kv = nullptr, *ks = nullptr, *kt = nullptr, *kph = nullptr, *ksh = nullptr, *kxo = nullptr,
            *kbo = nullptr;
     efem::dev::LaunchTuning tuning;  // efem_autotune / options
-    bool useGraph = true;    // option "h0_graph"
+    // option "h0_graph": replay the apply from a captured HIP graph.  Off by
+    // default: on this part hipGraphLaunch holds the host about as long as
+    // the 26 plain launches take, and the loop runs 2 % faster without it
+    bool useGraph = false;
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
     bool lookahead = true;       // option "lookahead"
     bool fwdTables = true;       // option "fwd_tables"

@@ -375,12 +375,15 @@ __device__ __forceinline__ void writePartials(double (&v)[K], double* work, unsi
 }
 
 // stage 2: the nb block partials of each component combined (lane t takes
-// blocks t, t + kRF, ...; then the block tree); every lane calls epi(v)
+// blocks t, t + kRF, ...; then the block tree); every lane calls epi(v, p),
+// p = epi.prefetch() -- what the epilogue reads besides the sums, loaded
+// before the partials so that the two loads overlap
 template <int K, typename Epi>
 __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, unsigned maxMask,
                                                      Epi epi)
 {
     __shared__ double sh[K][kRFW];
+    const typename Epi::Pre pre = epi.prefetch();
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
@@ -391,8 +394,13 @@ __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ 
             v[k] = isMax(maxMask, k) ? fmax(v[k], o) : v[k] + o;
         }
     blockReduce<K, kRFW>(v, sh, maxMask);
-    epi(v);
+    epi(v, pre);
 }
+
+struct NoPrefetch {
+    struct Pre {};
+    __device__ Pre prefetch() const { return {}; }
+};
 
 int reduceGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().reduceBlocks), (n + kRB - 1) / kRB)); }
 // reductions that stream the whole history (2 m + 4 vectors) read several
@@ -401,9 +409,9 @@ int historyGrid(int n) { return max(1, min(min(kReduceBlocks, tuning().historyBl
 
 // epilogue: out[k] = v[k]
 template <int K>
-struct EpiStore {
+struct EpiStore : NoPrefetch {
     double* out;
-    __device__ void operator()(const double (&v)[K]) const
+    __device__ void operator()(const double (&v)[K], Pre) const
     {
         if (threadIdx.x == 0) {
 #pragma unroll
@@ -413,9 +421,9 @@ struct EpiStore {
 };
 
 // epilogue: *p0 = v[0], *p1 = v[1]
-struct EpiStore2 {
+struct EpiStore2 : NoPrefetch {
     double *p0, *p1;
-    __device__ void operator()(const double (&v)[2]) const
+    __device__ void operator()(const double (&v)[2], Pre) const
     {
         if (threadIdx.x == 0) {
             *p0 = v[0];
@@ -482,11 +490,11 @@ __global__ __launch_bounds__(kRB) void kTwoLoop(double* __restrict__ q, const do
 }
 
 // epilogue of a two-loop step: r = v / *den; *cout = mode ? *aj - r : r
-struct EpiTwoLoopStep {
+struct EpiTwoLoopStep : NoPrefetch {
     const double *den, *aj;
     double* cout;
     int mode;
-    __device__ void operator()(const double (&v)[1]) const
+    __device__ void operator()(const double (&v)[1], Pre) const
     {
         if (threadIdx.x == 0) {
             const double r = v[0] / (*den);
@@ -565,9 +573,22 @@ __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double
 // / ys_i.
 template <int KC, int NC>
 struct EpiGramFirst {
+    static_assert(KC * KC <= kRF, "one Gram entry per lane");
     double *G, *ys, *yy, *theta, *alpha, *rs;
     int loc, mcap;
-    __device__ void operator()(const double (&v)[3 * NC + 1]) const
+    // lane t: stored Gram entry t and ys_t
+    struct Pre {
+        double g, y;
+    };
+    __device__ Pre prefetch() const
+    {
+        const int t = threadIdx.x, i = t / KC, k = t - i * KC;
+        Pre p{0.0, 1.0};
+        if (t < KC * KC && i < mcap && k < mcap) p.g = G[i * KG + k];
+        if (t < mcap) p.y = ys[t];
+        return p;
+    }
+    __device__ void operator()(const double (&v)[3 * NC + 1], Pre pre) const
     {
         constexpr int N = 3 * NC + 1;
         __shared__ double red[N];
@@ -587,19 +608,18 @@ struct EpiGramFirst {
         }
         __syncthreads();
         // Gram block in LDS: stored entries, with the new row / column
-        for (int t = threadIdx.x; t < KC * KC; t += kRF) {
-            const int i = t / KC, k = t - i * KC;
-            double g = 0.0;
+        if (threadIdx.x < KC * KC) {
+            const int t = threadIdx.x, i = t / KC, k = t - i * KC;
+            double g = pre.g;
             if (i < mcap && k < mcap) {
                 if (i == loc && k < NC) g = sA[k];
                 else if (k == loc && i < NC) g = sB[i];
-                else g = G[i * KG + k];
             }
             sG[i][k] = g;
         }
         if (threadIdx.x < KC) {
             const int k = threadIdx.x;
-            sys[k] = k == loc ? red[0] : (k < mcap ? ys[k] : 1.0);
+            sys[k] = k == loc ? red[0] : pre.y;
         }
         __syncthreads();
         if (threadIdx.x < NC) {
@@ -656,10 +676,26 @@ __global__ __launch_bounds__(kRB) void kTwoLoopDots(const double* __restrict__ q
 // dw . q + sum coef_k rs_k (rs_k = s_k . dw from kGramFirst).
 template <int KC, int NC>
 struct EpiTwoLoopDots {
+    static_assert(KC * KC <= kRF, "one Gram entry per lane");
     const double *G, *ys, *alphaIn, *rs;
     double *outCoef, *slope;
     int newest, mcap;
-    __device__ void operator()(const double (&v)[NC + 1]) const
+    // lane t: Gram entry t, ys_t and alpha_t
+    struct Pre {
+        double g, y, a;
+    };
+    __device__ Pre prefetch() const
+    {
+        const int t = threadIdx.x, i = t / KC, k = t - i * KC;
+        Pre p{0.0, 1.0, 0.0};
+        if (t < KC * KC && i < mcap && k < mcap) p.g = G[i * KG + k];
+        if (t < mcap) {
+            p.y = ys[t];
+            p.a = alphaIn[t];
+        }
+        return p;
+    }
+    __device__ void operator()(const double (&v)[NC + 1], Pre pre) const
     {
         __shared__ double red[NC + 1];
         __shared__ double sG[KC][KC], sys[KC], sa[KC], cs[KC];
@@ -668,13 +704,13 @@ struct EpiTwoLoopDots {
             for (int k = 0; k <= NC; ++k) red[k] = v[k];
         }
         // the recurrence reads the Gram block, ys and alpha from LDS
-        for (int t = threadIdx.x; t < KC * KC; t += kRF) {
-            const int i = t / KC, k = t - i * KC;
-            sG[i][k] = (i < mcap && k < mcap) ? G[i * KG + k] : 0.0;
+        if (threadIdx.x < KC * KC) {
+            const int t = threadIdx.x, i = t / KC, k = t - i * KC;
+            sG[i][k] = pre.g;
         }
         if (threadIdx.x < KC) {
-            sys[threadIdx.x] = threadIdx.x < mcap ? ys[threadIdx.x] : 1.0;
-            sa[threadIdx.x] = threadIdx.x < mcap ? alphaIn[threadIdx.x] : 0.0;
+            sys[threadIdx.x] = pre.y;
+            sa[threadIdx.x] = pre.a;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -995,13 +1031,13 @@ void launchDotMax(const double* a, const double* b, const double* c, int n, doub
     const int g = reduceGrid(n);
     if (!c) {
         kDotMax<false, false><<<g, kRB, 0, s>>>(a, b, c, nullptr, nullptr, n, work);
-        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{out});
+        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{{}, out});
     } else if (!xt) {
         kDotMax<true, false><<<g, kRB, 0, s>>>(a, b, c, cb, nullptr, n, work);
-        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{out});
+        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{{}, out});
     } else {
         kDotMax<true, true><<<g, kRB, 0, s>>>(a, b, c, cb, xt, n, work);
-        kReduceFinish<4><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<4>{out});
+        kReduceFinish<4><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<4>{{}, out});
     }
 }
 
@@ -1009,7 +1045,7 @@ void launchDot2(const double* a, const double* b, int n, double* work, double* a
 {
     const int g = reduceGrid(n);
     kDot2<<<g, kRB, 0, s>>>(a, b, n, work);
-    kReduceFinish<2><<<1, kRF, 0, s>>>(work, g, 0u, EpiStore2{ab, bb});
+    kReduceFinish<2><<<1, kRF, 0, s>>>(work, g, 0u, EpiStore2{{}, ab, bb});
 }
 
 void launchStep(double* x, const double* xp, const double* d, double alpha, int n, hipStream_t s)
@@ -1033,7 +1069,7 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
 {
     const int g = reduceGrid(n);
     kTwoLoop<<<g, kRB, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, n, work);
-    kReduceFinish<1><<<1, kRF, 0, s>>>(work, g, 0u, EpiTwoLoopStep{den, aj, cout, mode});
+    kReduceFinish<1><<<1, kRF, 0, s>>>(work, g, 0u, EpiTwoLoopStep{{}, den, aj, cout, mode});
 }
 
 // calls f(std::integral_constant<int, nc>) for 1 <= nc <= 16

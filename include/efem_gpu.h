@@ -215,8 +215,9 @@ int efem_host_constant_force(const efem_problem* pb, double* out, int n);
 const char* efem_host_dense_kernel(void);
 
 /* tuning / debug options:
- *   "h0_graph"          1 (default): the preconditioner apply replays a
- *                       captured HIP graph; 0: plain launches
+ *   "h0_graph"          0 (default): the preconditioner apply's level
+ *                       kernels are launched one by one; 1: replayed from a
+ *                       captured HIP graph (same kernels, same results)
  *   "two_loop_compact"  1 (default, history m <= 16): the two-loop recursion
  *                       is evaluated from Gram products s_i.y_k in two
  *                       multi-dot passes; 0: one reduction per step exactly

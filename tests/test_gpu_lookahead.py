@@ -43,7 +43,7 @@ def test_lookahead_identical_iterates():
     assert t.shape[0] > 100
     check_same(both(m, epsilon=1e-14, epsilon_rel=0.0, max_iterations=37), "iteration cap")
     check_same(both(m, options=[("two_loop_compact", 0)], epsilon=1e-8, max_iterations=3000), "reference order")
-    check_same(both(m, options=[("h0_graph", 0)], epsilon=1e-8, max_iterations=3000), "plain launches")
+    check_same(both(m, options=[("h0_graph", 1)], epsilon=1e-8, max_iterations=3000), "captured graph")
     check_same(both(m, epsilon=1e-8, max_iterations=400, use_precond=False), "H0 = I / theta")
     # the distance target stops the solve early
     with E.Context(m) as c:
