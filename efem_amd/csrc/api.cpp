@@ -128,6 +128,8 @@ struct efem_ctx {
     bool fwdTables = true;       // option "fwd_tables"
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
+    // host factor memory reused by the next factorisation of this problem
+    std::unique_ptr<efem::HostWorkspace> hostWs{new efem::HostWorkspace};
 };
 
 namespace {
@@ -680,7 +682,7 @@ int factorInternal(efem_ctx* c, int* failed)
         ok = efem::factorModels(c->parts, P, msg, c->denseSubtree);
         if (ok) efem::mergeFactors(P, F, snOffset);
     } else {
-        ok = efem::factorModel(m, F, msg, c->denseSubtree);
+        ok = efem::factorModel(m, F, msg, c->denseSubtree, c->hostWs.get());
     }
     if (!ok) {
         c->err = msg;
@@ -691,6 +693,7 @@ int factorInternal(efem_ctx* c, int* failed)
     c->factorPhases[1] = F.seconds - F.numericSeconds;
     c->factorPhases[2] = F.numericSeconds;
     int rc = uploadFactor(c, F, c->parts.empty() ? nullptr : &snOffset);
+    if (c->parts.empty()) c->hostWs->giveBlocks(std::move(F.B), static_cast<size_t>(F.Bsize));
     if (rc) return rc;
     c->factorFailed = false;
     if (failed) *failed = 0;

@@ -484,7 +484,7 @@ std::vector<double> freeCoords(const Model& m)
     return xyz;
 }
 
-bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int denseSubtree)
+bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int denseSubtree, HostWorkspace* ws)
 {
     if (m.nFree == 0) {
         msg = "no free DOF";
@@ -493,7 +493,7 @@ bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int dens
     Csc A;
     scalarLaplacian(m, A);
     const std::vector<int> perm = nestedDissection(A, freeCoords(m));
-    return supernodalCholesky(A, perm, F, msg, denseSubtree);
+    return supernodalCholesky(A, perm, F, msg, denseSubtree, ws);
 }
 
 bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,

@@ -71,7 +71,7 @@ std::vector<double> freeCoords(const Model& m);
 // approximate Jacobian -> nested dissection -> supernodal factor (false: not
 // SPD or no free DOF, msg says which)
 bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg,
-                 int denseSubtree = kDefaultDenseSubtree);
+                 int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr);
 // the same for independent parts, in parallel over the parts
 bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,
                   int denseSubtree = kDefaultDenseSubtree);

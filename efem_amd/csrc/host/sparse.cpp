@@ -225,8 +225,51 @@ void mergeFactors(std::vector<SupernodalFactor>& parts, SupernodalFactor& out, s
     out.valPtr.push_back(val);
 }
 
+HostWorkspace::~HostWorkspace()
+{
+    for (auto& kv : capacity_) delete[] kv.first;
+}
+
+std::unique_ptr<double[]> HostWorkspace::takeBlocks(size_t count)
+{
+    if (blocks_ && blockCount_ >= count) {
+        blockCount_ = 0;
+        return std::move(blocks_);
+    }
+    blocks_.reset();
+    blockCount_ = 0;
+    return std::unique_ptr<double[]>(new double[std::max<size_t>(count, 1)]);
+}
+
+void HostWorkspace::giveBlocks(std::unique_ptr<double[]> b, size_t count)
+{
+    blocks_ = std::move(b);
+    blockCount_ = blocks_ ? count : 0;
+}
+
+double* HostWorkspace::takeUpdate(size_t count)
+{
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = free_.lower_bound(count);
+    if (it != free_.end()) {
+        double* p = it->second;
+        free_.erase(it);
+        return p;
+    }
+    double* p = new double[std::max<size_t>(count, 1)];
+    capacity_[p] = count;
+    return p;
+}
+
+void HostWorkspace::giveUpdate(double* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(mu_);
+    free_.emplace(capacity_.at(p), p);
+}
+
 bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, SupernodalFactor& F,
-                        std::string& msg, int denseSubtree)
+                        std::string& msg, int denseSubtree, HostWorkspace* ws)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const int n = A.n;
@@ -499,9 +542,24 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     // subtrees; big fronts split their dense kernels into tasks too).  Each
     // supernode's update matrix lives in upd[s] until its parent absorbs it.
     // B is filled supernode by supernode inside the tasks (no serial zero-fill)
-    F.B.reset(new double[static_cast<size_t>(F.valPtr[ns])]);
+    F.B = ws ? ws->takeBlocks(static_cast<size_t>(F.valPtr[ns]))
+             : std::unique_ptr<double[]>(new double[static_cast<size_t>(std::max(F.valPtr[ns], 1LL))]);
     F.Bsize = F.valPtr[ns];
-    std::vector<std::unique_ptr<double[]>> upd(ns);
+    // update matrices: from the workspace's pool (pages already mapped by
+    // an earlier factorisation) or the heap
+    struct FreeUpdate {
+        HostWorkspace* ws;
+        void operator()(double* p) const
+        {
+            if (ws) ws->giveUpdate(p);
+            else delete[] p;
+        }
+    };
+    const FreeUpdate freeUpdate{ws};
+    using UpdPtr = std::unique_ptr<double[], FreeUpdate>;
+    std::vector<UpdPtr> upd;
+    upd.reserve(ns);
+    for (int s = 0; s < ns; ++s) upd.emplace_back(nullptr, freeUpdate);
     std::atomic<int> failed{0};
     const auto tn0 = std::chrono::steady_clock::now();
     // One supernode: the front's first w columns [F11; F21] are assembled,
@@ -521,9 +579,10 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         if (static_cast<int>(loc.size()) < n) loc.resize(n);
         for (int k = 0; k < m; ++k) loc[rows[k]] = k;
         std::fill(P, P + static_cast<size_t>(m) * w, 0.0);
-        std::unique_ptr<double[]> U;
+        UpdPtr U(nullptr, freeUpdate);
         if (nr > 0 && F.parent[s] >= 0) {
-            U.reset(new double[static_cast<size_t>(nr) * nr]);
+            const size_t cnt = static_cast<size_t>(nr) * nr;
+            U.reset(ws ? ws->takeUpdate(cnt) : new double[cnt]);
             // lower triangle plus the upper part of the 64 x 64 diagonal
             // tiles the update computes in full
             for (int b = 0; b < nr; ++b)

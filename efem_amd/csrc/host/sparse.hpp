@@ -15,7 +15,9 @@
 // t_s = L21 L11^-1 b'_s; backward z_s = L11^-T y_s - (L21 L11^-1)^T z_R).
 #pragma once
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +52,31 @@ struct SupernodalFactor {
 // order: nested dissection on the node graph using coordinates xyz[n][3]
 std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz);
 
+// Host memory kept between factorisations of one problem: the block storage
+// B and the update matrices come back to it when a factorisation is done,
+// so a refactorisation writes into pages that are already mapped instead of
+// faulting in fresh ones (hundreds of MB on the larger meshes).
+class HostWorkspace {
+public:
+    HostWorkspace() = default;
+    HostWorkspace(const HostWorkspace&) = delete;
+    HostWorkspace& operator=(const HostWorkspace&) = delete;
+    ~HostWorkspace();
+    // block storage of at least `count` doubles (contents undefined)
+    std::unique_ptr<double[]> takeBlocks(size_t count);
+    void giveBlocks(std::unique_ptr<double[]> b, size_t count);
+    // an update buffer of at least `count` doubles (thread safe)
+    double* takeUpdate(size_t count);
+    void giveUpdate(double* p);
+
+private:
+    std::unique_ptr<double[]> blocks_;
+    size_t blockCount_ = 0;
+    std::mutex mu_;
+    std::multimap<size_t, double*> free_;    // capacity -> buffer
+    std::map<double*, size_t> capacity_;     // every buffer it owns
+};
+
 // Every maximal elimination subtree of at most `denseSubtree` columns
 // becomes one dense supernode (fewer levels for the device triangular
 // solves, a few percent more stored entries); 1..kMaxSupernodeWidth.
@@ -57,7 +84,8 @@ constexpr int kDefaultDenseSubtree = 64;
 
 // returns false if a pivot is not positive (the reference's fallback case)
 bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F,
-                        std::string& msg, int denseSubtree = kDefaultDenseSubtree);
+                        std::string& msg, int denseSubtree = kDefaultDenseSubtree,
+                        HostWorkspace* ws = nullptr);
 
 // Forest of independent factors (batched instances): part i's columns,
 // supernodes, rows and blocks follow those of part i-1; levels stay per part.
