@@ -130,6 +130,14 @@ struct efem_ctx {
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
     // host factor memory reused by the next factorisation of this problem
     std::unique_ptr<efem::HostWorkspace> hostWs{new efem::HostWorkspace};
+    // approximate Jacobian assembled on the device (option
+    // "device_laplacian"): the host keeps its pattern, the device the
+    // contribution map (built at the first factorisation)
+    bool devLaplacian = true;
+    bool lapReady = false;
+    efem::Csc lapA;
+    int *lapPtr = nullptr, *lapSrc = nullptr;
+    double *lapKel = nullptr, *lapVals = nullptr;
 };
 
 namespace {
@@ -665,6 +673,46 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     return EFEM_OK;
 }
 
+// the approximate Jacobian's pattern and contribution map (once per
+// context), then its values by the device kernels into lapA.x
+int deviceLaplacian(efem_ctx* c)
+{
+    const efem::Model& m = c->model;
+    if (!c->lapReady) {
+        std::vector<int> ptr, src;
+        efem::laplacianMap(m, c->lapA, ptr, src);
+        int rc;
+        if ((rc = upload(c, &c->lapPtr, ptr))) return rc;
+        if ((rc = upload(c, &c->lapSrc, src))) return rc;
+        if ((rc = devAlloc(c, &c->lapKel, static_cast<size_t>(m.nElems) * m.nen * m.nen))) return rc;
+        if ((rc = devAlloc(c, &c->lapVals, c->lapA.x.size()))) return rc;
+        c->lapReady = true;
+    }
+    efem::dev::LapArgs a{};
+    a.E = m.nElems;
+    a.nen = m.nen;
+    a.ng = m.ng;
+    a.law = m.law;
+    a.ppe = m.paramsPerElem;
+    a.nnz = static_cast<int>(c->lapA.x.size());
+    a.connDof = c->connDof;
+    a.ginv = static_cast<const double*>(c->ginv64);
+    a.vol = c->vol;
+    a.gdN = static_cast<const double*>(c->gdN64);
+    a.gw = c->gw;
+    a.params = c->params;
+    a.kel = c->lapKel;
+    a.ptr = c->lapPtr;
+    a.src = c->lapSrc;
+    a.vals = c->lapVals;
+    efem::dev::launchLaplacian(a, c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    HIP_CHECK(c, hipMemcpyAsync(c->lapA.x.data(), c->lapVals, c->lapA.x.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
+}
+
 int factorInternal(efem_ctx* c, int* failed)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -682,7 +730,15 @@ int factorInternal(efem_ctx* c, int* failed)
         ok = efem::factorModels(c->parts, P, msg, c->denseSubtree);
         if (ok) efem::mergeFactors(P, F, snOffset);
     } else {
-        ok = efem::factorModel(m, F, msg, c->denseSubtree, c->hostWs.get());
+        // element codes e nen^2 + a nen + b must fit the map's ints
+        const bool dev = c->devLaplacian && static_cast<long long>(m.nElems) * m.nen * m.nen < (1LL << 31);
+        if (dev) {
+            int rc0 = deviceLaplacian(c);
+            if (rc0) return rc0;
+            ok = efem::factorLaplacian(m, c->lapA, F, msg, c->denseSubtree, c->hostWs.get());
+        } else {
+            ok = efem::factorModel(m, F, msg, c->denseSubtree, c->hostWs.get());
+        }
     }
     if (!ok) {
         c->err = msg;
@@ -1519,6 +1575,13 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
     }
     if (k == "lookahead") {
         c->lookahead = value != 0.0;
+        return EFEM_OK;
+    }
+    if (k == "device_laplacian") {
+        // assemble the approximate Jacobian on the device (1) or the host (0);
+        // the same values either way; takes effect at the next factorisation
+        c->devLaplacian = value != 0.0;
+        releaseFactor(c);
         return EFEM_OK;
     }
     if (k == "fwd_tables") {

@@ -319,6 +319,55 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
     }
 }
 
+// ---- approximate Jacobian ----------------------------------------------------
+// One lane per (element, column slot b): column b of the element matrix,
+// accumulated over the Gauss points as the host does (the gradients G are
+// formed per Gauss point; no multiply-add contraction anywhere).
+template <int NEN>
+__global__ __launch_bounds__(kThreads) void kLapElem(LapArgs a)
+{
+#pragma clang fp contract(off)
+    const long long t = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (t >= static_cast<long long>(a.E) * NEN) return;
+    const int e = static_cast<int>(t / NEN), b = static_cast<int>(t - static_cast<long long>(e) * NEN);
+    const int E = a.E;
+    const double ks = a.vol[e] * elasticityCoeff(a.law, a.params + (a.ppe ? 4 * static_cast<size_t>(e) : 0));
+    double kcol[NEN];
+#pragma unroll
+    for (int i = 0; i < NEN; ++i) kcol[i] = 0.0;
+    for (int q = 0; q < a.ng; ++q) {
+        double Ji[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ji[k] = a.ginv[(static_cast<size_t>(q) * 9 + k) * E + e];
+        const double* dN = a.gdN + static_cast<size_t>(q) * NEN * 3;
+        const double gb0 = Ji[0] * dN[3 * b] + Ji[3] * dN[3 * b + 1] + Ji[6] * dN[3 * b + 2];
+        const double gb1 = Ji[1] * dN[3 * b] + Ji[4] * dN[3 * b + 1] + Ji[7] * dN[3 * b + 2];
+        const double gb2 = Ji[2] * dN[3 * b] + Ji[5] * dN[3 * b + 1] + Ji[8] * dN[3 * b + 2];
+        const double w = ks * a.gw[q];
+#pragma unroll
+        for (int i = 0; i < NEN; ++i) {
+            const double g0 = Ji[0] * dN[3 * i] + Ji[3] * dN[3 * i + 1] + Ji[6] * dN[3 * i + 2];
+            const double g1 = Ji[1] * dN[3 * i] + Ji[4] * dN[3 * i + 1] + Ji[7] * dN[3 * i + 2];
+            const double g2 = Ji[2] * dN[3 * i] + Ji[5] * dN[3 * i + 1] + Ji[8] * dN[3 * i + 2];
+            kcol[i] += w * (g0 * gb0 + g1 * gb1 + g2 * gb2);
+        }
+    }
+    double* out = a.kel + static_cast<size_t>(e) * NEN * NEN;
+#pragma unroll
+    for (int i = 0; i < NEN; ++i) out[static_cast<size_t>(i) * NEN + b] = kcol[i];
+}
+
+// one lane per CSR entry: its contributions summed in the listed order
+__global__ __launch_bounds__(kThreads) void kLapSum(LapArgs a)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= a.nnz) return;
+    double sum = 0.0;
+    for (int p = a.ptr[k]; p < a.ptr[k + 1]; ++p) sum += a.kel[a.src[p]];
+    a.vals[k] = sum;
+}
+
 // ---- reductions -------------------------------------------------------------
 // Two launches per reduction.  Stage 1 streams the vectors with kRB-lane
 // blocks and writes each block's partials (wave butterflies, then the wave
@@ -1005,6 +1054,21 @@ void launchSumComp(const double* a, int n, double* work, double* out, hipStream_
     const int g = reduceGrid(n);
     kSumComp<<<g, kRB, 0, s>>>(a, n, work);
     kSumCompFinish<<<1, 64, 0, s>>>(work, g, out);
+}
+
+void launchLaplacian(const LapArgs& a, hipStream_t s)
+{
+    const long long lanes = static_cast<long long>(a.E) * a.nen;
+    const int g = static_cast<int>((lanes + kThreads - 1) / kThreads);
+    if (lanes > 0) {
+        switch (a.nen) {
+            case 4: kLapElem<4><<<g, kThreads, 0, s>>>(a); break;
+            case 8: kLapElem<8><<<g, kThreads, 0, s>>>(a); break;
+            case 10: kLapElem<10><<<g, kThreads, 0, s>>>(a); break;
+            default: kLapElem<20><<<g, kThreads, 0, s>>>(a); break;
+        }
+    }
+    if (a.nnz > 0) kLapSum<<<(a.nnz + kThreads - 1) / kThreads, kThreads, 0, s>>>(a);
 }
 
 void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s)

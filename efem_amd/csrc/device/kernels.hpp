@@ -78,6 +78,28 @@ struct GatherArgs {
 };
 void launchGather(const GatherArgs& a, hipStream_t s);
 
+// ---- approximate Jacobian ----------------------------------------------------
+// The scalar Laplacian L_ij = sum_e sum_q V_e c_e w_q grad N_i . grad N_j
+// (ElementConstraint::conEvaluteJacobi, ElementConstraint.cpp:164-205) on the
+// device: element matrices kel[e][a][b], then every CSR entry k sums its
+// contributions kel[src[ptr[k] .. ptr[k+1])] in the listed order.  The
+// arithmetic is that of the host assembly operation for operation (no
+// contracted multiply-adds), so the values are the host's bit for bit.
+struct LapArgs {
+    int E, nen, ng, law, ppe, nnz;
+    const int* connDof;    // unused by the kernels (kept for checks)
+    const double* ginv;    // [ng][9][E] FP64
+    const double* vol;     // [E]
+    const double* gdN;     // [ng][nen][3] FP64
+    const double* gw;      // [ng]
+    const double* params;  // [4] or [E][4]
+    double* kel;           // [E][nen][nen]
+    const int* ptr;        // [nnz + 1]
+    const int* src;        // e nen^2 + a nen + b
+    double* vals;          // [nnz]
+};
+void launchLaplacian(const LapArgs& a, hipStream_t s);
+
 // ---- vectors ----------------------------------------------------------------
 // deterministic reductions: results written to device scalars; the block
 // partials live in `work` (see kReduceWorkDoubles); two launches each

@@ -484,6 +484,81 @@ std::vector<double> freeCoords(const Model& m)
     return xyz;
 }
 
+void laplacianMap(const Model& m, Csc& L, std::vector<int>& ptr, std::vector<int>& src)
+{
+    // the same column lists as scalarLaplacian, with each contribution named
+    // (element e, row slot a, column slot b) -> e nen^2 + a nen + b instead
+    // of valued; stable sorting keeps element order within an entry
+    const int nf = m.nFree, nen = m.nen, E = m.nElems;
+    std::vector<int> eptr(nf + 1, 0), elist;
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < nen; ++a) {
+            const int j = m.connDof[static_cast<size_t>(a) * E + e];
+            if (j < nf) eptr[j + 1]++;
+        }
+    for (int j = 0; j < nf; ++j) eptr[j + 1] += eptr[j];
+    elist.resize(eptr[nf]);
+    {
+        std::vector<int> pos(eptr.begin(), eptr.end() - 1);
+        for (int e = 0; e < E; ++e)
+            for (int a = 0; a < nen; ++a) {
+                const int j = m.connDof[static_cast<size_t>(a) * E + e];
+                if (j < nf) elist[pos[j]++] = e * nen + a;
+            }
+    }
+    std::vector<std::vector<int>> ci(nf), cp(nf), cs(nf);
+#pragma omp parallel
+    {
+        std::vector<std::pair<int, int>> col;
+#pragma omp for schedule(dynamic, 256)
+        for (int j = 0; j < nf; ++j) {
+            col.clear();
+            for (int p = eptr[j]; p < eptr[j + 1]; ++p) {
+                const int e = elist[p] / nen, b = elist[p] % nen;
+                for (int a = 0; a < nen; ++a) {
+                    const int ia = m.connDof[static_cast<size_t>(a) * E + e];
+                    if (ia < nf) col.push_back({ia, (e * nen + a) * nen + b});
+                }
+            }
+            std::stable_sort(col.begin(), col.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (size_t k = 0; k < col.size();) {
+                size_t k2 = k;
+                ci[j].push_back(col[k].first);
+                cp[j].push_back(static_cast<int>(cs[j].size()));
+                while (k2 < col.size() && col[k2].first == col[k].first) cs[j].push_back(col[k2++].second);
+                k = k2;
+            }
+        }
+    }
+    L.n = nf;
+    L.p.assign(nf + 1, 0);
+    for (int j = 0; j < nf; ++j) L.p[j + 1] = L.p[j] + static_cast<int>(ci[j].size());
+    L.i.resize(L.p[nf]);
+    L.x.assign(L.p[nf], 0.0);
+    std::vector<long long> soff(nf + 1, 0);
+    for (int j = 0; j < nf; ++j) soff[j + 1] = soff[j] + static_cast<long long>(cs[j].size());
+    ptr.resize(static_cast<size_t>(L.p[nf]) + 1);
+    src.resize(static_cast<size_t>(soff[nf]));
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < nf; ++j) {
+        std::copy(ci[j].begin(), ci[j].end(), L.i.begin() + L.p[j]);
+        for (size_t k = 0; k < cp[j].size(); ++k) ptr[L.p[j] + k] = static_cast<int>(soff[j] + cp[j][k]);
+        std::copy(cs[j].begin(), cs[j].end(), src.begin() + soff[j]);
+    }
+    ptr[L.p[nf]] = static_cast<int>(soff[nf]);
+}
+
+bool factorLaplacian(const Model& m, const Csc& A, SupernodalFactor& F, std::string& msg, int denseSubtree,
+                     HostWorkspace* ws)
+{
+    if (m.nFree == 0) {
+        msg = "no free DOF";
+        return false;
+    }
+    const std::vector<int> perm = nestedDissection(A, freeCoords(m));
+    return supernodalCholesky(A, perm, F, msg, denseSubtree, ws);
+}
+
 bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int denseSubtree, HostWorkspace* ws)
 {
     if (m.nFree == 0) {

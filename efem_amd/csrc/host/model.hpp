@@ -64,6 +64,12 @@ bool concatModels(const std::vector<Model>& parts, Model& out, std::vector<doubl
 
 // scalar approximate Jacobian over free nodes (full symmetric CSC)
 void scalarLaplacian(const Model& m, Csc& L);
+// The pattern of scalarLaplacian (L.x zeroed) and, per entry k, its
+// contributions src[ptr[k] .. ptr[k+1]) = e nen^2 + a nen + b (element e's
+// row slot a, column slot b) in the order scalarLaplacian sums them: a
+// device kernel that forms the element matrices and sums them through this
+// map reproduces scalarLaplacian's values.
+void laplacianMap(const Model& m, Csc& L, std::vector<int>& ptr, std::vector<int>& src);
 
 // free-node coordinates [nFree][3]
 std::vector<double> freeCoords(const Model& m);
@@ -72,6 +78,9 @@ std::vector<double> freeCoords(const Model& m);
 // SPD or no free DOF, msg says which)
 bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg,
                  int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr);
+// the same from an assembled approximate Jacobian A (ordering + factor)
+bool factorLaplacian(const Model& m, const Csc& A, SupernodalFactor& F, std::string& msg,
+                     int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr);
 // the same for independent parts, in parallel over the parts
 bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,
                   int denseSubtree = kDefaultDenseSubtree);

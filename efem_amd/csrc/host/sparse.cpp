@@ -345,19 +345,31 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
     std::vector<int> Lp(n + 1, 0), Li;
     std::vector<double> Lx;
     {
-        for (int j = 0; j < n; ++j)
-            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
-                if (pinv[A.i[p]] >= pinv[j]) Lp[pinv[j] + 1]++;
-        for (int j = 0; j < n; ++j) Lp[j + 1] += Lp[j];
+        // column k of P A P^T is column perm[k] of A: counted and filled in
+        // permuted order, columns in parallel (each writes its own range;
+        // entries in A's order within the column)
+#pragma omp parallel for schedule(static, 1024)
+        for (int k = 0; k < n; ++k) {
+            const int col = perm[k];
+            int cnt = 0;
+            for (int p = A.p[col]; p < A.p[col + 1]; ++p) cnt += pinv[A.i[p]] >= k;
+            Lp[k + 1] = cnt;
+        }
+        for (int k = 0; k < n; ++k) Lp[k + 1] += Lp[k];
         Li.resize(Lp[n]);
         Lx.resize(Lp[n]);
-        std::vector<int> pos(Lp.begin(), Lp.end() - 1);
-        for (int j = 0; j < n; ++j)
-            for (int p = A.p[j]; p < A.p[j + 1]; ++p)
-                if (pinv[A.i[p]] >= pinv[j]) {
-                    Li[pos[pinv[j]]] = pinv[A.i[p]];
-                    Lx[pos[pinv[j]]++] = A.x[p];
+#pragma omp parallel for schedule(static, 1024)
+        for (int k = 0; k < n; ++k) {
+            const int col = perm[k];
+            int q = Lp[k];
+            for (int p = A.p[col]; p < A.p[col + 1]; ++p) {
+                const int i = pinv[A.i[p]];
+                if (i >= k) {
+                    Li[q] = i;
+                    Lx[q++] = A.x[p];
                 }
+            }
+        }
     }
     // column counts of L (diagonal included) by the row-subtree skeleton
     // method of Gilbert, Ng & Peyton (SIAM J. Matrix Anal. Appl. 15(4),

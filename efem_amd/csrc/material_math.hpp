@@ -32,6 +32,9 @@ enum { LAW_NH = 0, LAW_MR = 1, LAW_YEOH = 2 };
 // (NeoHookean.cpp:41, MooneyRivlin.cpp:48, Yeoh.cpp:47)
 EFEM_HD double elasticityCoeff(int law, const double* p)
 {
+    // evaluated the same way on the host and the device (the device
+    // assembly of the approximate Jacobian reproduces the host's bit for bit)
+#pragma clang fp contract(off)
     if (law == LAW_NH) return 2.94001 * p[0] + 2.0 / p[3];
     if (law == LAW_MR) return 3.818 * p[1] + 2.94001 * p[0] + 2.0 / p[3];
     return 2.94001 * p[0] + 3.51194 * p[1] + 3.3154 * p[2] + 2.0 / p[3];

@@ -256,6 +256,31 @@ def test_apply_h0_narrow_backward_items(prec, tol, rng):
         assert np.array_equal(ctx.apply_h0(b), x16)
 
 
+@pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8]), (1, [10, 2, 2, 0, 100.0])])
+def test_device_laplacian_matches_host_assembly(which, opts, rng):
+    # the approximate Jacobian assembled by the device kernels (element
+    # matrices, then each entry's contributions in the host's order, no
+    # contracted multiply-adds) is the host assembly bit for bit, so the
+    # factor and every apply are identical; per-element parameters too
+    m = E.Mesh.workload(which, opts)
+    P = np.atleast_2d(m.params)
+    P = np.repeat(P, m.n_elems, axis=0) if P.shape[0] == 1 else P
+    m = E.Mesh(m.X, m.conn, m.law, P * rng.uniform(0.5, 2.0, (m.n_elems, 1)), m.fixed, m.faces, m.pressure,
+               m.reduced)
+    b = rng.standard_normal(m.ndof)
+    out = {}
+    with E.Context(m, E.FP64) as ctx:
+        for dev in (0, 1):
+            ctx.set_option("device_laplacian", dev)
+            assert ctx.factor()
+            out[dev] = (ctx.apply_h0(b), ctx.factor_nnz)
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+    x_host, _ = E.host_factor_solve(m, b)
+    assert np.allclose(out[1][0], x_host, rtol=1e-10, atol=0) or \
+        np.linalg.norm(out[1][0] - x_host) <= 1e-10 * np.linalg.norm(x_host)
+
+
 @pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
 @pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8])])
 def test_apply_h0_forward_tables_match_lists(prec, which, opts, rng):
