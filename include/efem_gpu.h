@@ -227,6 +227,9 @@ const char* efem_host_dense_kernel(void);
  *                       direction and runs the next first trial, discarded
  *                       when the trial is rejected or the solve stops; the
  *                       iterates are those of 0 (no look-ahead) bit for bit
+ *   "device_laplacian"  1 (default): the approximate Jacobian is assembled on
+ *                       the device (bit-identical to the host assembly);
+ *                       0: on the host; releases the factor
  *   "fwd_tables"        1 (default): forward triangular-solve levels with at
  *                       most 4 contributions per front row read their
  *                       children's update values through per-item tables
