@@ -93,6 +93,7 @@ struct efem_ctx {
     int mcap = 0;
     double *x = nullptr, *r = nullptr, *xp = nullptr, *rp = nullptr, *d = nullptr, *tmp = nullptr;
     double *x3 = nullptr, *r3 = nullptr, *d2 = nullptr;  // spares of the solve loop
+    double* gdotWork = nullptr;  // partials of the line-search scalars fused into the gather
     hipEvent_t trialEv[2] = {nullptr, nullptr};          // trial scalars copied
     double *S = nullptr, *Y = nullptr;
     double* scal = nullptr;  // device scalars
@@ -251,6 +252,7 @@ int uploadModel(efem_ctx* c)
         if ((rc = devAlloc(c, v, np))) return rc;
         HIP_CHECK(c, hipMemsetAsync(*v, 0, np * sizeof(double), c->stream));
     }
+    if ((rc = devAlloc(c, &c->gdotWork, efem::dev::gatherDotWork(m.nFree)))) return rc;
     if ((rc = devAlloc(c, &c->scal, kNumScal))) return rc;
     HIP_CHECK(c, hipMemsetAsync(c->scal, 0, kNumScal * sizeof(double), c->stream));
     if ((rc = devAlloc(c, &c->work, efem::dev::kReduceWorkDoubles))) return rc;
@@ -322,7 +324,10 @@ efem::dev::GatherArgs gatherArgs(efem_ctx* c, double* r, bool f32, double ce, do
 }
 
 // r = f_int(x) - f_ext(x); x has 3 trailing zeros (the fixed-node slot)
-int residualInternal(efem_ctx* c, const double* x, double* r)
+// r(x); with dotOut, also the line search's scalars of the new r in the
+// gather launch: dotOut = {r . dotD, max|r|, max|x - dotCb|, |x - dotXt|^2}
+int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD = nullptr,
+                     const double* dotCb = nullptr, const double* dotXt = nullptr, double* dotOut = nullptr)
 {
     const efem::Model& m = c->model;
     const bool f32 = elem32(c) && !c->elemFp64;
@@ -339,7 +344,16 @@ int residualInternal(efem_ctx* c, const double* x, double* r)
         efem::dev::launchPressure(faceArgs(c, x, nullptr), c->stream);
         HIP_CHECK(c, hipGetLastError());
     }
-    efem::dev::launchGather(gatherArgs(c, r, f32, 1.0, -1.0, c->fext), c->stream);
+    efem::dev::GatherArgs ga = gatherArgs(c, r, f32, 1.0, -1.0, c->fext);
+    if (dotOut) {
+        ga.dotD = dotD;
+        ga.dotX = x;
+        ga.dotCb = dotCb;
+        ga.dotXt = dotXt;
+        ga.dotWork = c->gdotWork;
+        ga.dotOut = dotOut;
+    }
+    efem::dev::launchGather(ga, c->stream);
     HIP_CHECK(c, hipGetLastError());
     return EFEM_OK;
 }
@@ -1355,10 +1369,9 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     // copied to pinned slot `slot` behind an event
     auto enqueueTrial = [&](double step, int slot) -> int {
         efem::dev::launchStep(c->x, c->xp, c->d, step, n, c->stream);
-        int rc2 = residualInternal(c, c->x, c->r);
+        // the trial's scalars come with the residual's gather
+        int rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr, sc + kOut);
         if (rc2) return rc2;
-        efem::dev::launchDotMax(c->r, c->d, c->x, n, c->work, sc + kOut, c->stream,
-                                prm->x_target ? c->xtarget : nullptr, base);
         HIP_CHECK(c, hipMemcpyAsync(c->hostScal + kTrialSlot + 8 * slot, sc + kOut, 5 * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(c, hipEventRecord(c->trialEv[slot], c->stream));

@@ -27,6 +27,10 @@ namespace efem {
 namespace dev {
 namespace {
 
+// K block reductions over W waves (defined with the reductions below)
+template <int K, int W>
+__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W], unsigned maxMask);
+
 template <typename T, int LAW, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void kElemTet4(int E, const int* __restrict__ connDof,
                                                  const double* __restrict__ ginv,
@@ -276,11 +280,14 @@ __global__ __launch_bounds__(kThreads) void kElemScalar(ElemArgs a, const double
 }
 
 // LPN lanes per free node (they split its slot list, then combine in a fixed
-// butterfly), so LPN times as many independent loads are in flight
-template <typename T, int LPN>
+// butterfly), so LPN times as many independent loads are in flight.  DOT:
+// the block's partials of the line-search scalars of the new r as well
+// (K = 3, or 4 with the distance to a target), at dotWork[k nb + block]
+template <typename T, int LPN, int DOT = 0>
 __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
 {
-    const int t = xcdBlock() * kThreads + threadIdx.x;
+    const int blk = xcdBlock();
+    const int t = blk * kThreads + threadIdx.x;
     const int i = t / LPN, sub = t % LPN;
     const T* es = static_cast<const T*>(a.eslots);
     double se[3] = {0, 0, 0}, sf[3] = {0, 0, 0};
@@ -309,13 +316,41 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
             se[c] += __shfl_xor(se[c], off, 64);
             sf[c] += __shfl_xor(sf[c], off, 64);
         }
-    if (i >= a.nFree || sub != 0) return;
-    // residual: ce = 1, cf = -1 (r = f_int - f_pressure - f_const)
+    if constexpr (DOT == 0) {
+        if (i >= a.nFree || sub != 0) return;
+        // residual: ce = 1, cf = -1 (r = f_int - f_pressure - f_const)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double r = a.ce * se[c] + a.cf * sf[c];
-        if (a.fext) r -= a.fext[3 * i + c];
-        a.r[3 * i + c] = r;
+        for (int c = 0; c < 3; ++c) {
+            double r = a.ce * se[c] + a.cf * sf[c];
+            if (a.fext) r -= a.fext[3 * i + c];
+            a.r[3 * i + c] = r;
+        }
+    } else {
+        double v[DOT];
+#pragma unroll
+        for (int k = 0; k < DOT; ++k) v[k] = 0.0;
+        if (i < a.nFree && sub == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double r = a.ce * se[c] + a.cf * sf[c];
+                if (a.fext) r -= a.fext[3 * i + c];
+                a.r[3 * i + c] = r;
+                const double xi = a.dotX[3 * i + c];
+                v[0] += r * a.dotD[3 * i + c];
+                v[1] = fmax(v[1], fabs(r));
+                v[2] = fmax(v[2], fabs(a.dotCb ? xi - a.dotCb[3 * i + c] : xi));
+                if constexpr (DOT == 4) {
+                    const double e = xi - a.dotXt[3 * i + c];
+                    v[3] += e * e;
+                }
+            }
+        }
+        __shared__ double sh[DOT][kThreads / 64];
+        blockReduce<DOT, kThreads / 64>(v, sh, 0x6u);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < DOT; ++k) a.dotWork[static_cast<size_t>(k) * gridDim.x + blk] = v[k];
+        }
     }
 }
 
@@ -429,7 +464,7 @@ __device__ __forceinline__ void writePartials(double (&v)[K], double* work, unsi
 // before the partials so that the two loads overlap
 template <int K, typename Epi>
 __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, unsigned maxMask,
-                                                     Epi epi)
+                                                     Epi epi, int stride = kReduceBlocks)
 {
     __shared__ double sh[K][kRFW];
     const typename Epi::Pre pre = epi.prefetch();
@@ -439,7 +474,7 @@ __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ 
     for (int b = threadIdx.x; b < nb; b += kRF)
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double o = work[k * kReduceBlocks + b];
+            const double o = work[static_cast<size_t>(k) * stride + b];
             v[k] = isMax(maxMask, k) ? fmax(v[k], o) : v[k] + o;
         }
     blockReduce<K, kRFW>(v, sh, maxMask);
@@ -1076,16 +1111,31 @@ void launchAxpby(double* out, double a, const double* x, double b, const double*
     if (n > 0) kAxpby<<<blocks(n), kThreads, 0, s>>>(out, a, x, b, y, n);
 }
 
+size_t gatherDotWork(int nFree) { return 4 * (static_cast<size_t>(nFree) * 8 / kThreads + 2); }
+
+template <int DOT>
+void gatherLaunch(const GatherArgs& a, int lpn, int g, hipStream_t s)
+{
+    switch (lpn) {
+        case 1: a.mixed ? kGather<float, 1, DOT><<<g, kThreads, 0, s>>>(a) : kGather<double, 1, DOT><<<g, kThreads, 0, s>>>(a); break;
+        case 2: a.mixed ? kGather<float, 2, DOT><<<g, kThreads, 0, s>>>(a) : kGather<double, 2, DOT><<<g, kThreads, 0, s>>>(a); break;
+        case 8: a.mixed ? kGather<float, 8, DOT><<<g, kThreads, 0, s>>>(a) : kGather<double, 8, DOT><<<g, kThreads, 0, s>>>(a); break;
+        default: a.mixed ? kGather<float, 4, DOT><<<g, kThreads, 0, s>>>(a) : kGather<double, 4, DOT><<<g, kThreads, 0, s>>>(a);
+    }
+}
+
 void launchGather(const GatherArgs& a, hipStream_t s)
 {
     if (a.nFree <= 0) return;
     const int lpn = tuning().gatherLanes;
     const int g = static_cast<int>((static_cast<long long>(lpn) * a.nFree + kThreads - 1) / kThreads);
-    switch (lpn) {
-        case 1: a.mixed ? kGather<float, 1><<<g, kThreads, 0, s>>>(a) : kGather<double, 1><<<g, kThreads, 0, s>>>(a); break;
-        case 2: a.mixed ? kGather<float, 2><<<g, kThreads, 0, s>>>(a) : kGather<double, 2><<<g, kThreads, 0, s>>>(a); break;
-        case 8: a.mixed ? kGather<float, 8><<<g, kThreads, 0, s>>>(a) : kGather<double, 8><<<g, kThreads, 0, s>>>(a); break;
-        default: a.mixed ? kGather<float, 4><<<g, kThreads, 0, s>>>(a) : kGather<double, 4><<<g, kThreads, 0, s>>>(a);
+    if (!a.dotWork) return gatherLaunch<0>(a, lpn, g, s);
+    if (a.dotXt) {
+        gatherLaunch<4>(a, lpn, g, s);
+        kReduceFinish<4><<<1, kRF, 0, s>>>(a.dotWork, g, 0x6u, EpiStore<4>{{}, a.dotOut}, g);
+    } else {
+        gatherLaunch<3>(a, lpn, g, s);
+        kReduceFinish<3><<<1, kRF, 0, s>>>(a.dotWork, g, 0x6u, EpiStore<3>{{}, a.dotOut}, g);
     }
 }
 

@@ -75,8 +75,20 @@ struct GatherArgs {
     const double* fext;     // constant nodal forces (may be null)
     double ce, cf;          // r = ce sum(element slots) + cf sum(face slots) - fext
     double* r;
+    // optional (dotWork non-null): the line search's scalars of the new r in
+    // the same launch -- dotOut[0] = r . dotD, [1] = max|r|, [2] = max|dotX -
+    // dotCb| (dotCb may be null), [3] = |dotX - dotXt|^2 (dotXt non-null);
+    // dotWork holds 4 partials per gather block (see gatherDotWork)
+    const double* dotD = nullptr;
+    const double* dotX = nullptr;
+    const double* dotCb = nullptr;
+    const double* dotXt = nullptr;
+    double* dotWork = nullptr;
+    double* dotOut = nullptr;
 };
 void launchGather(const GatherArgs& a, hipStream_t s);
+// doubles of dotWork a gather over nFree nodes needs (any lanes per node)
+size_t gatherDotWork(int nFree);
 
 // ---- approximate Jacobian ----------------------------------------------------
 // The scalar Laplacian L_ij = sum_e sum_q V_e c_e w_q grad N_i . grad N_j
