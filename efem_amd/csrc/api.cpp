@@ -56,6 +56,10 @@ struct DeviceFactor {
 // levels with at least this many 4-column backward items use 16-column
 // items (default of the context option "bwd_narrow_items")
 constexpr int kNarrowMinBwdItems = 4096;
+// levels of wide (tall) supernodes with at most this many forward (backward)
+// items use 16-wave items (defaults of "wide_fwd_items" / "wide_bwd_items")
+constexpr int kWideMaxFwdItems = 500;
+constexpr int kWideMaxBwdItems = 1000;
 
 struct efem_ctx {
     std::string err;
@@ -129,6 +133,10 @@ struct efem_ctx {
     bool fwdTables = true;       // option "fwd_tables"
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
+    // options "wide_fwd_items" / "wide_bwd_items": levels of wide (tall)
+    // supernodes with at most this many items use 16-wave items
+    int wideFwdItems = kWideMaxFwdItems;
+    int wideBwdItems = kWideMaxBwdItems;
     // host factor memory reused by the next factorisation of this problem
     std::unique_ptr<efem::HostWorkspace> hostWs{new efem::HostWorkspace};
     // approximate Jacobian assembled on the device (option
@@ -146,8 +154,6 @@ namespace {
 // supernode shapes that switch a level to 16-wave triangular-solve items
 constexpr int kWideColumns = 128;
 constexpr int kTallRows = 256;
-constexpr int kWideMaxFwdItems = 500;
-constexpr int kWideMaxBwdItems = 1000;
 
 // scalar slots in ctx->scal
 constexpr int kOut = 0;       // [3] reduction outputs
@@ -554,8 +560,8 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
             maxm[l] = std::max(maxm[l], mm[s]);
         }
         for (int l = 0; l < F.nlevels; ++l) {
-            d.fwdWide[l] = maxw[l] >= kWideColumns && nf[l] <= kWideMaxFwdItems;
-            d.bwdWide[l] = maxm[l] >= kTallRows && nb[l] <= kWideMaxBwdItems;
+            d.fwdWide[l] = maxw[l] >= kWideColumns && nf[l] <= c->wideFwdItems;
+            d.bwdWide[l] = maxm[l] >= kTallRows && nb[l] <= c->wideBwdItems;
             // many 4-column items: 16-column items keep the level's waves
             // within what the chip holds at once (not in batches, whose
             // persistent kernel walks 4-column items)
@@ -1625,6 +1631,13 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         // levels with at least this many 4-column backward items use
         // 16-column items; takes effect at the next factorisation
         c->narrowMinItems = value >= 2147483647.0 ? 2147483647 : iv;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
+    if ((k == "wide_fwd_items" || k == "wide_bwd_items") && value >= 0.0) {
+        // takes effect at the next factorisation
+        (k == "wide_fwd_items" ? c->wideFwdItems : c->wideBwdItems) =
+            value >= 2147483647.0 ? 2147483647 : iv;
         releaseFactor(c);
         return EFEM_OK;
     }

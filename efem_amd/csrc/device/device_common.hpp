@@ -617,33 +617,87 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
         // stored zeros of L11^-1's upper triangle, so a sweep aligned to 32
         // rows reads whole lines at no cost to the result
         int r = (k & ~31) + part * 64 + lane;
-        // four rows at a time: their B values and row indices are loaded
-        // before the dependent gathers (same summation order as one by one)
-        for (; r + 3 * step < m; r += 4 * step) {
-            int rr[4];
+        // four rows at a time, their B values and row indices loaded before
+        // the dependent gathers (same summation order as one by one).  FP32:
+        // software-pipelined -- the next four rows' values and indices load
+        // while this four's gathers are in flight (measured: FP32 apply
+        // -1.7 %, FP64 +2 % from the extra registers, so FP64 keeps the plain
+        // batches)
+        if constexpr (sizeof(T) == 8) {
+            for (; r + 3 * step < m; r += 4 * step) {
+                int rr[4];
+                T bk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    rr[u] = r + u * step;
+                    bk[u] = cl[rr[u]];
+                }
+                int gi[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) gi[u] = rr[u] < w ? -1 : rows[rr[u]];
+                T sv[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const T* src =
+                        gi[u] < 0 ? y + 3 * static_cast<size_t>(j0 + rr[u]) : z + 3 * static_cast<size_t>(gi[u]);
+                    const T sg = gi[u] < 0 ? T(1) : T(-1);
+                    sv[u][0] = sg * src[0];
+                    sv[u][1] = sg * src[1];
+                    sv[u][2] = sg * src[2];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc0 += bk[u] * sv[u][0];
+                    acc1 += bk[u] * sv[u][1];
+                    acc2 += bk[u] * sv[u][2];
+                }
+            }
+        } else if (r + 3 * step < m) {
+            int rr[4], gi[4];
             T bk[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 rr[u] = r + u * step;
                 bk[u] = cl[rr[u]];
+                gi[u] = rr[u] < w ? -1 : rows[rr[u]];
             }
-            int gi[4];
+            for (;;) {
+                T sv[4][3];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) gi[u] = rr[u] < w ? -1 : rows[rr[u]];
-            T sv[4][3];
+                for (int u = 0; u < 4; ++u) {
+                    const T* src =
+                        gi[u] < 0 ? y + 3 * static_cast<size_t>(j0 + rr[u]) : z + 3 * static_cast<size_t>(gi[u]);
+                    const T sg = gi[u] < 0 ? T(1) : T(-1);
+                    sv[u][0] = sg * src[0];
+                    sv[u][1] = sg * src[1];
+                    sv[u][2] = sg * src[2];
+                }
+                const int rn = r + 4 * step;
+                const bool more = rn + 3 * step < m;
+                int rr2[4], gi2[4];
+                T bk2[4];
+                if (more) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const T* src = gi[u] < 0 ? y + 3 * static_cast<size_t>(j0 + rr[u]) : z + 3 * static_cast<size_t>(gi[u]);
-                const T sg = gi[u] < 0 ? T(1) : T(-1);
-                sv[u][0] = sg * src[0];
-                sv[u][1] = sg * src[1];
-                sv[u][2] = sg * src[2];
-            }
+                    for (int u = 0; u < 4; ++u) {
+                        rr2[u] = rn + u * step;
+                        bk2[u] = cl[rr2[u]];
+                        gi2[u] = rr2[u] < w ? -1 : rows[rr2[u]];
+                    }
+                }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                acc0 += bk[u] * sv[u][0];
-                acc1 += bk[u] * sv[u][1];
-                acc2 += bk[u] * sv[u][2];
+                for (int u = 0; u < 4; ++u) {
+                    acc0 += bk[u] * sv[u][0];
+                    acc1 += bk[u] * sv[u][1];
+                    acc2 += bk[u] * sv[u][2];
+                }
+                r = rn;
+                if (!more) break;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    rr[u] = rr2[u];
+                    bk[u] = bk2[u];
+                    gi[u] = gi2[u];
+                }
             }
         }
         for (; r < m; r += step) {

@@ -235,6 +235,9 @@ const char* efem_host_dense_kernel(void);
  *                       children's update values through per-item tables
  *                       (same sums, same order); 0: through the row lists;
  *                       releases the factor
+ *   "wide_fwd_items", "wide_bwd_items"  (defaults 500, 1000): levels of wide
+ *                       (tall) supernodes with at most this many forward
+ *                       (backward) items use 16-wave items; release the factor
  *   "dense_subtree"     1..768 (default 64): elimination subtrees of at most
  *                       this many columns become one dense supernode (fewer
  *                       triangular-solve levels, more stored entries);
