@@ -715,7 +715,6 @@ int deviceLaplacian(efem_ctx* c)
     a.law = m.law;
     a.ppe = m.paramsPerElem;
     a.nnz = static_cast<int>(c->lapA.x.size());
-    a.connDof = c->connDof;
     a.ginv = static_cast<const double*>(c->ginv64);
     a.vol = c->vol;
     a.gdN = static_cast<const double*>(c->gdN64);

@@ -99,7 +99,6 @@ size_t gatherDotWork(int nFree);
 // contracted multiply-adds), so the values are the host's bit for bit.
 struct LapArgs {
     int E, nen, ng, law, ppe, nnz;
-    const int* connDof;    // unused by the kernels (kept for checks)
     const double* ginv;    // [ng][9][E] FP64
     const double* vol;     // [E]
     const double* gdN;     // [ng][nen][3] FP64
