@@ -1287,7 +1287,9 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     // the next direction d2 = -H r (LBFGSMat::apply_Hv) and its slope r . d2
     // (kOut + 4).  Writes S/Y slot loc, the Gram row / column of loc, the
     // scalar slots and d2; reads x, xp, r, rp and the rest of the history.
+    bool preformed = false;  // the next first trial's x already formed
     auto nextDirection = [&]() -> int {
+        preformed = false;
         const int loc = ptr % mc;
         double* sl = c->S + static_cast<size_t>(loc) * n;
         double* yl = c->Y + static_cast<size_t>(loc) * n;
@@ -1312,8 +1314,12 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             } else {
                 efem::dev::launchDivDev(c->d2, sc + kTheta, n, c->stream);
             }
+            // the next first trial x + 1 d goes straight into the spare that
+            // becomes x after rotate()
             efem::dev::launchTwoLoopSecond(h, c->S, c->Y, n, ncorr, loc, mc, sc + kGram, sc + kYs, sc + kAlpha,
-                                           sc + kCoef, c->d2, c->work, c->stream, c->r, sc + kRS, sc + kOut + 4);
+                                           sc + kCoef, c->d2, c->work, c->stream, c->r, sc + kRS, sc + kOut + 4,
+                                           c->x, c->x3);
+            preformed = true;
         } else {
             efem::dev::launchDiff2(sl, yl, c->x, c->xp, c->r, c->rp, n, c->stream);
             efem::dev::launchDot2(sl, yl, n, c->work, sc + kYs + loc, sc + kYY + loc, c->stream);
@@ -1373,7 +1379,8 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     // ||x - u_c||_inf, ||x - x_target||^2, slope r . d of the iteration}
     // copied to pinned slot `slot` behind an event
     auto enqueueTrial = [&](double step, int slot) -> int {
-        efem::dev::launchStep(c->x, c->xp, c->d, step, n, c->stream);
+        if (!(preformed && step == 1.0)) efem::dev::launchStep(c->x, c->xp, c->d, step, n, c->stream);
+        preformed = false;
         // the trial's scalars come with the residual's gather
         int rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr, sc + kOut);
         if (rc2) return rc2;

@@ -824,11 +824,15 @@ struct EpiTwoLoopDots {
     }
 };
 
-// out = base * sgn_b + sum_{k < NC} coef_k R_k * sgn_c
+// out = base * sgn_b + sum_{k < NC} coef_k R_k * sgn_c; with xout, also
+// xout = xbase + out (a line-search step of length 1 from xbase: the same
+// value kStep forms from the stored out)
 template <int NC>
 __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict__ base, double sgnB,
                                                        const double* __restrict__ R, const double* coef,
-                                                       double sgnC, int n, double* __restrict__ out)
+                                                       double sgnC, int n, double* __restrict__ out,
+                                                       const double* __restrict__ xbase = nullptr,
+                                                       double* __restrict__ xout = nullptr)
 {
     __shared__ double c[NC];
     if (threadIdx.x < NC) c[threadIdx.x] = sgnC * coef[threadIdx.x];
@@ -843,6 +847,7 @@ __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict_
 #pragma unroll
     for (int k = 0; k < NC; ++k) v += c[k] * rk[k];
     out[i] = v;
+    if (xout) xout[i] = xbase[i] + 1.0 * v;
 }
 
 // ---- compensated sum --------------------------------------------------------
@@ -1234,7 +1239,8 @@ void launchGramFirst(double* S, double* Y, const double* x, const double* xp, co
 
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
                          const double* G, const double* ys, const double* alpha, double* coef, double* out,
-                         double* work, hipStream_t s, const double* dw, const double* rs, double* slope)
+                         double* work, hipStream_t s, const double* dw, const double* rs, double* slope,
+                         const double* xbase, double* xout)
 {
     withPairCount(nc, [&](auto ncc) {
         constexpr int NC = decltype(ncc)::value;
@@ -1248,7 +1254,7 @@ void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int 
             kReduceFinish<NC + 1><<<1, kRF, 0, s>>>(
                 work, g, 0u, EpiTwoLoopDots<16, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
         }
-        if (n > 0) kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(h, 1.0, S, coef, 1.0, n, out);
+        if (n > 0) kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(h, 1.0, S, coef, 1.0, n, out, xbase, xout);
     });
 }
 

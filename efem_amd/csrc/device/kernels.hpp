@@ -148,10 +148,12 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
 // output), then out = h + sum (alpha_i - beta_i) s_i and, with rs_k =
 // s_k . dw from launchGramFirst, *slope = dw . out, formed as
 // dw . h + sum (alpha_k - beta_k) rs_k (dw required; 1 <= nc <= 16)
+// xbase, xout (optional): also xout = xbase + out, the first line-search
+// trial x + 1 d formed with the direction
 void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest,
                          int mcap, const double* G, const double* ys, const double* alpha, double* coef,
                          double* out, double* work, hipStream_t s, const double* dw, const double* rs,
-                         double* slope);
+                         double* slope, const double* xbase = nullptr, double* xout = nullptr);
 // q += (*c) * v   (or q -= when neg)
 void launchAxpyDev(double* q, const double* v, const double* c, int neg, int n, hipStream_t s);
 // q /= (*theta)
