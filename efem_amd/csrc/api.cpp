@@ -29,6 +29,7 @@ struct DeviceFactor {
     int n = 0, nsuper = 0, nlevels = 0;
     long long nnz = 0;
     int* perm = nullptr;
+    int* pinv = nullptr;    // [n] old -> new
     int* rows = nullptr;
     void* B = nullptr;
     int *fptr = nullptr, *fidx = nullptr;
@@ -398,11 +399,12 @@ efem::dev::TrsvArgs trsvArgs(efem_ctx* c)
     return a;
 }
 
-void launchApplySweeps(efem_ctx* c, const double* v, double* out)
+// permuted: the caller already wrote the permuted right-hand side into f.bp
+void launchApplySweeps(efem_ctx* c, const double* v, double* out, bool permuted = false)
 {
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
-    efem::dev::launchPermute(a, v, f.bp, c->stream);
+    if (!permuted) efem::dev::launchPermute(a, v, f.bp, c->stream);
     for (int l = 0; l < f.nlevels; ++l)
         efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], f.fwdWide[l], c->stream,
                                      f.fwdK[l] ? f.fwdTab + f.fwdTabOff[l] : nullptr, f.fwdK[l], f.fwdW[l]);
@@ -440,10 +442,10 @@ int applyH0Graph(efem_ctx* c)
 // out = K^-1 v on the device (v, out: interleaved free-DOF vectors).  With
 // the graph option the forward/backward level launches are replayed from one
 // captured HIP graph reading `in` and writing `out` staging buffers.
-int applyH0Internal(efem_ctx* c, const double* v, double* out)
+int applyH0Internal(efem_ctx* c, const double* v, double* out, bool permuted = false)
 {
     DeviceFactor& f = c->fac;
-    if (c->useGraph) {
+    if (c->useGraph && !permuted) {
         const size_t nb = 3 * static_cast<size_t>(f.n) * sizeof(double);
         if (v != f.in) HIP_CHECK(c, hipMemcpyAsync(f.in, v, nb, hipMemcpyDeviceToDevice, c->stream));
         int rc = applyH0Graph(c);
@@ -453,7 +455,7 @@ int applyH0Internal(efem_ctx* c, const double* v, double* out)
     }
     const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
     if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
-    launchApplySweeps(c, v, out);
+    launchApplySweeps(c, v, out, permuted);
     if (prof) {
         HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used + 1], c->stream));
         c->evH0Used += 2;
@@ -644,6 +646,11 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     }
     int rc;
     if ((rc = upload(c, &d.perm, F.perm))) return rc;
+    {
+        std::vector<int> pinv(F.perm.size());
+        for (size_t k = 0; k < F.perm.size(); ++k) pinv[F.perm[k]] = static_cast<int>(k);
+        if ((rc = upload(c, &d.pinv, pinv))) return rc;
+    }
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
     {
         // block-inverse values, repacked to the device layout (leading
@@ -1300,16 +1307,19 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             // the new pair, its Gram row / column, theta and the first loop
             // in one pass; with the captured preconditioner the first pass
             // writes its staging input and the second reads its output
-            double* q0 = staged ? c->fac.in : c->d2;
+            // with launched (not captured) preconditioner sweeps the pass
+            // writes the permuted right-hand side the sweeps read
+            const bool permuted = h0 && !staged;
+            double* q0 = staged ? c->fac.in : permuted ? c->fac.bp : c->d2;
             efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, ncorr, loc, mc, sc + kGram,
                                        sc + kYs, sc + kYY + loc, sc + kTheta, sc + kAlpha, sc + kRS, q0, c->work,
-                                       c->stream);
+                                       c->stream, permuted ? c->fac.pinv : nullptr);
             const double* h = c->d2;
             if (staged) {
                 if ((rc2 = applyH0Graph(c))) return rc2;
                 h = c->fac.out;
             } else if (h0) {
-                if ((rc2 = applyH0Internal(c, c->d2, c->tmp))) return rc2;
+                if ((rc2 = applyH0Internal(c, nullptr, c->tmp, true))) return rc2;
                 h = c->tmp;
             } else {
                 efem::dev::launchDivDev(c->d2, sc + kTheta, n, c->stream);

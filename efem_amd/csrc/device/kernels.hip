@@ -827,12 +827,15 @@ struct EpiTwoLoopDots {
 // out = base * sgn_b + sum_{k < NC} coef_k R_k * sgn_c; with xout, also
 // xout = xbase + out (a line-search step of length 1 from xbase: the same
 // value kStep forms from the stored out)
+// With pinv, out is written in the factor's order instead: out[3 pinv[i / 3]
+// + i % 3] (the permuted right-hand side the triangular solves read).
 template <int NC>
 __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict__ base, double sgnB,
                                                        const double* __restrict__ R, const double* coef,
                                                        double sgnC, int n, double* __restrict__ out,
                                                        const double* __restrict__ xbase = nullptr,
-                                                       double* __restrict__ xout = nullptr)
+                                                       double* __restrict__ xout = nullptr,
+                                                       const int* __restrict__ pinv = nullptr)
 {
     __shared__ double c[NC];
     if (threadIdx.x < NC) c[threadIdx.x] = sgnC * coef[threadIdx.x];
@@ -846,7 +849,12 @@ __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict_
     double v = sgnB * b;
 #pragma unroll
     for (int k = 0; k < NC; ++k) v += c[k] * rk[k];
-    out[i] = v;
+    if (pinv) {
+        const int node = i / 3;
+        out[3 * static_cast<size_t>(pinv[node]) + (i - 3 * node)] = v;
+    } else {
+        out[i] = v;
+    }
     if (xout) xout[i] = xbase[i] + 1.0 * v;
 }
 
@@ -1219,7 +1227,7 @@ static_assert(kMaxCompact == 16, "withPairCount covers 1..kMaxCompact");
 
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
-                     double* alpha, double* rs, double* out, double* work, hipStream_t s)
+                     double* alpha, double* rs, double* out, double* work, hipStream_t s, const int* pinv)
 {
     withPairCount(nc, [&](auto ncc) {
         constexpr int NC = decltype(ncc)::value;
@@ -1233,7 +1241,8 @@ void launchGramFirst(double* S, double* Y, const double* x, const double* xp, co
             kReduceFinish<3 * NC + 1><<<1, kRF, 0, s>>>(
                 work, g, 0u, EpiGramFirst<16, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
         }
-        if (n > 0) kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, out);
+        if (n > 0)
+            kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, out, nullptr, nullptr, pinv);
     });
 }
 

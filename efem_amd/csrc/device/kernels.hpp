@@ -164,9 +164,11 @@ void launchTheta(const double* ys, const double* yy, double* theta, hipStream_t 
 // S[loc] = x - xp, Y[loc] = r - rp; Gram row / column of loc, ys[loc],
 // *yy = y.y, *theta = yy / ys[loc], rs[k] = s_k . r; alpha newest ->
 // oldest; then out = -r - sum alpha_i y_i
+// (pinv: out written in the factor's node order, out[3 pinv[node] + c])
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
-                     double* alpha, double* rs, double* out, double* work, hipStream_t s);
+                     double* alpha, double* rs, double* out, double* work, hipStream_t s,
+                     const int* pinv = nullptr);
 
 // ---- supernodal triangular solves (3 right-hand sides) ---------------------
 struct TrsvArgs {
