@@ -252,16 +252,33 @@ static bool potrfUnblocked(double* A, int n, int ld)
     return true;
 }
 
-bool potrf(double* A, int n, int ld)
+bool potrf(double* A, int n, int ld, bool par)
 {
     // left-looking by column blocks: A[j:, J] -= A[j:, :j] A[J, :j]^T, then
     // factor the diagonal block and solve the rows below it
     for (int j = 0; j < n; j += NB) {
         const int jb = std::min(NB, n - j);
         double* Ajj = A + j + static_cast<size_t>(j) * ld;
-        gemm(Ajj, ld, A + j, ld, A + j, ld, true, n - j, jb, j, -1.0);
+        const int rows = n - j;
+        if (par && rows > MC && j > 0) {
+            // row ranges of MC (the packing block): the same tiles per entry
+#pragma omp taskloop grainsize(1)
+            for (int r0 = 0; r0 < rows; r0 += MC)
+                gemm(Ajj + r0, ld, A + j + r0, ld, A + j, ld, true, std::min(MC, rows - r0), jb, j, -1.0);
+        } else {
+            gemm(Ajj, ld, A + j, ld, A + j, ld, true, rows, jb, j, -1.0);
+        }
         if (!potrfUnblocked(Ajj, jb, ld)) return false;
-        if (j + jb < n) trsmRightLT(Ajj, ld, Ajj, ld, jb, jb, n - j);
+        if (j + jb < n) {
+            if (par && rows - jb > MC) {
+                // independent rows
+#pragma omp taskloop grainsize(1)
+                for (int r0 = jb; r0 < rows; r0 += MC)
+                    trsmRightLT(Ajj, ld, Ajj, ld, jb, r0, std::min(rows, r0 + MC));
+            } else {
+                trsmRightLT(Ajj, ld, Ajj, ld, jb, jb, rows);
+            }
+        }
     }
     return true;
 }
@@ -288,7 +305,7 @@ void trsmRightLT(double* X, int ld, const double* L, int ldl, int n, int i0, int
     }
 }
 
-void trtri(const double* L, int ldl, double* Linv, int ldi, int n)
+void trtri(const double* L, int ldl, double* Linv, int ldi, int n, bool par)
 {
     // X = L^-1 by row blocks (forward substitution on the identity):
     // X[I, :i0] = -L[I, I]^-1 (L[I, :i0] X[:i0, :i0]), X[I, I] = L[I, I]^-1
@@ -297,18 +314,34 @@ void trtri(const double* L, int ldl, double* Linv, int ldi, int n)
     for (int i0 = 0; i0 < n; i0 += NB) {
         const int ib = std::min(NB, n - i0);
         double* XI = Linv + i0;  // rows I of X
-        // X[I, :i0] = -(L[I, :i0] X[:i0, :i0])
-        gemm(XI, ldi, L + i0, ldl, Linv, ldi, false, ib, i0, i0, -1.0);
+        // X[I, :i0] = -(L[I, :i0] X[:i0, :i0]), in column ranges of NC (the
+        // packing block: the same tiles per entry) when split into tasks
+        if (par && i0 > NC) {
+#pragma omp taskloop grainsize(1)
+            for (int c0 = 0; c0 < i0; c0 += NC)
+                gemm(XI + static_cast<size_t>(c0) * ldi, ldi, L + i0, ldl, Linv + static_cast<size_t>(c0) * ldi, ldi,
+                     false, ib, std::min(NC, i0 - c0), i0, -1.0);
+        } else {
+            gemm(XI, ldi, L + i0, ldl, Linv, ldi, false, ib, i0, i0, -1.0);
+        }
         // diagonal block: X[I, I] = L[I, I]^-1 (columns of the identity)
         for (int k = 0; k < ib; ++k) XI[k + static_cast<size_t>(i0 + k) * ldi] = 1.0;
         // X[I, :i0 + ib] <- L[I, I]^-1 X[I, :i0 + ib]  (lower solve per column)
-        for (int c = 0; c < i0 + ib; ++c) {
-            double* x = XI + static_cast<size_t>(c) * ldi;
-            for (int r = 0; r < ib; ++r) {
-                double v = x[r];
-                for (int p = 0; p < r; ++p) v -= L[(i0 + r) + static_cast<size_t>(i0 + p) * ldl] * x[p];
-                x[r] = v / L[(i0 + r) + static_cast<size_t>(i0 + r) * ldl];
+        auto solveCols = [&](int c0, int c1) {
+            for (int c = c0; c < c1; ++c) {
+                double* x = XI + static_cast<size_t>(c) * ldi;
+                for (int r = 0; r < ib; ++r) {
+                    double v = x[r];
+                    for (int p = 0; p < r; ++p) v -= L[(i0 + r) + static_cast<size_t>(i0 + p) * ldl] * x[p];
+                    x[r] = v / L[(i0 + r) + static_cast<size_t>(i0 + r) * ldl];
+                }
             }
+        };
+        if (par && i0 + ib > 2 * NC) {
+#pragma omp taskloop grainsize(1)
+            for (int c0 = 0; c0 < i0 + ib; c0 += NC) solveCols(c0, std::min(i0 + ib, c0 + NC));
+        } else {
+            solveCols(0, i0 + ib);
         }
     }
 }

@@ -18,7 +18,10 @@ void gemm(double* C, int ldc, const double* A, int lda, const double* B, int ldb
 
 // in-place lower Cholesky of the leading n x n block; false if a pivot is
 // not positive
-bool potrf(double* A, int n, int ld);
+// par: the panel updates and solves split into OpenMP tasks over 128-row
+// (96-column) ranges aligned to the packing blocks, so every entry sees the
+// same arithmetic as without them
+bool potrf(double* A, int n, int ld, bool par = false);
 
 // rows [i0, i1) of X (m x n block at X, ld) <- X L^-T, L the lower n x n
 // block at L (ldl): X = A21 L11^-T of the supernode
@@ -26,7 +29,7 @@ void trsmRightLT(double* X, int ld, const double* L, int ldl, int n, int i0, int
 
 // Linv (n x n, ldi, lower) <- L^-1, L lower n x n (ldl); the strict upper
 // part of Linv is set to 0
-void trtri(const double* L, int ldl, double* Linv, int ldi, int n);
+void trtri(const double* L, int ldl, double* Linv, int ldi, int n, bool par = false);
 
 // M (rows x n, ldm) += L21 Linv, L21 rows x n (ldl), Linv lower n x n (ldi):
 // M(i, k) = sum_{p >= k} L21(i, p) Linv(p, k)

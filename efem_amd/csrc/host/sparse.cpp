@@ -146,7 +146,7 @@ void syrkLower(const double* A, int n, int m, int ld, double* U, int ldu, bool p
 void blockInverse(double* P, int w, int m, bool par)
 {
     std::vector<double> Linv(static_cast<size_t>(w) * w);
-    dense::trtri(P, m, Linv.data(), w, w);
+    dense::trtri(P, m, Linv.data(), w, w, par && w >= 256);
     auto chunk = [&](int i0, int i1) {
         const int r = i1 - i0;
         std::vector<double> t(static_cast<size_t>(r) * w, 0.0);
@@ -624,7 +624,8 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
             }
             upd[c].reset();
         }
-        if (!dense::potrf(P, w, m)) {
+        const bool bigDiag = w >= 256;
+        if (!dense::potrf(P, w, m, bigDiag)) {
             failed = 1;
             return;
         }
@@ -634,7 +635,7 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
             syrkLower(P, w, nr, m, U.get(), nr, big);
             upd[s] = std::move(U);
         }
-        blockInverse(P, w, m, big);
+        blockInverse(P, w, m, big || w >= 256);
     };
     // postorder recursion with tasks for large subtrees
     std::function<void(int)> process = [&](int s) {
