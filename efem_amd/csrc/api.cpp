@@ -134,6 +134,12 @@ struct efem_ctx {
     bool fwdTables = true;       // option "fwd_tables"
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
+    // ordering + symbolic analysis of the approximate Jacobian: its pattern
+    // is fixed by the mesh and the fixed nodes, so it is computed at the
+    // first factorisation and reused by every refactorisation (option
+    // "reuse_symbolic" = 0: recomputed every time)
+    bool reuseSymbolic = true;
+    std::unique_ptr<efem::SupernodalSymbolic> symbolic;
     // options "wide_fwd_items" / "wide_bwd_items": levels of wide (tall)
     // supernodes with at most this many items use 16-wave items
     int wideFwdItems = kWideMaxFwdItems;
@@ -758,12 +764,14 @@ int factorInternal(efem_ctx* c, int* failed)
     } else {
         // element codes e nen^2 + a nen + b must fit the map's ints
         const bool dev = c->devLaplacian && static_cast<long long>(m.nElems) * m.nen * m.nen < (1LL << 31);
+        if (c->reuseSymbolic && !c->symbolic) c->symbolic.reset(new efem::SupernodalSymbolic());
+        efem::SupernodalSymbolic* sym = c->reuseSymbolic ? c->symbolic.get() : nullptr;
         if (dev) {
             int rc0 = deviceLaplacian(c);
             if (rc0) return rc0;
-            ok = efem::factorLaplacian(m, c->lapA, F, msg, c->denseSubtree, c->hostWs.get());
+            ok = efem::factorLaplacian(m, c->lapA, F, msg, c->denseSubtree, c->hostWs.get(), sym);
         } else {
-            ok = efem::factorModel(m, F, msg, c->denseSubtree, c->hostWs.get());
+            ok = efem::factorModel(m, F, msg, c->denseSubtree, c->hostWs.get(), sym);
         }
     }
     if (!ok) {
@@ -1616,6 +1624,15 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         // assemble the approximate Jacobian on the device (1) or the host (0);
         // the same values either way; takes effect at the next factorisation
         c->devLaplacian = value != 0.0;
+        c->symbolic.reset();
+        releaseFactor(c);
+        return EFEM_OK;
+    }
+    if (k == "reuse_symbolic") {
+        // keep the ordering and symbolic analysis between factorisations (1)
+        // or recompute them every time (0)
+        c->reuseSymbolic = value != 0.0;
+        c->symbolic.reset();
         releaseFactor(c);
         return EFEM_OK;
     }
@@ -1661,6 +1678,7 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         // elimination subtrees of at most this many columns become one dense
         // supernode; takes effect at the next factorisation
         c->denseSubtree = iv;
+        c->symbolic.reset();
         releaseFactor(c);
         return EFEM_OK;
     }

@@ -1,6 +1,8 @@
 // Host preprocessing (see model.hpp).
 #include "model.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -549,17 +551,37 @@ void laplacianMap(const Model& m, Csc& L, std::vector<int>& ptr, std::vector<int
 }
 
 bool factorLaplacian(const Model& m, const Csc& A, SupernodalFactor& F, std::string& msg, int denseSubtree,
-                     HostWorkspace* ws)
+                     HostWorkspace* ws, SupernodalSymbolic* symbolic)
 {
     if (m.nFree == 0) {
         msg = "no free DOF";
         return false;
     }
-    const std::vector<int> perm = nestedDissection(A, freeCoords(m));
-    return supernodalCholesky(A, perm, F, msg, denseSubtree, ws);
+    if (!symbolic) {
+        const std::vector<int> perm = nestedDissection(A, freeCoords(m));
+        return supernodalCholesky(A, perm, F, msg, denseSubtree, ws);
+    }
+    double symSeconds = 0.0;
+    if (symbolic->n == 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const std::vector<int> perm = nestedDissection(A, freeCoords(m));
+        if (!supernodalSymbolic(A, perm, *symbolic, msg, denseSubtree)) {
+            *symbolic = SupernodalSymbolic();
+            return false;
+        }
+        symSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else if (symbolic->n != A.n || symbolic->anz != static_cast<long long>(A.i.size()) ||
+               A.x.size() != A.i.size()) {
+        msg = "symbolic analysis does not match the matrix";
+        return false;
+    }
+    if (!supernodalNumeric(A, *symbolic, F, msg, ws)) return false;
+    F.seconds += symSeconds;
+    return true;
 }
 
-bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int denseSubtree, HostWorkspace* ws)
+bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int denseSubtree, HostWorkspace* ws,
+                 SupernodalSymbolic* symbolic)
 {
     if (m.nFree == 0) {
         msg = "no free DOF";
@@ -567,8 +589,7 @@ bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg, int dens
     }
     Csc A;
     scalarLaplacian(m, A);
-    const std::vector<int> perm = nestedDissection(A, freeCoords(m));
-    return supernodalCholesky(A, perm, F, msg, denseSubtree, ws);
+    return factorLaplacian(m, A, F, msg, denseSubtree, ws, symbolic);
 }
 
 bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,

@@ -77,10 +77,15 @@ std::vector<double> freeCoords(const Model& m);
 // approximate Jacobian -> nested dissection -> supernodal factor (false: not
 // SPD or no free DOF, msg says which)
 bool factorModel(const Model& m, SupernodalFactor& F, std::string& msg,
-                 int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr);
-// the same from an assembled approximate Jacobian A (ordering + factor)
+                 int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr,
+                 SupernodalSymbolic* symbolic = nullptr);
+// the same from an assembled approximate Jacobian A (ordering + factor).
+// With `symbolic`: the ordering and symbolic analysis are computed into it
+// when it is empty (n == 0) and reused as they are otherwise (A's pattern
+// must be the one they were computed from)
 bool factorLaplacian(const Model& m, const Csc& A, SupernodalFactor& F, std::string& msg,
-                     int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr);
+                     int denseSubtree = kDefaultDenseSubtree, HostWorkspace* ws = nullptr,
+                     SupernodalSymbolic* symbolic = nullptr);
 // the same for independent parts, in parallel over the parts
 bool factorModels(const std::vector<Model>& parts, std::vector<SupernodalFactor>& F, std::string& msg,
                   int denseSubtree = kDefaultDenseSubtree);

@@ -268,13 +268,14 @@ void HostWorkspace::giveUpdate(double* p)
     free_.emplace(capacity_.at(p), p);
 }
 
-bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, SupernodalFactor& F,
-                        std::string& msg, int denseSubtree, HostWorkspace* ws)
+bool supernodalSymbolic(const Csc& A, const std::vector<int>& permIn, SupernodalSymbolic& F, std::string& msg,
+                        int denseSubtree)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const int n = A.n;
-    F = SupernodalFactor();
+    F = SupernodalSymbolic();
     F.n = n;
+    F.anz = static_cast<long long>(A.i.size());
     if (n == 0) {
         msg = "empty matrix";
         return false;
@@ -342,8 +343,10 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         for (int k = 0; k < n; ++k) pinv[perm[k]] = k;
     }
     // lower pattern + values of P A P^T by columns
-    std::vector<int> Lp(n + 1, 0), Li;
-    std::vector<double> Lx;
+    std::vector<int>& Lp = F.Lp;
+    std::vector<int>& Li = F.Li;
+    std::vector<int>& Lsrc = F.Lsrc;
+    Lp.assign(n + 1, 0);
     {
         // column k of P A P^T is column perm[k] of A: counted and filled in
         // permuted order, columns in parallel (each writes its own range;
@@ -357,7 +360,7 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         }
         for (int k = 0; k < n; ++k) Lp[k + 1] += Lp[k];
         Li.resize(Lp[n]);
-        Lx.resize(Lp[n]);
+        Lsrc.resize(Lp[n]);
 #pragma omp parallel for schedule(static, 1024)
         for (int k = 0; k < n; ++k) {
             const int col = perm[k];
@@ -366,7 +369,7 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
                 const int i = pinv[A.i[p]];
                 if (i >= k) {
                     Li[q] = i;
-                    Lx[q++] = A.x[p];
+                    Lsrc[q++] = p;
                 }
             }
         }
@@ -483,14 +486,16 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         const int pj = parent[first[s + 1] - 1];
         F.parent[s] = pj == -1 ? -1 : snOf[pj];
     }
-    std::vector<std::vector<int>> children(ns);
+    std::vector<std::vector<int>>& children = F.children;
+    children.assign(ns, {});
     for (int s = 0; s < ns; ++s)
         if (F.parent[s] >= 0) children[F.parent[s]].push_back(s);
     // symbolic: row sets of the supernodes -- the rows of its columns below
     // the supernode and its children's update rows, sorted without
     // duplicates -- children before parents, subtrees of more than 512
     // columns as OpenMP tasks
-    std::vector<long long> subtreeCols(ns, 0);
+    std::vector<long long>& subtreeCols = F.subtreeCols;
+    subtreeCols.assign(ns, 0);
     for (int s = 0; s < ns; ++s) {
         subtreeCols[s] += first[s + 1] - first[s];
         if (F.parent[s] >= 0) subtreeCols[F.parent[s]] += subtreeCols[s];
@@ -550,6 +555,41 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         F.flops += static_cast<double>(w) * w * w / 3.0 + static_cast<double>(m - w) * w * w +
                    static_cast<double>(m - w) * (m - w) * w;
     }
+    F.level.assign(ns, 0);
+    F.nlevels = 0;
+    for (int s = 0; s < ns; ++s) {
+        if (F.parent[s] >= 0) F.level[F.parent[s]] = std::max(F.level[F.parent[s]], F.level[s] + 1);
+        F.nlevels = std::max(F.nlevels, F.level[s] + 1);
+    }
+    F.perm = perm;
+    F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+bool supernodalNumeric(const Csc& A, const SupernodalSymbolic& S, SupernodalFactor& F, std::string& msg,
+                       HostWorkspace* ws)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = S.n, ns = S.nsuper;
+    F = SupernodalFactor();
+    F.n = n;
+    F.perm = S.perm;
+    F.nsuper = ns;
+    F.first = S.first;
+    F.rowPtr = S.rowPtr;
+    F.rows = S.rows;
+    F.valPtr = S.valPtr;
+    F.parent = S.parent;
+    F.level = S.level;
+    F.nlevels = S.nlevels;
+    F.nnzL = S.nnzL;
+    F.flops = S.flops;
+    const std::vector<int>& first = S.first;
+    const std::vector<int>& Lp = S.Lp;
+    const std::vector<int>& Li = S.Li;
+    const std::vector<int>& Lsrc = S.Lsrc;
+    const std::vector<std::vector<int>>& children = S.children;
+    const std::vector<long long>& subtreeCols = S.subtreeCols;
     // numeric multifrontal factorisation, tree-parallel (OpenMP tasks over
     // subtrees; big fronts split their dense kernels into tasks too).  Each
     // supernode's update matrix lives in upd[s] until its parent absorbs it.
@@ -603,7 +643,7 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         }
         for (int j = j0; j < j0 + w; ++j) {
             double* col = P + static_cast<size_t>(j - j0) * m;
-            for (int p = Lp[j]; p < Lp[j + 1]; ++p) col[loc[Li[p]]] += Lx[p];
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) col[loc[Li[p]]] += A.x[Lsrc[p]];
         }
         for (int c : children[s]) {
             const double* Uc = upd[c].get();
@@ -665,14 +705,17 @@ bool supernodalCholesky(const Csc& A, const std::vector<int>& permIn, Supernodal
         return false;
     }
     F.numericSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tn0).count();
-    F.level.assign(ns, 0);
-    F.nlevels = 0;
-    for (int s = 0; s < ns; ++s) {
-        if (F.parent[s] >= 0) F.level[F.parent[s]] = std::max(F.level[F.parent[s]], F.level[s] + 1);
-        F.nlevels = std::max(F.nlevels, F.level[s] + 1);
-    }
-    F.perm = perm;
     F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F, std::string& msg,
+                        int denseSubtree, HostWorkspace* ws)
+{
+    SupernodalSymbolic S;
+    if (!supernodalSymbolic(A, perm, S, msg, denseSubtree)) return false;
+    if (!supernodalNumeric(A, S, F, msg, ws)) return false;
+    F.seconds += S.seconds;
     return true;
 }
 

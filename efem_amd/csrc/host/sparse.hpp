@@ -48,6 +48,26 @@ struct SupernodalFactor {
     double numericSeconds = 0;
 };
 
+// Symbolic analysis of P A P^T: what the numeric factorisation needs that
+// depends on A's pattern and the ordering only, not on A's values, so a
+// refactorisation of a matrix with the same pattern reuses it.
+struct SupernodalSymbolic {
+    int n = 0;
+    long long anz = 0;                   // entries of the A it was built from
+    std::vector<int> perm;               // final (postordered) ordering, perm[new] = old
+    std::vector<int> Lp, Li;             // lower pattern of P A P^T by columns
+    std::vector<int> Lsrc;               // entry of A.x behind each lower entry
+    int nsuper = 0;
+    std::vector<int> first, rowPtr, rows, parent, level;  // as in SupernodalFactor
+    std::vector<long long> valPtr;
+    std::vector<std::vector<int>> children;
+    std::vector<long long> subtreeCols;  // columns in each supernode's subtree
+    int nlevels = 0;
+    long long nnzL = 0;
+    double flops = 0;
+    double seconds = 0;
+};
+
 // scalar Laplacian from element gradients (see model.cpp)
 // order: nested dissection on the node graph using coordinates xyz[n][3]
 std::vector<int> nestedDissection(const Csc& A, const std::vector<double>& xyz);
@@ -82,7 +102,15 @@ private:
 // solves, a few percent more stored entries); 1..kMaxSupernodeWidth.
 constexpr int kDefaultDenseSubtree = 64;
 
-// returns false if a pivot is not positive (the reference's fallback case)
+// ordering-dependent analysis: elimination tree, postorder, column counts,
+// (relaxed) supernodes, row sets, block offsets, levels
+bool supernodalSymbolic(const Csc& A, const std::vector<int>& perm, SupernodalSymbolic& S, std::string& msg,
+                        int denseSubtree = kDefaultDenseSubtree);
+// the numeric multifrontal factorisation of A (whose pattern S was built
+// from); false if a pivot is not positive
+bool supernodalNumeric(const Csc& A, const SupernodalSymbolic& S, SupernodalFactor& F, std::string& msg,
+                       HostWorkspace* ws = nullptr);
+// both; returns false if a pivot is not positive (the reference's fallback case)
 bool supernodalCholesky(const Csc& A, const std::vector<int>& perm, SupernodalFactor& F,
                         std::string& msg, int denseSubtree = kDefaultDenseSubtree,
                         HostWorkspace* ws = nullptr);

@@ -283,6 +283,35 @@ def test_device_laplacian_matches_host_assembly(which, opts, rng):
 
 @pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
 @pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8])])
+def test_refactorisation_reuses_symbolic_analysis(prec, which, opts, rng):
+    # the ordering and symbolic analysis depend on the approximate
+    # Jacobian's pattern only: a refactorisation after new per-element
+    # parameters reuses the first one's and gives the factor (and every
+    # apply) that a fresh analysis gives, bit for bit
+    m = E.Mesh.workload(which, opts)
+    P = np.atleast_2d(m.params)
+    P = np.repeat(P, m.n_elems, axis=0) if P.shape[0] == 1 else P
+    m = E.Mesh(m.X, m.conn, m.law, P.copy(), m.fixed, m.faces, m.pressure, m.reduced)
+    P2 = P * rng.uniform(0.5, 2.0, (m.n_elems, 1))
+    b = rng.standard_normal(m.ndof)
+    with E.Context(m, prec) as reuse, E.Context(m, prec) as fresh:
+        fresh.set_option("reuse_symbolic", 0)
+        assert reuse.factor()
+        first = reuse.apply_h0(b)
+        reuse.set_params(P2)
+        fresh.set_params(P2)
+        assert reuse.factor() and fresh.factor()
+        assert reuse.factor_nnz == fresh.factor_nnz
+        x = reuse.apply_h0(b)
+        assert np.array_equal(x, fresh.apply_h0(b))
+        assert not np.array_equal(x, first)
+        u_r, st_r = reuse.solve(epsilon=1e-6, max_iterations=300)
+        u_f, st_f = fresh.solve(epsilon=1e-6, max_iterations=300)
+        assert st_r["iterations"] == st_f["iterations"] and np.array_equal(u_r, u_f)
+
+
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+@pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8])])
 def test_apply_h0_forward_tables_match_lists(prec, which, opts, rng):
     # the forward sweep's per-item extend-add tables sum the same update
     # values in the same order as the row lists: identical applies, bit for
