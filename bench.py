@@ -187,14 +187,14 @@ def run_config_solves(E, args):
         t = time.perf_counter()
         u_or, so = om.solve(epsilon=1e-6, max_iterations=args.max_iters)
         rec["oracle_cpu"] = {"s": round(time.perf_counter() - t, 4), "iterations": so["iterations"], "cores": 1}
-    for label, prec in (("fp64", E.FP64), ("mixed", E.MIXED)):
+    for label, prec, sw in (("fp64", E.FP64, 0.0), ("mixed", E.MIXED, args.fp64_switch)):
         with E.Context(m, prec) as c:
-            c.solve(epsilon=1e-6, max_iterations=args.max_iters)  # warm-up
+            c.solve(epsilon=1e-6, max_iterations=args.max_iters, fp64_switch=sw)  # warm-up
             walls, its = [], []
             for _ in range(3):
                 c.factor_release()
                 t = time.perf_counter()
-                u, st = c.solve(epsilon=1e-6, max_iterations=args.max_iters)
+                u, st = c.solve(epsilon=1e-6, max_iterations=args.max_iters, fp64_switch=sw)
                 walls.append(time.perf_counter() - t)
                 its.append(st["iterations"])
         r = {"s": round(float(np.median(walls)), 4), "iterations": int(np.median(its))}

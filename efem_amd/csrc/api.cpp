@@ -405,8 +405,10 @@ efem::dev::TrsvArgs trsvArgs(efem_ctx* c)
     return a;
 }
 
-// permuted: the caller already wrote the permuted right-hand side into f.bp
-void launchApplySweeps(efem_ctx* c, const double* v, double* out, bool permuted = false)
+// permuted: the caller already wrote the permuted right-hand side into f.bp;
+// unpermute = false: the solution stays in f.z in the factor's order (the
+// caller reads it through pinv) and out is not written
+void launchApplySweeps(efem_ctx* c, const double* v, double* out, bool permuted = false, bool unpermute = true)
 {
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
@@ -416,7 +418,7 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out, bool permuted 
                                      f.fwdK[l] ? f.fwdTab + f.fwdTabOff[l] : nullptr, f.fwdK[l], f.fwdW[l]);
     for (int l = f.nlevels - 1; l >= 0; --l)
         efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], f.bwdWide[l], c->stream);
-    efem::dev::launchUnpermute(a, out, c->stream);
+    if (unpermute) efem::dev::launchUnpermute(a, out, c->stream);
 }
 
 // Replay the captured apply: reads the staging buffer f.in, writes f.out
@@ -448,7 +450,7 @@ int applyH0Graph(efem_ctx* c)
 // out = K^-1 v on the device (v, out: interleaved free-DOF vectors).  With
 // the graph option the forward/backward level launches are replayed from one
 // captured HIP graph reading `in` and writing `out` staging buffers.
-int applyH0Internal(efem_ctx* c, const double* v, double* out, bool permuted = false)
+int applyH0Internal(efem_ctx* c, const double* v, double* out, bool permuted = false, bool unpermute = true)
 {
     DeviceFactor& f = c->fac;
     if (c->useGraph && !permuted) {
@@ -461,7 +463,7 @@ int applyH0Internal(efem_ctx* c, const double* v, double* out, bool permuted = f
     }
     const bool prof = c->profiling && c->evH0Used + 2 <= static_cast<int>(c->evsH0.size());
     if (prof) HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used], c->stream));
-    launchApplySweeps(c, v, out, permuted);
+    launchApplySweeps(c, v, out, permuted, unpermute);
     if (prof) {
         HIP_CHECK(c, hipEventRecord(c->evsH0[c->evH0Used + 1], c->stream));
         c->evH0Used += 2;
@@ -1322,13 +1324,15 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, ncorr, loc, mc, sc + kGram,
                                        sc + kYs, sc + kYY + loc, sc + kTheta, sc + kAlpha, sc + kRS, q0, c->work,
                                        c->stream, permuted ? c->fac.pinv : nullptr);
-            const double* h = c->d2;
+            efem::dev::HSrc h{c->d2};
             if (staged) {
                 if ((rc2 = applyH0Graph(c))) return rc2;
-                h = c->fac.out;
+                h = efem::dev::HSrc{c->fac.out};
             } else if (h0) {
-                if ((rc2 = applyH0Internal(c, nullptr, c->tmp, true))) return rc2;
-                h = c->tmp;
+                // the second loop reads the backward sweep's solution through
+                // the inverse permutation: no unpermute launch
+                if ((rc2 = applyH0Internal(c, nullptr, nullptr, true, false))) return rc2;
+                h = efem::dev::HSrc{nullptr, c->fac.pinv, c->fac.z, fac32(c) ? 1 : 0};
             } else {
                 efem::dev::launchDivDev(c->d2, sc + kTheta, n, c->stream);
             }
@@ -2253,7 +2257,7 @@ int efem_autotune(efem_ctx* c, double* report)
             efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, hm, hm - 1, hm, sc + kGram, sc + kYs,
                                        sc + kYY, sc + kTheta, sc + kAlpha, sc + kRS, c->tmp, c->work,
                                        c->stream);
-            efem::dev::launchTwoLoopSecond(c->tmp, c->S, c->Y, n, hm, hm - 1, hm, sc + kGram, sc + kYs, sc + kAlpha,
+            efem::dev::launchTwoLoopSecond(efem::dev::HSrc{c->tmp}, c->S, c->Y, n, hm, hm - 1, hm, sc + kGram, sc + kYs, sc + kAlpha,
                                            sc + kCoef, c->d, c->work, c->stream, c->r, sc + kRS,
                                            sc + kOut + 4);
             return EFEM_OK;

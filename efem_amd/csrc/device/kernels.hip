@@ -733,18 +733,27 @@ struct EpiGramFirst {
     }
 };
 
+// element i of an H0 output (see HSrc)
+__device__ __forceinline__ double hAt(const HSrc& q, int i)
+{
+    if (!q.pinv) return q.h[i];
+    const int node = i / 3;
+    const size_t j = 3 * static_cast<size_t>(q.pinv[node]) + (i - 3 * node);
+    return q.zF32 ? static_cast<double>(static_cast<const float*>(q.z)[j]) : static_cast<const double*>(q.z)[j];
+}
+
 // Second loop of the two-loop recursion: dots of Y_k with q (the H0 output,
 // stored pairs in slots 0..NC-1) and of dw with q in one sweep.
 template <int NC>
-__global__ __launch_bounds__(kRB) void kTwoLoopDots(const double* __restrict__ q, const double* __restrict__ R,
-                                                        int n, const double* __restrict__ dw, double* work)
+__global__ __launch_bounds__(kRB) void kTwoLoopDots(HSrc q, const double* __restrict__ R, int n,
+                                                        const double* __restrict__ dw, double* work)
 {
     static_assert(NC >= 1 && NC <= kMaxCompact, "pair count");
     double v[NC + 1];
 #pragma unroll
     for (int k = 0; k <= NC; ++k) v[k] = 0.0;
     for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
-        const double qi = q[i], dwi = dw[i];
+        const double qi = hAt(q, i), dwi = dw[i];
         double rk[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) rk[k] = R[static_cast<size_t>(k) * n + i];
@@ -828,11 +837,12 @@ struct EpiTwoLoopDots {
 // xout = xbase + out (a line-search step of length 1 from xbase: the same
 // value kStep forms from the stored out)
 // With pinv, out is written in the factor's order instead: out[3 pinv[i / 3]
-// + i % 3] (the permuted right-hand side the triangular solves read).
+// + i % 3] (the permuted right-hand side the triangular solves read).  The
+// base is read as an H0 output (HSrc).
 template <int NC>
-__global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict__ base, double sgnB,
-                                                       const double* __restrict__ R, const double* coef,
-                                                       double sgnC, int n, double* __restrict__ out,
+__global__ __launch_bounds__(kThreads) void kMultiAxpy(HSrc base, double sgnB, const double* __restrict__ R,
+                                                       const double* coef, double sgnC, int n,
+                                                       double* __restrict__ out,
                                                        const double* __restrict__ xbase = nullptr,
                                                        double* __restrict__ xout = nullptr,
                                                        const int* __restrict__ pinv = nullptr)
@@ -842,7 +852,7 @@ __global__ __launch_bounds__(kThreads) void kMultiAxpy(const double* __restrict_
     __syncthreads();
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    const double b = base[i];
+    const double b = hAt(base, i);
     double rk[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) rk[k] = R[static_cast<size_t>(k) * n + i];
@@ -1242,11 +1252,12 @@ void launchGramFirst(double* S, double* Y, const double* x, const double* xp, co
                 work, g, 0u, EpiGramFirst<16, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
         }
         if (n > 0)
-            kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(r, -1.0, Y, alpha, -1.0, n, out, nullptr, nullptr, pinv);
+            kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(HSrc{r}, -1.0, Y, alpha, -1.0, n, out, nullptr, nullptr,
+                                                           pinv);
     });
 }
 
-void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
+void launchTwoLoopSecond(HSrc h, const double* S, const double* Y, int n, int nc, int newest, int mcap,
                          const double* G, const double* ys, const double* alpha, double* coef, double* out,
                          double* work, hipStream_t s, const double* dw, const double* rs, double* slope,
                          const double* xbase, double* xout)

@@ -150,7 +150,17 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
 // dw . h + sum (alpha_k - beta_k) rs_k (dw required; 1 <= nc <= 16)
 // xbase, xout (optional): also xout = xbase + out, the first line-search
 // trial x + 1 d formed with the direction
-void launchTwoLoopSecond(const double* h, const double* S, const double* Y, int n, int nc, int newest,
+// The H0 output h of the second loop: a vector in DOF order, or (pinv set)
+// the backward sweep's solution z in the factor's order, read as
+// h[i] = z[3 pinv[i / 3] + i % 3] (FP32 or FP64 z) -- the same values the
+// unpermute launch would store, without that launch.
+struct HSrc {
+    const double* h = nullptr;
+    const int* pinv = nullptr;
+    const void* z = nullptr;
+    int zF32 = 0;
+};
+void launchTwoLoopSecond(HSrc h, const double* S, const double* Y, int n, int nc, int newest,
                          int mcap, const double* G, const double* ys, const double* alpha, double* coef,
                          double* out, double* work, hipStream_t s, const double* dw, const double* rs,
                          double* slope, const double* xbase = nullptr, double* xout = nullptr);

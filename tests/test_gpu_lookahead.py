@@ -66,3 +66,20 @@ def test_lookahead_mixed_fp64_switch():
     m = E.Mesh.workload(2, [30, 6, 6])
     t = check_same(both(m, E.MIXED, epsilon=1e-7, max_iterations=3000, fp64_switch=1e-3), "mixed switch")
     assert t.shape[0] > 10
+
+
+@pytest.mark.parametrize("precision", [E.FP64, E.MIXED])
+def test_second_loop_reads_sweep_output_through_permutation(precision):
+    # with plain preconditioner launches the second loop reads the backward
+    # sweep's solution through the inverse permutation; with the captured
+    # graph it reads the unpermuted copy: the same values in the same order,
+    # so the iterates are identical
+    m = E.Mesh.workload(1, [20, 4, 4, 0, 300.0])
+    res = []
+    for graph in (0, 1):
+        with E.Context(m, precision) as c:
+            c.set_option("h0_graph", graph)
+            u, st = c.solve(epsilon=1e-8, max_iterations=3000, fp64_switch=1e-3 if precision == E.MIXED else 0.0)
+            res.append((u, st, c.trace()))
+    t = check_same(res, f"permuted read vs graph copy, precision {precision}")
+    assert t.shape[0] > 100
