@@ -344,6 +344,9 @@ def main():
                     help="mixed: element kernels go FP64 once |r|inf <= this x |r0|inf")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cantilevers section")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
+    ap.add_argument("--profile-residual", type=int, default=0, metavar="WHICH",
+                    help="only time 200 residual evaluations of configuration WHICH (1-4) in FP64 and mixed "
+                         "(for a kernel trace)")
     ap.add_argument("--no-config-solves", action="store_true",
                     help="skip the mixed-vs-FP64 solves of the cantilever, cylinder and bar")
     ap.add_argument("--batch-load-scale", type=float, default=1.0 / 64.0,
@@ -362,6 +365,15 @@ def main():
 
     import efem_amd as E
 
+    if args.profile_residual:
+        m = E.Mesh.workload(args.profile_residual)
+        u = 1e-6 * np.random.default_rng(1).standard_normal(m.ndof)
+        rec = {"profile_residual": args.profile_residual, "elements": m.n_elems, "dof": m.ndof}
+        for prec, label in ((E.FP64, "fp64"), (E.MIXED, "mixed")):
+            with E.Context(m, prec) as c:
+                rec[label + "_us_per_eval"] = round(1e3 * c.time_residual(u, 200), 2)
+        print(json.dumps(rec))
+        return
     mesh = E.Mesh.workload(2, [200, 20, 20, 20240531, args.pressure])
     log(f"beam: {mesh.n_nodes} nodes, {mesh.n_elems} C3D4, {mesh.ndof} DOF, p = {args.pressure} Pa")
     if args.profile_run:
