@@ -24,6 +24,7 @@ EFEM_ERR_STATE = -4
 
 NEOHOOKEAN, MOONEY_RIVLIN, YEOH = 0, 1, 2
 FP64, F32_ELEMENTS, F32_PRECOND, MIXED = 0, 1, 2, 3
+LS_PROJECTION = 0
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -376,7 +377,8 @@ class Context:
         return out
 
     def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=0, max_linesearch=20,
-              m=6, lns_fea=1.0, use_precond=True, x_target=None, target_tol=0.0, fp64_switch=0.0):
+              m=6, lns_fea=1.0, use_precond=True, x_target=None, target_tol=0.0, fp64_switch=0.0,
+              line_search=LS_PROJECTION):
         """LBFGSSolver::solve.  epsilon_rel defaults to epsilon, as
         Integrator.cpp:13-14 sets both from residual_criterion.  x_target /
         target_tol: also stop once ||x - x_target|| <= tol ||x_target||."""
@@ -384,7 +386,7 @@ class Context:
         xt = None if x_target is None else np.ascontiguousarray(x_target, dtype=np.float64)
         prm = LbfgsParams(float(epsilon), float(epsilon if epsilon_rel is None else epsilon_rel),
                           int(max_iterations), int(max_linesearch), int(m), float(lns_fea),
-                          1 if use_precond else 0, 0, _ptr(xt, C.c_double), float(target_tol),
+                          1 if use_precond else 0, int(line_search), _ptr(xt, C.c_double), float(target_tol),
                           float(fp64_switch))
         st = SolveStats()
         rc = self._lib.efem_solve(self._h, _ptr(u, C.c_double), C.byref(prm), C.byref(st))

@@ -197,6 +197,36 @@ int devAlloc(efem_ctx* c, T** p, size_t count)
     return EFEM_OK;
 }
 
+// frees a buffer devAlloc recorded in the context's own list (after the
+// context stream has drained: queued kernels may still read it)
+template <typename T>
+int devFree(efem_ctx* c, T** p)
+{
+    if (!*p) return EFEM_OK;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    void* q = *p;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    HIP_CHECK(c, hipFree(q));
+    *p = nullptr;
+    return EFEM_OK;
+}
+
+// the L-BFGS history ring S / Y grown to hold m pairs (the old buffers are
+// freed, not kept until efem_destroy)
+int growHistory(efem_ctx* c, int m)
+{
+    if (m <= c->mcap) return EFEM_OK;
+    const size_t n = static_cast<size_t>(std::max(c->model.ndof(), 1));
+    int rc;
+    if ((rc = devFree(c, &c->S)) || (rc = devFree(c, &c->Y))) return rc;
+    c->mcap = 0;
+    if ((rc = devAlloc(c, &c->S, static_cast<size_t>(m) * n)) || (rc = devAlloc(c, &c->Y, static_cast<size_t>(m) * n)))
+        return rc;
+    c->mcap = m;
+    return EFEM_OK;
+}
+
 template <typename T>
 int upload(efem_ctx* c, T** p, const std::vector<T>& h)
 {
@@ -1112,7 +1142,12 @@ int efem_residual_d(efem_ctx* c, const double* u_d, double* r_d)
     efem::dev::setTuning(c->tuning);
     const size_t n = static_cast<size_t>(c->model.ndof());
     HIP_CHECK(c, hipMemcpyAsync(c->tmp, u_d, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    return residualInternal(c, c->tmp, r_d);
+    int rc = residualInternal(c, c->tmp, r_d);
+    if (rc) return rc;
+    // the context stream is not ordered against the caller's streams:
+    // r_d is complete (and u_d free for reuse) when this returns
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
 }
 
 int efem_factor(efem_ctx* c, int* failed)
@@ -1233,6 +1268,10 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         c->err = "efem_solve: invalid parameters (Param.h:110-127)";
         return EFEM_ERR_ARG;
     }
+    if (prm->line_search != EFEM_LS_PROJECTION) {
+        c->err = "efem_solve: unknown line-search rule";
+        return EFEM_ERR_ARG;
+    }
     efem_solve_stats local;
     efem_solve_stats& s = st ? *st : local;
     std::memset(&s, 0, sizeof(s));
@@ -1240,11 +1279,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     const int n = c->model.ndof();
     const int mc = prm->m;
     int rc;
-    if (mc > c->mcap) {
-        if ((rc = devAlloc(c, &c->S, static_cast<size_t>(mc) * std::max(n, 1)))) return rc;
-        if ((rc = devAlloc(c, &c->Y, static_cast<size_t>(mc) * std::max(n, 1)))) return rc;
-        c->mcap = mc;
-    }
+    if ((rc = growHistory(c, mc))) return rc;
     c->trace.clear();
     double* sc = c->scal;
     c->elemFp64 = false;
@@ -1943,6 +1978,11 @@ int efem_energy(efem_ctx* c, const double* u, double* W)
 int efem_set_params(efem_ctx* c, const double* params)
 {
     if (!c || !params) return EFEM_ERR_ARG;
+    if (!c->parts.empty()) {
+        // a batch's preconditioner is built from its instances' own models
+        c->err = "efem_set_params: batch context (parameters are fixed at efem_batch_create)";
+        return EFEM_ERR_STATE;
+    }
     std::string msg;
     if (!efem::setParams(c->model, params, msg)) {
         c->err = msg;
@@ -1960,6 +2000,11 @@ int efem_set_params(efem_ctx* c, const double* params)
 int efem_set_load(efem_ctx* c, double pressure, const double* pdir)
 {
     if (!c) return EFEM_ERR_ARG;
+    if (!c->parts.empty()) {
+        // a batch's faces carry their instances' own pressures
+        c->err = "efem_set_load: batch context (loads are fixed at efem_batch_create)";
+        return EFEM_ERR_STATE;
+    }
     efem::Model& m = c->model;
     if (m.nFaces == 0) {
         c->err = "efem_set_load: the problem has no pressure faces";
@@ -2059,16 +2104,16 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
         c->err = "efem_batch_solve: invalid parameters (history 1..16, max_iterations > 0, no x_target)";
         return EFEM_ERR_ARG;
     }
+    if (prm->line_search != EFEM_LS_PROJECTION) {
+        c->err = "efem_batch_solve: the batched solver runs the projection line search only";
+        return EFEM_ERR_ARG;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     const int count = static_cast<int>(c->parts.size());
     const int n = c->model.ndof();
     const int mc = prm->m;
     int rc;
-    if (mc > c->mcap) {
-        if ((rc = devAlloc(c, &c->S, static_cast<size_t>(mc) * std::max(n, 1)))) return rc;
-        if ((rc = devAlloc(c, &c->Y, static_cast<size_t>(mc) * std::max(n, 1)))) return rc;
-        c->mcap = mc;
-    }
+    if ((rc = growHistory(c, mc))) return rc;
     double factorSeconds = 0.0;
     if (prm->use_precond && !c->fac.ready) {
         int failed = 0;
@@ -2238,11 +2283,7 @@ int efem_autotune(efem_ctx* c, double* report)
     // their contents and the scalar slots are scratch outside a solve)
     double bestHist = 1e300;
     const int hm = 8;
-    if (rc == EFEM_OK && c->mcap < hm) {
-        if ((rc = devAlloc(c, &c->S, static_cast<size_t>(hm) * std::max(n, 1))) == EFEM_OK &&
-            (rc = devAlloc(c, &c->Y, static_cast<size_t>(hm) * std::max(n, 1))) == EFEM_OK)
-            c->mcap = hm;
-    }
+    if (rc == EFEM_OK) rc = growHistory(c, hm);
     if (rc == EFEM_OK) {
         HIP_CHECK(c, hipMemsetAsync(c->S, 0, static_cast<size_t>(hm) * n * sizeof(double), c->stream));
         HIP_CHECK(c, hipMemsetAsync(c->Y, 0, static_cast<size_t>(hm) * n * sizeof(double), c->stream));

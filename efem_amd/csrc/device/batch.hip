@@ -30,7 +30,7 @@ __device__ __forceinline__ void blockReduceK(double (&v)[K], const int (&op)[K],
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        v[k] = op[k] ? fmax(fmax(fmax(sh[k][0], sh[k][1]), sh[k][2]), sh[k][3])
+        v[k] = op[k] ? maxNan(maxNan(maxNan(sh[k][0], sh[k][1]), sh[k][2]), sh[k][3])
                      : ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
     __syncthreads();
 }
@@ -154,8 +154,8 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
         const int op[3] = {0, 1, 1};
         for (int i = d0 + tid; i < d1; i += kThreads) {
             v[0] += x[i] * x[i];
-            v[1] = fmax(v[1], fabs(x[i]));
-            v[2] = fmax(v[2], fabs(r[i]));
+            v[1] = maxNan(v[1], fabs(x[i]));
+            v[2] = maxNan(v[2], fabs(r[i]));
         }
         blockReduceK<3>(v, op, red);
         rinf = v[2];
@@ -204,8 +204,8 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
                 const int op[3] = {0, 1, 1};
                 for (int i = d0 + tid; i < d1; i += kThreads) {
                     v[0] += r[i] * d[i];
-                    v[1] = fmax(v[1], fabs(r[i]));
-                    v[2] = fmax(v[2], fabs(x[i]));
+                    v[1] = maxNan(v[1], fabs(r[i]));
+                    v[2] = maxNan(v[2], fabs(x[i]));
                 }
                 blockReduceK<3>(v, op, red);
                 rinf = v[1];

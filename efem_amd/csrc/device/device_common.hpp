@@ -36,10 +36,17 @@ __device__ __forceinline__ double waveSum(double v)
     return v;
 }
 
+// max that propagates NaN (fmax drops it): a norm over a vector holding a
+// NaN is NaN, so the non-finite checks of the solve see it
+__device__ __forceinline__ double maxNan(double a, double b)
+{
+    return (a != a) ? a : ((b != b) ? b : fmax(a, b));
+}
+
 __device__ __forceinline__ double waveMax(double v)
 {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = maxNan(v, __shfl_xor(v, off, 64));
     return v;
 }
 

@@ -337,8 +337,8 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
                 a.r[3 * i + c] = r;
                 const double xi = a.dotX[3 * i + c];
                 v[0] += r * a.dotD[3 * i + c];
-                v[1] = fmax(v[1], fabs(r));
-                v[2] = fmax(v[2], fabs(a.dotCb ? xi - a.dotCb[3 * i + c] : xi));
+                v[1] = maxNan(v[1], fabs(r));
+                v[2] = maxNan(v[2], fabs(a.dotCb ? xi - a.dotCb[3 * i + c] : xi));
                 if constexpr (DOT == 4) {
                     const double e = xi - a.dotXt[3 * i + c];
                     v[3] += e * e;
@@ -440,7 +440,7 @@ __device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W], uns
     for (int k = 0; k < K; ++k) {
         const bool mx = isMax(maxMask, k);
         double t = sh[k][0];
-        for (int w = 1; w < W; ++w) t = mx ? fmax(t, sh[k][w]) : t + sh[k][w];
+        for (int w = 1; w < W; ++w) t = mx ? maxNan(t, sh[k][w]) : t + sh[k][w];
         v[k] = t;
     }
     __syncthreads();
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ 
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double o = work[static_cast<size_t>(k) * stride + b];
-            v[k] = isMax(maxMask, k) ? fmax(v[k], o) : v[k] + o;
+            v[k] = isMax(maxMask, k) ? maxNan(v[k], o) : v[k] + o;
         }
     blockReduce<K, kRFW>(v, sh, maxMask);
     epi(v, pre);
@@ -531,10 +531,10 @@ __global__ __launch_bounds__(kRB) void kDotMax(const double* __restrict__ a, con
     for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
         const double ai = a[i];
         v[0] += ai * b[i];
-        v[1] = fmax(v[1], fabs(ai));
+        v[1] = maxNan(v[1], fabs(ai));
         if (HASC) {
             const double ci = c[i];
-            v[2] = fmax(v[2], fabs(cb ? ci - cb[i] : ci));
+            v[2] = maxNan(v[2], fabs(cb ? ci - cb[i] : ci));
             if (HASXT) {
                 const double e = ci - xt[i];
                 v[K - 1] += e * e;

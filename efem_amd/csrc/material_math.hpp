@@ -11,7 +11,9 @@
 //        + (2/D1)(J - 1) cof F                              [volumetric]
 // with c1 = C10 (NH, MR) or C10 + 2 C20 t + 3 C30 t^2, t = I1bar - 3 (Yeoh),
 // which equals NeoHookean.cpp:4-20, MooneyRivlin.cpp:4-24 and Yeoh.cpp:4-20
-// (dJ/dF = J F^-T = cof F, I1bar = I1 / I3^(1/3), I2bar = I2 / I3^(2/3)).
+// (dJ/dF = J F^-T = cof F, I1bar = I1 / I3^(1/3), I2bar = I2 / I3^(2/3),
+// I3 = J^2, so J^-2/3 stands for |J|^-2/3 throughout, inverted elements
+// included).
 // Strain energies (no counterpart in the reference, which never evaluates
 // one): W = c(I1bar-3)[+C01(I2bar-3)] + (J-1)^2/D1, Yeoh sum_i Ci0(I1bar-3)^i.
 #pragma once
@@ -88,6 +90,19 @@ template <typename S> EFEM_HD Dual<S> sqrt_(const Dual<S>& x)
     return Dual<S>(r, x.d / (S(2) * r));
 }
 
+EFEM_HD double valueOf(double x) { return x; }
+template <typename S> EFEM_HD S valueOf(const Dual<S>& x) { return x.v; }
+
+// log|J| from J - 1, accurate for J near 1.  The reference's invariants take
+// (J^2)^(1/3) (I1bar = I1 / I3^(1/3), I3 = J^2: NeoHookean.cpp:9-10), which
+// is |J|^(2/3) and stays finite for an inverted element (J < 0):
+// log(-J) = log1p(-2 - (J - 1)).
+template <typename D>
+EFEM_HD D logAbsJ(const D& jm1)
+{
+    return valueOf(jm1) > -1.0 ? log1p_(jm1) : log1p_(-2.0 - jm1);
+}
+
 // J - 1 from H without cancellation (exact identity det(I+H) - 1)
 template <typename D>
 EFEM_HD D detIpHm1(const D* h)
@@ -111,8 +126,8 @@ EFEM_HD void piolaGen(int law, const double* p, const D* hd, T* P)
     for (int k = 0; k < 9; ++k) H[k] = static_cast<T>(hd[k]);
     const D jm1 = detIpHm1(hd);
     const D J = 1.0 + jm1;
-    // J^-2/3 - 1 = expm1(-2/3 log1p(J - 1)), accurate for J near 1
-    const D em1 = expm1_(-2.0 / 3.0 * log1p_(jm1));
+    // |J|^-2/3 - 1 = expm1(-2/3 log|J|), accurate for J near 1
+    const D em1 = expm1_(-2.0 / 3.0 * logAbsJ(jm1));
     T F[9];
     for (int k = 0; k < 9; ++k) F[k] = H[k];
     F[0] += T(1);
@@ -231,7 +246,7 @@ template <typename D>
 EFEM_HD D energyGen(int law, const double* p, const D* H)
 {
     const D jm1 = detIpHm1(H);
-    const D em1 = expm1_(-2.0 / 3.0 * log1p_(jm1));
+    const D em1 = expm1_(-2.0 / 3.0 * logAbsJ(jm1));
     D E[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)

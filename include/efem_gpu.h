@@ -144,7 +144,9 @@ int efem_precision(const efem_ctx* ctx);
 
 /* r(u) = f_int(u) - f_ext(u) on the free DOFs (host pointers, synchronous) */
 int efem_residual(efem_ctx* ctx, const double* u, double* r);
-/* device-pointer variant, ordered on the context stream (not synchronised) */
+/* device-pointer variant: u_d and r_d are device buffers of ndof doubles.
+ * The caller's writes of u_d must be complete (its stream synchronised);
+ * the call returns once r_d is written and u_d is no longer read. */
 int efem_residual_d(efem_ctx* ctx, const double* u_d, double* r_d);
 
 /* assemble the approximate Jacobian, order and factor it on the host,
@@ -335,6 +337,8 @@ int efem_energy(efem_ctx* ctx, const double* u, double* W);
 /* replace the material parameters (same layout as efem_problem.params);
  * drops the factor, which depends on them */
 int efem_set_params(efem_ctx* ctx, const double* params);
+/* (both setters return EFEM_ERR_STATE on a batch context, whose instances'
+ * parameters and loads are fixed at efem_batch_create) */
 /* replace the follower pressure (pdir NULL) or the directed traction */
 int efem_set_load(efem_ctx* ctx, double pressure, const double* pdir);
 /* one material point, host only: P(F) (Material::getFirstPiola,

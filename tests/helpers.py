@@ -55,3 +55,47 @@ def face_objective(m, target=(0.0, 0.005, 0.0)):
         return out
 
     return J, g
+
+
+class DeviceBuffer:
+    """A device allocation made through the HIP runtime the library itself
+    links (libamdhip64 of /opt/rocm), for tests of the _d entry points.
+    torch ships a HIP runtime of its own, which must not share the process's
+    device with it, so torch tensors are not used for this."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes as C
+            E.lib()  # loads the library and with it its HIP runtime
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, n):
+        import ctypes as C
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        rc = self.hip().hipMalloc(C.byref(self.ptr), max(8 * self.n, 8))
+        assert rc == 0, rc
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        assert self.hip().hipMemcpy(self.ptr, a.ctypes.data, 8 * self.n, 1) == 0  # host -> device
+
+    def download(self):
+        out = np.empty(self.n)
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, 8 * self.n, 2) == 0  # device -> host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None

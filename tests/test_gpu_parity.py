@@ -93,6 +93,24 @@ def test_residual_full_beam_fp64_and_mixed(rng):
         assert np.linalg.norm(ctx.residual(u) - r_or) <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("which", [3, 4])
+def test_residual_full_cylinder_and_bar(which, rng):
+    # the configured meshes at full size: the cylinder (7,680 C3D20R, Yeoh,
+    # 1,920 curved 8-node pressure faces) and the bar (73,728 C3D10,
+    # per-element Neo-Hookean C10, 768 6-node faces)
+    m = E.Mesh.workload(which)
+    print(f"full workload {which} ndof={m.ndof}", flush=True)
+    om = O.Model(m)
+    for amp in (0.005, 0.05):
+        u = smooth_u(m, rng, amp)
+        r_or = om.residual(u)
+        scale = np.linalg.norm(om.internal_force(u)) + np.linalg.norm(om.external_force(u))
+        with E.Context(m, E.FP64) as ctx:
+            assert rel(ctx.residual(u), r_or) <= 1e-10, amp
+        with E.Context(m, E.MIXED) as ctx:
+            assert np.linalg.norm(ctx.residual(u) - r_or) <= 1e-5 * scale, amp
+
+
 def test_residual_directed_traction_and_nodal_force(rng):
     m0 = small(8, 0, 0)
     top = np.unique(m0.faces.ravel())

@@ -100,3 +100,22 @@ def test_objectivity(law, p, rng):
 def test_bad_arguments():
     with pytest.raises(E.EfemError):
         E.material_point(7, [1.0, 0, 0, 1.0], np.eye(3))
+
+
+@pytest.mark.parametrize("law,p", LAWS)
+def test_inverted_element_stress_matches_oracle(law, p, rng):
+    # J < 0 (an element turned inside out by a large trial step): the
+    # reference's invariants use (J^2)^(1/3) = |J|^(2/3) and stay finite
+    # (NeoHookean.cpp:9-10), so must the kernels' stress
+    import oracle as O
+    for _ in range(10):
+        F = rng.standard_normal((3, 3))
+        if np.linalg.det(F) > -0.2:
+            F[:, 0] *= -1
+        if abs(np.linalg.det(F)) < 0.2:
+            continue
+        P, _, _ = E.material_point(law, p, F)
+        Po = O.piola(law, np.asarray(p), F)
+        assert np.all(np.isfinite(P))
+        scale = np.abs(Po).max() + 1e-14 * np.abs(p[:3]).sum()
+        assert np.abs(P - Po).max() <= 1e-12 * scale

@@ -147,16 +147,48 @@ def test_volume_of_affine_elements_equals_jacobian(nen, rng):
         assert abs(O.element_volume(nen, Xe) - ref_vol) < 1e-13 * max(1, ref_vol)
 
 
-def test_volume_of_distorted_hex_is_polyhedral():
-    # a hex with a warped (non-planar) face: the tessellated-surface volume
-    # equals the sum of the volumes of the tets of the face-triangle fan
+def _closed_surface_volume(X, tris):
+    # divergence theorem over outward-oriented triangles: V = sum p0.(p1 x p2) / 6
+    return sum(np.dot(X[a], np.cross(X[b], X[c])) for a, b, c in tris) / 6.0
+
+
+def test_volume_of_warped_hex_is_one_diagonal_choice():
+    # a hex with one warped (non-planar) face.  Any triangulated-surface
+    # volume splits that face along one of its two diagonals; the two choices
+    # differ, and the isoparametric volume (integral of det J0, the bilinear
+    # face) is exactly their mean.  The oracle's volume must be one of the
+    # two choices and sit half their difference away from int det J0.  Which
+    # diagonal VTK's triangle filter picks is not fixed here (parity unpinned).
     X = HEX8_NODES.copy()
     X[6] += [0.0, 0.0, 0.4]
     v = O.element_volume(8, X)
-    c = X.mean(axis=0)
-    faces = [(0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2), (0, 3, 2, 1), (4, 5, 6, 7)]
-    tot = 0.0
-    for a, b, cc, d in faces:
-        for t in ((a, b, cc), (a, cc, d)):
-            tot += abs(np.linalg.det(np.array([X[t[0]] - c, X[t[1]] - c, X[t[2]] - c]))) / 6
-    assert abs(v - tot) < 1e-12
+    # outward faces of the VTK hexahedron; the warped one is (4, 5, 6, 7)
+    planar = [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    tris = [t for a, b, c, d in planar for t in ((a, b, c), (a, c, d))]
+    v02 = _closed_surface_volume(X, tris + [(4, 5, 6), (4, 6, 7)])
+    v13 = _closed_surface_volume(X, tris + [(4, 5, 7), (5, 6, 7)])
+    assert abs(v02 - v13) > 1e-3
+    pts, w = O.gauss(8, 0)
+    viso = 8.0 * sum(wq * np.linalg.det(X.T @ O.shape(8, p)[1]) for p, wq in zip(pts, w))
+    assert abs(viso - 0.5 * (v02 + v13)) < 1e-13
+    assert min(abs(v - v02), abs(v - v13)) < 1e-13
+    assert abs(abs(v - viso) - 0.5 * abs(v02 - v13)) < 1e-13
+
+
+def test_curved_c3d20_volume_bound_on_the_cylinder():
+    # the hollow cylinder's C3D20 elements have curved faces: the oracle's
+    # tessellated volume (each 8-node face as 6 triangles through its
+    # mid-edge nodes) is not the isoparametric volume; bound the difference
+    # (0.25 % mean, 0.30 % max on this mesh) and check that the isoparametric
+    # volumes add up to the annulus pi (ro^2 - ri^2) h
+    import efem_amd as E
+    m = E.Mesh.workload(3)
+    om = O.Model(m)
+    pts, w = O.gauss(20, 0)
+    dN = np.array([O.shape(20, p)[1] for p in pts])
+    J = np.einsum("eai,qaj->eqij", m.X[m.conn], dN)
+    viso = 8.0 * (np.linalg.det(J) * w).sum(1)
+    vt = np.array([om.volume(e) for e in range(m.n_elems)])
+    d = np.abs(vt - viso) / viso
+    assert d.max() < 3.1e-3 and d.mean() < 2.6e-3
+    assert abs(viso.sum() / (np.pi * (0.015 ** 2 - 0.010 ** 2) * 0.020) - 1) < 1e-5

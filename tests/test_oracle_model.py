@@ -82,10 +82,59 @@ def test_pressure_undeformed_closed_form(kind, reduced):
             tot_fixed += share * m.fixed[f].sum()
         assert abs(-fe[:, 2].sum() - (700.0 * A_face - tot_fixed)) < 1e-12
     else:
-        assert -fe[:, 2].sum() <= 700.0 * A_face + 1e-12
+        # quadratic faces: every node's exact share, see face_share_closed_form
+        want = face_share_closed_form(m, 700.0)
+        free = m.fixed == 0
+        assert np.allclose(fe[fid[free], 2], want[free, 2], rtol=0, atol=1e-12 * 700.0 * A_face)
     assert on_fixed.any()
     # nodes off the face get N_a(xi_q) = 0 up to rounding of 1 - xi - eta - zeta
     assert np.abs(fe[fid[~top & (m.fixed == 0)], 2]).max() < 1e-15 * 700.0 * A_face
+
+
+def face_share_closed_form(m, p):
+    """+f_ext per node of a follower pressure p on flat, straight-edged
+    faces at u = 0, from the exact integrals of the face shape functions:
+    f_a = p A_f int N_a dA / A_f along the inward normal, where
+    int N_a / A is 1/3 per corner of a 3-node triangle, 1/4 per corner of a
+    bilinear quad, 0 per corner and 1/3 per mid-edge node of a 6-node
+    triangle, and -1/12 per corner and 1/3 per mid-edge node of an 8-node
+    serendipity quad.  Corners are the first 3 (4) face nodes.  The inward
+    normal is -z for the top faces of the box meshes."""
+    fnn = m.faces.shape[1]
+    share = {3: [1 / 3] * 3, 4: [1 / 4] * 4, 6: [0.0] * 3 + [1 / 3] * 3,
+             8: [-1 / 12] * 4 + [1 / 3] * 4}[fnn]
+    nc = 3 if fnn in (3, 6) else 4
+    out = np.zeros((m.n_nodes, 3))
+    for f in m.faces:
+        P = m.X[f[:nc]]
+        if nc == 3:
+            n = 0.5 * np.cross(P[1] - P[0], P[2] - P[0])
+        else:
+            n = 0.5 * np.cross(P[2] - P[0], P[3] - P[1])
+        area = np.linalg.norm(n)
+        for a, s in zip(f, share):
+            out[a] += p * area * s * np.array([0.0, 0.0, -1.0])
+    return out
+
+
+@pytest.mark.parametrize("kind,reduced", [(8, 0), (4, 0), (10, 0), (20, 1), (20, 0)])
+def test_pressure_shares_under_homogeneous_deformation(kind, reduced, rng):
+    # u = (F - I) X: Nanson gives every face point the same cof(F) A0, so
+    # each node's follower force is cof(F) times its undeformed closed-form
+    # share (ExternalConstraint.cpp:829-836) -- for all four face kinds,
+    # with the face rule's points and weights entering only through the
+    # exact integrals of N_a
+    m = box(kind, reduced, p=700.0)
+    m = E.Mesh(m.X, m.conn, m.law, m.params, np.zeros(m.n_nodes, np.uint8), m.faces, m.pressure, m.reduced)
+    om = O.Model(m)
+    F = np.eye(3) + 0.15 * rng.standard_normal((3, 3))
+    assert np.linalg.det(F) > 0.3
+    u = (m.X @ (F - np.eye(3)).T).ravel()
+    got = -om.external_force(u).reshape(-1, 3)  # + f_ext
+    cof = np.linalg.det(F) * np.linalg.inv(F).T
+    want = face_share_closed_form(m, 700.0) @ cof.T
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 1e-12 * scale * 10
 
 
 def test_pressure_on_closed_surface_is_self_equilibrated(rng):
