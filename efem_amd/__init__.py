@@ -124,6 +124,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_newton": (C.c_int, [vp, _dp, C.c_double, C.c_int, C.POINTER(KrylovParams),
                                   C.POINTER(SolveStats)]),
         "efem_energy": (C.c_int, [vp, _dp, _dp]),
+        "efem_energy_noalloc": (C.c_int, [vp, vp, vp]),
         "efem_time_residual": (C.c_int, [vp, _dp, C.c_int, _dp]),
         "efem_autotune": (C.c_int, [vp, _dp]),
         "efem_set_params": (C.c_int, [vp, _dp]),

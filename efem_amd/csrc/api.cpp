@@ -66,6 +66,7 @@ struct efem_ctx {
     std::string err;
     efem::Model model;
     int precision = EFEM_FP64;
+    int device = -1;  // the HIP device current at efem_create
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     std::vector<void*>* allocList = &allocs;  // where devAlloc records
@@ -1055,6 +1056,10 @@ int initContext(std::unique_ptr<efem_ctx>& c)
         g_error = "efem_create: no HIP device";
         return EFEM_ERR_HIP;
     }
+    if (hipGetDevice(&c->device) != hipSuccess || efem::dev::scratchInit(c->device) != 0) {
+        g_error = "efem_create: device scratch of the allocation-free entry points unavailable";
+        return EFEM_ERR_HIP;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_error = "efem_create: hipStreamCreate failed";
         return EFEM_ERR_HIP;
@@ -1972,6 +1977,32 @@ int efem_energy(efem_ctx* c, const double* u, double* W)
     HIP_CHECK(c, hipGetLastError());
     if ((rc = readScalars(c, 1))) return rc;
     *W = c->hostScal[0];
+    return EFEM_OK;
+}
+
+// efem_energy without allocation: the context's model, the device's scratch
+// array (no hipMalloc / hipFree, no lazily allocated work vectors)
+int efem_energy_noalloc(efem_ctx* c, const double* u_d, double* W_d)
+{
+    if (!c || !u_d || !W_d) return EFEM_ERR_ARG;
+    const efem::Model& m = c->model;
+    const size_t n = static_cast<size_t>(m.ndof());
+    auto up = [](size_t v) { return (v + 31) / 32 * 32; };
+    const size_t need = up(n + 3) + up(static_cast<size_t>(m.nElems)) + 2 * efem::dev::kReduceBlocks;
+    if (need > efem::dev::kScratchDoubles) {
+        c->err = "efem_energy_noalloc: problem too large for the device scratch (" + std::to_string(need) +
+                 " doubles needed, " + std::to_string(efem::dev::kScratchDoubles) + " available)";
+        return EFEM_ERR_ARG;
+    }
+    double* sp = efem::dev::scratchOf(c->device);
+    if (!sp) {
+        c->err = "efem_energy_noalloc: device scratch not initialised";
+        return EFEM_ERR_STATE;
+    }
+    efem::dev::setTuning(c->tuning);
+    efem::dev::launchEnergyScratch(elemArgs(c, nullptr, false), u_d, static_cast<int>(n), sp, W_d, c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return EFEM_OK;
 }
 

@@ -19,6 +19,7 @@
 //    vectors), the backward sweep reads the ancestors' solved values.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -1112,6 +1113,58 @@ void launchSumComp(const double* a, int n, double* work, double* out, hipStream_
     const int g = reduceGrid(n);
     kSumComp<<<g, kRB, 0, s>>>(a, n, work);
     kSumCompFinish<<<1, 64, 0, s>>>(work, g, out);
+}
+
+// ---- allocation-free entry points -----------------------------------------
+namespace {
+
+// The scratch of the allocation-free entry points.  In host code `scratch`
+// only names the symbol: its device address comes from hipGetSymbolAddress
+// (scratchInit) and only that address is passed to kernels.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+double* g_scratchPtr[kMaxDevices] = {};
+
+// dst[i] = src[i] for i < n, 0 for n <= i < n + 3 (the fixed-node slot)
+__global__ void kCopyPad(double* __restrict__ dst, const double* __restrict__ src, int n)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n + 3) dst[i] = i < n ? src[i] : 0.0;
+}
+
+}  // namespace
+
+int scratchInit(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[device]) return 0;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+    g_scratchPtr[device] = static_cast<double*>(p);
+    return 0;
+}
+
+double* scratchOf(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return g_scratchPtr[device];
+}
+
+void launchEnergyScratch(ElemArgs a, const double* u, int n, double* sp, double* out, hipStream_t s)
+{
+    // layout: [x copy, n + 3 | per-element energies, E | sum partials, 2 x kReduceBlocks]
+    auto up = [](size_t v) { return (v + 31) / 32 * 32; };
+    double* xs = sp;
+    double* per = xs + up(static_cast<size_t>(n) + 3);
+    double* work = per + up(static_cast<size_t>(a.nElems));
+    kCopyPad<<<blocks(n + 3), kThreads, 0, s>>>(xs, u, n);
+    a.x = xs;
+    launchScalar<0>(a, nullptr, per, s);
+    launchSumComp(per, a.nElems, work, out, s);
 }
 
 void launchLaplacian(const LapArgs& a, hipStream_t s)

@@ -126,6 +126,19 @@ void launchDot2(const double* a, const double* b, int n, double* work, double* a
                 hipStream_t s);
 // *out = sum a (Neumaier lanes + TwoSum tree, reproducible)
 void launchSumComp(const double* a, int n, double* work, double* out, hipStream_t s);
+
+// ---- allocation-free entry points -----------------------------------------
+// One file-scope device array per GPU (kScratchDoubles doubles, 16 MiB) that
+// the allocation-free variants use instead of context buffers.
+constexpr size_t kScratchDoubles = size_t(2) << 20;
+// the array's device address on `device`: looked up once per device (with
+// that device current) by hipGetSymbolAddress, then cached
+int scratchInit(int device);
+double* scratchOf(int device);
+// total strain energy into out (device pointer): u (n values) copied into
+// the scratch with the fixed-node slot's three trailing zeros, per-element
+// energies there, compensated sum; everything on stream s
+void launchEnergyScratch(ElemArgs a, const double* u, int n, double* scratch, double* out, hipStream_t s);
 // out = a x + b y (y may be null)
 void launchAxpby(double* out, double a, const double* x, double b, const double* y, int n, hipStream_t s);
 // x = xp + alpha d

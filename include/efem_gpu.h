@@ -334,6 +334,18 @@ int efem_adjoint(efem_ctx* ctx, const double* u, const double* g, double* lambda
                  const efem_krylov_params* prm, efem_krylov_stats* stats);
 /* total strain energy sum_e sum_q w_q V_e W(F_q) (FP64, compensated sum) */
 int efem_energy(efem_ctx* ctx, const double* u, double* W);
+/* efem_energy for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  u_d: ndof doubles on the device; W_d:
+ * one double on the device; the same kernels and summation order as
+ * efem_energy, so the same value bit for bit.  Calls no hipMalloc / hipFree:
+ * its work memory is one 16 MiB array per GPU (2^21 doubles, held until the
+ * process ends), so it needs roundup32(ndof + 3) + roundup32(n_elems) + 512
+ * doubles to fit (EFEM_ERR_ARG otherwise).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (from several threads or
+ * contexts) race.  Returns once W_d is written (context stream
+ * synchronised). */
+int efem_energy_noalloc(efem_ctx* ctx, const double* u_d, double* W_d);
 /* replace the material parameters (same layout as efem_problem.params);
  * drops the factor, which depends on them */
 int efem_set_params(efem_ctx* ctx, const double* params);
