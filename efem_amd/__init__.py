@@ -24,7 +24,7 @@ EFEM_ERR_STATE = -4
 
 NEOHOOKEAN, MOONEY_RIVLIN, YEOH = 0, 1, 2
 FP64, F32_ELEMENTS, F32_PRECOND, MIXED = 0, 1, 2, 3
-LS_PROJECTION = 0
+LS_PROJECTION, LS_ENERGY = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

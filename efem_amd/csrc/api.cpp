@@ -102,6 +102,9 @@ struct efem_ctx {
     double* gdotWork = nullptr;  // partials of the line-search scalars fused into the gather
     hipEvent_t trialEv[2] = {nullptr, nullptr};          // trial scalars copied
     double *S = nullptr, *Y = nullptr;
+    // per-element strain energies of the energy line search: the accepted
+    // iterate's and the trial's (allocated by the first such solve)
+    double *wOld = nullptr, *wNew = nullptr;
     double* scal = nullptr;  // device scalars
     double* work = nullptr;
     double* hostScal = nullptr;  // pinned
@@ -162,6 +165,11 @@ namespace {
 // supernode shapes that switch a level to 16-wave triangular-solve items
 constexpr int kWideColumns = 128;
 constexpr int kTallRows = 256;
+
+// energy line search: Armijo constant, and the relative size below which a
+// change of the potential is rounding (then the projection rule decides)
+constexpr double kArmijoC1 = 1e-4;
+constexpr double kEnergyRtol = 1e-12;
 
 // scalar slots in ctx->scal
 constexpr int kOut = 0;       // [3] reduction outputs
@@ -1273,8 +1281,13 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         c->err = "efem_solve: invalid parameters (Param.h:110-127)";
         return EFEM_ERR_ARG;
     }
-    if (prm->line_search != EFEM_LS_PROJECTION) {
+    if (prm->line_search != EFEM_LS_PROJECTION && prm->line_search != EFEM_LS_ENERGY) {
         c->err = "efem_solve: unknown line-search rule";
+        return EFEM_ERR_ARG;
+    }
+    const bool energyLs = prm->line_search == EFEM_LS_ENERGY;
+    if (energyLs && c->model.nFaces > 0) {
+        c->err = "efem_solve: the energy line search needs conservative loads (no pressure faces)";
         return EFEM_ERR_ARG;
     }
     efem_solve_stats local;
@@ -1296,6 +1309,12 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     HIP_CHECK(c, hipMemcpyAsync(c->x, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = residualInternal(c, c->x, c->r))) return rc;
     s.r_evals = 1;
+    if (energyLs) {
+        // strain energies of the start, per element (FP64 in every mode)
+        const size_t ne = static_cast<size_t>(std::max(c->model.nElems, 1));
+        if (!c->wOld && ((rc = devAlloc(c, &c->wOld, ne)) || (rc = devAlloc(c, &c->wNew, ne)))) return rc;
+        efem::dev::launchElementEnergy(elemArgs(c, c->x, false), c->wOld, c->stream);
+    }
     // the relative criterion measures the solution from the committed
     // configuration: the reference's solver variable is the increment dx
     // on top of node->dx (Constraint.cpp:93-113)
@@ -1446,8 +1465,15 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         // the trial's scalars come with the residual's gather
         int rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr, sc + kOut);
         if (rc2) return rc2;
-        HIP_CHECK(c, hipMemcpyAsync(c->hostScal + kTrialSlot + 8 * slot, sc + kOut, 5 * sizeof(double),
-                                    hipMemcpyDeviceToHost, c->stream));
+        if (energyLs) {
+            // the potential's change from xp and its rounding scale into
+            // slots 5, 6: per-element energy differences (compensated)
+            efem::dev::launchElementEnergy(elemArgs(c, c->x, false), c->wNew, c->stream);
+            efem::dev::launchEnergyDiff(c->wNew, c->wOld, c->model.nElems, c->fext, c->x, c->xp, n, c->work,
+                                        sc + kOut + 5, c->stream);
+        }
+        HIP_CHECK(c, hipMemcpyAsync(c->hostScal + kTrialSlot + 8 * slot, sc + kOut,
+                                    (energyLs ? 7 : 5) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(c, hipEventRecord(c->trialEv[slot], c->stream));
         return EFEM_OK;
     };
@@ -1484,7 +1510,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     for (;;) {
         bool spec = false;
         LoopState before{};
-        if (c->lookahead && it == 0 && (prm->max_iterations == 0 || k < prm->max_iterations)) {
+        if (c->lookahead && !energyLs && it == 0 && (prm->max_iterations == 0 || k < prm->max_iterations)) {
             before = save();
             if ((rc = nextDirection())) return rc;
             std::swap(c->d, c->d2);
@@ -1501,14 +1527,21 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         ++s.r_evals;
         ++trials;
         const double used = cur;
-        if (!(std::fabs(dg) <= prm->lns_fea * std::fabs(dgInit)) && it + 1 < prm->max_linesearch) {
+        const bool proj = std::fabs(dg) <= prm->lns_fea * std::fabs(dgInit);
+        // energy rule (no counterpart in the reference): sufficient decrease
+        // of the potential, or a change within its rounding, as well
+        const bool armijo = !energyLs || hsc[5] <= kArmijoC1 * used * dgInit ||
+                            std::fabs(hsc[5]) <= kEnergyRtol * hsc[6];
+        if (!(proj && armijo) && it + 1 < prm->max_linesearch) {
             // rejected: the next trial of this iteration
             if (spec) restore(before);
-            cur *= dg * dgInit > 0 ? 1.5 : 0.5;
+            cur *= !armijo ? 0.5 : (dg * dgInit > 0 ? 1.5 : 0.5);
             ++it;
             if ((rc = enqueueTrial(cur, slot))) return rc;
             continue;
         }
+        // accepted: its energies are the next iterations' reference
+        if (energyLs) std::swap(c->wOld, c->wNew);
         // accepted (or the last trial allowed: LineSearchFEA.h:37-68)
         s.ls_trials += trials;
         c->trace.push_back(rinf);

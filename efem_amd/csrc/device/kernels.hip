@@ -922,6 +922,66 @@ __global__ void kSumCompFinish(const double* __restrict__ work, int nb, double* 
     out[0] = S + C;
 }
 
+// Potential difference of the energy line search, block partials:
+// compensated sum of (Wn[e] - Wo[e]) over elements and of -(x - xp) . fext
+// over the DOFs (the constant loads' work), plain sums of |Wo[e]| and of
+// xp . fext (the rounding scale).  work: [s | c | |W| | ext] x kReduceBlocks.
+__global__ __launch_bounds__(kRB) void kEnergyDiff(const double* __restrict__ Wn, const double* __restrict__ Wo,
+                                                   int E, const double* __restrict__ fext,
+                                                   const double* __restrict__ x, const double* __restrict__ xp,
+                                                   int n, double* work)
+{
+    __shared__ double ss[kRB], sc[kRB], sa[kRB], se[kRB];
+    const int tid = threadIdx.x;
+    double s = 0.0, c = 0.0, aw = 0.0, ex = 0.0;
+    for (int e = blockIdx.x * kRB + tid; e < E; e += gridDim.x * kRB) {
+        neumaier(s, c, Wn[e] - Wo[e]);
+        aw += fabs(Wo[e]);
+    }
+    if (fext)
+        for (int i = blockIdx.x * kRB + tid; i < n; i += gridDim.x * kRB) {
+            neumaier(s, c, -(x[i] - xp[i]) * fext[i]);
+            ex += xp[i] * fext[i];
+        }
+    ss[tid] = s;
+    sc[tid] = c;
+    sa[tid] = aw;
+    se[tid] = ex;
+    __syncthreads();
+    for (int w = kRB / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            double s1 = ss[tid], c1 = sc[tid];
+            combinePair(s1, c1, ss[tid + w], sc[tid + w]);
+            ss[tid] = s1;
+            sc[tid] = c1;
+            sa[tid] += sa[tid + w];
+            se[tid] += se[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        work[blockIdx.x] = ss[0];
+        work[kReduceBlocks + blockIdx.x] = sc[0];
+        work[2 * kReduceBlocks + blockIdx.x] = sa[0];
+        work[3 * kReduceBlocks + blockIdx.x] = se[0];
+    }
+}
+
+// out[0] = the potential difference, out[1] = its rounding scale
+// sum |Wo| + |xp . fext| (block partials combined in block order)
+__global__ void kEnergyDiffFinish(const double* __restrict__ work, int nb, double* out)
+{
+    if (threadIdx.x != 0) return;
+    double S = 0.0, C = 0.0, A = 0.0, X = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        combinePair(S, C, work[b], work[kReduceBlocks + b]);
+        A += work[2 * kReduceBlocks + b];
+        X += work[3 * kReduceBlocks + b];
+    }
+    out[0] = S + C;
+    out[1] = A + fabs(X);
+}
+
 // out = a x + b y (y may be null)
 __global__ void kAxpby(double* out, double a, const double* x, double b, const double* y, int n)
 {
@@ -1165,6 +1225,14 @@ void launchEnergyScratch(ElemArgs a, const double* u, int n, double* sp, double*
     a.x = xs;
     launchScalar<0>(a, nullptr, per, s);
     launchSumComp(per, a.nElems, work, out, s);
+}
+
+void launchEnergyDiff(const double* Wn, const double* Wo, int E, const double* fext, const double* x,
+                      const double* xp, int n, double* work, double* out, hipStream_t s)
+{
+    const int g = reduceGrid(max(E, n));
+    kEnergyDiff<<<g, kRB, 0, s>>>(Wn, Wo, E, fext, x, xp, n, work);
+    kEnergyDiffFinish<<<1, 64, 0, s>>>(work, g, out);
 }
 
 void launchLaplacian(const LapArgs& a, hipStream_t s)

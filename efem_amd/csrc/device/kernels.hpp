@@ -126,6 +126,11 @@ void launchDot2(const double* a, const double* b, int n, double* work, double* a
                 hipStream_t s);
 // *out = sum a (Neumaier lanes + TwoSum tree, reproducible)
 void launchSumComp(const double* a, int n, double* work, double* out, hipStream_t s);
+// energy line search: out[0] = sum_e (Wn[e] - Wo[e]) - (x - xp) . fext
+// (compensated), out[1] = sum_e |Wo[e]| + |xp . fext| (rounding scale);
+// fext may be null; work holds 4 kReduceBlocks doubles
+void launchEnergyDiff(const double* Wn, const double* Wo, int E, const double* fext, const double* x,
+                      const double* xp, int n, double* work, double* out, hipStream_t s);
 
 // ---- allocation-free entry points -----------------------------------------
 // One file-scope device array per GPU (kScratchDoubles doubles, 16 MiB) that

@@ -57,6 +57,15 @@ extern "C" {
 
 /* line-search rule */
 #define EFEM_LS_PROJECTION 0 /* the reference's |r.d| rule (LineSearchFEA.h:37-68) */
+/* energy rule (no counterpart in the reference; conservative loads only --
+ * nodal forces, gravity -- since a follower pressure has no potential):
+ * a trial is accepted when the projection rule holds and the total
+ * potential Pi = strain energy - constant loads' work decreased enough,
+ * Pi(x) - Pi(xp) <= 1e-4 step (r(xp) . d), or changed by less than
+ * 1e-12 of its size (rounding); without the decrease the step halves,
+ * otherwise it grows by 1.5 or halves as in the projection rule.  Pi(x) -
+ * Pi(xp) is summed element by element in FP64 with compensation. */
+#define EFEM_LS_ENERGY 1
 
 /*
  * Problem description, copied at efem_create (the caller keeps ownership).
@@ -104,7 +113,7 @@ typedef struct efem_lbfgs_params {
     int m;               /* history, 6 in the reference (1..64) */
     double lns_fea;      /* 1.0 */
     int use_precond;     /* 1: H0 = K^-1 (nl_solver "lbfgs_ah"), 0: H0 = I/theta */
-    int line_search;     /* EFEM_LS_PROJECTION */
+    int line_search;     /* EFEM_LS_PROJECTION or EFEM_LS_ENERGY */
     /* optional measurement hook (no counterpart in the reference): when
      * x_target is non-NULL the solve also stops as soon as
      * ||x - x_target||_2 <= target_tol ||x_target||_2 (checked on the device

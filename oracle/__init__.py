@@ -34,7 +34,8 @@ class OrParams(C.Structure):
     _fields_ = [
         ("epsilon", C.c_double), ("epsilon_rel", C.c_double), ("max_iterations", C.c_int),
         ("max_linesearch", C.c_int), ("m", C.c_int), ("lns_fea", C.c_double),
-        ("use_precond", C.c_int), ("factor_scalar", C.c_int),
+        ("use_precond", C.c_int), ("factor_scalar", C.c_int), ("line_search", C.c_int),
+        ("x_target", _dp), ("target_tol", C.c_double),
     ]
 
 
@@ -43,6 +44,7 @@ class OrStats(C.Structure):
         ("iterations", C.c_int), ("r_evals", C.c_int), ("ls_trials", C.c_int),
         ("precond_failed", C.c_int), ("rinf", C.c_double), ("xinf", C.c_double),
         ("factor_seconds", C.c_double), ("solve_seconds", C.c_double),
+        ("target_reached", C.c_int), ("target_err", C.c_double),
     ]
 
     def as_dict(self):
@@ -51,6 +53,7 @@ class OrStats(C.Structure):
 
 RESID_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp)
 H0_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp)
+ENERGY_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, _dp, _dp, _dp)
 
 _lib = None
 _ref = None
@@ -70,6 +73,11 @@ def lib():
             "or_element_volume": (C.c_double, [C.c_int, _dp]),
             "or_piola": (None, [C.c_int, _dp, _dp, _dp]),
             "or_elasticity_coeff": (C.c_double, [C.c_int, _dp]),
+            "or_energy_density": (C.c_double, [C.c_int, _dp, _dp]),
+            "or_potential": (C.c_double, [vp, _dp]),
+            "or_potential_diff": (C.c_double, [vp, _dp, _dp, _dp]),
+            "or_lbfgs_energy": (C.c_int, [C.c_int, RESID_FN, ENERGY_FN, H0_FN, vp, _dp, C.POINTER(OrParams),
+                                          C.POINTER(OrStats), _dp, C.c_int]),
             "or_model_create": (vp, [C.POINTER(OrProblem), C.c_char_p, C.c_int]),
             "or_model_destroy": (None, [vp]),
             "or_ndof": (C.c_int, [vp]),
@@ -143,6 +151,12 @@ def piola(law, p, F):
     P = np.zeros(9)
     lib().or_piola(law, _p(p), _p(F), _p(P))
     return P.reshape(3, 3)
+
+
+def energy_density(law, p, F):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    return lib().or_energy_density(law, _p(p), _p(F))
 
 
 def ref_piola(law, p, F):
@@ -220,6 +234,19 @@ class Model:
         lib().or_external_force(self._h, _p(u), _p(r))
         return r
 
+    def potential_diff(self, x, xp):
+        """(Pi(x) - Pi(xp) element by element, rounding scale at xp)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xp = np.ascontiguousarray(xp, dtype=np.float64)
+        sc = C.c_double()
+        d = lib().or_potential_diff(self._h, _p(x), _p(xp), C.byref(sc))
+        return d, sc.value
+
+    def potential(self, u):
+        """Strain energy minus the constant loads' work (NaN with pressure faces)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().or_potential(self._h, _p(u))
+
     def volume(self, e):
         return lib().or_volume_of(self._h, e)
 
@@ -249,13 +276,18 @@ class Model:
         return out
 
     def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=0, max_linesearch=20,
-              m=6, lns_fea=1.0, use_precond=True, factor_scalar=False, trace_cap=0):
+              m=6, lns_fea=1.0, use_precond=True, factor_scalar=False, trace_cap=0, line_search=0,
+              x_target=None, target_tol=0.0):
         u = np.zeros(self.ndof) if u0 is None else np.array(u0, dtype=np.float64)
+        xt = None if x_target is None else np.ascontiguousarray(x_target, dtype=np.float64)
         prm = OrParams(epsilon, epsilon if epsilon_rel is None else epsilon_rel, max_iterations,
-                       max_linesearch, m, lns_fea, int(use_precond), int(factor_scalar))
+                       max_linesearch, m, lns_fea, int(use_precond), int(factor_scalar), int(line_search),
+                       _p(xt), float(target_tol))
         st = OrStats()
         tr = np.zeros((max(trace_cap, 1), 3))
-        lib().or_lbfgs_solve_model(self._h, _p(u), C.byref(prm), C.byref(st), _p(tr), trace_cap)
+        rc = lib().or_lbfgs_solve_model(self._h, _p(u), C.byref(prm), C.byref(st), _p(tr), trace_cap)
+        if rc < 0:
+            raise ValueError("or_lbfgs_solve_model refused the parameters")
         d = st.as_dict()
         if trace_cap:
             d["trace"] = tr[: min(st.iterations, trace_cap)]
@@ -263,7 +295,7 @@ class Model:
 
 
 def lbfgs(n, resid, x0, h0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=0,
-          max_linesearch=20, m=6, lns_fea=1.0, trace_cap=0):
+          max_linesearch=20, m=6, lns_fea=1.0, trace_cap=0, energy=None):
     """Generic oracle L-BFGS on a Python residual callback r = resid(x)."""
     def _rf(ctx, xp, rp):
         x = np.ctypeslib.as_array(xp, shape=(n,))
@@ -275,14 +307,21 @@ def lbfgs(n, resid, x0, h0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=
         o = np.ctypeslib.as_array(op, shape=(n,))
         o[:] = h0(v.copy())
 
+    def _ef(ctx, xa, xb, scale):
+        ea = float(energy(np.ctypeslib.as_array(xa, shape=(n,)).copy()))
+        eb = float(energy(np.ctypeslib.as_array(xb, shape=(n,)).copy()))
+        scale[0] = abs(eb)
+        return ea - eb
+
     rf = RESID_FN(_rf)
     hf = H0_FN(_hf) if h0 is not None else H0_FN()
+    ef = ENERGY_FN(_ef) if energy is not None else ENERGY_FN()
     x = np.array(x0, dtype=np.float64)
     prm = OrParams(epsilon, epsilon if epsilon_rel is None else epsilon_rel, max_iterations,
-                   max_linesearch, m, lns_fea, 1 if h0 else 0, 0)
+                   max_linesearch, m, lns_fea, 1 if h0 else 0, 0, 1 if energy is not None else 0, None, 0.0)
     st = OrStats()
     tr = np.zeros((max(trace_cap, 1), 3))
-    k = lib().or_lbfgs(n, rf, hf, None, _p(x), C.byref(prm), C.byref(st), _p(tr), trace_cap)
+    k = lib().or_lbfgs_energy(n, rf, ef, hf, None, _p(x), C.byref(prm), C.byref(st), _p(tr), trace_cap)
     d = st.as_dict()
     d["return"] = k
     if trace_cap:

@@ -348,6 +348,37 @@ void or_piola(int law, const double p[4], const double F[9], double P[9])
         P[k] += kv * dJdF[k];
 }
 
+/* Strain energy density of the same invariants (an addition: the
+ * reference has no energy).  W = c1-part + (J-1)^2/D1 with
+ *   NH:   C10 (I1bar - 3)
+ *   MR:   C10 (I1bar - 3) + C01 (I2bar - 3)
+ *   Yeoh: C10 t + C20 t^2 + C30 t^3, t = I1bar - 3,
+ * I1bar = I1 / (J^2)^(1/3), I2bar = I2 / (J^2)^(2/3) as in or_piola, so that
+ * dW/dF = or_piola (pinned by central differences in the tests). */
+double or_energy_density(int law, const double p[4], const double F[9])
+{
+    double J = det3(F);
+    double C[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            C[3 * i + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
+    double I1 = C[0] + C[4] + C[8];
+    double I3 = J * J;
+    double t = I1 / pow(I3, 1.0 / 3.0) - 3.0;
+    double W = (J - 1.0) * (J - 1.0) / p[3];
+    if (law == OR_YEOH)
+        return W + p[0] * t + p[1] * t * t + p[2] * t * t * t;
+    W += p[0] * t;
+    if (law == OR_MOONEY_RIVLIN) {
+        double trCC = 0.0;
+        for (int k = 0; k < 9; k++)
+            trCC += C[k] * C[k];
+        double I2 = 0.5 * (I1 * I1 - trCC);
+        W += p[1] * (I2 / pow(I3 * I3, 1.0 / 3.0) - 3.0);
+    }
+    return W;
+}
+
 /* calibrated approximate-Jacobian coefficient, the "t=2.0" lines of
  * NeoHookean.cpp:41, MooneyRivlin.cpp:48, Yeoh.cpp:47 */
 double or_elasticity_coeff(int law, const double p[4])
@@ -902,6 +933,94 @@ void or_external_force(or_model* m, const double* u, double* r)
     }
     if (m->grav)
         for (int i = 0; i < m->ndof; i++) r[i] += m->grav[i];
+}
+
+/* Total potential (an addition): sum_e sum_q V w_q W(F_q), F = J(x) J0^-1
+ * as in or_internal_force, plus u . r_c, where r_c = the constant loads'
+ * part of the residual (nodal forces -f, gravity -M g), so grad = residual. */
+double or_potential(or_model* m, const double* u)
+{
+    if (m->n_faces > 0)
+        return NAN;
+    int nen = m->nen;
+    double xe[60], W = 0.0;
+    for (int e = 0; e < m->n_elems; e++) {
+        const int* c = m->conn + (size_t)e * nen;
+        for (int a = 0; a < nen; a++)
+            node_pos(m, u, c[a], xe + 3 * a);
+        for (int q = 0; q < m->ng; q++) {
+            const double* dN = m->gdN + (size_t)q * nen * 3;
+            double J[9] = { 0 }, F[9];
+            for (int a = 0; a < nen; a++)
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++)
+                        J[3 * i + j] += xe[3 * a + i] * dN[3 * a + j];
+            const double* Ji = m->J0inv + ((size_t)e * m->ng + q) * 9;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    F[3 * i + j] = J[3 * i] * Ji[j] + J[3 * i + 1] * Ji[3 + j] + J[3 * i + 2] * Ji[6 + j];
+            W += m->vol[e] * m->gw[q] * or_energy_density(m->law, m->params + 4 * (size_t)e, F);
+        }
+    }
+    double* rc = (double*)calloc((size_t)(m->ndof > 0 ? m->ndof : 1), sizeof(double));
+    or_external_force(m, u, rc);
+    for (int i = 0; i < m->ndof; i++)
+        W += u[i] * rc[i];
+    free(rc);
+    return W;
+}
+
+/* Potential difference Pi(x) - Pi(xp) formed element by element,
+ * sum_e sum_q V w_q (W(F_q(x)) - W(F_q(xp))) + (x - xp) . r_c, so that it
+ * keeps its relative accuracy when the two states are close; *scale gets
+ * sum_e sum_q V w_q |W(F_q(xp))| + |xp . r_c| (the size of Pi's rounding).
+ * NAN with pressure faces. */
+double or_potential_diff(or_model* m, const double* x, const double* xp, double* scale)
+{
+    if (m->n_faces > 0)
+        return NAN;
+    int nen = m->nen;
+    double xa[60], xb[60], D = 0.0, S = 0.0;
+    for (int e = 0; e < m->n_elems; e++) {
+        const int* c = m->conn + (size_t)e * nen;
+        for (int a = 0; a < nen; a++) {
+            node_pos(m, x, c[a], xa + 3 * a);
+            node_pos(m, xp, c[a], xb + 3 * a);
+        }
+        for (int q = 0; q < m->ng; q++) {
+            const double* dN = m->gdN + (size_t)q * nen * 3;
+            const double* Ji = m->J0inv + ((size_t)e * m->ng + q) * 9;
+            double Fa[9], Fb[9];
+            for (int s2 = 0; s2 < 2; s2++) {
+                const double* xe = s2 ? xb : xa;
+                double* F = s2 ? Fb : Fa;
+                double J[9] = { 0 };
+                for (int a = 0; a < nen; a++)
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++)
+                            J[3 * i + j] += xe[3 * a + i] * dN[3 * a + j];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++)
+                        F[3 * i + j] = J[3 * i] * Ji[j] + J[3 * i + 1] * Ji[3 + j] + J[3 * i + 2] * Ji[6 + j];
+            }
+            const double* p = m->params + 4 * (size_t)e;
+            double wa = or_energy_density(m->law, p, Fa), wb = or_energy_density(m->law, p, Fb);
+            double wv = m->vol[e] * m->gw[q];
+            D += wv * (wa - wb);
+            S += wv * fabs(wb);
+        }
+    }
+    double* rc = (double*)calloc((size_t)(m->ndof > 0 ? m->ndof : 1), sizeof(double));
+    or_external_force(m, x, rc);
+    double ext = 0.0;
+    for (int i = 0; i < m->ndof; i++) {
+        D += (x[i] - xp[i]) * rc[i];
+        ext += xp[i] * rc[i];
+    }
+    free(rc);
+    if (scale)
+        *scale = S + fabs(ext);
+    return D;
 }
 
 /* SoftObjective::residual -> Constraint::getResidual (Constraint.cpp:115-126):
@@ -1467,12 +1586,14 @@ static double dot(int n, const double* a, const double* b)
     return s;
 }
 
+/* ||a||_inf, NaN if any entry is NaN (the non-finite checks of
+ * Integrator.cpp:28-34 must see it) */
 static double norm_inf(int n, const double* a)
 {
     double s = 0;
     for (int i = 0; i < n; i++) {
         double v = fabs(a[i]);
-        s = v > s ? v : s;
+        s = (v != v || v > s) ? v : s;
     }
     return s;
 }
@@ -1514,7 +1635,14 @@ typedef struct {
     or_resid_fn f;
     or_h0_fn h0;
     void* ctx;
+    or_energy_diff_fn e; /* potential difference for the energy line search, or NULL */
 } solver_fn;
+
+/* energy line search (an addition): Armijo constant, and the relative
+ * size below which a potential difference is rounding (then only the
+ * projection rule decides, as in approximate-Wolfe line searches) */
+#define OR_ARMIJO_C1 1e-4
+#define OR_ENERGY_RTOL 1e-12
 
 /* LBFGSSolver::solve after the initial residual (LBFGS.h:126-171) with the
  * projection line search LineSearchFEA::LineSearch (LineSearchFEA.h:37-68).
@@ -1543,6 +1671,8 @@ static int lbfgs_loop(const solver_fn* fn, double* x, double* r, const or_lbfgs_
             drt[i] = -r[i];
     }
     int k = 1;
+    const int energy_ls = prm->line_search == 1 && fn->e;
+    double tnorm = prm->x_target ? sqrt(dot(n, prm->x_target, prm->x_target)) : 0.0;
     for (;;) {
         memcpy(xp, x, sizeof(double) * nn);
         memcpy(rp, r, sizeof(double) * nn);
@@ -1557,9 +1687,25 @@ static int lbfgs_loop(const solver_fn* fn, double* x, double* r, const or_lbfgs_
             trials++;
             used = cur;
             double dg = dot(n, r, drt);
-            if (fabs(dg) <= prm->lns_fea * fabs(dg_init))
+            int proj = fabs(dg) <= prm->lns_fea * fabs(dg_init);
+            if (!energy_ls) {
+                /* LineSearchFEA::LineSearch (LineSearchFEA.h:37-68) */
+                if (proj)
+                    break;
+                cur *= dg * dg_init > 0 ? 1.5 : 0.5;
+                continue;
+            }
+            /* energy rule (addition): sufficient decrease of the potential,
+             * dPi <= c1 step (r . d), or a dPi within rounding of the
+             * potential, and the projection rule; without the decrease the
+             * step halves, otherwise the projection rule's expansion /
+             * contraction */
+            double scale = 0.0;
+            double dpi = fn->e(fn->ctx, x, xp, &scale);
+            int armijo = dpi <= OR_ARMIJO_C1 * cur * dg_init || fabs(dpi) <= OR_ENERGY_RTOL * scale;
+            if (armijo && proj)
                 break;
-            cur *= dg * dg_init > 0 ? 1.5 : 0.5;
+            cur *= !armijo ? 0.5 : (dg * dg_init > 0 ? 1.5 : 0.5);
         }
         st->ls_trials += trials;
         double gnorm = norm_inf(n, r);
@@ -1570,6 +1716,16 @@ static int lbfgs_loop(const solver_fn* fn, double* x, double* r, const or_lbfgs_
         }
         st->iterations = k;
         st->rinf = gnorm;
+        if (prm->x_target) {
+            double e2 = 0.0;
+            for (int i = 0; i < n; i++)
+                e2 += (x[i] - prm->x_target[i]) * (x[i] - prm->x_target[i]);
+            st->target_err = sqrt(e2) / tnorm;
+            if (st->target_err <= prm->target_tol) {
+                st->target_reached = 1;
+                break;
+            }
+        }
         if (gnorm <= prm->epsilon || gnorm <= prm->epsilon_rel * norm_inf(n, x))
             break;
         if (prm->max_iterations != 0 && k >= prm->max_iterations)
@@ -1603,6 +1759,12 @@ static int lbfgs_loop(const solver_fn* fn, double* x, double* r, const or_lbfgs_
 int or_lbfgs(int n, or_resid_fn f, or_h0_fn h0, void* ctx, double* x, const or_lbfgs_params* prm,
     or_lbfgs_stats* st, double* trace, int trace_cap)
 {
+    return or_lbfgs_energy(n, f, NULL, h0, ctx, x, prm, st, trace, trace_cap);
+}
+
+int or_lbfgs_energy(int n, or_resid_fn f, or_energy_diff_fn e, or_h0_fn h0, void* ctx, double* x,
+    const or_lbfgs_params* prm, or_lbfgs_stats* st, double* trace, int trace_cap)
+{
     memset(st, 0, sizeof(*st));
     if (prm->m <= 0 || prm->m > 64 || prm->max_linesearch <= 0)
         return -1;
@@ -1620,7 +1782,11 @@ int or_lbfgs(int n, or_resid_fn f, or_h0_fn h0, void* ctx, double* x, const or_l
         st->solve_seconds = now_s() - t0;
         return 1;
     }
-    solver_fn fn = { n, f, h0, ctx };
+    if (prm->line_search == 1 && !e) {
+        free(r);
+        return -1;
+    }
+    solver_fn fn = { n, f, h0, ctx, e };
     int k = lbfgs_loop(&fn, x, r, prm, st, trace, trace_cap);
     free(r);
     st->solve_seconds = now_s() - t0;
@@ -1637,6 +1803,11 @@ static void model_h0_cb(void* ctx, const double* v, double* out)
     or_apply_h0((or_model*)ctx, v, out);
 }
 
+static double model_energy_cb(void* ctx, const double* x, const double* xp, double* scale)
+{
+    return or_potential_diff((or_model*)ctx, x, xp, scale);
+}
+
 /* Integrator::integral -> LBFGSSolver::solve for the model (LBFGS.h:96-174) */
 int or_lbfgs_solve_model(or_model* m, double* x, const or_lbfgs_params* prm, or_lbfgs_stats* st,
     double* trace, int trace_cap)
@@ -1644,6 +1815,8 @@ int or_lbfgs_solve_model(or_model* m, double* x, const or_lbfgs_params* prm, or_
     memset(st, 0, sizeof(*st));
     if (prm->m <= 0 || prm->m > 64 || prm->max_linesearch <= 0)
         return -1;
+    if (prm->line_search == 1 && m->n_faces > 0)
+        return -1; /* the energy rule needs a potential */
     int n = m->ndof;
     double t0 = now_s();
     double* r = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
@@ -1665,7 +1838,8 @@ int or_lbfgs_solve_model(or_model* m, double* x, const or_lbfgs_params* prm, or_
         st->factor_seconds = now_s() - tf;
         st->precond_failed = !have_h0;
     }
-    solver_fn fn = { n, model_resid_cb, have_h0 ? model_h0_cb : NULL, m };
+    solver_fn fn = { n, model_resid_cb, have_h0 ? model_h0_cb : NULL, m,
+        prm->line_search == 1 ? model_energy_cb : NULL };
     int k = lbfgs_loop(&fn, x, r, prm, st, trace, trace_cap);
     free(r);
     st->solve_seconds = now_s() - t0;

@@ -70,6 +70,9 @@ double or_element_volume(int nen, const double* Xe);
 /* ---- materials ---------------------------------------------------------- */
 void or_piola(int law, const double p[4], const double F[9], double P[9]);
 double or_elasticity_coeff(int law, const double p[4]);
+/* strain energy density whose derivative dW/dF is or_piola (an addition:
+ * the reference never evaluates an energy) */
+double or_energy_density(int law, const double p[4], const double F[9]);
 
 /* ---- model -------------------------------------------------------------- */
 or_model* or_model_create(const or_problem* pb, char* err, int errlen);
@@ -81,6 +84,15 @@ void or_residual(or_model* m, const double* u, double* r);
 void or_internal_force(or_model* m, const double* u, double* r);
 void or_external_force(or_model* m, const double* u, double* r);
 int or_face_parent(const or_model* m, int f);
+/* total potential of a conservative problem: strain energy
+ * sum_e sum_q V_e w_q W(F_q) minus the work of the constant loads (nodal
+ * forces, gravity), so that its gradient is or_residual; NAN when the model
+ * has pressure faces (a follower load has no potential) */
+double or_potential(or_model* m, const double* u);
+/* Pi(x) - Pi(xp) element by element (keeps its relative accuracy for close
+ * states); *scale (may be NULL) = sum of |energy terms| at xp, the size of
+ * Pi's rounding.  NAN with pressure faces. */
+double or_potential_diff(or_model* m, const double* x, const double* xp, double* scale);
 
 /* approximate Jacobian, full symmetric CSC with sorted rows.
  * scalar = 0: the reference's 3N_f matrix with explicit zero cross terms
@@ -103,6 +115,13 @@ typedef struct {
     double lns_fea;        /* projection acceptance factor (reference: 1.0) */
     int use_precond;       /* 1: H0 = K^{-1} (lbfgs_ah), 0: H0 = I/theta (lbfgs) */
     int factor_scalar;     /* factor mode, see or_factor */
+    /* additions (no counterpart in the reference): */
+    int line_search;       /* 0: the reference's projection rule; 1: energy
+                              rule (Armijo decrease of the potential with
+                              c1 = 1e-4, plus the projection rule) */
+    const double* x_target;/* non-NULL: also stop once ||x - x_target||_2 <=
+                              target_tol ||x_target||_2 after an iteration */
+    double target_tol;
 } or_lbfgs_params;
 
 typedef struct {
@@ -114,16 +133,28 @@ typedef struct {
     double xinf;
     double factor_seconds;
     double solve_seconds;
+    int target_reached;
+    double target_err;
 } or_lbfgs_stats;
 
 typedef void (*or_resid_fn)(void* ctx, const double* x, double* r);
 typedef void (*or_h0_fn)(void* ctx, const double* v, double* out);
+/* Pi(x) - Pi(xp), and in *scale the size of Pi's rounding at xp */
+typedef double (*or_energy_diff_fn)(void* ctx, const double* x, const double* xp, double* scale);
 
 /* generic solver: f(x) -> r; h0 NULL means H0 = I/theta.
  * trace (optional) receives per iteration {||r||_inf, step, trials}. */
 int or_lbfgs(int n, or_resid_fn f, or_h0_fn h0, void* ctx, double* x,
              const or_lbfgs_params* prm, or_lbfgs_stats* st,
              double* trace, int trace_cap);
+/* the same with the difference e(x, xp) of a potential whose gradient is f
+ * (needed by the energy line search; may be NULL for the projection rule).
+ * Energy rule: a trial is accepted when the projection rule holds and
+ * Pi(x) - Pi(xp) <= 1e-4 step (r(xp) . d) or |Pi(x) - Pi(xp)| <= 1e-12 scale;
+ * without the decrease the step halves, otherwise it grows by 1.5 or halves
+ * as the projection rule does. */
+int or_lbfgs_energy(int n, or_resid_fn f, or_energy_diff_fn e, or_h0_fn h0, void* ctx, double* x,
+                    const or_lbfgs_params* prm, or_lbfgs_stats* st, double* trace, int trace_cap);
 int or_lbfgs_solve_model(or_model* m, double* x, const or_lbfgs_params* prm,
                          or_lbfgs_stats* st, double* trace, int trace_cap);
 
