@@ -83,6 +83,24 @@ def cpu_baseline(mesh, n_evals):
     return n_evals / dt, dt
 
 
+def cpu_time_to_solution(mesh, u_star, target, max_iterations, factor_scalar=True):
+    """The oracle's solve from u = 0 (1 core): approximate-Jacobian assembly
+    + ordering + factorisation, then its L-BFGS (projection line search,
+    m = 6) until the displacement is within `target` (relative L2) of u*, or
+    `max_iterations` iterations (a bounded sample).  Returns the phases'
+    seconds and the iteration count."""
+    import oracle as O
+    om = O.Model(mesh)
+    t = time.perf_counter()
+    _, st = om.solve(epsilon=0.0, epsilon_rel=0.0, max_iterations=max_iterations, factor_scalar=factor_scalar,
+                     x_target=u_star, target_tol=target)
+    wall = time.perf_counter() - t
+    return {"wall_s": wall, "factor_s": st["factor_seconds"], "loop_s": wall - st["factor_seconds"],
+            "iterations": st["iterations"], "r_evals": st["r_evals"], "reached": bool(st["target_reached"]),
+            "target_err": st["target_err"], "factor_mode": "scalar Laplacian" if factor_scalar else
+            "interleaved 3N_f (the reference's matrix)"}
+
+
 def run_batch(E, args):
     """64 cantilevers (Lx4x4 C3D8, L ~ U{10..40}, NH, p ~ U[0.5, 5] kPa x scale):
     all solved in one persistent launch (mixed precision, per-instance

@@ -429,11 +429,11 @@ class Context:
 
     def autotune(self) -> dict:
         """Pick the fastest launch shapes for this problem (efem_autotune)."""
-        rep = np.zeros(7)
+        rep = np.zeros(9)
         self._check(self._lib.efem_autotune(self._h, _ptr(rep, C.c_double)), "efem_autotune")
         return {"gather_lanes": int(rep[0]), "tet4_threads": int(rep[1]), "reduce_blocks": int(rep[2]),
                 "history_blocks": int(rep[5]), "residual_ms": float(rep[3]), "dot_ms": float(rep[4]),
-                "history_ms": float(rep[6])}
+                "history_ms": float(rep[6]), "batch_threads": int(rep[7]), "batch_ms": float(rep[8])}
 
     def set_option(self, name: str, value: float):
         self._check(self._lib.efem_set_option(self._h, name.encode(), float(value)), "efem_set_option")
@@ -562,11 +562,14 @@ class Batch(Context):
         return [u[self.offsets[i]:self.offsets[i + 1]] for i in range(self.count)]
 
     def solve(self, u0=None, epsilon=1e-6, epsilon_rel=None, max_iterations=10000, max_linesearch=20, m=6,
-              lns_fea=1.0, use_precond=True, fp64_switch=0.0):
+              lns_fea=1.0, use_precond=True, fp64_switch=0.0, x_target=None, target_tol=0.0):
+        """efem_batch_solve; x_target (concatenated like u) stops each
+        instance at its own ||x_i - x_target_i|| <= target_tol ||x_target_i||."""
         u = np.zeros(self.ndof) if u0 is None else np.array(u0, dtype=np.float64)
+        xt = None if x_target is None else np.ascontiguousarray(x_target, dtype=np.float64)
         prm = LbfgsParams(float(epsilon), float(epsilon if epsilon_rel is None else epsilon_rel),
                           int(max_iterations), int(max_linesearch), int(m), float(lns_fea),
-                          1 if use_precond else 0, 0, None, 0.0, float(fp64_switch))
+                          1 if use_precond else 0, 0, _ptr(xt, C.c_double), float(target_tol), float(fp64_switch))
         st = (SolveStats * self.count)()
         rc = self._lib.efem_batch_solve(self._h, _ptr(u, C.c_double), C.byref(prm), st)
         if rc not in (EFEM_OK, EFEM_ERR_NONFINITE):

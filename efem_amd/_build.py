@@ -30,7 +30,10 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
 
 SOURCES = [
     ("device/kernels.hip", True),
-    ("device/batch.hip", True),
+    # the persistent batched solver, once per workgroup width
+    ("device/batch.hip", True, "256"),
+    ("device/batch.hip", True, "512"),
+    ("device/batch.hip", True, "1024"),
     ("api.cpp", True),
     ("host/model.cpp", False),
     ("host/element.cpp", False),
@@ -42,10 +45,12 @@ SOURCES = [
 ]
 
 
-def _compile(src: str, device: bool) -> Path:
+def _compile(src: str, device: bool, batch_width: str | None = None) -> Path:
     s = CSRC / src
-    obj = BUILD / (src.replace("/", "_") + ".o")
+    obj = BUILD / (src.replace("/", "_") + (f".{batch_width}" if batch_width else "") + ".o")
     cmd = [HIPCC, *COMMON]
+    if batch_width:
+        cmd += [f"-DEFEM_BATCH_BT={batch_width}"]
     if device:
         cmd += ["-x", "hip", *DEVICE]
     else:

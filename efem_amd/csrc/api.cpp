@@ -1737,6 +1737,10 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         c->tuning.historyBlocks = iv;
         return EFEM_OK;
     }
+    if (k == "batch_threads" && (iv == 256 || iv == 512 || iv == 1024)) {
+        c->tuning.batchThreads = iv;
+        return EFEM_OK;
+    }
     if (k == "bwd_narrow_items" && value >= 1.0) {
         // levels with at least this many 4-column backward items use
         // 16-column items; takes effect at the next factorisation
@@ -2136,7 +2140,7 @@ int efem_batch_create(efem_ctx** out, const efem_problem* problems, int count, i
         br[i] = {r.e0, r.e1, r.f0, r.f1, r.n0, r.n1};
     }
     if ((rc = upload(cp, &cp->dRanges, br)) || (!fpress.empty() && (rc = upload(cp, &cp->fpress, fpress))) ||
-        (rc = devAlloc(cp, &cp->batchStats, 8 * static_cast<size_t>(count)))) {
+        (rc = devAlloc(cp, &cp->batchStats, efem::dev::kBatchStats * static_cast<size_t>(count)))) {
         g_error = cp->err;
         efem_destroy(c.release());
         return rc;
@@ -2164,8 +2168,8 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
         return EFEM_ERR_STATE;
     }
     if (prm->m <= 0 || prm->m > efem::dev::kMaxCompact || prm->max_linesearch <= 0 || prm->epsilon < 0 ||
-        prm->epsilon_rel < 0 || prm->max_iterations <= 0 || prm->x_target) {
-        c->err = "efem_batch_solve: invalid parameters (history 1..16, max_iterations > 0, no x_target)";
+        prm->epsilon_rel < 0 || prm->max_iterations <= 0) {
+        c->err = "efem_batch_solve: invalid parameters (history 1..16, max_iterations > 0)";
         return EFEM_ERR_ARG;
     }
     if (prm->line_search != EFEM_LS_PROJECTION) {
@@ -2191,6 +2195,11 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
     }
     const size_t nb = static_cast<size_t>(n) * sizeof(double);
     HIP_CHECK(c, hipMemcpyAsync(c->x, u, nb, hipMemcpyHostToDevice, c->stream));
+    if (prm->x_target) {
+        // per-instance displacement targets (concatenated like u)
+        if (!c->xtarget && (rc = devAlloc(c, &c->xtarget, static_cast<size_t>(n) + 3))) return rc;
+        HIP_CHECK(c, hipMemcpyAsync(c->xtarget, prm->x_target, nb, hipMemcpyHostToDevice, c->stream));
+    }
     efem::dev::BatchArgs a;
     a.elem = elemArgs(c, c->x, false);
     a.face = faceArgs(c, c->x, nullptr);
@@ -2220,10 +2229,13 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
     a.epsRel = prm->epsilon_rel;
     a.lnsFea = prm->lns_fea;
     a.fp64Switch = elem32(c) ? prm->fp64_switch : 0.0;
+    a.xt = prm->x_target ? c->xtarget : nullptr;
+    a.targetTol = prm->target_tol;
     a.stats = c->batchStats;
     efem::dev::launchBatchSolve(a, count, c->stream);
     HIP_CHECK(c, hipGetLastError());
-    std::vector<double> st(8 * static_cast<size_t>(count));
+    const int ks = efem::dev::kBatchStats;
+    std::vector<double> st(ks * static_cast<size_t>(count));
     HIP_CHECK(c, hipMemcpyAsync(u, c->x, nb, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipMemcpyAsync(st.data(), c->batchStats, st.size() * sizeof(double), hipMemcpyDeviceToHost,
                                 c->stream));
@@ -2231,7 +2243,7 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     bool anyNonfinite = false;
     for (int i = 0; i < count; ++i) {
-        const double* q = &st[8 * static_cast<size_t>(i)];
+        const double* q = &st[ks * static_cast<size_t>(i)];
         const bool bad = q[5] != 0.0 || !std::isfinite(q[3]) || !std::isfinite(q[4]);
         if (bad) {
             anyNonfinite = true;
@@ -2248,6 +2260,8 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
             s.xinf = q[4];
             s.precond_failed = prm->use_precond && !h0;
             s.fp64_switch_iteration = static_cast<int>(q[6]);
+            s.target_reached = static_cast<int>(q[8]);
+            s.target_err = q[9];
             s.factor_seconds = factorSeconds;
             s.solve_seconds = wall;
             s.factor_nnz = h0 ? c->fac.nnz : 0;
@@ -2375,6 +2389,43 @@ int efem_autotune(efem_ctx* c, double* report)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    // batches: the persistent solver's workgroup width, timed on 40
+    // iterations of every instance from u = 0 (the history and iterate
+    // buffers are scratch outside a solve)
+    double bestBatch = 0.0;
+    if (!c->parts.empty()) {
+        bestBatch = 1e300;
+        std::vector<double> u0(static_cast<size_t>(std::max(n, 1)));
+        efem_lbfgs_params bp{};
+        bp.epsilon = 0.0;
+        bp.epsilon_rel = 0.0;
+        bp.max_iterations = 40;
+        bp.max_linesearch = 20;
+        bp.m = 6;
+        bp.lns_fea = 1.0;
+        bp.use_precond = 1;
+        bp.line_search = EFEM_LS_PROJECTION;
+        const int widths[3] = {256, 512, 1024};
+        efem::dev::LaunchTuning keep = best;
+        for (int wi = 0; wi < 3 && rc == EFEM_OK; ++wi) {
+            c->tuning = keep;
+            c->tuning.batchThreads = widths[wi];
+            double ms = 1e300;
+            for (int rep = 0; rep < 3 && rc == EFEM_OK; ++rep) {
+                std::fill(u0.begin(), u0.end(), 0.0);
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = efem_batch_solve(c, u0.data(), &bp, nullptr);
+                if (rc == EFEM_ERR_NONFINITE) rc = EFEM_OK;  // timing only
+                const double t = 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (rep > 0) ms = std::min(ms, t);  // the first run also factors
+            }
+            if (rc == EFEM_OK && ms < bestBatch) {
+                bestBatch = ms;
+                best.batchThreads = widths[wi];
+            }
+        }
+        if (rc) return rc;
+    }
     c->tuning = best;
     efem::dev::setTuning(best);
     if (report) {
@@ -2385,6 +2436,8 @@ int efem_autotune(efem_ctx* c, double* report)
         report[4] = bestRed;
         report[5] = best.historyBlocks;
         report[6] = bestHist;
+        report[7] = best.batchThreads;
+        report[8] = bestBatch;
     }
     return EFEM_OK;
 }

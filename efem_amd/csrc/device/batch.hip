@@ -12,14 +12,25 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include "kernels.hpp"
+
+// This file is compiled once per workgroup width (256, 512 and 1024 lanes,
+// one object each, -DEFEM_BATCH_BT=...): the autotuner picks the width.
+#ifndef EFEM_BATCH_BT
+#define EFEM_BATCH_BT 256
+#endif
 
 namespace efem {
 namespace dev {
 namespace {
 
-// K simultaneous block reductions over the 256 lanes; op 0 = sum, 1 = max
+constexpr int BT = EFEM_BATCH_BT;  // lanes per workgroup (one instance)
+constexpr int BW = BT / 64;        // waves per workgroup
+
+// K simultaneous block reductions over the BT lanes (wave trees, then the
+// waves in order); op 0 = sum, 1 = max
 template <int K>
-__device__ __forceinline__ void blockReduceK(double (&v)[K], const int (&op)[K], double (*sh)[4])
+__device__ __forceinline__ void blockReduceK(double (&v)[K], const int (&op)[K], double (*sh)[BW])
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
@@ -29,36 +40,39 @@ __device__ __forceinline__ void blockReduceK(double (&v)[K], const int (&op)[K],
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        v[k] = op[k] ? maxNan(maxNan(maxNan(sh[k][0], sh[k][1]), sh[k][2]), sh[k][3])
-                     : ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    for (int k = 0; k < K; ++k) {
+        double t = sh[k][0];
+#pragma unroll
+        for (int w = 1; w < BW; ++w) t = op[k] ? maxNan(t, sh[k][w]) : t + sh[k][w];
+        v[k] = t;
+    }
     __syncthreads();
 }
 
 template <int NEN, int NG, typename TF>
 union BatchLds {
-    ElemTileLds<float, NEN, NG, MODE_FORCE> ef;
-    ElemTileLds<double, NEN, NG, MODE_FORCE> ed;
-    TrsvFwdLds<TF, 4> tf;
+    ElemTileLds<float, NEN, NG, MODE_FORCE, BT> ef;
+    ElemTileLds<double, NEN, NG, MODE_FORCE, BT> ed;
+    TrsvFwdLds<TF, BW> tf;
 };
 
 template <typename T, int NEN, int NG, int LAW>
 __device__ __forceinline__ void blockElements(const BatchArgs& a, const BatchRange& I,
-                                              ElemTileLds<T, NEN, NG, MODE_FORCE>& L)
+                                              ElemTileLds<T, NEN, NG, MODE_FORCE, BT>& L)
 {
     const ElemArgs& ea = a.elem;
     T* slots = static_cast<T*>(ea.slots);
     const double* ginv = static_cast<const double*>(ea.ginv);
     if (NEN == 4) {
-        for (int e = I.e0 + threadIdx.x; e < I.e1; e += kThreads)
+        for (int e = I.e0 + threadIdx.x; e < I.e1; e += BT)
             tet4Lane<T, LAW, MODE_FORCE>(e, ea.nElems, ea.connDof, ginv, ea.vol, 1.0, ea.params, ea.paramsPerElem,
                                           a.x, nullptr, slots);
     } else {
         const double* gdN = static_cast<const double*>(ea.gdN);
-        for (int k = threadIdx.x; k < NG * NEN * 3; k += kThreads) L.sdN[k] = gdN[k];
-        for (int e0 = I.e0; e0 < I.e1; e0 += Tile<NG>::epb)
-            elemTile<T, NEN, NG, LAW, MODE_FORCE>(ea.nElems, e0, I.e1, ea.connDof, ginv, ea.vol, ea.gw, ea.params,
-                                                   ea.paramsPerElem, a.x, nullptr, slots, L);
+        for (int k = threadIdx.x; k < NG * NEN * 3; k += BT) L.sdN[k] = gdN[k];
+        for (int e0 = I.e0; e0 < I.e1; e0 += Tile<NG, BT>::epb)
+            elemTile<T, NEN, NG, LAW, MODE_FORCE, BT>(ea.nElems, e0, I.e1, ea.connDof, ginv, ea.vol, ea.gw,
+                                                       ea.params, ea.paramsPerElem, a.x, nullptr, slots, L);
     }
 }
 
@@ -66,28 +80,30 @@ __device__ __forceinline__ void blockElements(const BatchArgs& a, const BatchRan
 template <typename T, int NEN, int NG, typename TF>
 __device__ void blockResidual(const BatchArgs& a, const BatchRange& I, BatchLds<NEN, NG, TF>& U)
 {
-    ElemTileLds<T, NEN, NG, MODE_FORCE>& L =
-        *reinterpret_cast<ElemTileLds<T, NEN, NG, MODE_FORCE>*>(&U);
+    ElemTileLds<T, NEN, NG, MODE_FORCE, BT>& L =
+        *reinterpret_cast<ElemTileLds<T, NEN, NG, MODE_FORCE, BT>*>(&U);
     switch (a.elem.law) {
         case LAW_NH: blockElements<T, NEN, NG, LAW_NH>(a, I, L); break;
         case LAW_MR: blockElements<T, NEN, NG, LAW_MR>(a, I, L); break;
         default: blockElements<T, NEN, NG, LAW_YEOH>(a, I, L);
     }
-    for (int f = I.f0 + threadIdx.x; f < I.f1; f += kThreads) pressureFace<NEN, PMODE_FORCE>(a.face, f);
+    for (int f = I.f0 + threadIdx.x; f < I.f1; f += BT) pressureFace<NEN, PMODE_FORCE>(a.face, f);
     __syncthreads();
     GatherArgs g = a.gather;
     g.mixed = sizeof(T) == 4;
-    for (int i = I.n0 + threadIdx.x; i < I.n1; i += kThreads) gatherNode<T>(g, i);
+    for (int i = I.n0 + threadIdx.x; i < I.n1; i += BT) gatherNode<T>(g, i);
     __syncthreads();
 }
 
-// out = K^-1 in on the instance (its supernodes of the merged forest)
+// out = K^-1 in on the instance (its supernodes of the merged forest): the
+// forward items split their columns over the BW waves, the backward items
+// give each of their four columns BW / 4 waves
 template <typename TF, int NEN, int NG>
 __device__ void blockH0(const BatchArgs& a, int inst, const BatchRange& I, const double* in, double* out,
-                        BatchLds<NEN, NG, TF>& U)
+                        BatchLds<NEN, NG, TF>& U, TF (*bred)[3])
 {
     const TrsvArgs& t = a.trsv;
-    for (int j = I.n0 + threadIdx.x; j < I.n1; j += kThreads) {
+    for (int j = I.n0 + threadIdx.x; j < I.n1; j += BT) {
         const int o = t.perm[j];
         a.bp[3 * j] = in[3 * o];
         a.bp[3 * j + 1] = in[3 * o + 1];
@@ -98,17 +114,22 @@ __device__ void blockH0(const BatchArgs& a, int inst, const BatchRange& I, const
     for (int l = 0; l < a.nlev; ++l) {
         const int off = li[4 * l], cnt = li[4 * l + 1];
         for (int it = 0; it < cnt; ++it)
-            trsvFwdItem<TF, 4>(t, a.bp, a.items[2 * (off + it)], a.items[2 * (off + it) + 1], U.tf);
+            trsvFwdItem<TF, BW>(t, a.bp, a.items[2 * (off + it)], a.items[2 * (off + it) + 1], U.tf);
         __syncthreads();
     }
     for (int l = a.nlev - 1; l >= 0; --l) {
         const int off = li[4 * l + 2], cnt = li[4 * l + 3];
-        for (int it = 0; it < cnt; ++it)
-            trsvBwdItem<TF, 4>(t, a.items[2 * (off + it)], a.items[2 * (off + it) + 1]);
+        for (int it = 0; it < cnt; ++it) {
+            trsvBwdItem<TF, BW>(t, a.items[2 * (off + it)], a.items[2 * (off + it) + 1], bred);
+            // with several waves per column the item's partial sums sit in
+            // bred: the next item may overwrite them only after this one
+            // combined them
+            if (BW > 4) __syncthreads();
+        }
         __syncthreads();
     }
     const TF* z = static_cast<const TF*>(t.z);
-    for (int j = I.n0 + threadIdx.x; j < I.n1; j += kThreads) {
+    for (int j = I.n0 + threadIdx.x; j < I.n1; j += BT) {
         const int o = t.perm[j];
         out[3 * o] = static_cast<double>(z[3 * j]);
         out[3 * o + 1] = static_cast<double>(z[3 * j + 1]);
@@ -128,10 +149,11 @@ __device__ __forceinline__ void residualAt(const BatchArgs& a, const BatchRange&
 }
 
 template <int NEN, int NG, typename TF>
-__global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
+__global__ __launch_bounds__(BT) void kBatchLbfgs(BatchArgs a)
 {
     __shared__ BatchLds<NEN, NG, TF> U;
-    __shared__ double red[3][4];
+    __shared__ double red[4][BW];
+    __shared__ TF bred[BW][3];
     __shared__ double ys[kMaxCompact], alpha[kMaxCompact];
     const int inst = blockIdx.x;
     const BatchRange I = a.ranges[inst];
@@ -143,23 +165,27 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
     double* r = a.r;
     double* d = a.d;
     double* q = a.q;
+    const double* xt = a.xt;
     bool f32 = a.elemF32 != 0;
-    int rEvals = 0, lsTrials = 0, iters = 0, switchIt = 0, nonfinite = 0;
+    int rEvals = 0, lsTrials = 0, iters = 0, switchIt = 0, nonfinite = 0, reached = 0;
+    double tnorm2 = 0.0, terr = 0.0;
 
     residualAt<NEN, NG, TF>(a, I, f32, U);
     ++rEvals;
     double rinf, xinf;
     {
-        double v[3] = {0.0, 0.0, 0.0};
-        const int op[3] = {0, 1, 1};
-        for (int i = d0 + tid; i < d1; i += kThreads) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        const int op[4] = {0, 1, 1, 0};
+        for (int i = d0 + tid; i < d1; i += BT) {
             v[0] += x[i] * x[i];
             v[1] = maxNan(v[1], fabs(x[i]));
             v[2] = maxNan(v[2], fabs(r[i]));
+            if (xt) v[3] += xt[i] * xt[i];
         }
-        blockReduceK<3>(v, op, red);
+        blockReduceK<4>(v, op, red);
         rinf = v[2];
         xinf = v[1];
+        tnorm2 = v[3];
         const double xnorm2 = sqrt(v[0]);
         if (rinf <= a.eps || rinf <= a.epsRel * xnorm2) {
             iters = 1;
@@ -171,16 +197,16 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
         const bool h0 = a.usePrecond != 0;
         // d = -H0 r
         if (h0) {
-            blockH0<TF, NEN, NG>(a, inst, I, r, d, U);
-            for (int i = d0 + tid; i < d1; i += kThreads) d[i] = -d[i];
+            blockH0<TF, NEN, NG>(a, inst, I, r, d, U, bred);
+            for (int i = d0 + tid; i < d1; i += BT) d[i] = -d[i];
         } else {
-            for (int i = d0 + tid; i < d1; i += kThreads) d[i] = -r[i];
+            for (int i = d0 + tid; i < d1; i += BT) d[i] = -r[i];
         }
         __syncthreads();
         int ncorr = 0, ptr = m;
         double theta = 1.0;
         for (int k = 1;; ++k) {
-            for (int i = d0 + tid; i < d1; i += kThreads) {
+            for (int i = d0 + tid; i < d1; i += BT) {
                 a.xp[i] = x[i];
                 a.rp[i] = r[i];
             }
@@ -188,28 +214,31 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
             {
                 double v[1] = {0.0};
                 const int op[1] = {0};
-                for (int i = d0 + tid; i < d1; i += kThreads) v[0] += r[i] * d[i];
+                for (int i = d0 + tid; i < d1; i += BT) v[0] += r[i] * d[i];
                 blockReduceK<1>(v, op, red);
                 dgInit = v[0];
             }
             double cur = 1.0;
             int trials = 0;
+            double dist2 = 0.0;
             for (int it = 0; it < a.maxLs; ++it) {
-                for (int i = d0 + tid; i < d1; i += kThreads) x[i] = a.xp[i] + cur * d[i];
+                for (int i = d0 + tid; i < d1; i += BT) x[i] = a.xp[i] + cur * d[i];
                 __syncthreads();
                 residualAt<NEN, NG, TF>(a, I, f32, U);
                 ++rEvals;
                 ++trials;
-                double v[3] = {0.0, 0.0, 0.0};
-                const int op[3] = {0, 1, 1};
-                for (int i = d0 + tid; i < d1; i += kThreads) {
+                double v[4] = {0.0, 0.0, 0.0, 0.0};
+                const int op[4] = {0, 1, 1, 0};
+                for (int i = d0 + tid; i < d1; i += BT) {
                     v[0] += r[i] * d[i];
                     v[1] = maxNan(v[1], fabs(r[i]));
                     v[2] = maxNan(v[2], fabs(x[i]));
+                    if (xt) v[3] += (x[i] - xt[i]) * (x[i] - xt[i]);
                 }
-                blockReduceK<3>(v, op, red);
+                blockReduceK<4>(v, op, red);
                 rinf = v[1];
                 xinf = v[2];
+                dist2 = v[3];
                 if (fabs(v[0]) <= a.lnsFea * fabs(dgInit)) break;
                 cur *= v[0] * dgInit > 0.0 ? 1.5 : 0.5;
             }
@@ -218,6 +247,15 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
             if (!isfinite(rinf) || !isfinite(xinf)) {
                 nonfinite = 1;
                 break;
+            }
+            if (xt) {
+                // the displacement target (a measurement hook, as in the
+                // single-instance solve): distance of the accepted iterate
+                terr = sqrt(dist2 / tnorm2);
+                if (terr <= a.targetTol) {
+                    reached = 1;
+                    break;
+                }
             }
             if (rinf <= a.eps || rinf <= a.epsRel * xinf) break;
             if (k >= a.maxIt) break;
@@ -234,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
             {
                 double v[2] = {0.0, 0.0};
                 const int op[2] = {0, 0};
-                for (int i = d0 + tid; i < d1; i += kThreads) {
+                for (int i = d0 + tid; i < d1; i += BT) {
                     const double si = x[i] - a.xp[i], yi = r[i] - a.rp[i];
                     sl[i] = si;
                     yl[i] = yi;
@@ -248,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
             if (ncorr < m) ++ncorr;
             ptr = loc + 1;
             // two-loop recursion d = -H r (LBFGSMat::apply_Hv order)
-            for (int i = d0 + tid; i < d1; i += kThreads) d[i] = -r[i];
+            for (int i = d0 + tid; i < d1; i += BT) d[i] = -r[i];
             __syncthreads();
             int j = ptr % m;
             for (int c = 0; c < ncorr; ++c) {
@@ -257,18 +295,18 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
                 const double* yj = a.Y + j * nt;
                 double v[1] = {0.0};
                 const int op[1] = {0};
-                for (int i = d0 + tid; i < d1; i += kThreads) v[0] += sj[i] * d[i];
+                for (int i = d0 + tid; i < d1; i += BT) v[0] += sj[i] * d[i];
                 blockReduceK<1>(v, op, red);
                 const double aj = v[0] / ys[j];
                 if (tid == 0) alpha[j] = aj;
-                for (int i = d0 + tid; i < d1; i += kThreads) d[i] -= aj * yj[i];
+                for (int i = d0 + tid; i < d1; i += BT) d[i] -= aj * yj[i];
                 __syncthreads();
             }
             if (h0) {
-                blockH0<TF, NEN, NG>(a, inst, I, d, q, U);
-                for (int i = d0 + tid; i < d1; i += kThreads) d[i] = q[i];
+                blockH0<TF, NEN, NG>(a, inst, I, d, q, U, bred);
+                for (int i = d0 + tid; i < d1; i += BT) d[i] = q[i];
             } else {
-                for (int i = d0 + tid; i < d1; i += kThreads) d[i] = d[i] / theta;
+                for (int i = d0 + tid; i < d1; i += BT) d[i] = d[i] / theta;
             }
             __syncthreads();
             for (int c = 0; c < ncorr; ++c) {
@@ -276,10 +314,10 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
                 const double* yj = a.Y + j * nt;
                 double v[1] = {0.0};
                 const int op[1] = {0};
-                for (int i = d0 + tid; i < d1; i += kThreads) v[0] += yj[i] * d[i];
+                for (int i = d0 + tid; i < d1; i += BT) v[0] += yj[i] * d[i];
                 blockReduceK<1>(v, op, red);
                 const double coef = alpha[j] - v[0] / ys[j];
-                for (int i = d0 + tid; i < d1; i += kThreads) d[i] += coef * sj[i];
+                for (int i = d0 + tid; i < d1; i += BT) d[i] += coef * sj[i];
                 __syncthreads();
                 j = (j + 1) % m;
             }
@@ -287,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void kBatchLbfgs(BatchArgs a)
     }
 done:
     if (tid == 0) {
-        double* st = a.stats + 8 * static_cast<size_t>(inst);
+        double* st = a.stats + kBatchStats * static_cast<size_t>(inst);
         st[0] = iters;
         st[1] = rEvals;
         st[2] = lsTrials;
@@ -296,6 +334,8 @@ done:
         st[5] = nonfinite;
         st[6] = switchIt;
         st[7] = a.usePrecond;
+        st[8] = reached;
+        st[9] = terr;
     }
 }
 
@@ -303,18 +343,20 @@ template <typename TF>
 void launchBatchKind(const BatchArgs& a, int count, hipStream_t s)
 {
     const int nen = a.elem.nen, ng = a.elem.ng;
-    if (nen == 4) kBatchLbfgs<4, 1, TF><<<count, kThreads, 0, s>>>(a);
-    else if (nen == 8 && ng == 8) kBatchLbfgs<8, 8, TF><<<count, kThreads, 0, s>>>(a);
-    else if (nen == 8 && ng == 1) kBatchLbfgs<8, 1, TF><<<count, kThreads, 0, s>>>(a);
-    else if (nen == 10 && ng == 4) kBatchLbfgs<10, 4, TF><<<count, kThreads, 0, s>>>(a);
-    else if (nen == 10 && ng == 1) kBatchLbfgs<10, 1, TF><<<count, kThreads, 0, s>>>(a);
-    else if (nen == 20 && ng == 27) kBatchLbfgs<20, 27, TF><<<count, kThreads, 0, s>>>(a);
-    else if (nen == 20 && ng == 8) kBatchLbfgs<20, 8, TF><<<count, kThreads, 0, s>>>(a);
+    if (nen == 4) kBatchLbfgs<4, 1, TF><<<count, BT, 0, s>>>(a);
+    else if (nen == 8 && ng == 8) kBatchLbfgs<8, 8, TF><<<count, BT, 0, s>>>(a);
+    else if (nen == 8 && ng == 1) kBatchLbfgs<8, 1, TF><<<count, BT, 0, s>>>(a);
+    else if (nen == 10 && ng == 4) kBatchLbfgs<10, 4, TF><<<count, BT, 0, s>>>(a);
+    else if (nen == 10 && ng == 1) kBatchLbfgs<10, 1, TF><<<count, BT, 0, s>>>(a);
+    else if (nen == 20 && ng == 27) kBatchLbfgs<20, 27, TF><<<count, BT, 0, s>>>(a);
+    else if (nen == 20 && ng == 8) kBatchLbfgs<20, 8, TF><<<count, BT, 0, s>>>(a);
 }
 
 }  // namespace
 
-void launchBatchSolve(const BatchArgs& a, int count, hipStream_t s)
+#define EFEM_CAT2(a, b) a##b
+#define EFEM_CAT(a, b) EFEM_CAT2(a, b)
+void EFEM_CAT(launchBatchSolveW, EFEM_BATCH_BT)(const BatchArgs& a, int count, hipStream_t s)
 {
     if (count <= 0) return;
     if (a.trsv.mixed)

@@ -138,36 +138,36 @@ __device__ __forceinline__ void tet4Lane(int e, int E, const int* __restrict__ c
 
 
 // ---- general elements: LDS-staged (element, Gauss point) lanes ------------
-template <int NG>
+template <int NG, int BT = kThreads>
 struct Tile {
-    static constexpr int epb = NG >= 4 ? kThreads / NG : 64;  // elements per block
+    static constexpr int epb = NG >= 4 ? BT / NG : BT / 4;  // elements per block of BT lanes
 };
 
-// LDS of one element tile (EPB elements x NG Gauss points over 256 lanes)
-template <typename T, int NEN, int NG, int MODE>
+// LDS of one element tile (EPB elements x NG Gauss points over BT lanes)
+template <typename T, int NEN, int NG, int MODE, int BT = kThreads>
 struct ElemTileLds {
-    static constexpr int EPB = Tile<NG>::epb;
+    static constexpr int EPB = Tile<NG, BT>::epb;
     static constexpr int NV = MODE == MODE_TANGENT ? 2 : 1;
     double sdN[NG * NEN * 3];
     double su[NV][EPB * NEN * 3];
     T sQ[EPB * NG * 9];
 };
 
-// One tile of elements [e0, min(e0 + EPB, eEnd)), called by all 256 lanes of
+// One tile of elements [e0, min(e0 + EPB, eEnd)), called by all BT lanes of
 // a block (contains barriers).  The local-gradient table sdN must be written
 // before the first call (the tile's first barrier publishes it).
-template <typename T, int NEN, int NG, int LAW, int MODE>
+template <typename T, int NEN, int NG, int LAW, int MODE, int BT = kThreads>
 __device__ __forceinline__ void elemTile(int E, int e0, int eEnd, const int* __restrict__ connDof,
                                          const double* __restrict__ ginv, const double* __restrict__ vol,
                                          const double* __restrict__ gw, const double* __restrict__ params, int ppe,
                                          const double* __restrict__ x, const double* __restrict__ v,
-                                         T* __restrict__ slots, ElemTileLds<T, NEN, NG, MODE>& L)
+                                         T* __restrict__ slots, ElemTileLds<T, NEN, NG, MODE, BT>& L)
 {
-    constexpr int EPB = Tile<NG>::epb;
-    constexpr int NV = ElemTileLds<T, NEN, NG, MODE>::NV;
+    constexpr int EPB = Tile<NG, BT>::epb;
+    constexpr int NV = ElemTileLds<T, NEN, NG, MODE, BT>::NV;
     const int tid = threadIdx.x;
     // relative displacements u_a - u_0 (and the direction's v_a - v_0)
-    for (int k = tid; k < EPB * NEN; k += kThreads) {
+    for (int k = tid; k < EPB * NEN; k += BT) {
         const int el = k / NEN, a = k - el * NEN, e = e0 + el;
         if (e < eEnd) {
             const int id = connDof[a * E + e], id0 = connDof[e];
@@ -204,7 +204,7 @@ __device__ __forceinline__ void elemTile(int E, int e0, int eEnd, const int* __r
         }
     }
     __syncthreads();
-    for (int k = tid; k < EPB * NEN; k += kThreads) {
+    for (int k = tid; k < EPB * NEN; k += BT) {
         const int el = k / NEN, a = k - el * NEN, e = e0 + el;
         if (e >= eEnd) continue;
         T f[3] = {T(0), T(0), T(0)};

@@ -1255,6 +1255,15 @@ void launchAxpby(double* out, double a, const double* x, double b, const double*
     if (n > 0) kAxpby<<<blocks(n), kThreads, 0, s>>>(out, a, x, b, y, n);
 }
 
+void launchBatchSolve(const BatchArgs& a, int count, hipStream_t s)
+{
+    switch (tuning().batchThreads) {
+        case 1024: launchBatchSolveW1024(a, count, s); break;
+        case 512: launchBatchSolveW512(a, count, s); break;
+        default: launchBatchSolveW256(a, count, s);
+    }
+}
+
 size_t gatherDotWork(int nFree) { return 4 * (static_cast<size_t>(nFree) * 8 / kThreads + 2); }
 
 template <int DOT>

@@ -15,6 +15,8 @@ struct LaunchTuning {
     int historyBlocks = 256; // maximum blocks of the reductions over the
                              // L-BFGS history (Gram row + first loop, second
                              // loop): 32 .. kReduceBlocks
+    int batchThreads = 256;  // lanes per instance of the persistent batched
+                             // solver: 256, 512, 1024
 };
 void setTuning(const LaunchTuning& t);
 const LaunchTuning& tuning();
@@ -255,11 +257,20 @@ struct BatchArgs {
     double *S, *Y;               // history [m][ntot]
     int ntot, m, maxIt, maxLs, usePrecond, elemF32;
     double eps, epsRel, lnsFea, fp64Switch;
-    double* stats;               // [instance][8]
+    const double* xt;            // merged displacement targets, or null
+    double targetTol;            // stop an instance at ||x - xt|| <= tol ||xt||
+    double* stats;               // [instance][kBatchStats]
 };
 // stats per instance: iterations, r_evals, ls_trials, ||r||_inf, ||x||_inf,
-// non-finite flag, FP64-switch iteration, preconditioned flag
+// non-finite flag, FP64-switch iteration, preconditioned flag, target
+// reached, last ||x - xt|| / ||xt||
+constexpr int kBatchStats = 12;
+// one persistent launch, one workgroup of `width` lanes per instance
+// (tuning().batchThreads: 256, 512 or 1024)
 void launchBatchSolve(const BatchArgs& a, int count, hipStream_t s);
+void launchBatchSolveW256(const BatchArgs& a, int count, hipStream_t s);
+void launchBatchSolveW512(const BatchArgs& a, int count, hipStream_t s);
+void launchBatchSolveW1024(const BatchArgs& a, int count, hipStream_t s);
 
 }  // namespace dev
 }  // namespace efem

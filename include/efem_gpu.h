@@ -268,10 +268,14 @@ int efem_set_option(efem_ctx* ctx, const char* name, double value);
  * of the reductions over the L-BFGS history {32,64,128,256} blocks (one
  * compact two-loop pass pair with m = 8; grows the context's history buffers
  * to m = 8 if smaller).  The choice only changes summation orders, never the
- * formulation.  report[7] (may be NULL) = {lanes, tet4 threads, reduce
- * blocks, residual ms, dot ms, history blocks, history ms}.  The same shapes
- * can be set by efem_set_option("gather_lanes" | "tet4_threads" |
- * "reduce_blocks" | "history_blocks", v). */
+ * formulation.  On a batch context it also times the persistent solver
+ * (40 iterations of every instance from u = 0, with the preconditioner)
+ * at 256, 512 and 1024 lanes per instance and keeps the fastest.
+ * report[9] (may be NULL) = {lanes, tet4 threads, reduce blocks, residual
+ * ms, dot ms, history blocks, history ms, batch lanes (256 when not a
+ * batch), batch ms (0 when not a batch)}.  The same shapes can be set by
+ * efem_set_option("gather_lanes" | "tet4_threads" | "reduce_blocks" |
+ * "history_blocks" | "batch_threads", v). */
 int efem_autotune(efem_ctx* ctx, double* report);
 
 /* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */
@@ -375,9 +379,12 @@ int efem_material_point(int law, const double* params, const double* F,
  * concatenated free-DOF vector (instance i at dof_offsets[i]); each instance
  * is factored on its own, exactly as in a single-instance context.
  * efem_batch_solve runs LBFGSSolver::solve (LBFGS.h:96-174) for every
- * instance in ONE persistent launch, one workgroup per instance, with
- * per-instance L-BFGS state; prm->m <= 16, prm->max_iterations > 0, no
- * x_target.  u: concatenated in/out; stats: [count].  Returns
+ * instance in ONE persistent launch, one workgroup per instance (256, 512
+ * or 1024 lanes: option "batch_threads", picked by efem_autotune on a batch
+ * context), with per-instance L-BFGS state; prm->m <= 16,
+ * prm->max_iterations > 0; prm->x_target (concatenated like u) stops each
+ * instance at its own ||x_i - x_target_i|| <= target_tol ||x_target_i||.
+ * u: concatenated in/out; stats: [count].  Returns
  * EFEM_ERR_NONFINITE (with those instances' u zeroed) if any instance
  * produced NaN/Inf.  efem_solve is refused on a batch context. */
 int efem_batch_create(efem_ctx** out, const efem_problem* problems, int count,

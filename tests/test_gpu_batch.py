@@ -122,3 +122,54 @@ def test_batch_rejects_bad_input():
             E.Context.solve(b)
     with pytest.raises(E.EfemError):
         E.Batch([ms[0], E.Mesh.workload(2, [4, 2, 2])])  # C3D8 with C3D4
+
+
+@pytest.mark.parametrize("width", [256, 512, 1024])
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_batch_widths_match_oracle_and_are_batch_independent(width, prec):
+    # every workgroup width (wider workgroups split each item's columns and
+    # the vector loops over more lanes, so sums are ordered differently) gives
+    # the oracle's equilibrium, and an instance's result does not depend on
+    # the rest of the batch
+    ms = beams([0, 3, 5])
+    kw = dict(epsilon=1e-8, max_iterations=5000, fp64_switch=1e-3 if prec == E.MIXED else 0.0)
+    print(f"batch width {width} prec {prec}", flush=True)
+    with E.Batch(ms, prec) as b:
+        b.set_option("batch_threads", width)
+        u, st = b.solve(**kw)
+        parts = b.split(u)
+    for i, m in enumerate(ms):
+        assert st[i]["rinf"] <= 1e-8, (i, st[i])
+        uo, _ = O.Model(m).solve(epsilon=1e-9, max_iterations=5000)
+        assert rel(parts[i], uo) <= 1e-6, i
+    with E.Batch([ms[1]], prec) as b1:
+        b1.set_option("batch_threads", width)
+        u1, st1 = b1.solve(**kw)
+    assert np.array_equal(u1, parts[1]) and st1[0]["iterations"] == st[1]["iterations"]
+
+
+def test_batch_stops_each_instance_at_its_displacement_target():
+    # per-instance u* (FP64 grid solves to 1e-10): the batch stops every
+    # instance once it is within 1e-6 of its own u*
+    ms = beams(range(5))
+    us = []
+    for m in ms:
+        with E.Context(m) as c:
+            us.append(c.solve(epsilon=1e-10, max_iterations=5000)[0])
+    with E.Batch(ms, E.MIXED) as b:
+        u, st = b.solve(epsilon=0.0, epsilon_rel=0.0, max_iterations=5000, fp64_switch=1e-3,
+                        x_target=np.concatenate(us), target_tol=1e-6)
+        parts = b.split(u)
+    for i in range(len(ms)):
+        assert st[i]["target_reached"] == 1, (i, st[i])
+        err = rel(parts[i], us[i])
+        assert err <= 1e-6 and abs(err - st[i]["target_err"]) <= 1e-9, (i, err, st[i]["target_err"])
+
+
+def test_batch_autotune_picks_a_width():
+    ms = beams(range(8))
+    with E.Batch(ms, E.MIXED) as b:
+        rep = b.autotune()
+        assert rep["batch_threads"] in (256, 512, 1024) and rep["batch_ms"] > 0
+        u, st = b.solve(epsilon=1e-8, max_iterations=5000, fp64_switch=1e-3)
+        assert all(s["rinf"] <= 1e-8 for s in st)
