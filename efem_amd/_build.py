@@ -30,6 +30,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
 
 SOURCES = [
     ("device/kernels.hip", True),
+    ("device/factor.hip", True),
     # the persistent batched solver, once per workgroup width
     ("device/batch.hip", True, "256"),
     ("device/batch.hip", True, "512"),
@@ -42,6 +43,7 @@ SOURCES = [
     ("host/dense.cpp", False),
     ("host/input.cpp", False),
     ("host/vtk.cpp", False),
+    ("host/devplan.cpp", False),
 ]
 
 

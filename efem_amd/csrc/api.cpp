@@ -18,6 +18,7 @@
 #include "host/model.hpp"
 #include "host/sparse.hpp"
 #include "host/dense.hpp"
+#include "host/devplan.hpp"
 #include "host/vtk.hpp"
 
 namespace {
@@ -43,6 +44,13 @@ struct DeviceFactor {
     int* fwdTab = nullptr;
     std::vector<long long> fwdTabOff;
     std::vector<int> fwdK, fwdW;
+    // fused root (single-problem factors whose last level is one supernode
+    // R with no rows beyond its columns): M = L_RR^-T L_RR^-1 (ld mld) and
+    // its 64-row items, run in place of R's forward and backward levels
+    bool fusedRoot = false;
+    void* M = nullptr;
+    int mld = 0;
+    int rootOff = 0, rootCnt = 0;
     // the whole apply captured as one HIP graph over fixed staging buffers
     double *in = nullptr, *out = nullptr, *bp = nullptr;
     hipGraph_t graph = nullptr;
@@ -93,6 +101,7 @@ struct efem_ctx {
     std::vector<efem::PartRange> ranges;
     efem::dev::BatchRange* dRanges = nullptr;
     double* batchStats = nullptr;
+    float* batchSlots32 = nullptr;  // FP32 element slots of a batch (see BatchArgs)
     int *gptr = nullptr, *gslot = nullptr;
     double* fext = nullptr;
     // solver state
@@ -136,6 +145,7 @@ struct efem_ctx {
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
     bool lookahead = true;       // option "lookahead"
     bool fwdTables = true;       // option "fwd_tables"
+    bool fusedRootOpt = true;    // option "fused_root"
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
     // ordering + symbolic analysis of the approximate Jacobian: its pattern
@@ -154,6 +164,17 @@ struct efem_ctx {
     // "device_laplacian"): the host keeps its pattern, the device the
     // contribution map (built at the first factorisation)
     bool devLaplacian = true;
+    // numeric factorisation on the device (option "device_factor"; needs the
+    // device-assembled Laplacian): the plan (built once per pattern), its
+    // device copy and the arenas, kept between refactorisations
+    bool devFactorOpt = true;
+    std::unique_ptr<efem::DevFactorPlan> dplan;
+    int *dfW = nullptr, *dfM = nullptr, *dfRel = nullptr, *dfRelOff = nullptr, *dfPairs = nullptr,
+        *dfAsmSrc = nullptr, *dfFail = nullptr;
+    long long *dfDval = nullptr, *dfUoff = nullptr, *dfAsmDst = nullptr;
+    int4* dfTasks = nullptr;
+    double *dfL = nullptr, *dfX = nullptr, *dfU = nullptr;
+    std::vector<int> dfLaunches;
     bool lapReady = false;
     efem::Csc lapA;
     int *lapPtr = nullptr, *lapSrc = nullptr;
@@ -452,10 +473,15 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out, bool permuted 
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
     if (!permuted) efem::dev::launchPermute(a, v, f.bp, c->stream);
-    for (int l = 0; l < f.nlevels; ++l)
+    // with the fused root, the last level's forward and backward solves are
+    // one launch of z_R = M f_R
+    const int top = f.fusedRoot ? f.nlevels - 1 : f.nlevels;
+    for (int l = 0; l < top; ++l)
         efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], f.fwdWide[l], c->stream,
                                      f.fwdK[l] ? f.fwdTab + f.fwdTabOff[l] : nullptr, f.fwdK[l], f.fwdW[l]);
-    for (int l = f.nlevels - 1; l >= 0; --l)
+    if (f.fusedRoot)
+        efem::dev::launchTrsvRoot(a, f.bp, f.items + 2 * f.rootOff, f.rootCnt, f.M, f.mld, c->stream);
+    for (int l = top - 1; l >= 0; --l)
         efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], f.bwdWide[l], c->stream);
     if (unpermute) efem::dev::launchUnpermute(a, out, c->stream);
 }
@@ -548,7 +574,10 @@ int pinnedStage(efem_ctx* c, size_t bytes, void** p)
     return EFEM_OK;
 }
 
-int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset)
+// devX: the block inverses already on the device (FP64, device layout, the
+// context's X arena, not owned by the factor): F holds the structure only
+int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset,
+                 const double* devX = nullptr)
 {
     using clock = std::chrono::steady_clock;
     const auto u0 = clock::now();
@@ -665,6 +694,50 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
             }
         d.bwdCnt[l] = static_cast<int>(items.size() / 2) - d.bwdOff[l];
     }
+    // fused root: the last level is one supernode R whose rows are its
+    // columns; M = B11^T B11 = L_RR^-T L_RR^-1 on the host (FP64, column
+    // blocks in parallel, the zero upper triangle of B11 skipped)
+    std::vector<double> Mh;
+    int rootS = -1;
+    if (!snOffset && c->fusedRootOpt && F.nlevels >= 1 && byLevel[F.nlevels - 1].size() == 1) {
+        const int s = byLevel[F.nlevels - 1][0];
+        if (mm[s] == w[s] && F.parent[s] < 0) rootS = s;
+    }
+    if (rootS >= 0 && devX) {
+        // M on the device from the root's block (formed below)
+        d.fusedRoot = true;
+        d.mld = efem::dev::ldOf(w[rootS]);
+        d.rootOff = static_cast<int>(items.size() / 2);
+        for (int r0 = 0; r0 < w[rootS]; r0 += 64) {
+            items.push_back(make_int4(r0, w[rootS], w[rootS], F.first[rootS]));
+            items.push_back(make_int4(F.rowPtr[rootS], 0, 0, 0));
+        }
+        d.rootCnt = static_cast<int>(items.size() / 2) - d.rootOff;
+    } else if (rootS >= 0) {
+        const int wr = w[rootS];
+        const int ld = efem::dev::ldOf(wr);
+        const double* B11 = F.B.get() + F.valPtr[rootS];  // column-major, ld wr
+        std::vector<double> Bt(static_cast<size_t>(wr) * wr);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < wr; ++i)
+            for (int k = 0; k < wr; ++k) Bt[i + static_cast<size_t>(k) * wr] = B11[k + static_cast<size_t>(i) * wr];
+        Mh.assign(static_cast<size_t>(ld) * wr, 0.0);
+        const int nblk = (wr + 63) / 64;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int b = 0; b < nblk; ++b) {
+            const int j0 = 64 * b, j1 = std::min(wr, j0 + 64);
+            efem::dense::gemm(Mh.data() + static_cast<size_t>(j0) * ld, ld, Bt.data() + static_cast<size_t>(j0) * wr,
+                              wr, B11 + j0 + static_cast<size_t>(j0) * wr, wr, false, wr, j1 - j0, wr - j0, 1.0);
+        }
+        d.fusedRoot = true;
+        d.mld = ld;
+        d.rootOff = static_cast<int>(items.size() / 2);
+        for (int r0 = 0; r0 < wr; r0 += 64) {
+            items.push_back(make_int4(r0, wr, wr, F.first[rootS]));
+            items.push_back(make_int4(F.rowPtr[rootS], 0, 0, 0));
+        }
+        d.rootCnt = static_cast<int>(items.size() / 2) - d.rootOff;
+    }
     if (snOffset) {
         // items of one instance are contiguous within each level (supernodes
         // are numbered instance by instance and listed in order)
@@ -699,7 +772,40 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         if ((rc = upload(c, &d.pinv, pinv))) return rc;
     }
     if ((rc = upload(c, &d.rows, F.rows))) return rc;
-    {
+    if (devX) {
+        // computed on the device in FP64 in the device layout: used in place
+        // (FP64), or converted (FP32)
+        const long long nb = dval[F.nsuper];
+        c->factorPhases[3] = since(u0);
+        const auto p0 = clock::now();
+        if (fac32(c)) {
+            float* q = nullptr;
+            if ((rc = devAlloc(c, &q, static_cast<size_t>(std::max(nb, 1LL))))) return rc;
+            efem::dev::launchFactorToF32(devX, q, nb, c->stream);
+            HIP_CHECK(c, hipGetLastError());
+            d.B = q;
+        } else {
+            d.B = const_cast<double*>(devX);
+        }
+        if (d.fusedRoot) {
+            const int wr = w[rootS];
+            double* m64 = nullptr;
+            if ((rc = devAlloc(c, &m64, static_cast<size_t>(d.mld) * wr))) return rc;
+            HIP_CHECK(c, hipMemsetAsync(m64, 0, static_cast<size_t>(d.mld) * wr * sizeof(double), c->stream));
+            efem::dev::launchFactorGram(devX + dval[rootS], wr, d.mld, m64, c->stream);
+            if (fac32(c)) {
+                float* m32 = nullptr;
+                if ((rc = devAlloc(c, &m32, static_cast<size_t>(d.mld) * wr))) return rc;
+                efem::dev::launchFactorToF32(m64, m32, static_cast<long long>(d.mld) * wr, c->stream);
+                d.M = m32;
+            } else {
+                d.M = m64;
+            }
+            HIP_CHECK(c, hipGetLastError());
+        }
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        packS = since(p0);
+    } else {
         // block-inverse values, repacked to the device layout (leading
         // dimension ldOf(m), zero padding rows) and converted to FP32 if the
         // preconditioner runs in FP32, by all host threads (packBlocks) into
@@ -722,6 +828,18 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         HIP_CHECK(c, hipStreamSynchronize(c->stream));
         copyS = since(q0);
         d.B = q;
+    }
+    if (d.fusedRoot && !devX) {
+        if (fac32(c)) {
+            std::vector<float> m32(Mh.begin(), Mh.end());
+            float* q32 = nullptr;
+            if ((rc = upload(c, &q32, m32))) return rc;
+            d.M = q32;
+        } else {
+            double* q64 = nullptr;
+            if ((rc = upload(c, &q64, Mh))) return rc;
+            d.M = q64;
+        }
     }
     if ((rc = upload(c, &d.fptr, fptr))) return rc;
     if ((rc = upload(c, &d.fidx, fidx))) return rc;
@@ -749,7 +867,7 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
 
 // the approximate Jacobian's pattern and contribution map (once per
 // context), then its values by the device kernels into lapA.x
-int deviceLaplacian(efem_ctx* c)
+int deviceLaplacian(efem_ctx* c, bool download = true)
 {
     const efem::Model& m = c->model;
     if (!c->lapReady) {
@@ -780,10 +898,108 @@ int deviceLaplacian(efem_ctx* c)
     a.vals = c->lapVals;
     efem::dev::launchLaplacian(a, c->stream);
     HIP_CHECK(c, hipGetLastError());
-    HIP_CHECK(c, hipMemcpyAsync(c->lapA.x.data(), c->lapVals, c->lapA.x.size() * sizeof(double),
-                                hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (download) {
+        HIP_CHECK(c, hipMemcpyAsync(c->lapA.x.data(), c->lapVals, c->lapA.x.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
     return EFEM_OK;
+}
+
+// Numeric factorisation on the device from the symbolic analysis S: the
+// plan and its device copy are built once per pattern; the arenas are
+// zeroed and the plan's launches run on the context stream.  *notSpd = a
+// pivot was not positive (the reference's fallback case).
+int deviceNumeric(efem_ctx* c, const efem::SupernodalSymbolic& S, bool rebuild, bool* notSpd)
+{
+    if (!c->dplan || rebuild) {
+        auto P = std::make_unique<efem::DevFactorPlan>();
+        efem::buildDevFactorPlan(S, *P);
+        // plan arrays and arenas (replacing an earlier plan's)
+        for (int** q : {&c->dfW, &c->dfM, &c->dfRel, &c->dfRelOff, &c->dfPairs, &c->dfAsmSrc, &c->dfFail}) {
+            int rc0 = devFree(c, q);
+            if (rc0) return rc0;
+        }
+        for (long long** q : {&c->dfDval, &c->dfUoff, &c->dfAsmDst}) {
+            int rc0 = devFree(c, q);
+            if (rc0) return rc0;
+        }
+        for (double** q : {&c->dfL, &c->dfX, &c->dfU}) {
+            int rc0 = devFree(c, q);
+            if (rc0) return rc0;
+        }
+        int rc = devFree(c, &c->dfTasks);
+        if (rc) return rc;
+        std::vector<int4> t4(P->tasks.size() / 4);
+        for (size_t i = 0; i < t4.size(); ++i)
+            t4[i] = make_int4(P->tasks[4 * i], P->tasks[4 * i + 1], P->tasks[4 * i + 2], P->tasks[4 * i + 3]);
+        if ((rc = upload(c, &c->dfW, P->w)) || (rc = upload(c, &c->dfM, P->m)) ||
+            (rc = upload(c, &c->dfDval, P->dval)) || (rc = upload(c, &c->dfUoff, P->uOff)) ||
+            (rc = upload(c, &c->dfRel, P->rel)) || (rc = upload(c, &c->dfRelOff, P->relOff)) ||
+            (rc = upload(c, &c->dfPairs, P->pairs)) || (rc = upload(c, &c->dfAsmDst, P->asmDst)) ||
+            (rc = upload(c, &c->dfAsmSrc, P->asmSrc)) || (rc = upload(c, &c->dfTasks, t4)) ||
+            (rc = devAlloc(c, &c->dfFail, 1)) || (rc = devAlloc(c, &c->dfL, static_cast<size_t>(P->panelSize))) ||
+            (rc = devAlloc(c, &c->dfX, static_cast<size_t>(P->panelSize))) ||
+            (rc = devAlloc(c, &c->dfU, static_cast<size_t>(std::max(P->updSize, 1LL)))))
+            return rc;
+        c->dfLaunches.clear();
+        for (const efem::DevLaunch& l : P->launches) {
+            c->dfLaunches.push_back(l.kind);
+            c->dfLaunches.push_back(l.first);
+            c->dfLaunches.push_back(l.count);
+        }
+        c->dplan = std::move(P);
+    }
+    const efem::DevFactorPlan& P = *c->dplan;
+    HIP_CHECK(c, hipMemsetAsync(c->dfL, 0, static_cast<size_t>(P.panelSize) * sizeof(double), c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->dfX, 0, static_cast<size_t>(P.panelSize) * sizeof(double), c->stream));
+    if (P.updSize > 0)
+        HIP_CHECK(c, hipMemsetAsync(c->dfU, 0, static_cast<size_t>(P.updSize) * sizeof(double), c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->dfFail, 0, sizeof(int), c->stream));
+    efem::dev::DevFactorArgs a{};
+    a.L = c->dfL;
+    a.X = c->dfX;
+    a.U = c->dfU;
+    a.A = c->lapVals;
+    a.w = c->dfW;
+    a.m = c->dfM;
+    a.dval = c->dfDval;
+    a.uOff = c->dfUoff;
+    a.rel = c->dfRel;
+    a.relOff = c->dfRelOff;
+    a.pairs = c->dfPairs;
+    a.asmDst = c->dfAsmDst;
+    a.asmSrc = c->dfAsmSrc;
+    a.tasks = c->dfTasks;
+    a.fail = c->dfFail;
+    a.launches = c->dfLaunches.data();
+    a.nlaunch = static_cast<int>(c->dfLaunches.size() / 3);
+    efem::dev::launchDeviceFactor(a, c->stream);
+    HIP_CHECK(c, hipGetLastError());
+    int fail = 0;
+    HIP_CHECK(c, hipMemcpyAsync(&fail, c->dfFail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    *notSpd = fail != 0;
+    return EFEM_OK;
+}
+
+// the factor's structure (no values) from the symbolic analysis, for the
+// device upload of a factor computed on the device
+void factorStructure(const efem::SupernodalSymbolic& S, efem::SupernodalFactor& F)
+{
+    F = efem::SupernodalFactor();
+    F.n = S.n;
+    F.perm = S.perm;
+    F.nsuper = S.nsuper;
+    F.first = S.first;
+    F.rowPtr = S.rowPtr;
+    F.rows = S.rows;
+    F.valPtr = S.valPtr;
+    F.parent = S.parent;
+    F.level = S.level;
+    F.nlevels = S.nlevels;
+    F.nnzL = S.nnzL;
+    F.flops = S.flops;
 }
 
 int factorInternal(efem_ctx* c, int* failed)
@@ -807,6 +1023,47 @@ int factorInternal(efem_ctx* c, int* failed)
         const bool dev = c->devLaplacian && static_cast<long long>(m.nElems) * m.nen * m.nen < (1LL << 31);
         if (c->reuseSymbolic && !c->symbolic) c->symbolic.reset(new efem::SupernodalSymbolic());
         efem::SupernodalSymbolic* sym = c->reuseSymbolic ? c->symbolic.get() : nullptr;
+        if (dev && c->devFactorOpt) {
+            // assembly, numeric factorisation and block inverses on the
+            // device; ordering and symbolic analysis on the host (once per
+            // pattern unless reuse_symbolic = 0)
+            int rc0 = deviceLaplacian(c, false);
+            if (rc0) return rc0;
+            efem::SupernodalSymbolic fresh;
+            efem::SupernodalSymbolic& S = sym ? *sym : fresh;
+            const bool rebuild = !sym || S.n == 0;
+            double symS = 0.0;
+            if (S.n == 0) {
+                const auto ts = std::chrono::steady_clock::now();
+                const std::vector<int> perm = efem::nestedDissection(c->lapA, efem::freeCoords(m));
+                if (!efem::supernodalSymbolic(c->lapA, perm, S, msg, c->denseSubtree)) {
+                    S = efem::SupernodalSymbolic();
+                    c->err = msg;
+                    return EFEM_OK;
+                }
+                symS = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+            }
+            const auto tn = std::chrono::steady_clock::now();
+            bool notSpd = false;
+            if ((rc0 = deviceNumeric(c, S, rebuild, &notSpd))) return rc0;
+            const double numS = std::chrono::duration<double>(std::chrono::steady_clock::now() - tn).count();
+            if (notSpd) {
+                c->err = "approximate Jacobian is not positive definite";
+                c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                return EFEM_OK;  // the reference falls back to I/theta
+            }
+            factorStructure(S, F);
+            c->factorPhases[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            c->factorPhases[1] = symS;
+            c->factorPhases[2] = numS;
+            int rc = uploadFactor(c, F, nullptr, c->dfX);
+            if (rc) return rc;
+            c->factorFailed = false;
+            if (failed) *failed = 0;
+            c->factorSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            c->factorPhases[7] = c->factorSeconds;
+            return EFEM_OK;
+        }
         if (dev) {
             int rc0 = deviceLaplacian(c);
             if (rc0) return rc0;
@@ -1713,6 +1970,20 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         releaseFactor(c);
         return EFEM_OK;
     }
+    if (k == "device_factor") {
+        // numeric factorisation on the device (1) or the host (0); takes
+        // effect at the next factorisation
+        c->devFactorOpt = value != 0.0;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
+    if (k == "fused_root") {
+        // the last supernode's forward and backward solves as one launch
+        // z_R = M f_R; takes effect at the next factorisation
+        c->fusedRootOpt = value != 0.0;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
     if (k == "fwd_tables") {
         // per-item extend-add tables for the forward sweep; takes effect at
         // the next factorisation
@@ -2140,7 +2411,8 @@ int efem_batch_create(efem_ctx** out, const efem_problem* problems, int count, i
         br[i] = {r.e0, r.e1, r.f0, r.f1, r.n0, r.n1};
     }
     if ((rc = upload(cp, &cp->dRanges, br)) || (!fpress.empty() && (rc = upload(cp, &cp->fpress, fpress))) ||
-        (rc = devAlloc(cp, &cp->batchStats, efem::dev::kBatchStats * static_cast<size_t>(count)))) {
+        (rc = devAlloc(cp, &cp->batchStats, efem::dev::kBatchStats * static_cast<size_t>(count))) ||
+        (rc = devAlloc(cp, &cp->batchSlots32, static_cast<size_t>(cp->model.nElems) * cp->model.nen * 3))) {
         g_error = cp->err;
         efem_destroy(c.release());
         return rc;
@@ -2229,6 +2501,7 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
     a.epsRel = prm->epsilon_rel;
     a.lnsFea = prm->lns_fea;
     a.fp64Switch = elem32(c) ? prm->fp64_switch : 0.0;
+    a.slots32 = c->batchSlots32;
     a.xt = prm->x_target ? c->xtarget : nullptr;
     a.targetTol = prm->target_tol;
     a.stats = c->batchStats;

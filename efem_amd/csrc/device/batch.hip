@@ -61,7 +61,7 @@ __device__ __forceinline__ void blockElements(const BatchArgs& a, const BatchRan
                                               ElemTileLds<T, NEN, NG, MODE_FORCE, BT>& L)
 {
     const ElemArgs& ea = a.elem;
-    T* slots = static_cast<T*>(ea.slots);
+    T* slots = static_cast<T*>(sizeof(T) == 4 ? a.slots32 : ea.slots);
     const double* ginv = static_cast<const double*>(ea.ginv);
     if (NEN == 4) {
         for (int e = I.e0 + threadIdx.x; e < I.e1; e += BT)
@@ -91,6 +91,7 @@ __device__ void blockResidual(const BatchArgs& a, const BatchRange& I, BatchLds<
     __syncthreads();
     GatherArgs g = a.gather;
     g.mixed = sizeof(T) == 4;
+    if (sizeof(T) == 4) g.eslots = a.slots32;
     for (int i = I.n0 + threadIdx.x; i < I.n1; i += BT) gatherNode<T>(g, i);
     __syncthreads();
 }

@@ -599,6 +599,82 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     out[2] = static_cast<T>(f2) - acc2;
 }
 
+// One item of the fused root: 64 rows [r0, r0 + 64) of the last supernode
+// R (a root whose rows are its w columns) solved in one step from its front
+// f_R = its rows of the permuted right-hand side plus its children's update
+// vectors (the forward sweep's extend-add, in the same order):
+// z_R = M f_R, M = L_RR^-T L_RR^-1 = B11^T B11 (w x w, column-major, ld ldm),
+// which is the forward and the backward solve of R together.  f_R is staged
+// in LDS kMaxW columns at a time, so R may be wider than one staging
+// buffer.  Descriptor d0 = {r0, w, w, j0}, d1 = {front row offset, 0, 0, 0}.
+// Contains barriers; every lane must call it.
+template <typename T, int WAVES = 16>
+__device__ __forceinline__ void trsvRootItem(const TrsvArgs& a, const double* __restrict__ bp, int4 d0, int4 d1,
+                                             const T* __restrict__ M, int ldm, TrsvFwdLds<T, WAVES>& L)
+{
+    constexpr int NT = 64 * WAVES;
+    constexpr int NBT = 8;
+    const int r0 = d0.x, w = d0.y, j0 = d0.w, fr0 = d1.x;
+    const T* u = static_cast<const T*>(a.t);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = r0 + lane;
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    for (int c0 = 0; c0 < w; c0 += kMaxW) {
+        const int cw = min(kMaxW, w - c0);
+        for (int k = threadIdx.x; k < cw; k += NT) {
+            const double* bk = bp + 3 * static_cast<size_t>(j0 + c0 + k);
+            double v0 = bk[0], v1 = bk[1], v2 = bk[2];
+            const int p1 = a.fptr[fr0 + c0 + k + 1];
+            for (int p = a.fptr[fr0 + c0 + k]; p < p1; ++p) {
+                const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
+                v0 += static_cast<double>(uv[0]);
+                v1 += static_cast<double>(uv[1]);
+                v2 += static_cast<double>(uv[2]);
+            }
+            L.sb[3 * k] = static_cast<T>(v0);
+            L.sb[3 * k + 1] = static_cast<T>(v1);
+            L.sb[3 * k + 2] = static_cast<T>(v2);
+        }
+        __syncthreads();
+        const int kc = (cw + WAVES - 1) / WAVES;
+        const int kb = wave * kc;
+        const int ke = r < w ? min(cw, kb + kc) : kb;
+        const T* Mc = M + r + static_cast<size_t>(c0) * ldm;
+        for (int k = kb; k < ke; k += NBT) {
+            T b[NBT];
+#pragma unroll
+            for (int q = 0; q < NBT; ++q) b[q] = k + q < ke ? Mc[static_cast<size_t>(k + q) * ldm] : T(0);
+#pragma unroll
+            for (int q = 0; q < NBT; ++q) {
+                if (k + q < ke) {
+                    acc0 += b[q] * L.sb[3 * (k + q)];
+                    acc1 += b[q] * L.sb[3 * (k + q) + 1];
+                    acc2 += b[q] * L.sb[3 * (k + q) + 2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    L.red[wave][lane][0] = acc0;
+    L.red[wave][lane][1] = acc1;
+    L.red[wave][lane][2] = acc2;
+    __syncthreads();
+    if (wave != 0 || r >= w) return;
+    acc0 = L.red[0][lane][0];
+    acc1 = L.red[0][lane][1];
+    acc2 = L.red[0][lane][2];
+#pragma unroll
+    for (int q = 1; q < WAVES; ++q) {
+        acc0 += L.red[q][lane][0];
+        acc1 += L.red[q][lane][1];
+        acc2 += L.red[q][lane][2];
+    }
+    T* zo = static_cast<T*>(a.z) + 3 * static_cast<size_t>(j0 + r);
+    zo[0] = acc0;
+    zo[1] = acc1;
+    zo[2] = acc2;
+}
+
 // One backward item: four columns k0 .. k0 + 3 of supernode s.  With
 // WAVES = 4 one wave per column (no barrier); with WAVES = 16 four waves
 // per column split its rows and are summed in order through LDS (contains a

@@ -1051,6 +1051,15 @@ __global__ __launch_bounds__(64 * WAVES) void kTrsvFwdTab(TrsvArgs a, const doub
     trsvFwdItem<T, WAVES, KT>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L, ft);
 }
 
+// the fused root: z_R = M f_R, one 64-row item per block
+template <typename T>
+__global__ __launch_bounds__(1024) void kTrsvRoot(TrsvArgs a, const double* __restrict__ bp,
+                                                  const int4* __restrict__ items, const T* __restrict__ M, int ldm)
+{
+    __shared__ TrsvFwdLds<T, 16> L;
+    trsvRootItem<T, 16>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], M, ldm, L);
+}
+
 template <typename T, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void kTrsvBwd(TrsvArgs a, const int4* __restrict__ items)
 {
@@ -1462,6 +1471,16 @@ void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, i
         if (wide) kTrsvFwd<double, 16><<<nitems, 1024, 0, s>>>(a, bp, items);
         else kTrsvFwd<double, 4><<<nitems, 256, 0, s>>>(a, bp, items);
     }
+}
+
+void launchTrsvRoot(const TrsvArgs& a, const double* bp, const int4* items, int nitems, const void* M, int ldm,
+                    hipStream_t s)
+{
+    if (nitems <= 0) return;
+    if (a.mixed)
+        kTrsvRoot<float><<<nitems, 1024, 0, s>>>(a, bp, items, static_cast<const float*>(M), ldm);
+    else
+        kTrsvRoot<double><<<nitems, 1024, 0, s>>>(a, bp, items, static_cast<const double*>(M), ldm);
 }
 
 void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s)

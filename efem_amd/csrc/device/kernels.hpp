@@ -1,6 +1,7 @@
 // Device kernels of the solve loop (gfx950, wave64).  Launch wrappers only;
 // the kernels live in kernels.hip.
 #pragma once
+#include "../limits.hpp"
 #include <hip/hip_runtime.h>
 
 namespace efem {
@@ -27,8 +28,8 @@ const LaunchTuning& tuning();
 // every column starts on a 128-byte line in FP32 (256 bytes in FP64), so a
 // forward item's 64-row segment of a column is two whole lines and a
 // backward column sweep starts on a line
-constexpr int kLdAlign = 32;
-__host__ __device__ constexpr int ldOf(int m) { return (m + kLdAlign - 1) / kLdAlign * kLdAlign; }
+constexpr int kLdAlign = kBlockAlign;
+__host__ __device__ constexpr int ldOf(int m) { return blockLd(m); }
 
 // element internal force into per-(element, slot) buffers
 struct ElemArgs {
@@ -233,6 +234,34 @@ void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, i
 // column split its rows, for levels of tall supernodes; wide = kBwdNarrow:
 // sixteen columns per item, k0 = 0 mod 16, for levels of many short ones)
 constexpr int kBwdNarrow = 2;
+// the fused root (the last supernode R, whose rows are its columns):
+// z_R = M f_R with M = L_RR^-T L_RR^-1 (w x w column-major, ld ldm, FP32 or
+// FP64 as the factor), f_R its front (permuted right-hand side + children's
+// update vectors): R's forward and backward solve in one launch of 64-row
+// items (descriptors {r0, w, w, j0}, {front row offset, 0, 0, 0})
+void launchTrsvRoot(const TrsvArgs& a, const double* bp, const int4* items, int nitems, const void* M, int ldm,
+                    hipStream_t s);
+// device numeric factorisation (factor.hip): runs the plan's launch list
+// (kind, first task, count) on stream s; the L, X and U arenas must be zero
+// on entry; a pivot that is not positive sets *fail
+struct DevFactorArgs {
+    double *L, *X, *U;
+    const double* A;
+    const int *w, *m;
+    const long long *dval, *uOff;
+    const int *rel, *relOff, *pairs;
+    const long long* asmDst;
+    const int* asmSrc;
+    const int4* tasks;
+    int* fail;
+    const int* launches;  // host array, 3 ints per launch
+    int nlaunch;
+};
+void launchDeviceFactor(const DevFactorArgs& d, hipStream_t s);
+// B (float) = X (double), n values
+void launchFactorToF32(const double* X, void* B, long long n, hipStream_t s);
+// M = Xr^T Xr for a w x w lower-triangular block (column-major, ld), M with ld
+void launchFactorGram(const double* Xr, int wr, int ld, double* M, hipStream_t s);
 void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s);
 // out[3 perm[j] + c] = z[j][c]
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s);
@@ -257,6 +286,10 @@ struct BatchArgs {
     double *S, *Y;               // history [m][ntot]
     int ntot, m, maxIt, maxLs, usePrecond, elemF32;
     double eps, epsRel, lnsFea, fp64Switch;
+    // FP32 element slots of the batch: instances switch from FP32 to FP64
+    // elements one by one, so the two precisions keep separate slot
+    // buffers (elem.slots / gather.eslots are the FP64 ones)
+    void* slots32;
     const double* xt;            // merged displacement targets, or null
     double targetTol;            // stop an instance at ||x - xt|| <= tol ||xt||
     double* stats;               // [instance][kBatchStats]

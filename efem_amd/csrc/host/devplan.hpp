@@ -1,0 +1,61 @@
+// Plan of the device numeric factorisation: everything the device
+// multifrontal LL^T needs that depends on the approximate Jacobian's pattern
+// only (so a context builds it once, beside the reused symbolic analysis).
+//
+// The device factors the same supernodal structure as the host
+// (SupernodalSymbolic): per supernode s (w columns, m rows, nr = m - w) the
+// panel [L11; L21] lives in the L arena at dval[s] (column-major, leading
+// dimension ldOf(m), the device B layout), its update matrix U_s (nr x nr,
+// lower, ld nr) in the U arena at uOff[s], and the block inverse
+// B_s = [L11^-1; L21 L11^-1] is written to the X arena at the same dval[s]
+// (X is the FP64 B the triangular solves read).  Levels run bottom-up; each
+// is a fixed sequence of batched launches over 64 x 64 tiles:
+//   assemble A's lower entries -> extend-add children's update matrices ->
+//   per 64-column step k: Cholesky of the diagonal tile, triangular solve
+//   of the tiles below, update of the panel's remaining columns ->
+//   U_s -= L21 L21^T -> per 64-column step J (last to first): the block
+//   inverse X = [I; L21] L11^-1 by a right-side triangular solve.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "sparse.hpp"
+
+namespace efem {
+
+// one launch of the device factorisation: kind, first task, task count
+struct DevLaunch {
+    int kind;
+    int first;
+    int count;
+};
+
+enum DevLaunchKind {
+    DF_ASM = 0,     // tasks = A entries (asmDst, asmSrc)
+    DF_EXTADD = 1,  // tasks {s, target front column, pair begin, pair end}
+    DF_POTRF = 2,   // tasks {s, k, 0, 0}
+    DF_TRSM = 3,    // tasks {s, k, first row, 0}
+    DF_UPDATE = 4,  // tasks {s, k, row tile, column tile}
+    DF_SYRK = 5,    // tasks {s, row tile, column tile, 0} of U_s
+    DF_BSOLVE = 6,  // tasks {s, J, row tile, 0}
+};
+
+struct DevFactorPlan {
+    int n = 0, nsuper = 0;
+    long long panelSize = 0;  // doubles of the L and X arenas (the device B layout)
+    long long updSize = 0;    // doubles of the U arena
+    std::vector<int> w, m;               // per supernode
+    std::vector<long long> dval, uOff;   // per supernode (uOff -1: no update matrix)
+    std::vector<long long> asmDst;       // L arena position of each A lower entry
+    std::vector<int> asmSrc;             // its index in A's values
+    std::vector<int> pairs;              // extend-add pairs {child, child column} (2 ints)
+    std::vector<int> rel;                // child update rows -> parent front rows
+    std::vector<int> relOff;             // per supernode offset into rel (-1: none)
+    std::vector<int> tasks;              // 4 ints per task (asm tasks are not listed)
+    std::vector<DevLaunch> launches;     // in execution order
+};
+
+// builds the plan from the symbolic analysis (host, once per pattern)
+void buildDevFactorPlan(const SupernodalSymbolic& S, DevFactorPlan& P);
+
+}  // namespace efem

@@ -7,4 +7,9 @@ namespace efem {
 // right-hand sides (3 x width values) in LDS
 constexpr int kMaxSupernodeWidth = 768;
 
+// device layout of the factor blocks: an m-row column occupies
+// blockLd(m) = m rounded up to kBlockAlign rows (zero padding rows)
+constexpr int kBlockAlign = 32;
+constexpr int blockLd(int m) { return (m + kBlockAlign - 1) / kBlockAlign * kBlockAlign; }
+
 }  // namespace efem

@@ -246,6 +246,19 @@ const char* efem_host_dense_kernel(void);
  *                       children's update values through per-item tables
  *                       (same sums, same order); 0: through the row lists;
  *                       releases the factor
+ *   "device_factor"     1 (default): with "device_laplacian", the numeric
+ *                       factorisation (assembly of the fronts, dense
+ *                       partial Cholesky, update matrices, block inverses)
+ *                       runs on the device from the host's ordering and
+ *                       symbolic analysis (the same factor up to rounding);
+ *                       0: on the host; releases the factor
+ *   "fused_root"        1 (default): when the elimination tree's last level
+ *                       is one supernode R whose rows are its columns, its
+ *                       forward and backward solves run as one launch
+ *                       z_R = M f_R with M = L_RR^-T L_RR^-1 formed at the
+ *                       factorisation (the same K^-1 up to rounding, two
+ *                       launches fewer per apply); 0: level by level;
+ *                       releases the factor
  *   "wide_fwd_items", "wide_bwd_items"  (defaults 500, 1000): levels of wide
  *                       (tall) supernodes with at most this many forward
  *                       (backward) items use 16-wave items; release the factor

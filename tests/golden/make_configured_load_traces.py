@@ -55,7 +55,9 @@ def run(which, opts, iterations):
         "r0_inf": r0, "iterations": st["iterations"], "r_evals": st["r_evals"], "status": status,
         "final_rinf": st["rinf"], "max_rinf": float(np.nanmax(tr[:, 0])), "wall_s": round(wall, 1),
         "factor_s": round(st["factor_seconds"], 2),
-        "trace": [[float(a), float(b), int(c)] for a, b, c in tr],
+        # the first 200 iterations in full, the last 20 as well
+        "trace": [[float(a), float(b), int(c)] for a, b, c in tr[:200]],
+        "trace_tail": [[float(a), float(b), int(c)] for a, b, c in tr[-20:]],
     }
 
 

@@ -59,6 +59,10 @@ def test_batch_apply_h0_matches_single(prec, rng):
         hv = b.split(b.apply_h0(v))
         for i, (m, vi) in enumerate(zip(ms, b.split(v))):
             with E.Context(m, prec) as c:
+                # a batch factors its instances on the host and sweeps
+                # every level, root included
+                c.set_option("device_factor", 0)
+                c.set_option("fused_root", 0)
                 assert c.factor()
                 assert np.array_equal(hv[i], c.apply_h0(vi)), i
 
@@ -173,3 +177,21 @@ def test_batch_autotune_picks_a_width():
         assert rep["batch_threads"] in (256, 512, 1024) and rep["batch_ms"] > 0
         u, st = b.solve(epsilon=1e-8, max_iterations=5000, fp64_switch=1e-3)
         assert all(s["rinf"] <= 1e-8 for s in st)
+
+
+def test_batch_mixed_instances_switching_at_different_times():
+    # instances of very different lengths switch from FP32 to FP64 elements
+    # at different iterations; their element force slots must not overlap
+    # (each precision has its own buffer): every instance equals the same
+    # instance solved in a batch of its own, bit for bit
+    ms = beams([0, 1, 2, 3, 4, 5, 6, 7], shorten=False)
+    kw = dict(epsilon=1e-9, max_iterations=5000, fp64_switch=1e-2)
+    with E.Batch(ms, E.MIXED) as b:
+        u, st = b.solve(**kw)
+        parts = b.split(u)
+    sw = sorted(s["fp64_switch_iteration"] for s in st)
+    assert sw[0] != sw[-1]
+    for i in (0, 3, 7):
+        with E.Batch([ms[i]], E.MIXED) as b1:
+            ui, si = b1.solve(**kw)
+        assert np.array_equal(ui, parts[i]) and si[0]["iterations"] == st[i]["iterations"], i

@@ -288,6 +288,7 @@ def test_device_laplacian_matches_host_assembly(which, opts, rng):
     b = rng.standard_normal(m.ndof)
     out = {}
     with E.Context(m, E.FP64) as ctx:
+        ctx.set_option("device_factor", 0)  # host numeric factorisation for both
         for dev in (0, 1):
             ctx.set_option("device_laplacian", dev)
             assert ctx.factor()
@@ -370,6 +371,77 @@ def test_apply_h0_dense_subtree_sizes(prec, tol, rng):
         with pytest.raises(E.EfemError):
             ctx.set_option("dense_subtree", 0)
     assert nnz == sorted(nnz) and nnz[-1] > nnz[0]
+
+
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+@pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8]), (1, None), (2, None)])
+def test_device_factorisation_matches_host(prec, which, opts, rng):
+    # the numeric factorisation on the device (fronts, tiled partial
+    # Cholesky, update matrices, block inverses) against the host's:
+    # the same factor up to rounding, a direct solve of K, and a
+    # refactorisation after new per-element parameters
+    m = E.Mesh.workload(which, opts)
+    print(f"device factor workload {which} {opts} ndof {m.ndof}", flush=True)
+    P = np.atleast_2d(m.params)
+    P = np.repeat(P, m.n_elems, axis=0) if P.shape[0] == 1 else P
+    m = E.Mesh(m.X, m.conn, m.law, P.copy(), m.fixed, m.faces, m.pressure, m.reduced)
+    b = rng.standard_normal(m.ndof)
+    out = {}
+    with E.Context(m, prec) as ctx:
+        for dev in (0, 1):
+            ctx.set_option("device_factor", dev)
+            assert ctx.factor()
+            out[dev] = ctx.apply_h0(b)
+            ph = ctx.factor_phases()
+            print(f"  device_factor={dev}: numeric {1e3 * ph['numeric']:.1f} ms, total {1e3 * ph['total']:.1f} ms",
+                  flush=True)
+        assert rel(out[1], out[0]) <= (1e-12 if prec == E.FP64 else 1e-6)
+        if m.ndof <= 20000:
+            K = O.Model(m).K(scalar=False)
+            x_ref = spla.spsolve(K.tocsc(), b)
+            assert rel(out[1], x_ref) <= (1e-10 if prec == E.FP64 else 1e-4)
+        # a refactorisation with new parameters (same plan, new values)
+        P2 = P * rng.uniform(0.5, 2.0, (m.n_elems, 1))
+        ctx.set_params(P2)
+        assert ctx.factor()
+        x1 = ctx.apply_h0(b)
+        ctx.set_option("device_factor", 0)
+        assert ctx.factor()
+        assert rel(x1, ctx.apply_h0(b)) <= (1e-12 if prec == E.FP64 else 1e-6)
+
+
+def test_device_factorisation_reports_indefinite_matrices():
+    # elasticity_coeff < 0: K is negative definite; the device's first pivot
+    # is not positive -> the reference's fallback (no factor, H0 = I/theta)
+    m0 = E.Mesh.workload(1, [10, 2, 2, 0, 1.0])
+    m = E.Mesh(m0.X, m0.conn, m0.law, np.array([-2000.0, 0.0, 0.0, 1e-2]), m0.fixed, m0.faces, m0.pressure,
+               m0.reduced)
+    with E.Context(m, E.FP64) as ctx:
+        ctx.set_option("device_factor", 1)
+        assert not ctx.factor()
+        assert ctx.factor_nnz == 0
+
+
+@pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
+@pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8]), (1, None)])
+def test_apply_h0_fused_root(prec, tol, which, opts, rng):
+    # the last supernode solved as z_R = M f_R (M = L_RR^-T L_RR^-1) in
+    # place of its forward and backward levels: the same K^-1 to rounding,
+    # in plain launches and in the captured graph
+    m = E.Mesh.workload(which, opts)
+    K = O.Model(m).K(scalar=False)
+    b = rng.standard_normal(m.ndof)
+    x_ref = spla.spsolve(K.tocsc(), b)
+    out = {}
+    with E.Context(m, prec) as ctx:
+        for fused in (0, 1):
+            ctx.set_option("fused_root", fused)
+            assert ctx.factor()
+            out[fused] = ctx.apply_h0(b)
+            assert rel(out[fused], x_ref) <= tol, fused
+        ctx.set_option("h0_graph", 1)
+        assert np.array_equal(ctx.apply_h0(b), out[1])
+    assert rel(out[1], out[0]) <= (1e-12 if prec == E.FP64 else 1e-5)
 
 
 @pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
