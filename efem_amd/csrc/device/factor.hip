@@ -18,7 +18,8 @@ namespace efem {
 namespace dev {
 namespace {
 
-constexpr int TB = 64;  // tile edge
+constexpr int TB = 64;           // tile edge
+constexpr int kTrsmRows = 256;   // rows of one triangular-solve task (the plan's)
 
 struct FacArgs {
     double* L;               // panels [L11; L21]
@@ -66,80 +67,131 @@ __global__ __launch_bounds__(256) void kFacExtAdd(FacArgs a, int first)
     }
 }
 
-// Cholesky of the diagonal tile k of the panel (kw x kw), in LDS
-__global__ __launch_bounds__(256) void kFacPotrf(FacArgs a, int first)
+// one lane's copy of a value held by another lane of the wave
+__device__ __forceinline__ double readLane(double v, int lane)
 {
-    __shared__ double T[TB][TB + 1];
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u), lane));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// PIN(v) marks a register value as updated where it stands: placed after
+// each column of a right-looking elimination it keeps the compiler from
+// sinking the column's updates to their first use (it would then hold every
+// loaded factor entry live until there, and spill); emits no instruction
+#define PIN(v) asm volatile("" : "+v"(v))
+
+// Cholesky of the diagonal tile k of the panel (kw x kw): one wave, lane =
+// row, right-looking in registers; each finished column is broadcast
+// through LDS for the trailing update
+__global__ __launch_bounds__(TB) void kFacPotrf(FacArgs a, int first)
+{
+    __shared__ double Lc[TB][TB];
     const int4 t4 = a.tasks[first + blockIdx.x];
     const int s = t4.x, k = t4.y;
     const int ws = a.w[s], ld = blockLd(a.m[s]);
     const int c0 = TB * k, kw = min(TB, ws - c0);
     double* P = a.L + a.dval[s] + c0 + static_cast<size_t>(c0) * ld;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < TB * TB; e += 256) {
-        const int i = e % TB, j = e / TB;
-        T[i][j] = (i < kw && j < kw && i >= j) ? P[i + static_cast<size_t>(j) * ld] : 0.0;
+    const int r = threadIdx.x;
+    // the tile's lower part (zeros elsewhere) staged through LDS: coalesced
+    // column loads, then each lane takes its row
+#pragma unroll 16
+    for (int j = 0; j < TB; ++j) {
+        const double e = P[min(r, kw - 1) + static_cast<size_t>(min(j, kw - 1)) * ld];
+        Lc[r][j] = (r < kw && j <= r) ? e : 0.0;
     }
     __syncthreads();
-    for (int c = 0; c < kw; ++c) {
-        if (tid == 0) {
-            const double d = T[c][c];
-            if (!(d > 0.0)) {
-                atomicOr(a.fail, 1);
-                T[c][c] = 1.0;
-            } else {
-                T[c][c] = sqrt(d);
-            }
+    double v[TB];
+#pragma unroll
+    for (int j = 0; j < TB; ++j) v[j] = Lc[r][j];
+    __syncthreads();
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < TB; ++c) {
+        const bool pad = c >= kw;
+        const double d = readLane(v[c], c);  // the updated diagonal entry
+        const bool notPos = !pad && !(d > 0.0);
+        bad = bad || notPos;
+        // sqrt(d) and its reciprocal from the hardware 1/sqrt estimate by
+        // Newton steps (a shorter dependent chain than sqrt and a division:
+        // the column loop is latency bound)
+        const double dd = pad || notPos ? 1.0 : d;
+        double y = __builtin_amdgcn_rsq(dd);
+        y = fma(0.5 * y, fma(-dd * y, y, 1.0), y);
+        double l = dd * y;
+        l = fma(0.5 * y, fma(-l, l, dd), l);
+        y = fma(y, fma(-l, y, 1.0), y);
+        const double inv = pad ? 0.0 : y;
+        // lanes below the diagonal get L(r, c); lanes above only touch their
+        // own upper part, which nothing reads
+        const double lc = r == c ? (pad ? 0.0 : l) : v[c] * inv;
+        v[c] = lc;
+        Lc[c][r] = lc;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the trailing update, scheduled in chunks of 16 (bounds the LDS
+        // reads in flight)
+#pragma unroll
+        for (int q = c + 1; q < TB; ++q) {
+            v[q] -= lc * Lc[c][q];
+            if ((q & 15) == 15) __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads();
-        const double dc = T[c][c];
-        for (int r = c + 1 + tid; r < kw; r += 256) T[r][c] /= dc;
-        __syncthreads();
-        // trailing update of the tile's lower part
-        const int nt = kw - c - 1;
-        for (int e = tid; e < nt * nt; e += 256) {
-            const int r = c + 1 + e % nt, q = c + 1 + e / nt;
-            if (r >= q) T[r][q] -= T[r][c] * T[q][c];
-        }
-        __syncthreads();
+#pragma unroll
+        for (int q = c + 1; q < TB; ++q) PIN(v[q]);
     }
-    for (int e = tid; e < TB * TB; e += 256) {
-        const int i = e % TB, j = e / TB;
-        if (i < kw && j < kw && i >= j) P[i + static_cast<size_t>(j) * ld] = T[i][j];
-    }
+    if (r == 0 && bad) atomicOr(a.fail, 1);
+    if (r < kw)
+#pragma unroll
+        for (int j = 0; j < TB; ++j)
+            if (j <= r) P[r + static_cast<size_t>(j) * ld] = v[j];
 }
 
-// rows [r0, r0 + 64) of the panel's column tile k <- (rows) L_kk^-T
-__global__ __launch_bounds__(64) void kFacTrsm(FacArgs a, int first)
+// rows [r0, r0 + kTrsmRows) of the panel's column tile k (all below the
+// diagonal tile) <- (rows) L_kk^-T: thread = row, right-looking forward
+// substitution against L_kk staged in LDS column by column
+__global__ __launch_bounds__(kTrsmRows) void kFacTrsm(FacArgs a, int first)
 {
-    __shared__ double Lk[TB][TB + 1];
+    __shared__ double Lc[TB][TB];  // Lc[c][j] = L_kk(j, c)
+    __shared__ double invd[TB];    // 1 / L_kk(c, c) (0 for padding columns)
     const int4 t4 = a.tasks[first + blockIdx.x];
     const int s = t4.x, k = t4.y, r0 = t4.z;
     const int ws = a.w[s], ms = a.m[s], ld = blockLd(ms);
     const int c0 = TB * k, kw = min(TB, ws - c0);
     double* P = a.L + a.dval[s];
-    const int lane = threadIdx.x;
-    for (int j = 0; j < TB; ++j) Lk[lane][j] = (lane < kw && j < kw && lane >= j) ?
-                                                   P[c0 + lane + static_cast<size_t>(c0 + j) * ld] : 0.0;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < TB * TB; e += kTrsmRows) {
+        const int j = e % TB, c = e / TB;
+        Lc[c][j] = (j < kw && c < kw && j > c) ? P[c0 + j + static_cast<size_t>(c0 + c) * ld] : 0.0;
+        if (j == c) invd[c] = c < kw ? 1.0 / P[c0 + c + static_cast<size_t>(c0 + c) * ld] : 0.0;
+    }
     __syncthreads();
-    const int r = r0 + lane;
-    if (r >= ms) return;
-    double x[TB];
-#pragma unroll
-    for (int j = 0; j < TB; ++j) x[j] = j < kw ? P[r + static_cast<size_t>(c0 + j) * ld] : 0.0;
-    // forward substitution: x_j = (x_j - sum_{c<j} x_c L_jc) / L_jj
+    const int gr = r0 + tid;
+    if (gr >= ms) return;
+    double b[TB];
 #pragma unroll
     for (int j = 0; j < TB; ++j) {
-        if (j < kw) {
-            double v = x[j];
-#pragma unroll
-            for (int c = 0; c < j; ++c) v -= x[c] * Lk[j][c];
-            x[j] = v / Lk[j][j];
-        }
+        const double e = P[gr + static_cast<size_t>(c0 + min(j, kw - 1)) * ld];
+        b[j] = j < kw ? e : 0.0;
     }
 #pragma unroll
-    for (int j = 0; j < TB; ++j)
-        if (j < kw) P[r + static_cast<size_t>(c0 + j) * ld] = x[j];
+    for (int c = 0; c < TB; ++c) {
+        const double x = b[c] * invd[c];
+        b[c] = x;
+#pragma unroll
+        for (int j = c + 1; j < TB; ++j) b[j] -= x * Lc[c][j];
+#pragma unroll
+        for (int j = c + 1; j < TB; ++j) PIN(b[j]);
+    }
+    if (kw == TB) {
+#pragma unroll
+        for (int j = 0; j < TB; ++j) P[gr + static_cast<size_t>(c0 + j) * ld] = b[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < TB; ++j)
+            if (j < kw) P[gr + static_cast<size_t>(c0 + j) * ld] = b[j];
+    }
 }
 
 // 64 x 64 tile product C -= A B^T over kk columns; A rows ra.., B rows rb..
@@ -224,27 +276,24 @@ __global__ __launch_bounds__(256) void kFacSyrk(FacArgs a, int first)
         }
 }
 
-// block inverse, column tile J, rows of tile it: X = R L11^-1 with
-// R = [I; L21]: y = R(rows, J) - sum_{K > J} X(rows, K) L11(K, J), then
-// x L_JJ = y by backward substitution over the tile's columns
+// L11^-1, column tile J, rows of tile it >= J (inside the w columns):
+// y = I(rows, J) - sum_{J < K <= it} X(rows, K) L11(K, J), then x L_JJ = y by
+// right-looking backward substitution, lane = row
 __global__ __launch_bounds__(256) void kFacBsolve(FacArgs a, int first)
 {
     __shared__ double As[TB][TB + 1], Bs[TB][TB + 1];
     const int4 t4 = a.tasks[first + blockIdx.x];
     const int s = t4.x, J = t4.y, it = t4.z;
-    const int ws = a.w[s], ms = a.m[s], ld = blockLd(ms);
+    const int ws = a.w[s], ld = blockLd(a.m[s]);
     const int c0 = TB * J, kw = min(TB, ws - c0);
-    const int ri = TB * it, rows = min(TB, ms - ri);
+    const int ri = TB * it, rows = min(TB, ws - ri);
     const double* P = a.L + a.dval[s];
     double* X = a.X + a.dval[s];
-    // sum_{K > J} X(rows, K) L11(K, J) = X(rows, c0 + kw .. ws) * (L11(c0 + kw .., J))
-    // as C = A B^T with A = X rows, B = L11(J cols) transposed: B(j, k) =
-    // L11(c0 + kw + k, c0 + j); done here with an explicit loop over the
-    // trailing columns through the same tile machinery on transposed staging
     TileAcc acc{};
     const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
-    for (int k0 = c0 + kw; k0 < ws; k0 += TB) {
-        const int kc = min(TB, ws - k0);
+    // X(rows, k) vanishes for k past the tile's last row (L11^-1 is lower)
+    for (int k0 = c0 + kw; k0 < ri + rows; k0 += TB) {
+        const int kc = min(TB, ri + rows - k0);
         __syncthreads();
         for (int e = tid; e < TB * TB; e += 256) {
             const int i = e % TB, j = e / TB;
@@ -267,42 +316,87 @@ __global__ __launch_bounds__(256) void kFacBsolve(FacArgs a, int first)
         }
     }
     __syncthreads();
-    // y into As (rows x kw), L_JJ into Bs
+    // y into As, L_JJ row-major into Bs with its reciprocal diagonal on the
+    // diagonal (unit on padding columns)
     for (int e = tid; e < TB * TB; e += 256) {
         const int i = e % TB, j = e / TB;
-        Bs[i][j] = (i < kw && j < kw && i >= j) ? P[c0 + i + static_cast<size_t>(c0 + j) * ld] : 0.0;
+        const double l = (i < kw && j < kw && j >= i) ? P[c0 + j + static_cast<size_t>(c0 + i) * ld]
+                                                      : (i == j ? 1.0 : 0.0);
+        Bs[j][i] = i == j ? 1.0 / l : l;
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 4 * tr + p, j = 4 * tc + q, gi = ri + i, gj = c0 + j;
-            double r = 0.0;
-            if (i < rows && j < kw) r = gi < ws ? (gi == gj ? 1.0 : 0.0) : P[gi + static_cast<size_t>(gj) * ld];
-            As[i][j] = r - acc.v[p][q];
+            const int i = 4 * tr + p, j = 4 * tc + q;
+            const double d = (i < rows && j < kw && ri + i == c0 + j) ? 1.0 : 0.0;
+            As[i][j] = (i < rows && j < kw) ? d - acc.v[p][q] : 0.0;
         }
     __syncthreads();
     if (tid >= TB) return;
     const int i = tid;
-    if (i >= rows) return;
-    double x[TB];
+    double y[TB];
 #pragma unroll
-    for (int j = 0; j < TB; ++j) x[j] = As[i][j];
-    // x_j = (y_j - sum_{c > j} x_c L_cj) / L_jj, j from the last column down
+    for (int j = 0; j < TB; ++j) y[j] = As[i][j];
 #pragma unroll
     for (int jj = 0; jj < TB; ++jj) {
         const int j = TB - 1 - jj;
-        if (j < kw) {
-            double v = x[j];
+        const double x = y[j] * Bs[j][j];
+        y[j] = x;
 #pragma unroll
-            for (int c = j + 1; c < TB; ++c)
-                if (c < kw) v -= x[c] * Bs[c][j];
-            x[j] = v / Bs[j][j];
+        for (int c = 0; c < j; ++c) y[c] -= x * Bs[j][c];
+#pragma unroll
+        for (int c = 0; c < j; ++c) PIN(y[c]);
+    }
+    if (i >= rows) return;
+#pragma unroll
+    for (int j = 0; j < TB; ++j)
+        if (j < kw) X[ri + i + static_cast<size_t>(c0 + j) * ld] = y[j];
+}
+
+// L21 L11^-1, tile (it, jt): rows w + 64 it.., columns 64 jt.. of X, the
+// sum over k >= 64 jt (L11^-1 is lower)
+__global__ __launch_bounds__(256) void kFacGemm21(FacArgs a, int first)
+{
+    __shared__ double As[TB][TB + 1], Bs[TB][TB + 1];
+    const int4 t4 = a.tasks[first + blockIdx.x];
+    const int s = t4.x, it = t4.y, jt = t4.z;
+    const int ws = a.w[s], ms = a.m[s], nr = ms - ws, ld = blockLd(ms);
+    const int ri = TB * it, rows = min(TB, nr - ri), cj = TB * jt, cols = min(TB, ws - cj);
+    const double* P = a.L + a.dval[s] + ws + ri;  // L21 rows of the tile
+    double* X = a.X + a.dval[s];
+    TileAcc acc{};
+    const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
+    for (int k0 = cj; k0 < ws; k0 += TB) {
+        const int kc = min(TB, ws - k0);
+        __syncthreads();
+        for (int e = tid; e < TB * TB; e += 256) {
+            const int i = e % TB, j = e / TB;
+            // As[i][j] = L21(ri + i, k0 + j); Bs[j][i] = X11(k0 + i, cj + j)
+            As[i][j] = (i < rows && j < kc) ? P[i + static_cast<size_t>(k0 + j) * ld] : 0.0;
+            Bs[j][i] = (j < cols && i < kc) ? X[k0 + i + static_cast<size_t>(cj + j) * ld] : 0.0;
+        }
+        __syncthreads();
+        for (int j = 0; j < kc; ++j) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = As[4 * tr + q][j];
+                bv[q] = Bs[4 * tc + q][j];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc.v[p][q] += av[p] * bv[q];
         }
     }
 #pragma unroll
-    for (int j = 0; j < TB; ++j)
-        if (j < kw) X[ri + i + static_cast<size_t>(c0 + j) * ld] = x[j];
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * tr + p, j = 4 * tc + q;
+            if (i < rows && j < cols) X[ws + ri + i + static_cast<size_t>(cj + j) * ld] = acc.v[p][q];
+        }
 }
 
 __global__ __launch_bounds__(256) void kFacToF32(const double* __restrict__ X, float* __restrict__ B, long long n)
@@ -377,11 +471,13 @@ void launchDeviceFactor(const DevFactorArgs& d, hipStream_t s)
         switch (kind) {
             case 0: kFacAsm<<<(count + 255) / 256, 256, 0, s>>>(a, first, count); break;
             case 1: kFacExtAdd<<<count, 256, 0, s>>>(a, first); break;
-            case 2: kFacPotrf<<<count, 256, 0, s>>>(a, first); break;
-            case 3: kFacTrsm<<<count, 64, 0, s>>>(a, first); break;
+            case 2: kFacPotrf<<<count, TB, 0, s>>>(a, first); break;
+            case 3: kFacTrsm<<<count, kTrsmRows, 0, s>>>(a, first); break;
             case 4: kFacUpdate<<<count, 256, 0, s>>>(a, first); break;
             case 5: kFacSyrk<<<count, 256, 0, s>>>(a, first); break;
-            default: kFacBsolve<<<count, 256, 0, s>>>(a, first); break;
+            case 6: kFacBsolve<<<count, 256, 0, s>>>(a, first); break;
+            case 7: kFacGemm21<<<count, 256, 0, s>>>(a, first); break;
+            default: break;
         }
     }
 }

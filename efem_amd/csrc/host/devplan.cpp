@@ -8,7 +8,8 @@
 namespace efem {
 
 namespace {
-constexpr int kTile = 64;  // tile edge of the device dense kernels
+constexpr int kTile = 64;        // tile edge of the device dense kernels
+constexpr int kTrsmRows = 256;   // rows of one triangular-solve task
 inline int tiles(int v) { return (v + kTile - 1) / kTile; }
 }  // namespace
 
@@ -118,7 +119,7 @@ void buildDevFactorPlan(const SupernodalSymbolic& S, DevFactorPlan& P)
             for (int s : sl) {
                 if (tiles(P.w[s]) <= k) continue;
                 const int kw = std::min(kTile, P.w[s] - kTile * k);
-                for (int r = kTile * k + kw; r < P.m[s]; r += kTile) task(s, k, r, 0);
+                for (int r = kTile * k + kw; r < P.m[s]; r += kTrsmRows) task(s, k, r, 0);
             }
             launch(DF_TRSM, t0, ntask() - t0);
             t0 = ntask();
@@ -140,14 +141,21 @@ void buildDevFactorPlan(const SupernodalSymbolic& S, DevFactorPlan& P)
             }
             launch(DF_SYRK, t0, ntask() - t0);
         }
-        // block inverse, last column tile first
+        // L11^-1, last column tile first, then L21 L11^-1
         for (int J = nbwMax - 1; J >= 0; --J) {
             const int t0 = ntask();
             for (int s : sl) {
                 if (tiles(P.w[s]) <= J) continue;
-                for (int it = J; it < tiles(P.m[s]); ++it) task(s, J, it, 0);
+                for (int it = J; it < tiles(P.w[s]); ++it) task(s, J, it, 0);
             }
             launch(DF_BSOLVE, t0, ntask() - t0);
+        }
+        {
+            const int t0 = ntask();
+            for (int s : sl)
+                for (int it = 0; it < tiles(P.m[s] - P.w[s]); ++it)
+                    for (int jt = 0; jt < tiles(P.w[s]); ++jt) task(s, it, jt, 0);
+            launch(DF_GEMM21, t0, ntask() - t0);
         }
     }
 }

@@ -12,9 +12,9 @@
 // is a fixed sequence of batched launches over 64 x 64 tiles:
 //   assemble A's lower entries -> extend-add children's update matrices ->
 //   per 64-column step k: Cholesky of the diagonal tile, triangular solve
-//   of the tiles below, update of the panel's remaining columns ->
-//   U_s -= L21 L21^T -> per 64-column step J (last to first): the block
-//   inverse X = [I; L21] L11^-1 by a right-side triangular solve.
+//   of the rows below it, update of the panel's remaining columns -> U_s -= L21 L21^T -> per 64-column step J (last to first):
+//   L11^-1 by a right-side triangular solve of L11 X = I -> L21 L11^-1 as
+//   one tile product.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -33,11 +33,13 @@ struct DevLaunch {
 enum DevLaunchKind {
     DF_ASM = 0,     // tasks = A entries (asmDst, asmSrc)
     DF_EXTADD = 1,  // tasks {s, target front column, pair begin, pair end}
-    DF_POTRF = 2,   // tasks {s, k, 0, 0}
-    DF_TRSM = 3,    // tasks {s, k, first row, 0}
+    DF_POTRF = 2,   // tasks {s, k, 0, 0}: Cholesky of the diagonal tile k
+    DF_TRSM = 3,    // tasks {s, k, first row, 0}: 256 rows below the tile
+                    // solved against its factor
     DF_UPDATE = 4,  // tasks {s, k, row tile, column tile}
     DF_SYRK = 5,    // tasks {s, row tile, column tile, 0} of U_s
-    DF_BSOLVE = 6,  // tasks {s, J, row tile, 0}
+    DF_BSOLVE = 6,  // tasks {s, J, row tile < w / 64, 0}: L11^-1 by column tiles
+    DF_GEMM21 = 7,  // tasks {s, row tile of L21, column tile, 0}: L21 L11^-1
 };
 
 struct DevFactorPlan {
