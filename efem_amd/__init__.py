@@ -109,6 +109,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_host_factor_solve": (C.c_int, [C.POINTER(Problem), _dp, _dp, C.POINTER(C.c_longlong)]),
         "efem_host_dense_kernel": (C.c_char_p, []),
         "efem_host_constant_force": (C.c_int, [C.POINTER(Problem), _dp, C.c_int]),
+        "efem_host_jacobi": (C.c_int, [C.POINTER(Problem), C.POINTER(C.c_longlong), _ip, _ip, _dp, C.c_longlong]),
         "efem_device_memory": (C.c_int, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "efem_commit": (C.c_int, [vp, _dp]),
         "efem_committed": (C.c_int, [vp, _dp]),
@@ -621,6 +622,26 @@ def host_constant_force(mesh: Mesh) -> np.ndarray:
     if rc != EFEM_OK:
         raise EfemError(f"efem_host_constant_force failed ({rc}): {lib().efem_global_error().decode()}")
     return out
+
+
+def host_jacobi(mesh: Mesh):
+    """Host-only: the approximate Jacobian's triplets (efem_host_jacobi) as
+    (rows, cols, vals): every 3 x 3 block of the elasticity-weighted scalar
+    Laplacian, explicit zeros off the block diagonals."""
+    L = lib()
+    pb = mesh.problem(FP64)
+    nnz = C.c_longlong()
+    rc = L.efem_host_jacobi(C.byref(pb), C.byref(nnz), None, None, None, 0)
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_host_jacobi failed ({rc}): {L.efem_global_error().decode()}")
+    r = np.empty(nnz.value, np.int32)
+    c = np.empty(nnz.value, np.int32)
+    v = np.empty(nnz.value)
+    rc = L.efem_host_jacobi(C.byref(pb), C.byref(nnz), _ptr(r, C.c_int), _ptr(c, C.c_int), _ptr(v, C.c_double),
+                            nnz.value)
+    if rc != EFEM_OK:
+        raise EfemError(f"efem_host_jacobi failed ({rc}): {L.efem_global_error().decode()}")
+    return r, c, v
 
 
 def write_vtk(mesh: Mesh, u, path) -> None:

@@ -44,7 +44,12 @@ SOURCES = [
     ("host/input.cpp", False),
     ("host/vtk.cpp", False),
     ("host/devplan.cpp", False),
+    ("host/fe.cpp", False),
 ]
+
+# the C++ interfaces' test driver (tests/cpp), linked against the library
+FE_DRIVER_SRC = ROOT / "tests" / "cpp" / "fe_driver.cpp"
+FE_DRIVER = ROOT / "tests" / "cpp" / "fe_driver"
 
 
 def _compile(src: str, device: bool, batch_width: str | None = None) -> Path:
@@ -80,6 +85,20 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_fe_driver(verbose: bool = False) -> Path:
+    """tests/cpp/fe_driver: a host program written against include/efem/fe.hpp."""
+    tmp = FE_DRIVER.with_name(FE_DRIVER.name + ".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", str(FE_DRIVER_SRC), "-o", str(tmp), f"-L{PKG}", "-lefem_gpu",
+           "-Wl,-rpath,$ORIGIN/../../efem_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"fe_driver build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, FE_DRIVER)
+    if verbose:
+        print(f"built {FE_DRIVER}")
+    return FE_DRIVER
+
+
 def build_oracle(verbose: bool = False) -> None:
     """The CPU oracle (test infrastructure) and, when the reference tree is
     present, oracle/_ref (the reference's material sources)."""
@@ -99,5 +118,6 @@ def build_oracle(verbose: bool = False) -> None:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_fe_driver(verbose=True)
     if "--oracle" in sys.argv:
         build_oracle(verbose=True)

@@ -146,6 +146,7 @@ struct efem_ctx {
     bool lookahead = true;       // option "lookahead"
     bool fwdTables = true;       // option "fwd_tables"
     bool fusedRootOpt = true;    // option "fused_root"
+    int residualTerms = 7;       // option "residual_terms" (efem_residual only)
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
     // ordering + symbolic analysis of the approximate Jacobian: its pattern
@@ -396,28 +397,42 @@ efem::dev::GatherArgs gatherArgs(efem_ctx* c, double* r, bool f32, double ce, do
     return ga;
 }
 
+// residual terms (option "residual_terms")
+constexpr int kTermElements = 1;  // element internal forces
+constexpr int kTermPressure = 2;  // follower pressure / directed traction
+constexpr int kTermConstant = 4;  // constant nodal forces and gravity
+constexpr int kTermsAll = 7;
+
 // r = f_int(x) - f_ext(x); x has 3 trailing zeros (the fixed-node slot)
 // r(x); with dotOut, also the line search's scalars of the new r in the
 // gather launch: dotOut = {r . dotD, max|r|, max|x - dotCb|, |x - dotXt|^2}
+// terms: kTermsAll, or a subset for one constraint's part (option
+// "residual_terms" of efem_residual)
 int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD = nullptr,
-                     const double* dotCb = nullptr, const double* dotXt = nullptr, double* dotOut = nullptr)
+                     const double* dotCb = nullptr, const double* dotXt = nullptr, double* dotOut = nullptr,
+                     int terms = kTermsAll)
 {
     const efem::Model& m = c->model;
     const bool f32 = elem32(c) && !c->elemFp64;
-    const efem::dev::ElemArgs ea = elemArgs(c, x, f32);
-    const bool prof = c->profiling && c->evUsed + 2 <= static_cast<int>(c->evs.size());
-    if (prof) HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed], c->stream));
-    efem::dev::launchElementForce(ea, c->stream);
-    if (prof) {
-        HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed + 1], c->stream));
-        c->evUsed += 2;
+    if (terms & kTermElements) {
+        const efem::dev::ElemArgs ea = elemArgs(c, x, f32);
+        const bool prof = c->profiling && c->evUsed + 2 <= static_cast<int>(c->evs.size());
+        if (prof) HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed], c->stream));
+        efem::dev::launchElementForce(ea, c->stream);
+        if (prof) {
+            HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed + 1], c->stream));
+            c->evUsed += 2;
+        }
+        HIP_CHECK(c, hipGetLastError());
     }
-    HIP_CHECK(c, hipGetLastError());
-    if (m.nFaces > 0) {
+    if (m.nFaces > 0 && (terms & kTermPressure)) {
         efem::dev::launchPressure(faceArgs(c, x, nullptr), c->stream);
         HIP_CHECK(c, hipGetLastError());
     }
-    efem::dev::GatherArgs ga = gatherArgs(c, r, f32, 1.0, -1.0, c->fext);
+    // a zero coefficient makes the gather skip those slots
+    efem::dev::GatherArgs ga = gatherArgs(c, r, f32, (terms & kTermElements) ? 1.0 : 0.0,
+                                          (terms & kTermPressure) ? -1.0 : 0.0,
+                                          (terms & kTermConstant) ? c->fext : nullptr);
     if (dotOut) {
         ga.dotD = dotD;
         ga.dotX = x;
@@ -1399,7 +1414,7 @@ int efem_residual(efem_ctx* c, const double* u, double* r)
     efem::dev::setTuning(c->tuning);
     const size_t n = static_cast<size_t>(c->model.ndof());
     HIP_CHECK(c, hipMemcpyAsync(c->tmp, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    int rc = residualInternal(c, c->tmp, c->r);
+    int rc = residualInternal(c, c->tmp, c->r, nullptr, nullptr, nullptr, nullptr, c->residualTerms);
     if (rc) return rc;
     HIP_CHECK(c, hipMemcpyAsync(r, c->r, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -1880,6 +1895,44 @@ int efem_host_constant_force(const efem_problem* pb, double* out, int n)
     return EFEM_OK;
 }
 
+int efem_host_jacobi(const efem_problem* pb, long long* nnz, int* rows, int* cols, double* vals, long long cap)
+{
+    if (!pb || !nnz) return EFEM_ERR_ARG;
+    efem::Model m;
+    std::string msg;
+    efem::Csc L;
+    try {
+        if (!efem::buildModel(*pb, m, msg)) {
+            g_error = msg;
+            return EFEM_ERR_ARG;
+        }
+        efem::scalarLaplacian(m, L);
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return EFEM_ERR_ARG;
+    }
+    // every 3 x 3 block of a nonzero L_ij: L_ij on its diagonal, explicit
+    // zeros off it (the pattern ElementConstraint.cpp:193-204 pushes)
+    const long long need = 9LL * static_cast<long long>(L.i.size());
+    *nnz = need;
+    if (!rows || !cols || !vals) return EFEM_OK;
+    if (cap < need) {
+        g_error = "efem_host_jacobi: cap < " + std::to_string(need);
+        return EFEM_ERR_ARG;
+    }
+    long long q = 0;
+    for (int j = 0; j < L.n; ++j)
+        for (int p = L.p[j]; p < L.p[j + 1]; ++p)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    rows[q] = 3 * L.i[p] + a;
+                    cols[q] = 3 * j + b;
+                    vals[q] = a == b ? L.x[p] : 0.0;
+                    ++q;
+                }
+    return EFEM_OK;
+}
+
 int efem_device_memory(long long* free_bytes, long long* total_bytes)
 {
     if (!free_bytes || !total_bytes) return EFEM_ERR_ARG;
@@ -1992,6 +2045,13 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         return EFEM_OK;
     }
     const int iv = static_cast<int>(value);
+    if (k == "residual_terms" && iv >= 1 && iv <= 7 && static_cast<double>(iv) == value) {
+        // efem_residual evaluates only these terms (1 element internal
+        // forces, 2 pressure faces, 4 constant loads): one constraint's part
+        // of the residual; solves always use all terms
+        c->residualTerms = iv;
+        return EFEM_OK;
+    }
     if (k == "gather_lanes" && (iv == 1 || iv == 2 || iv == 4 || iv == 8)) {
         c->tuning.gatherLanes = iv;
         return EFEM_OK;

@@ -400,6 +400,7 @@ __device__ __forceinline__ void gatherNode(const GatherArgs& a, int i)
             se[1] += static_cast<double>(v[1]);
             se[2] += static_cast<double>(v[2]);
         } else {
+            if (a.cf == 0.0) continue;
             const double* v = a.fslots + 3 * static_cast<size_t>(s - a.nElemSlots);
             sf[0] += v[0];
             sf[1] += v[1];

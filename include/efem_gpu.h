@@ -214,6 +214,15 @@ int efem_host_factor_solve(const efem_problem* pb, const double* b, double* x,
  * bytes; lets a caller check that refactorisations replace, not add to, the
  * factor's device buffers */
 int efem_device_memory(long long* free_bytes, long long* total_bytes);
+/* the approximate Jacobian as triplets, host only
+ * (ElementConstraint::conEvaluteJacobi / conGetJacobi,
+ * src/fe/ElementConstraint.cpp:164-210): K = L (x) I3 over the free DOFs, L
+ * the elasticity_coeff-weighted scalar Laplacian; every 3 x 3 block of a
+ * nonzero L_ij with L_ij on its diagonal and explicit zeros off it, each
+ * entry once (the matrix setFromTriplets builds from the reference's
+ * per-element triplets).  *nnz = the entry count; rows, cols, vals (cap
+ * entries) NULL to query it.  Errors through efem_global_error. */
+int efem_host_jacobi(const efem_problem* pb, long long* nnz, int* rows, int* cols, double* vals, long long cap);
 
 /* host-only diagnostic (no GPU): the constant part of the external force on
  * the free DOFs as the residual subtracts it -- nodal forces
@@ -270,6 +279,12 @@ const char* efem_host_dense_kernel(void);
  *                       at least this many 4-column work items use
  *                       16-column items instead; releases the factor (the
  *                       next solve factors again)
+ *   "residual_terms"    1..7 (default 7): efem_residual evaluates only the
+ *                       terms in the mask -- 1 element internal forces, 2
+ *                       follower pressure / directed traction, 4 constant
+ *                       nodal forces and gravity -- one constraint's part
+ *                       (Constrainable::conEvaluteResidual,
+ *                       src/fe/Constraint.h:26); solves use every term
  *   "gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks":
  *                       launch shapes (see efem_autotune) */
 int efem_set_option(efem_ctx* ctx, const char* name, double value);
