@@ -483,6 +483,10 @@ def main():
                          "u* saved by --save-ustar (no GPU needed; minutes); --cpu-tts-interleaved for the "
                          "reference's 3N_f factor")
     ap.add_argument("--cpu-tts-interleaved", action="store_true")
+    ap.add_argument("--profile-factor", type=int, default=0, metavar="N",
+                    help="only N refactorisations of the beam (kernel traces)")
+    ap.add_argument("--profile-batch", type=int, default=0, metavar="ITERS",
+                    help="only one batched C5 solve capped at ITERS iterations (kernel traces)")
     args = ap.parse_args()
     if args.cpu_tts_from:
         import efem_amd as E
@@ -509,6 +513,34 @@ def main():
             with E.Context(m, prec) as c:
                 rec[label + "_us_per_eval"] = round(1e3 * c.time_residual(u, 200), 2)
         print(json.dumps(rec))
+        return
+    if args.profile_factor:
+        # refactorisations of the beam (device assembly + device numeric
+        # factorisation), mixed precision, for a kernel trace
+        m = E.Mesh.workload(2, [200, 20, 20, 20240531, args.pressure])
+        with E.Context(m, E.MIXED) as c:
+            c.factor()  # ordering + symbolic analysis, not profiled separately
+            t = time.perf_counter()
+            for _ in range(args.profile_factor):
+                c.factor()
+            dt = (time.perf_counter() - t) / args.profile_factor
+            print(json.dumps({"profile_factor": args.profile_factor, "s_per_factor": round(dt, 4),
+                              "phases_ms": {k: round(1e3 * v, 2) for k, v in c.factor_phases().items()}}))
+        return
+    if args.profile_batch:
+        # one persistent batched solve of the 64 cantilevers (loads x
+        # batch-load-scale, mixed, the autotuned width, capped at
+        # profile_batch iterations), for a kernel trace
+        meshes = [E.Mesh.workload(5, [i, 20240531, args.batch_load_scale]) for i in range(64)]
+        with E.Batch(meshes, E.MIXED) as b:
+            tune = b.autotune()
+            b.factor_release()
+            t = time.perf_counter()
+            _, st = b.solve(epsilon=1e-6, max_iterations=args.profile_batch, fp64_switch=1e-3)
+            dt = time.perf_counter() - t
+            print(json.dumps({"profile_batch": args.profile_batch, "batch_threads": tune["batch_threads"],
+                              "s": round(dt, 4), "iterations_max": max(s["iterations"] for s in st),
+                              "iterations_mean": float(np.mean([s["iterations"] for s in st]))}))
         return
     if args.save_ustar:
         p1 = round(args.pressure * (1.0 + 1e-6), 6)
