@@ -318,7 +318,11 @@ int uploadModel(efem_ctx* c)
     }
     if ((rc = upload(c, &c->gptr, m.gptr))) return rc;
     if ((rc = upload(c, &c->gslot, m.gslot))) return rc;
-    if ((rc = upload(c, &c->fext, m.fext))) return rc;
+    // constant loads: without any (no nodal force, no gravity) the vector
+    // stays off the device and the gather and energy kernels skip its reads
+    // (subtracting a zero changes no value)
+    if (std::any_of(m.fext.begin(), m.fext.end(), [](double v) { return v != 0.0; }))
+        if ((rc = upload(c, &c->fext, m.fext))) return rc;
     const size_t np = static_cast<size_t>(m.ndof()) + 3;  // + dummy zero node
     // zero fills on the context stream: it is a non-blocking stream, so a
     // null-stream hipMemset would not be ordered before its work
