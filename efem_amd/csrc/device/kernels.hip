@@ -29,8 +29,8 @@ namespace dev {
 namespace {
 
 // K block reductions over W waves (defined with the reductions below)
-template <int K, int W>
-__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W], unsigned maxMask);
+template <int K, int W, unsigned MM>
+__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W]);
 
 template <typename T, int LAW, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void kElemTet4(int E, const int* __restrict__ connDof,
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
             }
         }
         __shared__ double sh[DOT][kThreads / 64];
-        blockReduce<DOT, kThreads / 64>(v, sh, 0x6u);
+        blockReduce<DOT, kThreads / 64, 0x6u>(v, sh);
         if (threadIdx.x == 0) {
 #pragma unroll
             for (int k = 0; k < DOT; ++k) a.dotWork[static_cast<size_t>(k) * gridDim.x + blk] = v[k];
@@ -422,38 +422,39 @@ constexpr int kRF = 256;  // stage-2 block (one lane per block partial)
 constexpr int kRFW = kRF / 64;
 
 // K simultaneous block reductions over W waves: wave butterflies, then the
-// W wave results in wave order.  Bit k of maxMask: component k is a maximum
-// (else a sum; components from 32 on are sums).  All lanes return the block
-// result.
-__device__ __forceinline__ bool isMax(unsigned maxMask, int k) { return k < 32 && ((maxMask >> k) & 1u); }
+// W wave results in wave order.  Bit k of the mask MM (a template argument):
+// component k is a maximum (else a sum; components from 32 on are sums).
+// All lanes return the block result.
+__host__ __device__ constexpr bool isMax(unsigned maxMask, int k) { return k < 32 && ((maxMask >> k) & 1u); }
 
-template <int K, int W>
-__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W], unsigned maxMask)
+template <int K, int W, unsigned MM>
+__device__ __forceinline__ void blockReduce(double (&v)[K], double (*sh)[W])
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const bool mx = isMax(maxMask, k);
-        const double t = mx ? waveMax(v[k]) : waveSum(v[k]);
+        // the mask is a compile-time constant, so only one butterfly is
+        // emitted per component (with a run-time mask both the sum and the
+        // NaN-propagating max were computed and one selected)
+        const double t = isMax(MM, k) ? waveMax(v[k]) : waveSum(v[k]);
         if (lane == 0) sh[k][wave] = t;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const bool mx = isMax(maxMask, k);
         double t = sh[k][0];
-        for (int w = 1; w < W; ++w) t = mx ? maxNan(t, sh[k][w]) : t + sh[k][w];
+        for (int w = 1; w < W; ++w) t = isMax(MM, k) ? maxNan(t, sh[k][w]) : t + sh[k][w];
         v[k] = t;
     }
     __syncthreads();
 }
 
 // stage 1: this block's partials of v[0..K) to work
-template <int K>
-__device__ __forceinline__ void writePartials(double (&v)[K], double* work, unsigned maxMask)
+template <int K, unsigned MM>
+__device__ __forceinline__ void writePartials(double (&v)[K], double* work)
 {
     __shared__ double sh[K][kRBW];
-    blockReduce<K, kRBW>(v, sh, maxMask);
+    blockReduce<K, kRBW, MM>(v, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) work[k * kReduceBlocks + blockIdx.x] = v[k];
@@ -464,9 +465,9 @@ __device__ __forceinline__ void writePartials(double (&v)[K], double* work, unsi
 // blocks t, t + kRF, ...; then the block tree); every lane calls epi(v, p),
 // p = epi.prefetch() -- what the epilogue reads besides the sums, loaded
 // before the partials so that the two loads overlap
-template <int K, typename Epi>
-__global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, unsigned maxMask,
-                                                     Epi epi, int stride = kReduceBlocks)
+template <int K, unsigned MM, typename Epi>
+__global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, Epi epi,
+                                                     int stride = kReduceBlocks)
 {
     __shared__ double sh[K][kRFW];
     const typename Epi::Pre pre = epi.prefetch();
@@ -477,9 +478,9 @@ __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ 
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double o = work[static_cast<size_t>(k) * stride + b];
-            v[k] = isMax(maxMask, k) ? maxNan(v[k], o) : v[k] + o;
+            v[k] = isMax(MM, k) ? maxNan(v[k], o) : v[k] + o;
         }
-    blockReduce<K, kRFW>(v, sh, maxMask);
+    blockReduce<K, kRFW, MM>(v, sh);
     epi(v, pre);
 }
 
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(kRB) void kDotMax(const double* __restrict__ a, con
             }
         }
     }
-    writePartials<K>(v, work, 0x6u);
+    writePartials<K, 0x6u>(v, work);
 }
 
 __global__ __launch_bounds__(kRB) void kDot2(const double* __restrict__ a, const double* __restrict__ b, int n,
@@ -555,7 +556,7 @@ __global__ __launch_bounds__(kRB) void kDot2(const double* __restrict__ a, const
         v[0] += a[i] * bi;
         v[1] += bi * bi;
     }
-    writePartials<2>(v, work, 0u);
+    writePartials<2, 0u>(v, work);
 }
 
 __global__ __launch_bounds__(kRB) void kTwoLoop(double* __restrict__ q, const double* __restrict__ vprev,
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(kRB) void kTwoLoop(double* __restrict__ q, const do
         }
         v[0] += w[i] * qi;
     }
-    writePartials<1>(v, work, 0u);
+    writePartials<1, 0u>(v, work);
 }
 
 // epilogue of a two-loop step: r = v / *den; *cout = mode ? *aj - r : r
@@ -650,7 +651,7 @@ __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double
         sl[i] = si;
         yl[i] = yi;
     }
-    writePartials<N>(v, work, 0u);
+    writePartials<N, 0u>(v, work);
 }
 
 // Epilogue of kGramFirst: writes the Gram row / column of loc, ys_loc, yy,
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(kRB) void kTwoLoopDots(HSrc q, const double* __rest
         for (int k = 0; k < NC; ++k) v[k] += rk[k] * qi;
         v[NC] += dwi * qi;
     }
-    writePartials<NC + 1>(v, work, 0u);
+    writePartials<NC + 1, 0u>(v, work);
 }
 
 // Epilogue of kTwoLoopDots: the beta recurrence oldest -> newest, coef =
@@ -1295,10 +1296,10 @@ void launchGather(const GatherArgs& a, hipStream_t s)
     if (!a.dotWork) return gatherLaunch<0>(a, lpn, g, s);
     if (a.dotXt) {
         gatherLaunch<4>(a, lpn, g, s);
-        kReduceFinish<4><<<1, kRF, 0, s>>>(a.dotWork, g, 0x6u, EpiStore<4>{{}, a.dotOut}, g);
+        kReduceFinish<4, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<4>{{}, a.dotOut}, g);
     } else {
         gatherLaunch<3>(a, lpn, g, s);
-        kReduceFinish<3><<<1, kRF, 0, s>>>(a.dotWork, g, 0x6u, EpiStore<3>{{}, a.dotOut}, g);
+        kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<3>{{}, a.dotOut}, g);
     }
 }
 
@@ -1308,13 +1309,13 @@ void launchDotMax(const double* a, const double* b, const double* c, int n, doub
     const int g = reduceGrid(n);
     if (!c) {
         kDotMax<false, false><<<g, kRB, 0, s>>>(a, b, c, nullptr, nullptr, n, work);
-        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{{}, out});
+        kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(work, g, EpiStore<3>{{}, out});
     } else if (!xt) {
         kDotMax<true, false><<<g, kRB, 0, s>>>(a, b, c, cb, nullptr, n, work);
-        kReduceFinish<3><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<3>{{}, out});
+        kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(work, g, EpiStore<3>{{}, out});
     } else {
         kDotMax<true, true><<<g, kRB, 0, s>>>(a, b, c, cb, xt, n, work);
-        kReduceFinish<4><<<1, kRF, 0, s>>>(work, g, 0x6u, EpiStore<4>{{}, out});
+        kReduceFinish<4, 0x6u><<<1, kRF, 0, s>>>(work, g, EpiStore<4>{{}, out});
     }
 }
 
@@ -1322,7 +1323,7 @@ void launchDot2(const double* a, const double* b, int n, double* work, double* a
 {
     const int g = reduceGrid(n);
     kDot2<<<g, kRB, 0, s>>>(a, b, n, work);
-    kReduceFinish<2><<<1, kRF, 0, s>>>(work, g, 0u, EpiStore2{{}, ab, bb});
+    kReduceFinish<2, 0u><<<1, kRF, 0, s>>>(work, g, EpiStore2{{}, ab, bb});
 }
 
 void launchStep(double* x, const double* xp, const double* d, double alpha, int n, hipStream_t s)
@@ -1346,7 +1347,7 @@ void launchTwoLoopStep(double* q, const double* vprev, const double* cprev, cons
 {
     const int g = reduceGrid(n);
     kTwoLoop<<<g, kRB, 0, s>>>(q, vprev, cprev, mode ? 1.0 : -1.0, w, n, work);
-    kReduceFinish<1><<<1, kRF, 0, s>>>(work, g, 0u, EpiTwoLoopStep{{}, den, aj, cout, mode});
+    kReduceFinish<1, 0u><<<1, kRF, 0, s>>>(work, g, EpiTwoLoopStep{{}, den, aj, cout, mode});
 }
 
 // calls f(std::integral_constant<int, nc>) for 1 <= nc <= 16
@@ -1385,11 +1386,11 @@ void launchGramFirst(double* S, double* Y, const double* x, const double* xp, co
         kGramFirst<NC><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
         if (mcap <= 8) {
             if constexpr (NC <= 8)
-                kReduceFinish<3 * NC + 1><<<1, kRF, 0, s>>>(
-                    work, g, 0u, EpiGramFirst<8, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
+                kReduceFinish<3 * NC + 1, 0u><<<1, kRF, 0, s>>>(
+                    work, g, EpiGramFirst<8, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
         } else {
-            kReduceFinish<3 * NC + 1><<<1, kRF, 0, s>>>(
-                work, g, 0u, EpiGramFirst<16, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
+            kReduceFinish<3 * NC + 1, 0u><<<1, kRF, 0, s>>>(
+                work, g, EpiGramFirst<16, NC>{G, ys, yy, theta, alpha, rs, loc, mcap});
         }
         if (n > 0)
             kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(HSrc{r}, -1.0, Y, alpha, -1.0, n, out, nullptr, nullptr,
@@ -1408,11 +1409,11 @@ void launchTwoLoopSecond(HSrc h, const double* S, const double* Y, int n, int nc
         kTwoLoopDots<NC><<<g, kRB, 0, s>>>(h, Y, n, dw, work);
         if (mcap <= 8) {
             if constexpr (NC <= 8)
-                kReduceFinish<NC + 1><<<1, kRF, 0, s>>>(
-                    work, g, 0u, EpiTwoLoopDots<8, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
+                kReduceFinish<NC + 1, 0u><<<1, kRF, 0, s>>>(
+                    work, g, EpiTwoLoopDots<8, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
         } else {
-            kReduceFinish<NC + 1><<<1, kRF, 0, s>>>(
-                work, g, 0u, EpiTwoLoopDots<16, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
+            kReduceFinish<NC + 1, 0u><<<1, kRF, 0, s>>>(
+                work, g, EpiTwoLoopDots<16, NC>{G, ys, alpha, rs, coef, slope, newest, mcap});
         }
         if (n > 0) kMultiAxpy<NC><<<blocks(n), kThreads, 0, s>>>(h, 1.0, S, coef, 1.0, n, out, xbase, xout);
     });
