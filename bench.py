@@ -5,8 +5,9 @@ of the L-BFGS solve loop on the 265,923-DOF beam (200x20x20 hexes split into
 C10=80000 C01=20000 D1=1e-6, x=0 fixed, follower pressure on z=max), mixed
 precision vs FP64, on one MI355X.
 
-A step is one complete solve from u = 0: approximate-Jacobian assembly,
-ordering and factorisation (host) + upload + the L-BFGS loop on the device,
+A step is one complete solve from u = 0: approximate-Jacobian assembly and
+numeric factorisation (device; the ordering and symbolic analysis are done
+once per context, at its autotuning) + the L-BFGS loop on the device,
 stopped when the displacement is within 1e-6 (relative L2) of a tightly
 converged FP64 solution u*.  The reference's own stopping test is kept too.
 
@@ -56,18 +57,22 @@ def element_work(m, prec):
 
 
 def committed_cpu_tts():
-    """The full single-thread oracle solve of the beam at 100.0001 Pa to 1e-6
-    displacement error (scalar-Laplacian factor), from its committed record:
-    {iterations, wall_s, factor_s, host}."""
+    """The full single-thread oracle solves of the beam at 100.0001 Pa to 1e-6
+    displacement error, from their committed record: the scalar-Laplacian
+    factor run {iterations, wall_s, factor_s, r_evals, host} and, under
+    "interleaved", the run with the reference's own 3N_f matrix."""
     f = ROOT / "profiles" / "cpu_oracle_tts_beam_c3d4.md"
     out = {}
+    cur = out
     try:
         for line in f.read_text().splitlines():
+            if line.startswith("Interleaved mode"):
+                cur = out.setdefault("interleaved", {})
             for key in ("iterations", "wall_s", "factor_s", "r_evals"):
-                if line.startswith(f"{key}:"):
-                    out[key] = float(line.split(":")[1])
-            if line.startswith("host:"):
-                out["host"] = line.split(":", 1)[1].strip()
+                if line.startswith(f"{key}:") and key not in cur:
+                    cur[key] = float(line.split(":")[1])
+            if line.startswith("host:") and "host" not in cur:
+                cur["host"] = line.split(":", 1)[1].strip()
     except OSError:
         pass
     return out
@@ -292,6 +297,7 @@ def run_config_solves(E, args):
     # 20,000 iterations
     for label, prec, sw in (("fp64", E.FP64, 0.0),):
         with E.Context(m, prec) as c:
+            rec["dense_subtree"] = c.autotune_factor()["dense_subtree"]
             c.solve(epsilon=1e-6, max_iterations=args.max_iters, fp64_switch=sw)  # warm-up
             walls, its = [], []
             for _ in range(3):
@@ -332,6 +338,8 @@ def run_config_solves(E, args):
         m = E.Mesh.workload(which, opts)
         rec = {"elements": m.n_elems, "dof": m.ndof, "pressure_pa": m.pressure, "pressure_configured_pa": p_conf}
         with E.Context(m, E.FP64) as c64, E.Context(m, E.MIXED) as cmx:
+            rec["dense_subtree"] = {"fp64": c64.autotune_factor()["dense_subtree"],
+                                    "mixed": cmx.autotune_factor()["dense_subtree"]}
             fext = np.abs(c64.residual(np.zeros(m.ndof))).max()
             us, st_l = c64.solve(epsilon=1e-8 * fext, epsilon_rel=0.0, max_iterations=args.max_iters)
             us, st_n = c64.newton(us, epsilon=1e-13 * fext, max_iterations=12, tol=1e-10)
@@ -483,6 +491,8 @@ def main():
                          "u* saved by --save-ustar (no GPU needed; minutes); --cpu-tts-interleaved for the "
                          "reference's 3N_f factor")
     ap.add_argument("--cpu-tts-interleaved", action="store_true")
+    ap.add_argument("--dense-subtree", type=int, default=192,
+                    help="--profile-run / --profile-factor: the factor's dense-subtree size")
     ap.add_argument("--profile-factor", type=int, default=0, metavar="N",
                     help="only N refactorisations of the beam (kernel traces)")
     ap.add_argument("--profile-batch", type=int, default=0, metavar="ITERS",
@@ -519,6 +529,7 @@ def main():
         # factorisation), mixed precision, for a kernel trace
         m = E.Mesh.workload(2, [200, 20, 20, 20240531, args.pressure])
         with E.Context(m, E.MIXED) as c:
+            c.set_option("dense_subtree", args.dense_subtree)
             c.factor()  # ordering + symbolic analysis, not profiled separately
             t = time.perf_counter()
             for _ in range(args.profile_factor):
@@ -556,6 +567,10 @@ def main():
     log(f"beam: {mesh.n_nodes} nodes, {mesh.n_elems} C3D4, {mesh.ndof} DOF, p = {args.pressure} Pa")
     if args.profile_run:
         c = E.Context(mesh, E.MIXED)
+        # the dense-subtree size the bench's autotuner picks on the beam
+        # (efem_autotune_factor), set directly so its trial applies stay out
+        # of the trace
+        c.set_option("dense_subtree", args.dense_subtree)
         c.profile_reset()
         t = time.perf_counter()
         u, st = c.solve(epsilon=0.0, epsilon_rel=0.0, max_iterations=args.profile_run)
@@ -605,6 +620,8 @@ def main():
         return st
 
     tuned = {name: c.autotune() for name, c in ctx.items()}
+    for name, c in ctx.items():
+        tuned[name].update(c.autotune_factor())
     log(f"autotune: {tuned}")
     u_star, ustar_info = reference_solution(args.pressure)
     for _ in range(args.warmup):
@@ -682,7 +699,8 @@ def main():
         "fp64_switch": args.fp64_switch,
         "fp64_switch_iteration": int(mix[-1]["fp64_switch_iteration"]),
         "u_star": ustar_info,
-        "launch_shapes": {k: {kk: v[kk] for kk in ("gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks")}
+        "launch_shapes": {k: {kk: v[kk] for kk in ("gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks",
+                                                   "dense_subtree")}
                           for k, v in tuned.items()},
         "configured_load": configured,
         "load_samples": [round(args.pressure * (1.0 + 1e-6 * (k + 1)), 6) for k in range(args.steps)],
@@ -784,6 +802,12 @@ def main():
             rec["tts_note"] = ("factorisation + the sample's seconds per iteration x the iteration count of "
                                "the committed full run (profiles/cpu_oracle_tts_beam_c3d4.md, wall "
                                f"{full.get('wall_s', float('nan')):.1f} s on {full.get('host', '?')})")
+            il = full.get("interleaved", {})
+            if "wall_s" in il:
+                # the same solve with the reference's own matrix (Eigen's
+                # SimplicialLLT factors the 3N_f interleaved K), committed run
+                rec["reference_matrix_full_run_s"] = il["wall_s"]
+                rec["reference_matrix_factor_s"] = il.get("factor_s")
             # the device's mixed solve at the same load for the ratio
             st1 = one_solve("mixed", p1, u1)
             rec["tts_mixed_gpu_s"] = round(st1["wall"], 4)

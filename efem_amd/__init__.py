@@ -128,6 +128,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_energy_noalloc": (C.c_int, [vp, vp, vp]),
         "efem_time_residual": (C.c_int, [vp, _dp, C.c_int, _dp]),
         "efem_autotune": (C.c_int, [vp, _dp]),
+        "efem_autotune_factor": (C.c_int, [vp, _dp]),
         "efem_set_params": (C.c_int, [vp, _dp]),
         "efem_set_load": (C.c_int, [vp, C.c_double, _dp]),
         "efem_material_point": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -435,6 +436,14 @@ class Context:
         return {"gather_lanes": int(rep[0]), "tet4_threads": int(rep[1]), "reduce_blocks": int(rep[2]),
                 "history_blocks": int(rep[5]), "residual_ms": float(rep[3]), "dot_ms": float(rep[4]),
                 "history_ms": float(rep[6]), "batch_threads": int(rep[7]), "batch_ms": float(rep[8])}
+
+    def autotune_factor(self) -> dict:
+        """Pick the dense-subtree size with the fastest preconditioner apply
+        (efem_autotune_factor); the context stays factored with it."""
+        rep = np.zeros(4)
+        self._check(self._lib.efem_autotune_factor(self._h, _ptr(rep, C.c_double)), "efem_autotune_factor")
+        return {"dense_subtree": int(rep[0]), "apply_ms": float(rep[1]), "factor_s": float(rep[2]),
+                "factor_nnz": int(rep[3])}
 
     def set_option(self, name: str, value: float):
         self._check(self._lib.efem_set_option(self._h, name.encode(), float(value)), "efem_set_option")

@@ -2634,6 +2634,76 @@ int efem_time_residual(efem_ctx* c, const double* u, int count, double* ms_per_e
     return rc;
 }
 
+int efem_autotune_factor(efem_ctx* c, double* report)
+{
+    if (!c) return EFEM_ERR_ARG;
+    if (!c->parts.empty()) {
+        c->err = "efem_autotune_factor: single-problem contexts only";
+        return EFEM_ERR_STATE;
+    }
+    const int n = c->model.ndof();
+    if (n == 0) {
+        c->err = "efem_autotune_factor: no free DOF";
+        return EFEM_ERR_STATE;
+    }
+    hipEvent_t e0, e1;
+    HIP_CHECK(c, hipEventCreate(&e0));
+    HIP_CHECK(c, hipEventCreate(&e1));
+    // the apply's time does not depend on the values: a zero right-hand side
+    HIP_CHECK(c, hipMemsetAsync(c->tmp, 0, static_cast<size_t>(n + 3) * sizeof(double), c->stream));
+    const int cands[4] = {64, 128, 192, 256};
+    const int keep = c->denseSubtree;
+    int bestDs = keep;
+    double bestMs = 1e300, bestFactor = 0.0;
+    long long bestNnz = 0;
+    int rc = EFEM_OK;
+    for (int i = 0; i < 4 && rc == EFEM_OK; ++i) {
+        c->denseSubtree = cands[i];
+        c->symbolic.reset();
+        int failed = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = factorInternal(c, &failed);
+        const double fs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != EFEM_OK) break;
+        if (failed) {
+            c->err = "efem_autotune_factor: the approximate Jacobian is not SPD";
+            rc = EFEM_ERR_STATE;
+            break;
+        }
+        // one warm-up apply, then 20 timed
+        if ((rc = applyH0Internal(c, c->tmp, c->d))) break;
+        HIP_CHECK(c, hipEventRecord(e0, c->stream));
+        for (int k = 0; k < 20 && rc == EFEM_OK; ++k) rc = applyH0Internal(c, c->tmp, c->d);
+        HIP_CHECK(c, hipEventRecord(e1, c->stream));
+        HIP_CHECK(c, hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_CHECK(c, hipEventElapsedTime(&t, e0, e1));
+        const double ms = t / 20.0;
+        // a larger subtree must win by 1 %: it stores more entries
+        if (ms < 0.99 * bestMs) {
+            bestMs = ms;
+            bestDs = cands[i];
+            bestFactor = fs;
+            bestNnz = c->fac.nnz;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->denseSubtree = rc == EFEM_OK ? bestDs : keep;
+    c->symbolic.reset();
+    releaseFactor(c);
+    if (rc != EFEM_OK) return rc;
+    int failed = 0;
+    if ((rc = factorInternal(c, &failed))) return rc;
+    if (report) {
+        report[0] = bestDs;
+        report[1] = bestMs;
+        report[2] = bestFactor;
+        report[3] = static_cast<double>(bestNnz);
+    }
+    return EFEM_OK;
+}
+
 int efem_autotune(efem_ctx* c, double* report)
 {
     if (!c) return EFEM_ERR_ARG;

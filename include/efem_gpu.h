@@ -305,6 +305,16 @@ int efem_set_option(efem_ctx* ctx, const char* name, double value);
  * efem_set_option("gather_lanes" | "tet4_threads" | "reduce_blocks" |
  * "history_blocks" | "batch_threads", v). */
 int efem_autotune(efem_ctx* ctx, double* report);
+/* autotuner of the preconditioner's structure (no counterpart in the
+ * reference; single-problem contexts): factors the approximate Jacobian
+ * with dense elimination subtrees of at most 64, 128, 192 and 256 columns
+ * (option "dense_subtree"), times 20 applies of each, keeps the fastest (a
+ * larger size must be 1 % faster) and leaves the context factored with it.
+ * Every candidate is the same K^-1 up to rounding.  report[4] (may be NULL)
+ * = {dense subtree, apply ms, factorisation s (with its ordering and
+ * symbolic analysis), stored factor entries}.  EFEM_ERR_STATE when the
+ * matrix is not SPD. */
+int efem_autotune_factor(efem_ctx* ctx, double* report);
 
 /* per-iteration trace of the last efem_solve: {||r||_inf, step, trials} */
 int efem_trace(const efem_ctx* ctx, double* out, int cap);

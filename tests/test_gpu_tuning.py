@@ -73,3 +73,32 @@ def test_invalid_shapes_refused():
                         ("history_blocks", 0), ("history_blocks", 1000)):
             with pytest.raises(E.EfemError):
                 c.set_option(name, v)
+
+
+@pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
+@pytest.mark.parametrize("which,opts", [(2, [40, 8, 8]), (3, [24, 2, 8]), (1, None)])
+def test_autotune_factor_keeps_the_preconditioner(prec, tol, which, opts, rng):
+    # every dense-subtree size is the same K^-1 up to rounding: the tuned
+    # context is factored, its apply solves K, and a refactorisation keeps
+    # the tuned structure
+    import scipy.sparse.linalg as spla
+    m = E.Mesh.workload(which, opts)
+    K = O.Model(m).K(scalar=False)
+    b = rng.standard_normal(m.ndof)
+    x_ref = spla.spsolve(K.tocsc(), b)
+    with E.Context(m, prec) as c:
+        rep = c.autotune_factor()
+        print("autotune_factor", rep, flush=True)
+        assert rep["dense_subtree"] in (64, 128, 192, 256) and rep["apply_ms"] > 0
+        assert c.factor_nnz == rep["factor_nnz"] > 0
+        assert rel(c.apply_h0(b), x_ref) <= tol
+        assert c.factor()
+        assert c.factor_nnz == rep["factor_nnz"]
+        assert rel(c.apply_h0(b), x_ref) <= tol
+
+
+def test_autotune_factor_refused_on_batches():
+    ms = [E.Mesh.workload(5, [i, 3, 1.0 / 64]) for i in range(2)]
+    with E.Batch(ms, E.FP64) as b:
+        with pytest.raises(E.EfemError):
+            b.autotune_factor()
