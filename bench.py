@@ -78,6 +78,15 @@ def committed_cpu_tts():
     return out
 
 
+def apply_bytes(lay, n_free, s):
+    """Algorithmic bytes of one preconditioner apply with s-byte factor
+    values: the level sweeps read the blocks of the supernodes outside the
+    fused top twice (forward, backward), the fused top reads Z (nT^2), and
+    the right-hand side, the solution and the sweeps' vectors cost 48 bytes
+    per free node."""
+    return 2 * lay["sweep_entries"] * s + lay["top_columns"] ** 2 * s + 48 * n_free
+
+
 def measured_traffic():
     """HBM bytes per preconditioner apply from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE of one apply, copy-kernel calibrated)."""
@@ -642,7 +651,7 @@ def main():
             if k == args.steps - 1:
                 ems, ek = ctx[name].profile_read()
                 hms, hk = ctx[name].profile_read_h0()
-                prof[name] = (ems, ek, hms, hk, ctx[name].factor_nnz)
+                prof[name] = (ems, ek, hms, hk, ctx[name].factor_layout)
 
     # the configured 3 kPa: what the reference algorithm does there (the
     # oracle's own record: tests/golden/configured_load_traces.json)
@@ -719,12 +728,12 @@ def main():
         })
     # roofline of the dominant kernel family: the preconditioner apply
     # (forward + backward supernodal triangular solves), HBM-bound
-    _, _, hms, hk, nnz = prof["mixed"]
+    _, _, hms, hk, lay = prof["mixed"]
     ems, ek = elem["mixed"]
     h0_ms = hms / max(hk, 1)
-    h0_bytes = 2 * nnz * 4 + 48 * mesh.n_free
+    h0_bytes = apply_bytes(lay, mesh.n_free, 4)
     out["roofline"] = {
-        "kernel": "H0 apply (kTrsvFwd + kTrsvBwd levels), mixed",
+        "kernel": "H0 apply (kTrsvFwd + kTrsvBwd levels + kTrsvTop), mixed",
         "bound": "hbm",
         "achieved": round(h0_bytes / (h0_ms * 1e-3) / 1e9, 1),
         "peak": HBM_PEAK_GBS,
@@ -735,7 +744,8 @@ def main():
                           "calibrated on a copy kernel)",
         "bytes_per_launch": h0_bytes,
         "avg_ms": round(h0_ms, 4),
-        "factor_nnz": nnz,
+        "factor_nnz": lay["nnz"],
+        "layout": lay,
     }
     B, F = element_work(mesh, "mixed")
     e_us = 1e3 * ems / max(ek, 1)

@@ -99,6 +99,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "efem_residual_d": (C.c_int, [vp, vp, vp]),
         "efem_factor": (C.c_int, [vp, _ip]),
         "efem_factor_nnz": (C.c_longlong, [vp]),
+        "efem_factor_layout": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
         "efem_factor_release": (C.c_int, [vp]),
         "efem_factor_phases": (C.c_int, [vp, _dp]),
         "efem_profile_read_h0": (C.c_int, [vp, _dp, _ip]),
@@ -371,6 +372,16 @@ class Context:
     @property
     def factor_nnz(self) -> int:
         return int(self._lib.efem_factor_nnz(self._h))
+
+    @property
+    def factor_layout(self) -> dict:
+        """The factor's apply: stored entries, those the level sweeps read,
+        the fused top (columns, Z's leading dimension, depth), launches per
+        apply inside the solve loop, levels, supernodes."""
+        out = (C.c_longlong * 8)()
+        self._check(self._lib.efem_factor_layout(self._h, out), "efem_factor_layout")
+        keys = ("nnz", "sweep_entries", "top_columns", "top_ld", "top_depth", "launches", "levels", "supernodes")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def apply_h0(self, v) -> np.ndarray:
         v = np.ascontiguousarray(v, dtype=np.float64)

@@ -44,13 +44,17 @@ struct DeviceFactor {
     int* fwdTab = nullptr;
     std::vector<long long> fwdTabOff;
     std::vector<int> fwdK, fwdW;
-    // fused root (single-problem factors whose last level is one supernode
-    // R with no rows beyond its columns): M = L_RR^-T L_RR^-1 (ld mld) and
-    // its 64-row items, run in place of R's forward and backward levels
-    bool fusedRoot = false;
-    void* M = nullptr;
-    int mld = 0;
-    int rootOff = 0, rootCnt = 0;
+    // fused top (single-problem factors): T = the supernodes within depth
+    // topDepth of the roots, solved as z_T = Z f_T in one launch in place of
+    // their forward and backward items; Z = (L_TT L_TT^T)^-1 (topN x topN,
+    // ld topLd, FP32 or FP64 as the factor), topCol the factor column of
+    // each T-local index, topTab the K x topN update positions of f_T
+    bool fusedTop = false;
+    int topDepth = -1, topN = 0, topLd = 0, topK = 0;
+    long long sweepEntries = 0;  // stored block entries of the supernodes outside T
+    void* Z = nullptr;
+    int* topCol = nullptr;
+    int* topTab = nullptr;
     // the whole apply captured as one HIP graph over fixed staging buffers
     double *in = nullptr, *out = nullptr, *bp = nullptr;
     hipGraph_t graph = nullptr;
@@ -62,6 +66,10 @@ struct DeviceFactor {
 
 }  // namespace
 
+// columns the fused top of the elimination tree may hold (default of the
+// context option "fused_top_max"): one FP32 pass of the top kernel, a Z of
+// at most 51 MB in FP32
+constexpr int kFusedTopMaxCols = 3584;
 // levels with at least this many 4-column backward items use 16-column
 // items (default of the context option "bwd_narrow_items")
 constexpr int kNarrowMinBwdItems = 4096;
@@ -145,7 +153,15 @@ struct efem_ctx {
     bool compactTwoLoop = true;  // option "two_loop_compact" (history <= 16)
     bool lookahead = true;       // option "lookahead"
     bool fwdTables = true;       // option "fwd_tables"
-    bool fusedRootOpt = true;    // option "fused_root"
+    bool fusedRootOpt = true;    // option "fused_root": the fused top (below)
+    // option "fused_top_max": the fused top takes the deepest depth whose
+    // supernodes (with all shallower ones) hold at most this many columns
+    int fusedTopMax = kFusedTopMaxCols;
+    int topWaves = 16;           // option "top_waves": columns per block of the fused top, 8 or 16
+    // option "bwd_stage": backward sweep items that stage [y_s; -z[rows]]
+    // in LDS, 4, 8 or 16 columns each (0: the per-wave gathers), on every
+    // level but those of many short supernodes (16-column narrow items)
+    int bwdStage = 16;
     int residualTerms = 7;       // option "residual_terms" (efem_residual only)
     int narrowMinItems = kNarrowMinBwdItems;  // option "bwd_narrow_items"
     int denseSubtree = efem::kDefaultDenseSubtree;  // option "dense_subtree"
@@ -492,15 +508,24 @@ void launchApplySweeps(efem_ctx* c, const double* v, double* out, bool permuted 
     DeviceFactor& f = c->fac;
     const efem::dev::TrsvArgs a = trsvArgs(c);
     if (!permuted) efem::dev::launchPermute(a, v, f.bp, c->stream);
-    // with the fused root, the last level's forward and backward solves are
-    // one launch of z_R = M f_R
-    const int top = f.fusedRoot ? f.nlevels - 1 : f.nlevels;
-    for (int l = 0; l < top; ++l)
+    // with the fused top, the levels hold only the supernodes outside T
+    // (levels wholly inside T launch nothing) and T is one launch of
+    // z_T = Z f_T between the two sweeps
+    for (int l = 0; l < f.nlevels; ++l)
         efem::dev::launchTrsvForward(a, f.bp, f.items + 2 * f.fwdOff[l], f.fwdCnt[l], f.fwdWide[l], c->stream,
                                      f.fwdK[l] ? f.fwdTab + f.fwdTabOff[l] : nullptr, f.fwdK[l], f.fwdW[l]);
-    if (f.fusedRoot)
-        efem::dev::launchTrsvRoot(a, f.bp, f.items + 2 * f.rootOff, f.rootCnt, f.M, f.mld, c->stream);
-    for (int l = top - 1; l >= 0; --l)
+    if (f.fusedTop) {
+        efem::dev::TopArgs t;
+        t.nT = f.topN;
+        t.ldz = f.topLd;
+        t.K = f.topK;
+        t.waves = c->topWaves;
+        t.Z = f.Z;
+        t.tcol = f.topCol;
+        t.tab = f.topTab;
+        efem::dev::launchTrsvTop(a, f.bp, t, c->stream);
+    }
+    for (int l = f.nlevels - 1; l >= 0; --l)
         efem::dev::launchTrsvBackward(a, f.items + 2 * f.bwdOff[l], f.bwdCnt[l], f.bwdWide[l], c->stream);
     if (unpermute) efem::dev::launchUnpermute(a, out, c->stream);
 }
@@ -593,6 +618,60 @@ int pinnedStage(efem_ctx* c, size_t bytes, void** p)
     return EFEM_OK;
 }
 
+// Z of the fused top on the device (FP64 from the FP64 blocks of T's
+// supernodes, `super` giving each one's offset and leading dimension, then
+// FP32 for an FP32 factor) and the top kernel's tables; every buffer is the
+// factor's (the caller's AllocInto)
+int formTop(efem_ctx* c, const efem::TopPlan& P, const std::vector<efem::dev::TopSuper>& super, const double* src,
+            bool f32)
+{
+    DeviceFactor& d = c->fac;
+    int rc;
+    const size_t nz = static_cast<size_t>(P.ld) * P.nT;
+    std::vector<int4> tasks4(P.tasks.size() / 4);
+    for (size_t i = 0; i < tasks4.size(); ++i)
+        tasks4[i] = make_int4(P.tasks[4 * i], P.tasks[4 * i + 1], P.tasks[4 * i + 2], P.tasks[4 * i + 3]);
+    double* z64 = nullptr;
+    efem::dev::TopSuper* sn = nullptr;
+    int* rtl = nullptr;
+    int4* tasks = nullptr;
+    if ((rc = devAlloc(c, &z64, nz)) || (rc = upload(c, &sn, super)) || (rc = upload(c, &rtl, P.rtl)) ||
+        (rc = upload(c, &tasks, tasks4)))
+        return rc;
+    // the padding rows and the blocks between different roots stay zero
+    HIP_CHECK(c, hipMemsetAsync(z64, 0, nz * sizeof(double), c->stream));
+    efem::dev::TopBuild b;
+    b.Z = z64;
+    b.nT = P.nT;
+    b.ldz = P.ld;
+    b.B = src;
+    b.sn = sn;
+    b.rtl = rtl;
+    b.tasks = tasks;
+    for (const efem::TopPlan::Step& st : P.steps) {
+        if (st.kind == 0) efem::dev::launchTopGemm(b, st.first, st.count, c->stream);
+        else if (st.kind == 1) efem::dev::launchTopMirror(b, st.p0, st.p1, c->stream);
+        else efem::dev::launchTopDiag(b, st.first, st.count, c->stream);
+    }
+    HIP_CHECK(c, hipGetLastError());
+    if (f32) {
+        float* z32 = nullptr;
+        if ((rc = devAlloc(c, &z32, nz))) return rc;
+        efem::dev::launchFactorToF32(z64, z32, static_cast<long long>(nz), c->stream);
+        HIP_CHECK(c, hipGetLastError());
+        d.Z = z32;
+    } else {
+        d.Z = z64;
+    }
+    if ((rc = upload(c, &d.topCol, P.tcol)) || (rc = upload(c, &d.topTab, P.tab))) return rc;
+    d.fusedTop = true;
+    d.topDepth = P.depth;
+    d.topN = P.nT;
+    d.topLd = P.ld;
+    d.topK = P.K;
+    return EFEM_OK;
+}
+
 // devX: the block inverses already on the device (FP64, device layout, the
 // context's X arena, not owned by the factor): F holds the structure only
 int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>* snOffset,
@@ -628,9 +707,17 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     // forward-sweep extend-add map (children's update rows -> front rows)
     std::vector<int> fptr, fidx;
     efem::frontGatherMap(F, tOff, fptr, fidx);
+    // the fused top (single problems): its supernodes leave the levels
+    efem::TopPlan top;
+    const bool fused = !snOffset && c->fusedRootOpt &&
+                       efem::planTop(F, nt, c->fusedTopMax, efem::dev::kTopMaxK, efem::dev::kLdAlign, top);
+    auto inTop = [&](int s) { return fused && top.inT[s]; };
+    for (int s = 0; s < F.nsuper; ++s)
+        if (!inTop(s)) d.sweepEntries += static_cast<long long>(w[s]) * mm[s];
     // work-item descriptors per level
     std::vector<std::vector<int>> byLevel(F.nlevels);
-    for (int s = 0; s < F.nsuper; ++s) byLevel[F.level[s]].push_back(s);
+    for (int s = 0; s < F.nsuper; ++s)
+        if (!inTop(s)) byLevel[F.level[s]].push_back(s);
     std::vector<int4> items;
     auto desc1 = [&](int a0, int a1, long long val) {
         return make_int4(a0, a1, static_cast<int>(static_cast<unsigned>(val & 0xffffffffLL)),
@@ -650,6 +737,7 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         // 16-wave items lose on levels of >500 forward / >1000 backward items)
         std::vector<int> nf(F.nlevels, 0), nb(F.nlevels, 0), maxw(F.nlevels, 0), maxm(F.nlevels, 0);
         for (int s = 0; s < F.nsuper; ++s) {
+            if (inTop(s)) continue;
             const int l = F.level[s];
             nf[l] += (mm[s] + 63) / 64;
             nb[l] += (w[s] + 3) / 4;
@@ -663,6 +751,12 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
             // within what the chip holds at once (not in batches, whose
             // persistent kernel walks 4-column items)
             if (!snOffset && nb[l] >= c->narrowMinItems) d.bwdWide[l] = efem::dev::kBwdNarrow;
+            // staged backward items on the other levels (batches: 4
+            // columns, the items their persistent kernel walks)
+            if (c->bwdStage > 0 && d.bwdWide[l] != efem::dev::kBwdNarrow)
+                d.bwdWide[l] = snOffset || c->bwdStage == 4 ? efem::dev::kBwdStage4
+                               : c->bwdStage == 8           ? efem::dev::kBwdStage8
+                                                            : efem::dev::kBwdStage16;
         }
     }
     for (int l = 0; l < F.nlevels; ++l) {
@@ -705,57 +799,15 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     }
     for (int l = 0; l < F.nlevels; ++l) {
         d.bwdOff[l] = static_cast<int>(items.size() / 2);
-        const int cols = d.bwdWide[l] == efem::dev::kBwdNarrow ? 16 : 4;
+        const int cols = d.bwdWide[l] == efem::dev::kBwdNarrow ? 16
+                         : d.bwdWide[l] >= efem::dev::kBwdStage4 ? efem::dev::bwdStageCols(d.bwdWide[l])
+                                                                 : 4;
         for (int s : byLevel[l])
             for (int k0 = 0; k0 < w[s]; k0 += cols) {
                 items.push_back(make_int4(k0, w[s], mm[s], F.first[s]));
                 items.push_back(desc1(F.rowPtr[s], 0, dval[s]));
             }
         d.bwdCnt[l] = static_cast<int>(items.size() / 2) - d.bwdOff[l];
-    }
-    // fused root: the last level is one supernode R whose rows are its
-    // columns; M = B11^T B11 = L_RR^-T L_RR^-1 on the host (FP64, column
-    // blocks in parallel, the zero upper triangle of B11 skipped)
-    std::vector<double> Mh;
-    int rootS = -1;
-    if (!snOffset && c->fusedRootOpt && F.nlevels >= 1 && byLevel[F.nlevels - 1].size() == 1) {
-        const int s = byLevel[F.nlevels - 1][0];
-        if (mm[s] == w[s] && F.parent[s] < 0) rootS = s;
-    }
-    if (rootS >= 0 && devX) {
-        // M on the device from the root's block (formed below)
-        d.fusedRoot = true;
-        d.mld = efem::dev::ldOf(w[rootS]);
-        d.rootOff = static_cast<int>(items.size() / 2);
-        for (int r0 = 0; r0 < w[rootS]; r0 += 64) {
-            items.push_back(make_int4(r0, w[rootS], w[rootS], F.first[rootS]));
-            items.push_back(make_int4(F.rowPtr[rootS], 0, 0, 0));
-        }
-        d.rootCnt = static_cast<int>(items.size() / 2) - d.rootOff;
-    } else if (rootS >= 0) {
-        const int wr = w[rootS];
-        const int ld = efem::dev::ldOf(wr);
-        const double* B11 = F.B.get() + F.valPtr[rootS];  // column-major, ld wr
-        std::vector<double> Bt(static_cast<size_t>(wr) * wr);
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < wr; ++i)
-            for (int k = 0; k < wr; ++k) Bt[i + static_cast<size_t>(k) * wr] = B11[k + static_cast<size_t>(i) * wr];
-        Mh.assign(static_cast<size_t>(ld) * wr, 0.0);
-        const int nblk = (wr + 63) / 64;
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int b = 0; b < nblk; ++b) {
-            const int j0 = 64 * b, j1 = std::min(wr, j0 + 64);
-            efem::dense::gemm(Mh.data() + static_cast<size_t>(j0) * ld, ld, Bt.data() + static_cast<size_t>(j0) * wr,
-                              wr, B11 + j0 + static_cast<size_t>(j0) * wr, wr, false, wr, j1 - j0, wr - j0, 1.0);
-        }
-        d.fusedRoot = true;
-        d.mld = ld;
-        d.rootOff = static_cast<int>(items.size() / 2);
-        for (int r0 = 0; r0 < wr; r0 += 64) {
-            items.push_back(make_int4(r0, wr, wr, F.first[rootS]));
-            items.push_back(make_int4(F.rowPtr[rootS], 0, 0, 0));
-        }
-        d.rootCnt = static_cast<int>(items.size() / 2) - d.rootOff;
     }
     if (snOffset) {
         // items of one instance are contiguous within each level (supernodes
@@ -806,22 +858,6 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         } else {
             d.B = const_cast<double*>(devX);
         }
-        if (d.fusedRoot) {
-            const int wr = w[rootS];
-            double* m64 = nullptr;
-            if ((rc = devAlloc(c, &m64, static_cast<size_t>(d.mld) * wr))) return rc;
-            HIP_CHECK(c, hipMemsetAsync(m64, 0, static_cast<size_t>(d.mld) * wr * sizeof(double), c->stream));
-            efem::dev::launchFactorGram(devX + dval[rootS], wr, d.mld, m64, c->stream);
-            if (fac32(c)) {
-                float* m32 = nullptr;
-                if ((rc = devAlloc(c, &m32, static_cast<size_t>(d.mld) * wr))) return rc;
-                efem::dev::launchFactorToF32(m64, m32, static_cast<long long>(d.mld) * wr, c->stream);
-                d.M = m32;
-            } else {
-                d.M = m64;
-            }
-            HIP_CHECK(c, hipGetLastError());
-        }
         HIP_CHECK(c, hipStreamSynchronize(c->stream));
         packS = since(p0);
     } else {
@@ -848,17 +884,36 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
         copyS = since(q0);
         d.B = q;
     }
-    if (d.fusedRoot && !devX) {
-        if (fac32(c)) {
-            std::vector<float> m32(Mh.begin(), Mh.end());
-            float* q32 = nullptr;
-            if ((rc = upload(c, &q32, m32))) return rc;
-            d.M = q32;
-        } else {
-            double* q64 = nullptr;
-            if ((rc = upload(c, &q64, Mh))) return rc;
-            d.M = q64;
+    if (fused) {
+        // Z from the FP64 block inverses of T: the X arena in place (device
+        // factorisation), else T's blocks of the host factor uploaded
+        const double* src = devX;
+        std::vector<efem::dev::TopSuper> super(top.super.size());
+        std::vector<double> hb;
+        for (size_t q = 0; q < top.super.size(); ++q) {
+            const efem::TopSuperInfo& t = top.super[q];
+            efem::dev::TopSuper& o = super[q];
+            o.cs = t.cs;
+            o.w = t.w;
+            o.m = t.m;
+            o.rOff = t.rOff;
+            o.pad = 0;
+            if (devX) {
+                o.ld = efem::dev::ldOf(t.m);
+                o.boff = dval[t.s];
+            } else {
+                o.ld = t.m;
+                o.boff = static_cast<long long>(hb.size());
+                const double* b0 = F.B.get() + F.valPtr[t.s];
+                hb.insert(hb.end(), b0, b0 + static_cast<size_t>(t.w) * t.m);
+            }
         }
+        if (!devX) {
+            double* q64 = nullptr;
+            if ((rc = upload(c, &q64, hb))) return rc;
+            src = q64;
+        }
+        if ((rc = formTop(c, top, super, src, fac32(c)))) return rc;
     }
     if ((rc = upload(c, &d.fptr, fptr))) return rc;
     if ((rc = upload(c, &d.fidx, fidx))) return rc;
@@ -1446,6 +1501,25 @@ int efem_factor(efem_ctx* c, int* failed)
 }
 
 long long efem_factor_nnz(const efem_ctx* c) { return c && c->fac.ready ? c->fac.nnz : 0; }
+
+int efem_factor_layout(const efem_ctx* c, long long* out8)
+{
+    if (!c || !out8) return EFEM_ERR_ARG;
+    const DeviceFactor& f = c->fac;
+    std::fill(out8, out8 + 8, 0LL);
+    if (!f.ready) return EFEM_ERR_STATE;
+    int launches = f.fusedTop ? 1 : 0;
+    for (int l = 0; l < f.nlevels; ++l) launches += (f.fwdCnt[l] > 0) + (f.bwdCnt[l] > 0);
+    out8[0] = f.nnz;
+    out8[1] = f.sweepEntries;
+    out8[2] = f.fusedTop ? f.topN : 0;
+    out8[3] = f.fusedTop ? f.topLd : 0;
+    out8[4] = f.fusedTop ? f.topDepth : -1;
+    out8[5] = launches;
+    out8[6] = f.nlevels;
+    out8[7] = f.nsuper;
+    return EFEM_OK;
+}
 
 int efem_factor_phases(const efem_ctx* c, double* out8)
 {
@@ -2035,8 +2109,8 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         return EFEM_OK;
     }
     if (k == "fused_root") {
-        // the last supernode's forward and backward solves as one launch
-        // z_R = M f_R; takes effect at the next factorisation
+        // the top of the elimination tree (at least the roots) solved as one
+        // launch z_T = Z f_T; takes effect at the next factorisation
         c->fusedRootOpt = value != 0.0;
         releaseFactor(c);
         return EFEM_OK;
@@ -2087,6 +2161,24 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         // takes effect at the next factorisation
         (k == "wide_fwd_items" ? c->wideFwdItems : c->wideBwdItems) =
             value >= 2147483647.0 ? 2147483647 : iv;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
+    if (k == "bwd_stage" && (iv == 0 || iv == 4 || iv == 8 || iv == 16) && static_cast<double>(iv) == value) {
+        // takes effect at the next factorisation
+        c->bwdStage = iv;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
+    if (k == "top_waves" && (iv == 8 || iv == 16) && static_cast<double>(iv) == value) {
+        c->topWaves = iv;
+        return EFEM_OK;
+    }
+    if (k == "fused_top_max" && iv >= 1 && iv <= 65536 && static_cast<double>(iv) == value) {
+        // columns the fused top may hold: it takes the deepest depth whose
+        // supernodes, with all shallower ones, fit (none if the roots alone
+        // do not); takes effect at the next factorisation
+        c->fusedTopMax = iv;
         releaseFactor(c);
         return EFEM_OK;
     }

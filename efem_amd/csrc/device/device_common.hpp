@@ -600,82 +600,6 @@ __device__ __forceinline__ void trsvFwdItem(const TrsvArgs& a, const double* __r
     out[2] = static_cast<T>(f2) - acc2;
 }
 
-// One item of the fused root: 64 rows [r0, r0 + 64) of the last supernode
-// R (a root whose rows are its w columns) solved in one step from its front
-// f_R = its rows of the permuted right-hand side plus its children's update
-// vectors (the forward sweep's extend-add, in the same order):
-// z_R = M f_R, M = L_RR^-T L_RR^-1 = B11^T B11 (w x w, column-major, ld ldm),
-// which is the forward and the backward solve of R together.  f_R is staged
-// in LDS kMaxW columns at a time, so R may be wider than one staging
-// buffer.  Descriptor d0 = {r0, w, w, j0}, d1 = {front row offset, 0, 0, 0}.
-// Contains barriers; every lane must call it.
-template <typename T, int WAVES = 16>
-__device__ __forceinline__ void trsvRootItem(const TrsvArgs& a, const double* __restrict__ bp, int4 d0, int4 d1,
-                                             const T* __restrict__ M, int ldm, TrsvFwdLds<T, WAVES>& L)
-{
-    constexpr int NT = 64 * WAVES;
-    constexpr int NBT = 8;
-    const int r0 = d0.x, w = d0.y, j0 = d0.w, fr0 = d1.x;
-    const T* u = static_cast<const T*>(a.t);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = r0 + lane;
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    for (int c0 = 0; c0 < w; c0 += kMaxW) {
-        const int cw = min(kMaxW, w - c0);
-        for (int k = threadIdx.x; k < cw; k += NT) {
-            const double* bk = bp + 3 * static_cast<size_t>(j0 + c0 + k);
-            double v0 = bk[0], v1 = bk[1], v2 = bk[2];
-            const int p1 = a.fptr[fr0 + c0 + k + 1];
-            for (int p = a.fptr[fr0 + c0 + k]; p < p1; ++p) {
-                const T* uv = u + 3 * static_cast<size_t>(a.fidx[p]);
-                v0 += static_cast<double>(uv[0]);
-                v1 += static_cast<double>(uv[1]);
-                v2 += static_cast<double>(uv[2]);
-            }
-            L.sb[3 * k] = static_cast<T>(v0);
-            L.sb[3 * k + 1] = static_cast<T>(v1);
-            L.sb[3 * k + 2] = static_cast<T>(v2);
-        }
-        __syncthreads();
-        const int kc = (cw + WAVES - 1) / WAVES;
-        const int kb = wave * kc;
-        const int ke = r < w ? min(cw, kb + kc) : kb;
-        const T* Mc = M + r + static_cast<size_t>(c0) * ldm;
-        for (int k = kb; k < ke; k += NBT) {
-            T b[NBT];
-#pragma unroll
-            for (int q = 0; q < NBT; ++q) b[q] = k + q < ke ? Mc[static_cast<size_t>(k + q) * ldm] : T(0);
-#pragma unroll
-            for (int q = 0; q < NBT; ++q) {
-                if (k + q < ke) {
-                    acc0 += b[q] * L.sb[3 * (k + q)];
-                    acc1 += b[q] * L.sb[3 * (k + q) + 1];
-                    acc2 += b[q] * L.sb[3 * (k + q) + 2];
-                }
-            }
-        }
-        __syncthreads();
-    }
-    L.red[wave][lane][0] = acc0;
-    L.red[wave][lane][1] = acc1;
-    L.red[wave][lane][2] = acc2;
-    __syncthreads();
-    if (wave != 0 || r >= w) return;
-    acc0 = L.red[0][lane][0];
-    acc1 = L.red[0][lane][1];
-    acc2 = L.red[0][lane][2];
-#pragma unroll
-    for (int q = 1; q < WAVES; ++q) {
-        acc0 += L.red[q][lane][0];
-        acc1 += L.red[q][lane][1];
-        acc2 += L.red[q][lane][2];
-    }
-    T* zo = static_cast<T*>(a.z) + 3 * static_cast<size_t>(j0 + r);
-    zo[0] = acc0;
-    zo[1] = acc1;
-    zo[2] = acc2;
-}
-
 // One backward item: four columns k0 .. k0 + 3 of supernode s.  With
 // WAVES = 4 one wave per column (no barrier); with WAVES = 16 four waves
 // per column split its rows and are summed in order through LDS (contains a
@@ -825,6 +749,103 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
         zo[0] = s0;
         zo[1] = s1;
         zo[2] = s2;
+    }
+}
+
+// Staged backward item: columns k0 .. k0 + W - 1 of supernode s by W waves,
+// one column each.  The right-hand side v = [y_s; -z[rows]] is staged in
+// LDS once per item, CH rows at a time from the 64-row line holding k0
+// (rows above a column meet the zero upper triangle of L11^-1), and shared
+// by the W columns; each wave issues its column's loads of a chunk before
+// the block gathers the chunk's v (every load of the chunk in flight at
+// once).  Row r goes to lane r mod 64 in increasing order, so a column's
+// sum does not depend on how columns are grouped into items.  Descriptor
+// d0 = {k0, w, m, j0} (k0 a multiple of W), d1 = {row offset, 0, B offset
+// lo, hi}.  Contains barriers; every lane must call it.
+template <typename T>
+struct BwdStage {
+    static constexpr int NB = sizeof(T) == 4 ? 32 : 16;  // column loads per lane per chunk
+    static constexpr int CH = 64 * NB;                    // rows per chunk (LDS 24 KB)
+};
+template <typename T>
+struct TrsvBwdStageLds {
+    T v[3][BwdStage<T>::CH];
+};
+
+template <typename T, int W>
+__device__ __forceinline__ void trsvBwdStageItem(const TrsvArgs& a, int4 d0, int4 d1, TrsvBwdStageLds<T>& L)
+{
+    constexpr int NT = 64 * W, NB = BwdStage<T>::NB, CH = BwdStage<T>::CH;
+    constexpr int RPT = (CH + NT - 1) / NT;  // rows of v each lane stages per chunk
+    const int k0 = d0.x, w = d0.y, m = d0.z, j0 = d0.w;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = k0 + wave;
+    const bool own = k < w;
+    const int* rows = a.rows + d1.x;
+    const T* y = static_cast<const T*>(a.y);
+    T* z = static_cast<T*>(a.z);
+    const T* cl = static_cast<const T*>(a.B) + itemVal(d1) + static_cast<size_t>(min(k, w - 1)) * ldOf(m);
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    const int base = k0 & ~63;
+    for (int c0 = base; c0 < m; c0 += CH) {
+        const int cn = min(CH, m - c0);
+        T bk[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int r = c0 + lane + 64 * i;
+            bk[i] = own && r < m ? cl[r] : T(0);
+        }
+        if (c0 != base) __syncthreads();  // the previous chunk is done with L.v
+        const int cr = (cn + 63) & ~63;   // rows staged (zeros past cn): the rows the columns read
+        int gi[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int i = threadIdx.x + j * NT, r = c0 + i;
+            gi[j] = i < cn && r >= w ? rows[r] : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int i = threadIdx.x + j * NT, r = c0 + i;
+            if (i >= cr) break;
+            T v0 = T(0), v1 = T(0), v2 = T(0);
+            if (i < cn) {
+                if (r < w) {
+                    const T* src = y + 3 * static_cast<size_t>(j0 + r);
+                    v0 = src[0];
+                    v1 = src[1];
+                    v2 = src[2];
+                } else {
+                    const T* src = z + 3 * static_cast<size_t>(gi[j]);
+                    v0 = -src[0];
+                    v1 = -src[1];
+                    v2 = -src[2];
+                }
+            }
+            L.v[0][i] = v0;
+            L.v[1][i] = v1;
+            L.v[2][i] = v2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            if (64 * i >= cn) break;
+            const int q = lane + 64 * i;
+            acc0 += bk[i] * L.v[0][q];
+            acc1 += bk[i] * L.v[1][q];
+            acc2 += bk[i] * L.v[2][q];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+        acc2 += __shfl_xor(acc2, off, 64);
+    }
+    if (lane == 0 && own) {
+        T* zo = z + 3 * static_cast<size_t>(j0 + k);
+        zo[0] = acc0;
+        zo[1] = acc1;
+        zo[2] = acc2;
     }
 }
 

@@ -405,22 +405,29 @@ __global__ __launch_bounds__(256) void kFacToF32(const double* __restrict__ X, f
     if (i < n) B[i] = static_cast<float>(X[i]);
 }
 
-// M = X^T X of one w x w block (ld), tile (it, jt); M column-major, ld
-__global__ __launch_bounds__(256) void kFacGram(const double* __restrict__ Xr, int wr, int ld, double* __restrict__ M)
+// ---- the fused top: Z = (L_TT L_TT^T)^-1 by selected inversion -----------
+// Tile (r0, j0) of the column block of T supernode s:
+// Z[r, cs + j] = -sum_k Z[r, rtl[rOff + k]] B21_s[k, j] for r0 <= r < row end,
+// k over the rows of s below its columns (their columns of Z are complete)
+__global__ __launch_bounds__(256) void kTopGemm(TopBuild b, int first)
 {
     __shared__ double As[TB][TB + 1], Bs[TB][TB + 1];
-    const int it = blockIdx.x, jt = blockIdx.y;
-    const int ri = TB * it, rj = TB * jt;
+    const int4 t4 = b.tasks[first + blockIdx.x];
+    const TopSuper sn = b.sn[t4.x];
+    const int r0 = t4.y, j0 = t4.z;
+    const int nr = sn.m - sn.w;
+    const int arows = min(TB, t4.w - r0), bcols = min(TB, sn.w - j0);
+    const int* rt = b.rtl + sn.rOff;
+    const double* B21 = b.B + sn.boff + sn.w;
     const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
     TileAcc acc{};
-    // M(i, j) = sum_k X(k, i) X(k, j), k >= max(i, j) (X lower triangular)
-    for (int k0 = min(ri, rj) / TB * TB; k0 < wr; k0 += TB) {
-        const int kc = min(TB, wr - k0);
+    for (int k0 = 0; k0 < nr; k0 += TB) {
+        const int kc = min(TB, nr - k0);
         __syncthreads();
         for (int e = tid; e < TB * TB; e += 256) {
             const int i = e % TB, j = e / TB;
-            As[i][j] = (ri + i < wr && j < kc) ? Xr[k0 + j + static_cast<size_t>(ri + i) * ld] : 0.0;
-            Bs[i][j] = (rj + i < wr && j < kc) ? Xr[k0 + j + static_cast<size_t>(rj + i) * ld] : 0.0;
+            As[i][j] = (i < arows && j < kc) ? b.Z[(r0 + i) + static_cast<size_t>(rt[k0 + j]) * b.ldz] : 0.0;
+            Bs[j][i] = (i < kc && j < bcols) ? B21[(k0 + i) + static_cast<size_t>(j0 + j) * sn.ld] : 0.0;
         }
         __syncthreads();
         for (int j = 0; j < kc; ++j) {
@@ -440,12 +447,95 @@ __global__ __launch_bounds__(256) void kFacGram(const double* __restrict__ Xr, i
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = ri + 4 * tr + p, j = rj + 4 * tc + q;
-            if (i < wr && j < wr) M[i + static_cast<size_t>(j) * ld] = acc.v[p][q];
+            const int i = 4 * tr + p, j = 4 * tc + q;
+            if (i < arows && j < bcols)
+                b.Z[(r0 + i) + static_cast<size_t>(sn.cs + j0 + j) * b.ldz] = -acc.v[p][q];
+        }
+}
+
+// Z[j, i] = Z[i, j] for i < p0 <= j < p1 (64 x 64 tiles through LDS)
+__global__ __launch_bounds__(256) void kTopMirror(TopBuild b, int p0, int p1)
+{
+    __shared__ double t[TB][TB + 1];
+    const int i0 = blockIdx.x * TB, j0 = p0 + blockIdx.y * TB;
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int i = e % TB, j = e / TB;
+        if (i0 + i < p0 && j0 + j < p1) t[i][j] = b.Z[(i0 + i) + static_cast<size_t>(j0 + j) * b.ldz];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int j = e % TB, i = e / TB;
+        if (i0 + i < p0 && j0 + j < p1) b.Z[(j0 + j) + static_cast<size_t>(i0 + i) * b.ldz] = t[i][j];
+    }
+}
+
+// Tile (i0, j0) of the diagonal block of s: Z[cs + i, cs + j] =
+// sum_{r < m} B_s[r, i] V[r, j], V[r, j] = B_s[r, j] (r < w: B11^T B11, the
+// zero upper triangle included) or -Z[rtl[rOff + r - w], cs + j] (r >= w:
+// -B21^T Z[R_s, s], the column block's rows on R_s formed before)
+__global__ __launch_bounds__(256) void kTopDiag(TopBuild b, int first)
+{
+    __shared__ double As[TB][TB + 1], Vs[TB][TB + 1];
+    const int4 t4 = b.tasks[first + blockIdx.x];
+    const TopSuper sn = b.sn[t4.x];
+    const int i0 = t4.y, j0 = t4.z;
+    const int ni = min(TB, sn.w - i0), nj = min(TB, sn.w - j0);
+    const int* rt = b.rtl + sn.rOff;
+    const double* Bp = b.B + sn.boff;
+    const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
+    TileAcc acc{};
+    for (int k0 = 0; k0 < sn.m; k0 += TB) {
+        const int kc = min(TB, sn.m - k0);
+        __syncthreads();
+        for (int e = tid; e < TB * TB; e += 256) {
+            const int kk = e % TB, c = e / TB, r = k0 + kk;
+            As[c][kk] = (kk < kc && c < ni) ? Bp[r + static_cast<size_t>(i0 + c) * sn.ld] : 0.0;
+            double v = 0.0;
+            if (kk < kc && c < nj)
+                v = r < sn.w ? Bp[r + static_cast<size_t>(j0 + c) * sn.ld]
+                             : -b.Z[rt[r - sn.w] + static_cast<size_t>(sn.cs + j0 + c) * b.ldz];
+            Vs[c][kk] = v;
+        }
+        __syncthreads();
+        for (int k = 0; k < kc; ++k) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = As[4 * tr + q][k];
+                bv[q] = Vs[4 * tc + q][k];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc.v[p][q] += av[p] * bv[q];
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * tr + p, j = 4 * tc + q;
+            if (i < ni && j < nj)
+                b.Z[(sn.cs + i0 + i) + static_cast<size_t>(sn.cs + j0 + j) * b.ldz] = acc.v[p][q];
         }
 }
 
 }  // namespace
+
+void launchTopGemm(const TopBuild& b, int first, int count, hipStream_t s)
+{
+    if (count > 0) kTopGemm<<<count, 256, 0, s>>>(b, first);
+}
+
+void launchTopMirror(const TopBuild& b, int p0, int p1, hipStream_t s)
+{
+    if (p0 > 0 && p1 > p0) kTopMirror<<<dim3((p0 + TB - 1) / TB, (p1 - p0 + TB - 1) / TB), 256, 0, s>>>(b, p0, p1);
+}
+
+void launchTopDiag(const TopBuild& b, int first, int count, hipStream_t s)
+{
+    if (count > 0) kTopDiag<<<count, 256, 0, s>>>(b, first);
+}
 
 void launchDeviceFactor(const DevFactorArgs& d, hipStream_t s)
 {
@@ -485,13 +575,6 @@ void launchDeviceFactor(const DevFactorArgs& d, hipStream_t s)
 void launchFactorToF32(const double* X, void* B, long long n, hipStream_t s)
 {
     if (n > 0) kFacToF32<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(X, static_cast<float*>(B), n);
-}
-
-void launchFactorGram(const double* Xr, int wr, int ld, double* M, hipStream_t s)
-{
-    if (wr <= 0) return;
-    const int nt = (wr + TB - 1) / TB;
-    kFacGram<<<dim3(nt, nt), 256, 0, s>>>(Xr, wr, ld, M);
 }
 
 }  // namespace dev

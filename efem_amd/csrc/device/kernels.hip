@@ -1053,13 +1053,110 @@ __global__ __launch_bounds__(64 * WAVES) void kTrsvFwdTab(TrsvArgs a, const doub
     trsvFwdItem<T, WAVES, KT>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L, ft);
 }
 
-// the fused root: z_R = M f_R, one 64-row item per block
+// The fused top: z[tcol[k]] = sum_i Z[i, k] f[i], one wave per column k,
+// W columns per block.  Each pass covers RC rows: the block first loads the
+// indices of its rows of f (each lane RPT rows: the table's update
+// positions and the factor column), then each wave issues its column
+// segment (NQ 16-byte loads per lane, VE consecutive rows each), then the
+// values of f (the permuted right-hand side plus the children's update
+// rows, summed in FP64 in table order) go to LDS, and the column is
+// multiplied and reduced in a fixed butterfly.  Rows past nT read Z's zero
+// padding against zeros in LDS.
 template <typename T>
-__global__ __launch_bounds__(1024) void kTrsvRoot(TrsvArgs a, const double* __restrict__ bp,
-                                                  const int4* __restrict__ items, const T* __restrict__ M, int ldm)
+struct TopVec;
+template <>
+struct TopVec<float> {
+    using V = float4;
+};
+template <>
+struct TopVec<double> {
+    using V = double2;
+};
+
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void kTrsvTop(TrsvArgs a, const double* __restrict__ bp, TopArgs t)
 {
-    __shared__ TrsvFwdLds<T, 16> L;
-    trsvRootItem<T, 16>(a, bp, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], M, ldm, L);
+    using V = typename TopVec<T>::V;
+    constexpr int NT = 64 * W;
+    constexpr int VE = 16 / sizeof(T);      // values per 16-byte load
+    constexpr int NQ = 14;                  // loads per lane per pass
+    constexpr int RC = NQ * 64 * VE;        // rows per pass: 3584 (FP32), 1792 (FP64); LDS 43 KB
+    constexpr int RPT = (RC + NT - 1) / NT; // rows of f each lane stages per pass
+    __shared__ T sf[3][RC];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * W + wave;
+    const T* col = static_cast<const T*>(t.Z) + static_cast<size_t>(min(k, t.nT - 1)) * t.ldz;
+    const T* u = static_cast<const T*>(a.t);
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    for (int p0 = 0; p0 < t.nT; p0 += RC) {
+        const int pn = min(RC, t.nT - p0);
+        int tc[RPT], q[RPT][kTopMaxK];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = threadIdx.x + r * NT;
+            const bool on = i < pn;
+            tc[r] = on ? t.tcol[p0 + i] : -1;
+#pragma unroll
+            for (int j = 0; j < kTopMaxK; ++j) q[r][j] = on && j < t.K ? t.tab[static_cast<size_t>(j) * t.nT + p0 + i] : -1;
+        }
+        V zq[NQ];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            const int r = p0 + (c * 64 + lane) * VE;
+            // ldz is a multiple of 32 rows: a 16-byte vector lies wholly
+            // inside the column or wholly past it
+            zq[c] = r < t.ldz ? *reinterpret_cast<const V*>(col + r) : V{};
+        }
+        if (p0 > 0) __syncthreads();  // the previous pass is done with sf
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = threadIdx.x + r * NT;
+            if (i >= RC) break;
+            double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+            if (tc[r] >= 0) {
+                const double* bk = bp + 3 * static_cast<size_t>(tc[r]);
+                f0 = bk[0];
+                f1 = bk[1];
+                f2 = bk[2];
+#pragma unroll
+                for (int j = 0; j < kTopMaxK; ++j) {
+                    if (q[r][j] >= 0) {
+                        const T* up = u + 3 * static_cast<size_t>(q[r][j]);
+                        f0 += static_cast<double>(up[0]);
+                        f1 += static_cast<double>(up[1]);
+                        f2 += static_cast<double>(up[2]);
+                    }
+                }
+            }
+            sf[0][i] = static_cast<T>(f0);
+            sf[1][i] = static_cast<T>(f1);
+            sf[2][i] = static_cast<T>(f2);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            const int r = (c * 64 + lane) * VE;
+            const T* zv = reinterpret_cast<const T*>(&zq[c]);
+#pragma unroll
+            for (int v = 0; v < VE; ++v) {
+                acc0 += zv[v] * sf[0][r + v];
+                acc1 += zv[v] * sf[1][r + v];
+                acc2 += zv[v] * sf[2][r + v];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+        acc2 += __shfl_xor(acc2, off, 64);
+    }
+    if (lane == 0 && k < t.nT) {
+        T* zo = static_cast<T*>(a.z) + 3 * static_cast<size_t>(t.tcol[k]);
+        zo[0] = acc0;
+        zo[1] = acc1;
+        zo[2] = acc2;
+    }
 }
 
 template <typename T, int WAVES>
@@ -1067,6 +1164,13 @@ __global__ __launch_bounds__(64 * WAVES) void kTrsvBwd(TrsvArgs a, const int4* _
 {
     __shared__ T red[WAVES][3];
     trsvBwdItem<T, WAVES>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], red);
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void kTrsvBwdStage(TrsvArgs a, const int4* __restrict__ items)
+{
+    __shared__ TrsvBwdStageLds<T> L;
+    trsvBwdStageItem<T, W>(a, items[2 * blockIdx.x], items[2 * blockIdx.x + 1], L);
 }
 
 template <typename T>
@@ -1475,19 +1579,43 @@ void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, i
     }
 }
 
-void launchTrsvRoot(const TrsvArgs& a, const double* bp, const int4* items, int nitems, const void* M, int ldm,
-                    hipStream_t s)
+void launchTrsvTop(const TrsvArgs& a, const double* bp, const TopArgs& t, hipStream_t s)
 {
-    if (nitems <= 0) return;
+    if (t.nT <= 0) return;
+    if (t.waves == 8) {
+        const int grid = (t.nT + 7) / 8;
+        if (a.mixed)
+            kTrsvTop<float, 8><<<grid, 512, 0, s>>>(a, bp, t);
+        else
+            kTrsvTop<double, 8><<<grid, 512, 0, s>>>(a, bp, t);
+        return;
+    }
+    const int grid = (t.nT + 15) / 16;
     if (a.mixed)
-        kTrsvRoot<float><<<nitems, 1024, 0, s>>>(a, bp, items, static_cast<const float*>(M), ldm);
+        kTrsvTop<float, 16><<<grid, 1024, 0, s>>>(a, bp, t);
     else
-        kTrsvRoot<double><<<nitems, 1024, 0, s>>>(a, bp, items, static_cast<const double*>(M), ldm);
+        kTrsvTop<double, 16><<<grid, 1024, 0, s>>>(a, bp, t);
+}
+
+template <typename T>
+void bwdStageLaunch(const TrsvArgs& a, const int4* items, int nitems, int W, hipStream_t s)
+{
+    switch (W) {
+        case 4: kTrsvBwdStage<T, 4><<<nitems, 256, 0, s>>>(a, items); break;
+        case 8: kTrsvBwdStage<T, 8><<<nitems, 512, 0, s>>>(a, items); break;
+        default: kTrsvBwdStage<T, 16><<<nitems, 1024, 0, s>>>(a, items); break;
+    }
 }
 
 void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s)
 {
     if (nitems <= 0) return;
+    if (wide >= kBwdStage4) {
+        const int W = bwdStageCols(wide);
+        if (a.mixed) bwdStageLaunch<float>(a, items, nitems, W, s);
+        else bwdStageLaunch<double>(a, items, nitems, W, s);
+        return;
+    }
     if (wide == kBwdNarrow) {
         if (a.mixed)
             kTrsvBwdNarrow<float><<<nitems, 256, 0, s>>>(a, items);

@@ -234,13 +234,56 @@ void launchTrsvForward(const TrsvArgs& a, const double* bp, const int4* items, i
 // column split its rows, for levels of tall supernodes; wide = kBwdNarrow:
 // sixteen columns per item, k0 = 0 mod 16, for levels of many short ones)
 constexpr int kBwdNarrow = 2;
-// the fused root (the last supernode R, whose rows are its columns):
-// z_R = M f_R with M = L_RR^-T L_RR^-1 (w x w column-major, ld ldm, FP32 or
-// FP64 as the factor), f_R its front (permuted right-hand side + children's
-// update vectors): R's forward and backward solve in one launch of 64-row
-// items (descriptors {r0, w, w, j0}, {front row offset, 0, 0, 0})
-void launchTrsvRoot(const TrsvArgs& a, const double* bp, const int4* items, int nitems, const void* M, int ldm,
-                    hipStream_t s);
+// wide = kBwdStage4 / 8 / 16: staged items of 4, 8 or 16 columns (k0 a
+// multiple of that), one wave per column, the right-hand side [y_s;
+// -z[rows]] gathered into LDS once per item
+constexpr int kBwdStage4 = 3, kBwdStage8 = 4, kBwdStage16 = 5;
+__host__ __device__ constexpr int bwdStageCols(int wide) { return wide == kBwdStage4 ? 4 : wide == kBwdStage8 ? 8 : 16; }
+// The fused top of the elimination tree.  T = the supernodes within depth D
+// of the roots (closed under ancestors), nT columns numbered T-locally by
+// depth, then factor column.  With L_TT the factor's TT block, the TT block
+// of (P A P^T)^-1 is Z = (L_TT L_TT^T)^-1 (dense, symmetric, column-major,
+// ld ldz, zero padding rows), and T's forward and backward solves are one
+// product z_T = Z f_T, f_T = the permuted right-hand side on T plus the
+// update rows of T's children outside T (their extend-add):
+//   z[tcol[k]] = sum_i Z[i, k] f[i],
+//   f[i] = bp[tcol[i]] + sum_{j < K} u[tab[j nT + i]]   (j in order; absent
+//   entries name the zero slot after the last update row)
+// One wave per column k, `waves` (8 or 16) columns per block.
+constexpr int kTopMaxK = 4;   // contributions per T row the table holds
+struct TopArgs {
+    int nT, ldz, K, waves;
+    const void* Z;            // FP32 or FP64, as the factor
+    const int* tcol;          // [nT]
+    const int* tab;           // [K][nT]
+};
+void launchTrsvTop(const TrsvArgs& a, const double* bp, const TopArgs& t, hipStream_t s);
+// Formation of Z at each factorisation (factor.hip, FP64), from the FP64
+// block inverses B_s = [L11^-1; L21 L11^-1] of T's supernodes, depth by
+// depth from the roots (selected inversion: with R_s the rows of s below
+// its columns, Z[:, s] = -Z[:, R_s] B21_s on the columns already formed
+// and Z_ss = B11^T B11 - B21^T Z[R_s, s]).
+struct TopSuper {
+    int cs, w, m, ld;         // first T-local column, width, rows, block leading dimension
+    long long boff;           // block offset in the source
+    int rOff, pad;            // rows below the columns as T-local indices: rtl[rOff .. rOff + m - w)
+};
+struct TopBuild {
+    double* Z;
+    int nT, ldz;
+    const double* B;          // source blocks
+    const TopSuper* sn;
+    const int* rtl;
+    const int4* tasks;        // tile tasks, see the launchers
+};
+// tasks {supernode, row tile start, column tile start, row end}:
+// Z[r, cs + j] = -sum_k Z[r, rtl[rOff + k]] B21[k, j] for the tile's rows < row end
+void launchTopGemm(const TopBuild& b, int first, int count, hipStream_t s);
+// Z[j, i] = Z[i, j] for i < p0 <= j < p1
+void launchTopMirror(const TopBuild& b, int p0, int p1, hipStream_t s);
+// tasks {supernode, i tile start, j tile start, 0}: the diagonal block
+// Z[cs + i, cs + j] = sum_{r < m} B[r, i] V[r, j], V = [B11; -Z[R_s, s]]
+void launchTopDiag(const TopBuild& b, int first, int count, hipStream_t s);
 // device numeric factorisation (factor.hip): runs the plan's launch list
 // (kind, first task, count) on stream s; the L, X and U arenas must be zero
 // on entry; a pivot that is not positive sets *fail
@@ -260,8 +303,6 @@ struct DevFactorArgs {
 void launchDeviceFactor(const DevFactorArgs& d, hipStream_t s);
 // B (float) = X (double), n values
 void launchFactorToF32(const double* X, void* B, long long n, hipStream_t s);
-// M = Xr^T Xr for a w x w lower-triangular block (column-major, ld), M with ld
-void launchFactorGram(const double* Xr, int wr, int ld, double* M, hipStream_t s);
 void launchTrsvBackward(const TrsvArgs& a, const int4* items, int nitems, int wide, hipStream_t s);
 // out[3 perm[j] + c] = z[j][c]
 void launchUnpermute(const TrsvArgs& a, double* out, hipStream_t s);

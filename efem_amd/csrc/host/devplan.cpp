@@ -161,3 +161,128 @@ void buildDevFactorPlan(const SupernodalSymbolic& S, DevFactorPlan& P)
 }
 
 }  // namespace efem
+
+namespace efem {
+
+bool planTop(const SupernodalFactor& F, int zeroSlot, int maxCols, int maxK, int align, TopPlan& P)
+{
+    P = TopPlan();
+    const int ns = F.nsuper;
+    if (ns == 0 || maxCols <= 0) return false;
+    std::vector<int> w(ns), m(ns), tOff(ns);
+    int nt = 0;
+    for (int s = 0; s < ns; ++s) {
+        w[s] = F.first[s + 1] - F.first[s];
+        m[s] = F.rowPtr[s + 1] - F.rowPtr[s];
+        tOff[s] = nt;
+        nt += m[s] - w[s];
+    }
+    // supernodes are numbered in postorder: a parent follows its children
+    std::vector<int> depth(ns, 0);
+    int maxDepth = 0;
+    for (int s = ns - 1; s >= 0; --s) {
+        depth[s] = F.parent[s] < 0 ? 0 : depth[F.parent[s]] + 1;
+        maxDepth = std::max(maxDepth, depth[s]);
+    }
+    std::vector<long long> colsAt(maxDepth + 1, 0);
+    for (int s = 0; s < ns; ++s) colsAt[depth[s]] += w[s];
+    long long cum = 0;
+    for (int d = 0; d <= maxDepth && cum + colsAt[d] <= maxCols; ++d) {
+        cum += colsAt[d];
+        P.depth = d;
+    }
+    if (P.depth < 0) return false;
+    P.nT = static_cast<int>(cum);
+    P.ld = (P.nT + align - 1) / align * align;
+    P.inT.assign(ns, 0);
+    std::vector<int> tlOf(F.n, -1);
+    std::vector<int> depthStart(P.depth + 2, 0);  // first T-local column of each depth
+    for (int d = 0; d <= P.depth; ++d) {
+        depthStart[d] = static_cast<int>(P.tcol.size());
+        for (int s = 0; s < ns; ++s) {
+            if (depth[s] != d) continue;
+            P.inT[s] = 1;
+            P.super.push_back({s, static_cast<int>(P.tcol.size()), w[s], m[s], 0});
+            for (int j = F.first[s]; j < F.first[s + 1]; ++j) {
+                tlOf[j] = static_cast<int>(P.tcol.size());
+                P.tcol.push_back(j);
+            }
+        }
+    }
+    depthStart[P.depth + 1] = P.nT;
+    // rows below each T supernode's columns: its ancestors' columns, in T
+    for (TopSuperInfo& t : P.super) {
+        t.rOff = static_cast<int>(P.rtl.size());
+        for (int r = F.rowPtr[t.s] + t.w; r < F.rowPtr[t.s + 1]; ++r) {
+            const int q = tlOf[F.rows[r]];
+            if (q < 0) return false;  // not closed under ancestors (not a supernodal tree)
+            P.rtl.push_back(q);
+        }
+    }
+    // f_T: the update rows of T's children outside T, children in order
+    // (the extend-add of the forward sweep)
+    std::vector<int> cnt(P.nT, 0);
+    int K = 0;
+    for (int s = 0; s < ns; ++s) {
+        if (P.inT[s] || F.parent[s] < 0 || !P.inT[F.parent[s]]) continue;
+        for (int r = F.rowPtr[s] + w[s]; r < F.rowPtr[s + 1]; ++r) {
+            const int q = tlOf[F.rows[r]];
+            if (q < 0) return false;
+            K = std::max(K, ++cnt[q]);
+        }
+    }
+    if (K > maxK) {
+        P = TopPlan();
+        return false;
+    }
+    P.K = std::max(K, 1);
+    P.tab.assign(static_cast<size_t>(P.K) * P.nT, zeroSlot);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int s = 0; s < ns; ++s) {
+        if (P.inT[s] || F.parent[s] < 0 || !P.inT[F.parent[s]]) continue;
+        for (int i = 0; i < m[s] - w[s]; ++i) {
+            const int q = tlOf[F.rows[F.rowPtr[s] + w[s] + i]];
+            P.tab[static_cast<size_t>(cnt[q]++) * P.nT + q] = tOff[s] + i;
+        }
+    }
+    // formation of Z, depth by depth from the roots (64 x 64 tiles)
+    constexpr int TB = 64;
+    auto task = [&](int a, int b, int c, int d) {
+        P.tasks.push_back(a);
+        P.tasks.push_back(b);
+        P.tasks.push_back(c);
+        P.tasks.push_back(d);
+    };
+    auto ntasks = [&]() { return static_cast<int>(P.tasks.size() / 4); };
+    for (int d = 0; d <= P.depth; ++d) {
+        const int p0 = depthStart[d], p1 = depthStart[d + 1];
+        auto addGemm = [&](int r0, int r1) {
+            TopPlan::Step st{0, ntasks(), 0, 0, 0};
+            for (size_t q = 0; q < P.super.size(); ++q) {
+                const TopSuperInfo& t = P.super[q];
+                if (t.cs < p0 || t.cs >= p1 || t.m == t.w) continue;
+                for (int r = r0; r < r1; r += TB)
+                    for (int j = 0; j < t.w; j += TB) task(static_cast<int>(q), r, j, r1);
+            }
+            st.count = ntasks() - st.first;
+            if (st.count > 0) P.steps.push_back(st);
+        };
+        if (d > 0) {
+            addGemm(0, p0);
+            P.steps.push_back({1, 0, 0, p0, p1});
+            addGemm(p0, p1);
+        }
+        TopPlan::Step st{2, ntasks(), 0, 0, 0};
+        for (size_t q = 0; q < P.super.size(); ++q) {
+            const TopSuperInfo& t = P.super[q];
+            if (t.cs < p0 || t.cs >= p1) continue;
+            for (int i = 0; i < t.w; i += TB)
+                for (int j = 0; j < t.w; j += TB) task(static_cast<int>(q), i, j, 0);
+        }
+        st.count = ntasks() - st.first;
+        if (st.count > 0) P.steps.push_back(st);
+    }
+    return true;
+}
+
+}  // namespace efem

@@ -61,3 +61,48 @@ struct DevFactorPlan {
 void buildDevFactorPlan(const SupernodalSymbolic& S, DevFactorPlan& P);
 
 }  // namespace efem
+
+namespace efem {
+
+// The fused top of the elimination tree: T = the supernodes within depth D
+// of the roots, D the deepest depth whose supernodes (with all shallower
+// ones) hold at most maxCols columns; T is closed under ancestors.  With
+// L_TT the factor's TT block, Z = (L_TT L_TT^T)^-1 is the TT block of
+// (P A P^T)^-1, so T's forward and backward solves are the one product
+// z_T = Z f_T, f_T the right-hand side on T plus the update rows of T's
+// children outside T.  T-local numbering: by depth, then factor column (a
+// supernode's columns stay contiguous).
+struct TopSuperInfo {
+    int s;      // supernode
+    int cs;     // first T-local column
+    int w, m;   // width, rows
+    int rOff;   // rows below its columns, T-local, at rtl[rOff .. rOff + m - w)
+};
+struct TopPlan {
+    int depth = -1, nT = 0, ld = 0, K = 0;
+    std::vector<char> inT;              // per supernode
+    std::vector<int> tcol;              // [nT] factor column of each T-local index
+    std::vector<int> tab;               // [K][nT] update-vector positions of f_T (zeroSlot if absent)
+    std::vector<TopSuperInfo> super;    // T supernodes in T-local order
+    std::vector<int> rtl;
+    // Z is formed depth by depth from the roots (selected inversion, with
+    // R_s the rows of s below its columns and B21_s = L21 L11^-1):
+    //   Z[r, s] = -Z[r, R_s] B21_s for the rows r of shallower depths (kind 0),
+    //   Z[s, r] = Z[r, s] (kind 1, mirror of the rows < p0 into [p0, p1)),
+    //   the same product for the rows of this depth (kind 0), then the
+    //   diagonal block Z_ss = B11^T B11 - B21^T Z[R_s, s] (kind 2).
+    // Tile tasks, 4 ints each: kind 0 {T supernode, row tile start, column
+    // tile start, row end}; kind 2 {T supernode, i tile start, j tile
+    // start, 0}; tiles of 64.
+    std::vector<int> tasks;
+    struct Step {
+        int kind, first, count, p0, p1;
+    };
+    std::vector<Step> steps;
+};
+// false (and no fused top) if not even the roots fit, or a T row has more
+// than maxK children's update rows; zeroSlot = the index of the all-zero
+// update row; Z's leading dimension = nT rounded up to `align` rows
+bool planTop(const SupernodalFactor& F, int zeroSlot, int maxCols, int maxK, int align, TopPlan& P);
+
+}  // namespace efem

@@ -424,24 +424,80 @@ def test_device_factorisation_reports_indefinite_matrices():
 
 @pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
 @pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8]), (1, None)])
-def test_apply_h0_fused_root(prec, tol, which, opts, rng):
-    # the last supernode solved as z_R = M f_R (M = L_RR^-T L_RR^-1) in
-    # place of its forward and backward levels: the same K^-1 to rounding,
-    # in plain launches and in the captured graph
+def test_apply_h0_fused_top(prec, tol, which, opts, rng):
+    # the top of the elimination tree (the supernodes within some depth of
+    # the roots) solved as z_T = Z f_T, Z = (L_TT L_TT^T)^-1 formed on the
+    # device by selected inversion, in place of their forward and backward
+    # levels: the same K^-1 to rounding for every depth the column limit
+    # selects (none, the root only, part of the tree, the whole tree), from
+    # the device factorisation and from the host one, in plain launches and
+    # in the captured graph
     m = E.Mesh.workload(which, opts)
     K = O.Model(m).K(scalar=False)
     b = rng.standard_normal(m.ndof)
     x_ref = spla.spsolve(K.tocsc(), b)
     out = {}
     with E.Context(m, prec) as ctx:
-        for fused in (0, 1):
-            ctx.set_option("fused_root", fused)
-            assert ctx.factor()
-            out[fused] = ctx.apply_h0(b)
-            assert rel(out[fused], x_ref) <= tol, fused
+        for dev in (1, 0):
+            ctx.set_option("device_factor", dev)
+            for fused, top in ((0, 3584), (1, 1), (1, 120), (1, 600), (1, 3584)):
+                ctx.set_option("fused_root", fused)
+                ctx.set_option("fused_top_max", top)
+                assert ctx.factor()
+                out[dev, fused, top] = ctx.apply_h0(b)
+                assert rel(out[dev, fused, top], x_ref) <= tol, (dev, fused, top)
         ctx.set_option("h0_graph", 1)
-        assert np.array_equal(ctx.apply_h0(b), out[1])
-    assert rel(out[1], out[0]) <= (1e-12 if prec == E.FP64 else 1e-5)
+        assert np.array_equal(ctx.apply_h0(b), out[0, 1, 3584])
+    for key, v in out.items():
+        assert rel(v, out[1, 0, 3584]) <= (1e-12 if prec == E.FP64 else 1e-5), key
+
+
+@pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
+@pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8])])
+def test_apply_h0_staged_backward(prec, tol, which, opts, rng):
+    # backward sweeps whose items stage [y_s; -z[rows]] in LDS, 4, 8 or 16
+    # columns per item, against spsolve (without the fused top, so every
+    # level sweeps); a column's sum does not depend on the grouping, so the
+    # three widths agree bit for bit
+    m = E.Mesh.workload(which, opts)
+    K = O.Model(m).K(scalar=False)
+    b = rng.standard_normal(m.ndof)
+    x_ref = spla.spsolve(K.tocsc(), b)
+    out = {}
+    with E.Context(m, prec) as ctx:
+        ctx.set_option("fused_root", 0)
+        for W in (0, 4, 8, 16):
+            ctx.set_option("bwd_stage", W)
+            assert ctx.factor()
+            out[W] = ctx.apply_h0(b)
+            assert rel(out[W], x_ref) <= tol, W
+    assert np.array_equal(out[4], out[8]) and np.array_equal(out[4], out[16])
+    assert rel(out[4], out[0]) <= (1e-12 if prec == E.FP64 else 1e-5)
+
+
+def test_apply_h0_fused_top_full_beam(rng):
+    # the full 480,000-element beam, whose fused top is the three levels
+    # nearest the root (3,565 columns), with and without the fused top:
+    # FP64 K (apply_h0 b) = b through the oracle's K (no factorisation on the
+    # host); the FP32 factor's apply as close to the FP64 one with the fused
+    # top as without it
+    m = E.Mesh.workload(2, [200, 20, 20])
+    K = O.Model(m).K(scalar=False).tocsr()
+    b = rng.standard_normal(m.ndof)
+    xs = {}
+    for prec in (E.FP64, E.MIXED):
+        with E.Context(m, prec) as ctx:
+            ctx.set_option("dense_subtree", 192)
+            for fused in (0, 1):
+                ctx.set_option("fused_root", fused)
+                assert ctx.factor()
+                xs[prec, fused] = ctx.apply_h0(b)
+    for fused in (0, 1):
+        assert rel(K @ xs[E.FP64, fused], b) <= 1e-9, fused
+    assert rel(xs[E.FP64, 1], xs[E.FP64, 0]) <= 1e-12
+    e0 = rel(xs[E.MIXED, 0], xs[E.FP64, 0])
+    e1 = rel(xs[E.MIXED, 1], xs[E.FP64, 0])
+    assert e1 <= 2 * e0 + 1e-7, (e0, e1)
 
 
 @pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
