@@ -158,6 +158,10 @@ struct efem_ctx {
     // supernodes (with all shallower ones) hold at most this many columns
     int fusedTopMax = kFusedTopMaxCols;
     int topWaves = 16;           // option "top_waves": columns per block of the fused top, 8 or 16
+    // option "tet4_geometry": C3D4 elements form J0^-1 from the nodes'
+    // coordinates (Xd) instead of reading the stored inverse
+    bool tet4Geometry = true;
+    double* Xd = nullptr;
     // option "bwd_stage": backward sweep items that stage [y_s; -z[rows]]
     // in LDS, 4, 8 or 16 columns each (0: the per-wave gathers), on every
     // level but those of many short supernodes (16-column narrow items)
@@ -321,6 +325,15 @@ int uploadModel(efem_ctx* c)
     }
     c->ginv64 = c->ginv;
     c->gdN64 = c->gdN;
+    if (m.nen == 4) {
+        // reference coordinates per free DOF: C3D4 elements form J0^-1 from
+        // them instead of streaming 72 bytes each
+        std::vector<double> xd(3 * static_cast<size_t>(m.nFree) + 3, 0.0);
+        for (int n = 0; n < m.nNodes; ++n)
+            if (m.fid[n] >= 0)
+                for (int d = 0; d < 3; ++d) xd[3 * static_cast<size_t>(m.fid[n]) + d] = m.X[3 * static_cast<size_t>(n) + d];
+        if ((rc = upload(c, &c->Xd, xd))) return rc;
+    }
     if (m.nFaces > 0) {
         std::vector<int> fdof(static_cast<size_t>(m.nen) * m.nFaces);
         for (int f = 0; f < m.nFaces; ++f)
@@ -375,6 +388,8 @@ efem::dev::ElemArgs elemArgs(efem_ctx* c, const double* x, bool f32)
     ea.params = c->params;
     ea.x = x;
     ea.slots = c->eslots;
+    ea.Xd = c->tet4Geometry ? c->Xd : nullptr;
+    ea.nFree = m.nFree;
     return ea;
 }
 
@@ -2168,6 +2183,10 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         // takes effect at the next factorisation
         c->bwdStage = iv;
         releaseFactor(c);
+        return EFEM_OK;
+    }
+    if (k == "tet4_geometry") {
+        c->tet4Geometry = value != 0.0;
         return EFEM_OK;
     }
     if (k == "top_waves" && (iv == 8 || iv == 16) && static_cast<double>(iv) == value) {

@@ -109,11 +109,53 @@ __device__ __forceinline__ void tet4Grad(const double* __restrict__ x, const int
         for (int i = 0; i < 3; ++i) Gu[3 * i + k] = x[3 * id[k] + i] - u3[i];
 }
 
+// J0^-1 of a C3D4 element from its nodes' reference coordinates (Xd: per
+// free DOF, the node's coordinate), by the host's operations in the host's
+// order without contraction -- J0 = sum_{a>=1} (X_a - X_0) (x) dN_a with the
+// local gradients dN_0..3 = e_1, e_2, e_3, (-1, -1, -1), then the cofactor
+// inverse -- so it equals the stored J0^-1 bit for bit
+__device__ __forceinline__ void tet4Ginv(const double* __restrict__ Xd, const int* id, double* B)
+{
+#pragma clang fp contract(off)
+    double X0[3], d[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) X0[i] = Xd[3 * id[0] + i];
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) d[a - 1][i] = Xd[3 * id[a] + i] - X0[i];
+    double A[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        A[3 * i] = -d[2][i];
+        A[3 * i + 1] = d[0][i] - d[2][i];
+        A[3 * i + 2] = d[1][i] - d[2][i];
+    }
+    const double c00 = A[4] * A[8] - A[5] * A[7];
+    const double c01 = A[5] * A[6] - A[3] * A[8];
+    const double c02 = A[3] * A[7] - A[4] * A[6];
+    const double det = A[0] * c00 + A[1] * c01 + A[2] * c02;
+    const double id0 = 1.0 / det;
+    B[0] = c00 * id0;
+    B[1] = (A[2] * A[7] - A[1] * A[8]) * id0;
+    B[2] = (A[1] * A[5] - A[2] * A[4]) * id0;
+    B[3] = c01 * id0;
+    B[4] = (A[0] * A[8] - A[2] * A[6]) * id0;
+    B[5] = (A[2] * A[3] - A[0] * A[5]) * id0;
+    B[6] = c02 * id0;
+    B[7] = (A[1] * A[6] - A[0] * A[7]) * id0;
+    B[8] = (A[0] * A[4] - A[1] * A[3]) * id0;
+}
+
+// Xd (may be null): J0^-1 recomputed from the nodes' coordinates when all
+// four nodes are free (nF free nodes), else read from ginv (every fixed
+// node shares one DOF slot, so its coordinates are not in Xd)
 template <typename T, int LAW, int MODE>
 __device__ __forceinline__ void tet4Lane(int e, int E, const int* __restrict__ connDof,
                                          const double* __restrict__ ginv, const double* __restrict__ vol, double w0,
                                          const double* __restrict__ params, int ppe, const double* __restrict__ x,
-                                         const double* __restrict__ v, T* __restrict__ slots)
+                                         const double* __restrict__ v, T* __restrict__ slots,
+                                         const double* __restrict__ Xd = nullptr, int nF = 0)
 {
     int id[4];
 #pragma unroll
@@ -122,8 +164,12 @@ __device__ __forceinline__ void tet4Lane(int e, int E, const int* __restrict__ c
     tet4Grad<MODE>(x, id, Gu);
     if (MODE == MODE_TANGENT) tet4Grad<MODE>(v, id, Gv);
     double Jd[9];
+    if (Xd && id[0] < nF && id[1] < nF && id[2] < nF && id[3] < nF) {
+        tet4Ginv(Xd, id, Jd);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + e];
+        for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + e];
+    }
     T Q[9];
     stressMoment<T, MODE>(LAW, params + (ppe ? 4 * e : 0), Gu, Gv, Jd, vol[e] * w0, Q);
     T* out = slots + 12 * static_cast<size_t>(e);

@@ -39,11 +39,11 @@ __global__ __launch_bounds__(TPB) void kElemTet4(int E, const int* __restrict__ 
                                                  const double* __restrict__ params, int ppe,
                                                  const double* __restrict__ x,
                                                  const double* __restrict__ v,
-                                                 T* __restrict__ slots)
+                                                 T* __restrict__ slots, const double* __restrict__ Xd, int nF)
 {
     const int e = xcdBlock() * TPB + threadIdx.x;
     if (e >= E) return;
-    tet4Lane<T, LAW, MODE>(e, E, connDof, ginv, vol, w0, params, ppe, x, v, slots);
+    tet4Lane<T, LAW, MODE>(e, E, connDof, ginv, vol, w0, params, ppe, x, v, slots, Xd, nF);
 }
 
 template <typename T, int NEN, int NG, int LAW, int MODE>
@@ -100,15 +100,16 @@ void launchTet4Tpb(const ElemArgs& a, const double* v, hipStream_t s)
     switch (a.law) {
         case LAW_NH:
             kElemTet4<T, LAW_NH, MODE, TPB><<<grid, TPB, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                                 a.paramsPerElem, a.x, v, slots);
+                                                                 a.paramsPerElem, a.x, v, slots, a.Xd, a.nFree);
             break;
         case LAW_MR:
             kElemTet4<T, LAW_MR, MODE, TPB><<<grid, TPB, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0, a.params,
-                                                                 a.paramsPerElem, a.x, v, slots);
+                                                                 a.paramsPerElem, a.x, v, slots, a.Xd, a.nFree);
             break;
         default:
             kElemTet4<T, LAW_YEOH, MODE, TPB><<<grid, TPB, 0, s>>>(a.nElems, a.connDof, ginv, a.vol, 1.0,
-                                                                   a.params, a.paramsPerElem, a.x, v, slots);
+                                                                   a.params, a.paramsPerElem, a.x, v, slots, a.Xd,
+                                                                   a.nFree);
     }
 }
 

@@ -42,6 +42,12 @@ struct ElemArgs {
     const double* params;   // [4] or [E][4]
     const double* x;        // displacement, free DOFs + 3 trailing zeros
     void* slots;            // [E][nen][3] (double or float)
+    // C3D4 only (may be null): the reference coordinates per free DOF
+    // (node of free id i at Xd[3 i]); an element whose four nodes are free
+    // forms its J0^-1 from them (bit for bit the stored one), the others
+    // read ginv
+    const double* Xd = nullptr;
+    int nFree = 0;
 };
 void launchElementForce(const ElemArgs& a, hipStream_t s);
 // tangent product of the element forces: slots <- (d f_e / d u)[v] (the

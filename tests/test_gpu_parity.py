@@ -422,6 +422,30 @@ def test_device_factorisation_reports_indefinite_matrices():
         assert ctx.factor_nnz == 0
 
 
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+@pytest.mark.parametrize("opts", [[30, 6, 6], [200, 20, 20]])
+def test_tet4_geometry_from_coordinates_is_bitwise(prec, opts, rng):
+    # C3D4 elements form J0^-1 from their nodes' reference coordinates (the
+    # host's operations, no contraction) instead of reading the stored
+    # inverse: residual and tangent product identical bit for bit, and the
+    # residual still the oracle's
+    m = E.Mesh.workload(2, opts)
+    u = smooth_u(m, rng, amp=0.01)
+    v = rng.standard_normal(m.ndof)
+    out = {}
+    with E.Context(m, prec) as ctx:
+        for g in (0, 1):
+            ctx.set_option("tet4_geometry", g)
+            out[g] = (ctx.residual(u), ctx.tangent(u, v))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    if opts[0] <= 30:
+        om = O.Model(m)
+        r_or = om.residual(u)
+        scale = np.linalg.norm(r_or) if prec == E.FP64 else (
+            np.linalg.norm(om.internal_force(u)) + np.linalg.norm(om.external_force(u)))
+        assert np.linalg.norm(out[1][0] - r_or) <= (1e-10 if prec == E.FP64 else 1e-5) * scale
+
+
 @pytest.mark.parametrize("prec,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-4)])
 @pytest.mark.parametrize("which,opts", [(2, [30, 6, 6]), (4, [16, 4, 4]), (3, [24, 2, 8]), (1, None)])
 def test_apply_h0_fused_top(prec, tol, which, opts, rng):
