@@ -125,6 +125,7 @@ struct efem_ctx {
     double* scal = nullptr;  // device scalars
     double* work = nullptr;
     double* hostScal = nullptr;  // pinned
+    double* hostScalDev = nullptr;  // its device address (kernels write the trials' scalars there)
     DeviceFactor fac;
     bool factorFailed = false;
     double factorSeconds = 0.0;
@@ -365,6 +366,7 @@ int uploadModel(efem_ctx* c)
     if ((rc = devAlloc(c, &c->work, efem::dev::kReduceWorkDoubles))) return rc;
     HIP_CHECK(c, hipHostMalloc(reinterpret_cast<void**>(&c->hostScal), kHostScal * sizeof(double),
                                hipHostMallocDefault));
+    HIP_CHECK(c, hipHostGetDevicePointer(reinterpret_cast<void**>(&c->hostScalDev), c->hostScal, 0));
     for (hipEvent_t& e : c->trialEv) HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return EFEM_OK;
@@ -445,22 +447,28 @@ constexpr int kTermsAll = 7;
 // "residual_terms" of efem_residual)
 int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD = nullptr,
                      const double* dotCb = nullptr, const double* dotXt = nullptr, double* dotOut = nullptr,
-                     int terms = kTermsAll)
+                     int terms = kTermsAll, const double* dotCarry = nullptr)
 {
     const efem::Model& m = c->model;
     const bool f32 = elem32(c) && !c->elemFp64;
+    // the element forces and the pressure faces go out together (one
+    // launch for C3D4); the profiled element time then holds the faces
+    const bool faces = m.nFaces > 0 && (terms & kTermPressure);
     if (terms & kTermElements) {
         const efem::dev::ElemArgs ea = elemArgs(c, x, f32);
         const bool prof = c->profiling && c->evUsed + 2 <= static_cast<int>(c->evs.size());
         if (prof) HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed], c->stream));
-        efem::dev::launchElementForce(ea, c->stream);
+        if (faces)
+            efem::dev::launchElementForceFaces(ea, faceArgs(c, x, nullptr), c->stream);
+        else
+            efem::dev::launchElementForce(ea, c->stream);
         if (prof) {
             HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed + 1], c->stream));
             c->evUsed += 2;
         }
         HIP_CHECK(c, hipGetLastError());
     }
-    if (m.nFaces > 0 && (terms & kTermPressure)) {
+    if (faces && !(terms & kTermElements)) {
         efem::dev::launchPressure(faceArgs(c, x, nullptr), c->stream);
         HIP_CHECK(c, hipGetLastError());
     }
@@ -475,6 +483,7 @@ int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD
         ga.dotXt = dotXt;
         ga.dotWork = c->gdotWork;
         ga.dotOut = dotOut;
+        ga.dotCarry = dotCarry;
     }
     efem::dev::launchGather(ga, c->stream);
     HIP_CHECK(c, hipGetLastError());
@@ -1827,8 +1836,12 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     auto enqueueTrial = [&](double step, int slot) -> int {
         if (!(preformed && step == 1.0)) efem::dev::launchStep(c->x, c->xp, c->d, step, n, c->stream);
         preformed = false;
-        // the trial's scalars come with the residual's gather
-        int rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr, sc + kOut);
+        // the trial's scalars come with the residual's gather; without the
+        // energy rule its finishing block writes them, and the slope, into
+        // the pinned slot itself
+        double* pinnedSlot = c->hostScalDev + kTrialSlot + 8 * slot;
+        int rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr,
+                                   energyLs ? sc + kOut : pinnedSlot, kTermsAll, energyLs ? nullptr : sc + kOut + 4);
         if (rc2) return rc2;
         if (energyLs) {
             // the potential's change from xp and its rounding scale into
@@ -1837,8 +1850,9 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             efem::dev::launchEnergyDiff(c->wNew, c->wOld, c->model.nElems, c->fext, c->x, c->xp, n, c->work,
                                         sc + kOut + 5, c->stream);
         }
-        HIP_CHECK(c, hipMemcpyAsync(c->hostScal + kTrialSlot + 8 * slot, sc + kOut,
-                                    (energyLs ? 7 : 5) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (energyLs)
+            HIP_CHECK(c, hipMemcpyAsync(c->hostScal + kTrialSlot + 8 * slot, sc + kOut, 7 * sizeof(double),
+                                        hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(c, hipEventRecord(c->trialEv[slot], c->stream));
         return EFEM_OK;
     };

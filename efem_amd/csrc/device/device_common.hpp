@@ -21,12 +21,12 @@ constexpr int kThreads = 256;
 // work items share data (nodes sharing elements), give each XCD one
 // contiguous range of logical blocks instead of every 8th block.
 constexpr int kXcds = 8;
-__device__ __forceinline__ int xcdBlock()
+__device__ __forceinline__ int xcdBlockOf(int b, int g)
 {
-    const int b = blockIdx.x, g = gridDim.x;
     const int x = b % kXcds, k = b / kXcds, per = g / kXcds, rem = g % kXcds;
     return x * per + min(x, rem) + k;
 }
+__device__ __forceinline__ int xcdBlock() { return xcdBlockOf(blockIdx.x, gridDim.x); }
 
 // wave64 butterfly sum (fixed order)
 __device__ __forceinline__ double waveSum(double v)

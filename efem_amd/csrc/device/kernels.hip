@@ -154,15 +154,19 @@ struct PressureTile {
     static constexpr int fpb = 28;  // 28 x 9 Gauss points <= 256 lanes
 };
 
-template <int NEN>
-__global__ __launch_bounds__(kThreads) void kPressureTile(FaceArgs a)
+// One tile of FPB = BT / kMaxQ faces by the BT lanes of a block (contains
+// barriers; every lane must call it): relative nodal displacements staged
+// in LDS, one lane per (face, Gauss point) forms cof(F) A0, then one lane
+// per (face, node) its nodal forces p sum_q WN_q v_q into the face slots.
+template <int NEN, int BT>
+__device__ __forceinline__ void pressureTileBlock(const FaceArgs& a, int blk)
 {
-    constexpr int FPB = PressureTile<NEN>::fpb;
+    constexpr int FPB = BT / PressureTile<NEN>::kMaxQ;
     __shared__ double sdu[FPB][NEN][3];
     __shared__ double sv[FPB][PressureTile<NEN>::kMaxQ][3];
     const int F = a.nFaces, fq = a.fq, tid = threadIdx.x;
-    const int f0 = blockIdx.x * FPB;
-    for (int k = tid; k < FPB * NEN; k += kThreads) {
+    const int f0 = blk * FPB;
+    for (int k = tid; k < FPB * NEN; k += BT) {
         const int fl = k / NEN, n = k - fl * NEN, f = f0 + fl;
         if (f < F) {
             const int id = a.faceDof[n * F + f], id0 = a.faceDof[f];
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(kThreads) void kPressureTile(FaceArgs a)
         }
     }
     __syncthreads();
-    for (int k = tid; k < FPB * NEN; k += kThreads) {
+    for (int k = tid; k < FPB * NEN; k += BT) {
         const int fl = k / NEN, n = k - fl * NEN, f = f0 + fl;
         if (f >= F) continue;
         double o0 = 0.0, o1 = 0.0, o2 = 0.0;
@@ -214,6 +218,58 @@ __global__ __launch_bounds__(kThreads) void kPressureTile(FaceArgs a)
         dst[0] = pr * o0;
         dst[1] = pr * o1;
         dst[2] = pr * o2;
+    }
+}
+
+template <int NEN>
+__global__ __launch_bounds__(kThreads) void kPressureTile(FaceArgs a)
+{
+    pressureTileBlock<NEN, kThreads>(a, blockIdx.x);
+}
+
+// C3D4 element forces and the pressure faces in one launch: blocks
+// [0, nEB) are element lanes (remapped per XCD over that range), the rest
+// face tiles of TPB / kMaxQ faces
+template <typename T, int LAW, int TPB>
+__global__ __launch_bounds__(TPB) void kElemTet4Faces(ElemArgs ea, FaceArgs fa, int nEB)
+{
+    if (static_cast<int>(blockIdx.x) < nEB) {
+        const int e = xcdBlockOf(blockIdx.x, nEB) * TPB + threadIdx.x;
+        if (e < ea.nElems)
+            tet4Lane<T, LAW, MODE_FORCE>(e, ea.nElems, ea.connDof, static_cast<const double*>(ea.ginv), ea.vol,
+                                         1.0, ea.params, ea.paramsPerElem, ea.x, nullptr,
+                                         static_cast<T*>(ea.slots), ea.Xd, ea.nFree);
+        return;
+    }
+    pressureTileBlock<4, TPB>(fa, blockIdx.x - nEB);
+}
+
+template <typename T, int LAW, int TPB>
+void launchTet4FacesTpb(const ElemArgs& ea, const FaceArgs& fa, hipStream_t s)
+{
+    constexpr int FPB = TPB / PressureTile<4>::kMaxQ;
+    const int nEB = (ea.nElems + TPB - 1) / TPB;
+    const int nFB = (fa.nFaces + FPB - 1) / FPB;
+    kElemTet4Faces<T, LAW, TPB><<<nEB + nFB, TPB, 0, s>>>(ea, fa, nEB);
+}
+
+template <typename T, int TPB>
+void launchTet4FacesLaw(const ElemArgs& ea, const FaceArgs& fa, hipStream_t s)
+{
+    switch (ea.law) {
+        case LAW_NH: launchTet4FacesTpb<T, LAW_NH, TPB>(ea, fa, s); break;
+        case LAW_MR: launchTet4FacesTpb<T, LAW_MR, TPB>(ea, fa, s); break;
+        default: launchTet4FacesTpb<T, LAW_YEOH, TPB>(ea, fa, s);
+    }
+}
+
+template <typename T>
+void launchTet4Faces(const ElemArgs& ea, const FaceArgs& fa, hipStream_t s)
+{
+    switch (tuning().tet4Threads) {
+        case 64: launchTet4FacesLaw<T, 64>(ea, fa, s); break;
+        case 128: launchTet4FacesLaw<T, 128>(ea, fa, s); break;
+        default: launchTet4FacesLaw<T, 256>(ea, fa, s);
     }
 }
 
@@ -475,12 +531,24 @@ __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ 
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kRF)
+    // a few components: each lane loads NB of its partials at once (the
+    // same order of combination as one by one)
+    constexpr int NB = K <= 4 ? 8 : 1;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += NB * kRF) {
+        double o[NB][K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double o = work[static_cast<size_t>(k) * stride + b];
-            v[k] = isMax(MM, k) ? maxNan(v[k], o) : v[k] + o;
+        for (int q = 0; q < NB; ++q) {
+            const int b = b0 + q * kRF;
+#pragma unroll
+            for (int k = 0; k < K; ++k) o[q][k] = b < nb ? work[static_cast<size_t>(k) * stride + b] : 0.0;
         }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            if (b0 + q * kRF >= nb) break;
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k] = isMax(MM, k) ? maxNan(v[k], o[q][k]) : v[k] + o[q][k];
+        }
+    }
     blockReduce<K, kRFW, MM>(v, sh);
     epi(v, pre);
 }
@@ -504,6 +572,22 @@ struct EpiStore : NoPrefetch {
         if (threadIdx.x == 0) {
 #pragma unroll
             for (int k = 0; k < K; ++k) out[k] = v[k];
+        }
+    }
+};
+
+// epilogue: out[k] = v[k] and out[4] = *extra (a device scalar carried
+// along: the trial's slope), out in pinned host memory
+template <int K>
+struct EpiStoreCarry : NoPrefetch {
+    double* out;
+    const double* extra;
+    __device__ void operator()(const double (&v)[K], Pre) const
+    {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] = v[k];
+            out[4] = *extra;
         }
     }
 };
@@ -1222,6 +1306,20 @@ void launchElementForce(const ElemArgs& a, hipStream_t s)
         launchElem<double, MODE_FORCE>(a, nullptr, s);
 }
 
+void launchElementForceFaces(const ElemArgs& a, const FaceArgs& f, hipStream_t s)
+{
+    // one launch for the C3D4 kernel and faces whose Gauss points fit the tile
+    if (a.nen == 4 && f.nen == 4 && f.nFaces > 0 && f.fq <= PressureTile<4>::kMaxQ) {
+        if (a.mixed)
+            launchTet4Faces<float>(a, f, s);
+        else
+            launchTet4Faces<double>(a, f, s);
+        return;
+    }
+    launchElementForce(a, s);
+    launchPressure(f, s);
+}
+
 void launchElementTangent(const ElemArgs& a, const double* v, hipStream_t s)
 {
     if (a.mixed)
@@ -1401,10 +1499,16 @@ void launchGather(const GatherArgs& a, hipStream_t s)
     if (!a.dotWork) return gatherLaunch<0>(a, lpn, g, s);
     if (a.dotXt) {
         gatherLaunch<4>(a, lpn, g, s);
-        kReduceFinish<4, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<4>{{}, a.dotOut}, g);
+        if (a.dotCarry)
+            kReduceFinish<4, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStoreCarry<4>{{}, a.dotOut, a.dotCarry}, g);
+        else
+            kReduceFinish<4, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<4>{{}, a.dotOut}, g);
     } else {
         gatherLaunch<3>(a, lpn, g, s);
-        kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<3>{{}, a.dotOut}, g);
+        if (a.dotCarry)
+            kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStoreCarry<3>{{}, a.dotOut, a.dotCarry}, g);
+        else
+            kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<3>{{}, a.dotOut}, g);
     }
 }
 

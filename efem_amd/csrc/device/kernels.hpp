@@ -72,6 +72,10 @@ struct FaceArgs {
     double* slots;          // [F][nen][3]
 };
 void launchPressure(const FaceArgs& a, hipStream_t s);
+// the element forces and the pressure faces (f.nFaces > 0): one launch for
+// C3D4 elements (face tiles after the element blocks), else the two
+// launches; the slots are those of the two kernels
+void launchElementForceFaces(const ElemArgs& a, const FaceArgs& f, hipStream_t s);
 // load stiffness of the follower pressure applied to a.v, or its transpose
 void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s);
 
@@ -94,6 +98,9 @@ struct GatherArgs {
     const double* dotXt = nullptr;
     double* dotWork = nullptr;
     double* dotOut = nullptr;
+    // optional: dotOut[4] = *dotCarry as well (dotOut then pinned host
+    // memory: the host reads the trial's scalars without a copy)
+    const double* dotCarry = nullptr;
 };
 void launchGather(const GatherArgs& a, hipStream_t s);
 // doubles of dotWork a gather over nFree nodes needs (any lanes per node)
