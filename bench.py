@@ -288,56 +288,55 @@ def run_config_solves(E, args):
     single-thread oracle and for the device in FP64; the displacements are
     compared with the oracle's."""
     out = {}
-    # cantilever: at the configured 2 kPa the reference L-BFGS stalls; 1 kPa
-    # converges (its iterates then stall below ~1e-7 N, so no tighter u* is
-    # formed: the comparison is at the reference criterion)
-    m = E.Mesh.workload(1, [20, 4, 4, 0, 1000.0])
-    rec = {"elements": m.n_elems, "dof": m.ndof, "pressure_pa": m.pressure, "pressure_configured_pa": 2000.0,
-           "criterion": "|r|inf <= 1e-6 N (reference)"}
-    if not args.no_cpu:
-        import oracle as O
-        om = O.Model(m)
-        t = time.perf_counter()
-        u_or, so = om.solve(epsilon=1e-6, max_iterations=args.max_iters)
-        rec["oracle_cpu"] = {"s": round(time.perf_counter() - t, 4), "iterations": so["iterations"], "cores": 1}
-    # FP64 only, as configured: at 1 kPa the cantilever sits near the limit
-    # of what the reference iteration solves, and the mixed solve (FP32
-    # noise in the first iterations) does not reach the criterion within
-    # 20,000 iterations
-    for label, prec, sw in (("fp64", E.FP64, 0.0),):
-        with E.Context(m, prec) as c:
+    # cantilever: at the configured 2 kPa the reference L-BFGS stalls, and at
+    # 1 kPa its iteration is chaotic (over loads 1e-6 apart the oracle itself
+    # reaches different equilibria or none within 20,000 iterations); at
+    # 500 Pa it converges reliably, so the comparison is there, as medians
+    # over five loads 500 (1 + k 1e-6) Pa, at the reference criterion
+    # (|r|inf <= 1e-6 N: below ~1e-7 N the iterates stall, so no tighter u*)
+    loads = [500.0 * (1.0 + 1e-6 * k) for k in range(5)]
+    rec = {"elements": 320, "dof": 1500, "pressure_pa": loads[0], "pressure_samples": len(loads),
+           "pressure_configured_pa": 2000.0, "criterion": "|r|inf <= 1e-6 N (reference)"}
+    rows = {"oracle": [], "oracle_m10": [], "fp64": [], "fp64_m10": []}
+    for p in loads:
+        m = E.Mesh.workload(1, [20, 4, 4, 0, p])
+        ref = {}
+        if not args.no_cpu:
+            import oracle as O
+            om = O.Model(m)
+            for key, mm in (("oracle", 6), ("oracle_m10", 10)):
+                t = time.perf_counter()
+                u_or, so = om.solve(epsilon=1e-6, max_iterations=args.max_iters, m=mm)
+                rows[key].append((time.perf_counter() - t, so["iterations"], 0.0))
+                ref[mm] = u_or
+        # FP64 only, as configured (the mixed solve's FP32 first iterations
+        # push this near-limit load off the FP64 path)
+        with E.Context(m, E.FP64) as c:
             rec["dense_subtree"] = c.autotune_factor()["dense_subtree"]
-            c.solve(epsilon=1e-6, max_iterations=args.max_iters, fp64_switch=sw)  # warm-up
-            walls, its = [], []
-            for _ in range(3):
+            for key, mm in (("fp64", 6), ("fp64_m10", 10)):
+                c.solve(epsilon=1e-6, max_iterations=args.max_iters, m=mm)  # warm-up
                 c.factor_release()
                 t = time.perf_counter()
-                u, st = c.solve(epsilon=1e-6, max_iterations=args.max_iters, fp64_switch=sw)
-                walls.append(time.perf_counter() - t)
-                its.append(st["iterations"])
-        r = {"s": round(float(np.median(walls)), 4), "iterations": int(np.median(its))}
-        if not args.no_cpu:
-            r["disp_rel_diff_vs_oracle"] = float(np.linalg.norm(u - u_or) / np.linalg.norm(u_or))
-        rec[label] = r
-    # history 10 as the configuration states (the reference fixes m = 6,
-    # Param.h:102): device FP64 and oracle
-    with E.Context(m, E.FP64) as c:
-        c.solve(epsilon=1e-6, max_iterations=args.max_iters, m=10)
-        c.factor_release()
-        t = time.perf_counter()
-        u10, st10 = c.solve(epsilon=1e-6, max_iterations=args.max_iters, m=10)
-        rec["fp64_m10"] = {"s": round(time.perf_counter() - t, 4), "iterations": st10["iterations"],
-                           "converged": bool(st10["rinf"] <= 1e-6)}
-    if not args.no_cpu:
-        t = time.perf_counter()
-        uo10, so10 = om.solve(epsilon=1e-6, max_iterations=args.max_iters, m=10)
-        rec["oracle_cpu_m10"] = {"s": round(time.perf_counter() - t, 4), "iterations": so10["iterations"],
-                                 "cores": 1}
-        rec["fp64_m10"]["disp_rel_diff_vs_oracle"] = float(np.linalg.norm(u10 - uo10) / np.linalg.norm(uo10))
-    if "oracle_cpu" in rec:
-        rec["speedup_fp64_vs_oracle_cpu"] = round(rec["oracle_cpu"]["s"] / rec["fp64"]["s"], 2)
+                u, st = c.solve(epsilon=1e-6, max_iterations=args.max_iters, m=mm)
+                wall = time.perf_counter() - t
+                d = float(np.linalg.norm(u - ref[mm]) / np.linalg.norm(ref[mm])) if mm in ref else float("nan")
+                rows[key].append((wall, st["iterations"], d))
+    for key, v in rows.items():
+        if not v:
+            continue
+        r = {"s": round(float(np.median([x[0] for x in v])), 4),
+             "iterations": int(np.median([x[1] for x in v])),
+             "iterations_per_load": [x[1] for x in v]}
+        if key.startswith("oracle"):
+            r["cores"] = 1
+        elif not args.no_cpu:
+            r["max_disp_rel_diff_vs_oracle"] = float(max(x[2] for x in v))
+        rec[key] = r
+    if "oracle" in rec:
+        rec["speedup_fp64_vs_oracle_cpu"] = round(rec["oracle"]["s"] / rec["fp64"]["s"], 2)
+        rec["speedup_fp64_m10_vs_oracle_cpu"] = round(rec["oracle_m10"]["s"] / rec["fp64_m10"]["s"], 2)
     out["C1 cantilever 20x4x4 C3D8 NH"] = rec
-    log(f"C1 at {m.pressure:g} Pa: oracle {rec.get('oracle_cpu')}, fp64 {rec['fp64']}")
+    log(f"C1 at {loads[0]:g} Pa (median of {len(loads)} loads): oracle {rec.get('oracle')}, fp64 {rec['fp64']}")
 
     specs = (
         ("C3 cylinder 48x4x40 C3D20R Yeoh", 3, [48, 4, 40, 0, 10000.0], 10000.0),
@@ -463,7 +462,7 @@ def run_adjoint(E, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pressure", type=float, default=100.0)
     ap.add_argument("--target", type=float, default=1e-6)
