@@ -736,8 +736,11 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
     const bool fused = !snOffset && c->fusedRootOpt &&
                        efem::planTop(F, nt, c->fusedTopMax, efem::dev::kTopMaxK, efem::dev::kLdAlign, top);
     auto inTop = [&](int s) { return fused && top.inT[s]; };
+    // entries of L the sweeps read (the diagonal block's lower triangle and
+    // the rows below it), as nnzL counts them
     for (int s = 0; s < F.nsuper; ++s)
-        if (!inTop(s)) d.sweepEntries += static_cast<long long>(w[s]) * mm[s];
+        if (!inTop(s))
+            d.sweepEntries += static_cast<long long>(w[s]) * (w[s] + 1) / 2 + static_cast<long long>(mm[s] - w[s]) * w[s];
     // work-item descriptors per level
     std::vector<std::vector<int>> byLevel(F.nlevels);
     for (int s = 0; s < F.nsuper; ++s)

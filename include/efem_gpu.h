@@ -163,8 +163,9 @@ int efem_residual_d(efem_ctx* ctx, const double* u_d, double* r_d);
 int efem_factor(efem_ctx* ctx, int* failed);
 long long efem_factor_nnz(const efem_ctx* ctx);
 /* layout of the factor's apply (diagnostic, no counterpart in the
- * reference): {stored block entries, of which those the level sweeps read
- * (the supernodes outside the fused top), fused-top columns nT (0: none),
+ * reference): {entries of L (nnz, lower triangles of the diagonal blocks),
+ * of which those the level sweeps read (the supernodes outside the fused
+ * top), fused-top columns nT (0: none),
  * its Z's leading dimension, its depth (-1: none), kernel launches per
  * apply inside the solve loop, elimination-tree levels, supernodes} */
 int efem_factor_layout(const efem_ctx* ctx, long long* out8);
