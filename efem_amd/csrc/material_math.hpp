@@ -17,6 +17,7 @@
 // Strain energies (no counterpart in the reference, which never evaluates
 // one): W = c(I1bar-3)[+C01(I2bar-3)] + (J-1)^2/D1, Yeoh sum_i Ci0(I1bar-3)^i.
 #pragma once
+#include <type_traits>
 
 #ifdef __HIPCC__
 #define EFEM_HD __host__ __device__ __forceinline__
@@ -126,8 +127,31 @@ EFEM_HD void piolaGen(int law, const double* p, const D* hd, T* P)
     for (int k = 0; k < 9; ++k) H[k] = static_cast<T>(hd[k]);
     const D jm1 = detIpHm1(hd);
     const D J = 1.0 + jm1;
-    // |J|^-2/3 - 1 = expm1(-2/3 log|J|), accurate for J near 1
-    const D em1 = expm1_(-2.0 / 3.0 * logAbsJ(jm1));
+    // |J|^-2/3 - 1 = expm1(-2/3 log|J|), accurate for J near 1, consumed in
+    // T.  With FP32 stresses and the Neo-Hookean or Mooney-Rivlin law it only
+    // enters J^-2/3 = 1 + (J^-2/3 - 1), so it is formed in FP32 from the FP64
+    // J - 1 (the FP64-critical step is J - 1 itself, free of cancellation;
+    // the FP32 error of the small difference stays below an ulp of J^-2/3).
+    // Yeoh's t = 3 (J^-2/3 - 1) + tr E J^-2/3 cancels to second order, so
+    // there (and with FP64 stresses) it is formed in D
+    D em1 = D(0.0);  // formed in D where Yeoh's t needs it
+    T jm23;
+    if constexpr (std::is_same<D, double>::value && std::is_same<T, float>::value) {
+        if (law != LAW_YEOH) {
+            const float j1 = static_cast<float>(jm1);
+            jm23 = 1.0f + expm1f(-2.0f / 3.0f * (j1 > -1.0f ? log1pf(j1) : log1pf(-2.0f - j1)));
+        } else {
+            em1 = expm1_(-2.0 / 3.0 * logAbsJ(jm1));
+            jm23 = static_cast<T>(1.0 + em1);
+        }
+    } else {
+        em1 = expm1_(-2.0 / 3.0 * logAbsJ(jm1));
+        jm23 = static_cast<T>(1.0 + em1);
+    }
+    // 1 / J and the volumetric coefficient (2/D1)(J - 1) formed in D, then
+    // rounded once: the volumetric term dominates near incompressibility
+    const T invJ = static_cast<T>(1.0 / J);
+    const T kv = static_cast<T>(2.0 / p[3] * jm1);
     T F[9];
     for (int k = 0; k < 9; ++k) F[k] = H[k];
     F[0] += T(1);
@@ -152,14 +176,12 @@ EFEM_HD void piolaGen(int law, const double* p, const D* hd, T* P)
     const T trE = Em[0] + Em[4] + Em[8];
     T A[9];  // deviatoric argument of the I1bar (and I2bar) terms
     for (int k = 0; k < 9; ++k) A[k] = Em[k];
-    const T jm23 = static_cast<T>(1.0 + em1);
     T c1 = static_cast<T>(p[0]);
     if (law == LAW_YEOH) {
         // t = I1 J^-2/3 - 3 = 3 (J^-2/3 - 1) + tr E J^-2/3
         const T t = static_cast<T>(3.0 * em1) + trE * jm23;
         c1 = static_cast<T>(p[0]) + T(2) * static_cast<T>(p[1]) * t + T(3) * static_cast<T>(p[2]) * t * t;
     }
-    const T invJ = static_cast<T>(1.0 / J);
     T s1 = T(2) * c1 * jm23 * invJ;  // 2 c1 J^-2/3 / J  (F^-T = cof / J)
     T dev[9];
     {
@@ -185,7 +207,6 @@ EFEM_HD void piolaGen(int law, const double* p, const D* hd, T* P)
         dev[8] -= s2 * mB;
     }
     // P = cof(F) dev + (2/D1)(J-1) cof(F)
-    const T kv = static_cast<T>(2.0 / p[3] * jm1);
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             P[3 * i + j] = cf[3 * i] * dev[j] + cf[3 * i + 1] * dev[3 + j] + cf[3 * i + 2] * dev[6 + j] +
