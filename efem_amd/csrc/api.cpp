@@ -163,6 +163,15 @@ struct efem_ctx {
     // coordinates (Xd) instead of reading the stored inverse
     bool tet4Geometry = true;
     double* Xd = nullptr;
+    // option "tet4_patches": the C3D4 residual sums each patch's element
+    // forces in LDS and the gather reads one FP64 partial per (patch, node)
+    // instead of one slot per (element, node); option "patch_gather_lanes"
+    // sets the gather's lanes per node on that path
+    bool tet4Patches = true;
+    int patchLanes = 2;
+    efem::dev::PatchArgs patch;  // device arrays (patch.part null: no plan)
+    int *pRptr = nullptr, *pFptr = nullptr, *pFslot = nullptr;
+    int pRec = 0;
     // option "bwd_stage": backward sweep items that stage [y_s; -z[rows]]
     // in LDS, 4, 8 or 16 columns each (0: the per-wave gathers), on every
     // level but those of many short supernodes (16-column narrow items)
@@ -348,6 +357,30 @@ int uploadModel(efem_ctx* c)
     }
     if ((rc = upload(c, &c->gptr, m.gptr))) return rc;
     if ((rc = upload(c, &c->gslot, m.gslot))) return rc;
+    if (m.nen == 4 && c->parts.empty() && m.nElems > 0) {
+        efem::Tet4Patches tp;
+        efem::buildTet4Patches(m, efem::dev::kPatch, tp);
+        efem::dev::PatchArgs& pa = c->patch;
+        pa.E = m.nElems;
+        pa.nPatch = tp.nPatch;
+        int *conn = nullptr, *order = nullptr, *roff = nullptr, *rec = nullptr;
+        double *vol = nullptr, *part = nullptr;
+        uint16_t* ent = nullptr;
+        if ((rc = upload(c, &conn, tp.conn)) || (rc = upload(c, &order, tp.order)) || (rc = upload(c, &vol, tp.vol)) ||
+            (rc = upload(c, &roff, tp.roff)) || (rc = upload(c, &rec, tp.rec)) || (rc = upload(c, &ent, tp.ent)) ||
+            (rc = devAlloc(c, &part, 3 * static_cast<size_t>(std::max(tp.nRec, 1)))) ||
+            (rc = upload(c, &c->pRptr, tp.rptr)) || (rc = upload(c, &c->pFptr, tp.fptr)) ||
+            (rc = upload(c, &c->pFslot, tp.fslot)))
+            return rc;
+        pa.conn = conn;
+        pa.order = order;
+        pa.vol = vol;
+        pa.roff = roff;
+        pa.rec = rec;
+        pa.ent = ent;
+        pa.part = part;
+        c->pRec = tp.nRec;
+    }
     // constant loads: without any (no nodal force, no gravity) the vector
     // stays off the device and the gather and energy kernels skip its reads
     // (subtracting a zero changes no value)
@@ -454,14 +487,20 @@ int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD
     // the element forces and the pressure faces go out together (one
     // launch for C3D4); the profiled element time then holds the faces
     const bool faces = m.nFaces > 0 && (terms & kTermPressure);
+    const bool patches = (terms & kTermElements) && c->tet4Patches && c->patch.part;
     if (terms & kTermElements) {
         const efem::dev::ElemArgs ea = elemArgs(c, x, f32);
         const bool prof = c->profiling && c->evUsed + 2 <= static_cast<int>(c->evs.size());
         if (prof) HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed], c->stream));
-        if (faces)
+        if (patches) {
+            efem::dev::FaceArgs fa = faceArgs(c, x, nullptr);
+            if (!faces) fa.nFaces = 0;
+            efem::dev::launchElementPatches(ea, c->patch, fa, c->stream);
+        } else if (faces) {
             efem::dev::launchElementForceFaces(ea, faceArgs(c, x, nullptr), c->stream);
-        else
+        } else {
             efem::dev::launchElementForce(ea, c->stream);
+        }
         if (prof) {
             HIP_CHECK(c, hipEventRecord(c->evs[c->evUsed + 1], c->stream));
             c->evUsed += 2;
@@ -476,6 +515,17 @@ int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD
     efem::dev::GatherArgs ga = gatherArgs(c, r, f32, (terms & kTermElements) ? 1.0 : 0.0,
                                           (terms & kTermPressure) ? -1.0 : 0.0,
                                           (terms & kTermConstant) ? c->fext : nullptr);
+    if (patches) {
+        // the records' FP64 partials (node-major) stand for the element
+        // slots; the lists hold the face slots only
+        ga.nElemSlots = 0;
+        ga.mixed = 0;
+        ga.rptr = c->pRptr;
+        ga.gptr = c->pFptr;
+        ga.gslot = c->pFslot;
+        ga.eslots = c->patch.part;
+        ga.lanes = c->patchLanes;
+    }
     if (dotOut) {
         ga.dotD = dotD;
         ga.dotX = x;
@@ -2202,6 +2252,14 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         releaseFactor(c);
         return EFEM_OK;
     }
+    if (k == "tet4_patches") {
+        c->tet4Patches = value != 0.0;
+        return EFEM_OK;
+    }
+    if (k == "patch_gather_lanes" && (iv == 1 || iv == 2 || iv == 4 || iv == 8)) {
+        c->patchLanes = iv;
+        return EFEM_OK;
+    }
     if (k == "tet4_geometry") {
         c->tet4Geometry = value != 0.0;
         return EFEM_OK;
@@ -2860,20 +2918,29 @@ int efem_autotune(efem_ctx* c, double* report)
     double bestRes = 1e300;
     const int lanes[4] = {1, 2, 4, 8};
     const int tpbs[3] = {64, 128, 256};
+    // with C3D4 patches the residual's shapes are the record gather's lanes
+    const bool patchPath = c->tet4Patches && c->patch.part;
+    int bestPatchLanes = c->patchLanes;
     for (int li = 0; li < 4 && rc == EFEM_OK; ++li)
-        for (int ti = 0; ti < (c->model.nen == 4 ? 3 : 1) && rc == EFEM_OK; ++ti) {
+        for (int ti = 0; ti < (c->model.nen == 4 && !patchPath ? 3 : 1) && rc == EFEM_OK; ++ti) {
             efem::dev::LaunchTuning t = best;
-            t.gatherLanes = lanes[li];
-            if (c->model.nen == 4) t.tet4Threads = tpbs[ti];
+            if (patchPath) {
+                c->patchLanes = lanes[li];
+            } else {
+                t.gatherLanes = lanes[li];
+                if (c->model.nen == 4) t.tet4Threads = tpbs[ti];
+            }
             efem::dev::setTuning(t);
             double ms = 0.0;
             rc = timeIt([&]() { return residualInternal(c, c->tmp, c->r); }, 10, &ms);
             if (rc == EFEM_OK && ms < bestRes) {
                 bestRes = ms;
+                bestPatchLanes = lanes[li];
                 best.gatherLanes = t.gatherLanes;
                 best.tet4Threads = t.tet4Threads;
             }
         }
+    c->patchLanes = bestPatchLanes;
     // reductions: the two-vector dot + norm of the line search
     double bestRed = 1e300;
     const int grids[4] = {32, 64, 128, 256};
@@ -2964,7 +3031,7 @@ int efem_autotune(efem_ctx* c, double* report)
     c->tuning = best;
     efem::dev::setTuning(best);
     if (report) {
-        report[0] = best.gatherLanes;
+        report[0] = patchPath ? c->patchLanes : best.gatherLanes;
         report[1] = best.tet4Threads;
         report[2] = best.reduceBlocks;
         report[3] = bestRes;

@@ -150,6 +150,36 @@ __device__ __forceinline__ void tet4Ginv(const double* __restrict__ Xd, const in
 // Xd (may be null): J0^-1 recomputed from the nodes' coordinates when all
 // four nodes are free (nF free nodes), else read from ginv (every fixed
 // node shares one DOF slot, so its coordinates are not in Xd)
+// The four nodal forces (or tangent products) of one C3D4 element with
+// free ids id[]: f[3 a + c] for node a, component c.  eg indexes ginv
+// (SoA, stride E) and the per-element parameters; volw = V_e w.
+template <typename T, int LAW, int MODE>
+__device__ __forceinline__ void tet4Forces(const int (&id)[4], int eg, int E, const double* __restrict__ ginv,
+                                           double volw, const double* __restrict__ params, int ppe,
+                                           const double* __restrict__ x, const double* __restrict__ v,
+                                           const double* __restrict__ Xd, int nF, T (&f)[12])
+{
+    double Gu[9], Gv[9];
+    tet4Grad<MODE>(x, id, Gu);
+    if (MODE == MODE_TANGENT) tet4Grad<MODE>(v, id, Gv);
+    double Jd[9];
+    if (Xd && id[0] < nF && id[1] < nF && id[2] < nF && id[3] < nF) {
+        tet4Ginv(Xd, id, Jd);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + eg];
+    }
+    T Q[9];
+    stressMoment<T, MODE>(LAW, params + (ppe ? 4 * eg : 0), Gu, Gv, Jd, volw, Q);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        f[i] = Q[3 * i];
+        f[3 + i] = Q[3 * i + 1];
+        f[6 + i] = Q[3 * i + 2];
+        f[9 + i] = -(Q[3 * i] + Q[3 * i + 1] + Q[3 * i + 2]);
+    }
+}
+
 template <typename T, int LAW, int MODE>
 __device__ __forceinline__ void tet4Lane(int e, int E, const int* __restrict__ connDof,
                                          const double* __restrict__ ginv, const double* __restrict__ vol, double w0,
@@ -160,26 +190,11 @@ __device__ __forceinline__ void tet4Lane(int e, int E, const int* __restrict__ c
     int id[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) id[a] = connDof[a * E + e];
-    double Gu[9], Gv[9];
-    tet4Grad<MODE>(x, id, Gu);
-    if (MODE == MODE_TANGENT) tet4Grad<MODE>(v, id, Gv);
-    double Jd[9];
-    if (Xd && id[0] < nF && id[1] < nF && id[2] < nF && id[3] < nF) {
-        tet4Ginv(Xd, id, Jd);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Jd[k] = ginv[k * E + e];
-    }
-    T Q[9];
-    stressMoment<T, MODE>(LAW, params + (ppe ? 4 * e : 0), Gu, Gv, Jd, vol[e] * w0, Q);
+    T f[12];
+    tet4Forces<T, LAW, MODE>(id, e, E, ginv, vol[e] * w0, params, ppe, x, v, Xd, nF, f);
     T* out = slots + 12 * static_cast<size_t>(e);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        out[i] = Q[3 * i];
-        out[3 + i] = Q[3 * i + 1];
-        out[6 + i] = Q[3 * i + 2];
-        out[9 + i] = -(Q[3 * i] + Q[3 * i + 1] + Q[3 * i + 2]);
-    }
+    for (int k = 0; k < 12; ++k) out[k] = f[k];
 }
 
 

@@ -244,6 +244,93 @@ __global__ __launch_bounds__(TPB) void kElemTet4Faces(ElemArgs ea, FaceArgs fa, 
     pressureTileBlock<4, TPB>(fa, blockIdx.x - nEB);
 }
 
+// C3D4 element forces summed per patch in LDS (see PatchArgs): blocks
+// [0, nEB) are patches (remapped per XCD), the rest face tiles
+template <typename T, int LAW>
+__global__ __launch_bounds__(kPatch) void kElemTet4Patch(ElemArgs ea, PatchArgs pa, FaceArgs fa, int nEB)
+{
+    if (static_cast<int>(blockIdx.x) >= nEB) {
+        pressureTileBlock<4, kPatch>(fa, blockIdx.x - nEB);
+        return;
+    }
+    __shared__ T sq[kPatch * 12];
+    const int p = xcdBlockOf(blockIdx.x, nEB);
+    const int t = threadIdx.x;
+    const int pe = p * kPatch + t;
+    const int r0 = pa.roff[p], nr = pa.roff[p + 1] - r0;
+    // the lane's first record and its first 16 entries, loaded before the
+    // element's work
+    int rb = 0, rn = 0, rp = 0;
+    uint4 c0 = make_uint4(0, 0, 0, 0), c1 = make_uint4(0, 0, 0, 0);
+    if (t < nr) {
+        const int4 rc = reinterpret_cast<const int4*>(pa.rec)[r0 + t];
+        rb = rc.x;
+        rn = rc.y;
+        rp = rc.z;
+        const uint4* ep = reinterpret_cast<const uint4*>(pa.ent + rb);
+        c0 = ep[0];
+        if (rn > 8) c1 = ep[1];
+    }
+    if (pe < pa.E) {
+        int id[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) id[a] = pa.conn[a * pa.E + pe];
+        T f[12];
+        tet4Forces<T, LAW, MODE_FORCE>(id, pa.order[pe], ea.nElems, static_cast<const double*>(ea.ginv),
+                                       pa.vol[pe] * 1.0, ea.params, ea.paramsPerElem, ea.x, nullptr, ea.Xd,
+                                       ea.nFree, f);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) sq[12 * t + k] = f[k];
+    }
+    __syncthreads();
+    for (int j = t; j < nr; j += kPatch) {
+        if (j != t) {
+            const int4 rc = reinterpret_cast<const int4*>(pa.rec)[r0 + j];
+            rb = rc.x;
+            rn = rc.y;
+            rp = rc.z;
+            const uint4* ep = reinterpret_cast<const uint4*>(pa.ent + rb);
+            c0 = ep[0];
+            c1 = rn > 8 ? ep[1] : make_uint4(0, 0, 0, 0);
+        }
+        const unsigned w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < rn) {
+                const unsigned q = (w[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+                s0 += static_cast<double>(sq[3 * q]);
+                s1 += static_cast<double>(sq[3 * q + 1]);
+                s2 += static_cast<double>(sq[3 * q + 2]);
+            }
+        }
+        for (int i = 16; i < rn; ++i) {
+            const unsigned q = pa.ent[rb + i];
+            s0 += static_cast<double>(sq[3 * q]);
+            s1 += static_cast<double>(sq[3 * q + 1]);
+            s2 += static_cast<double>(sq[3 * q + 2]);
+        }
+        double* o = pa.part + 3 * static_cast<size_t>(rp);
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
+    }
+}
+
+template <typename T>
+void launchPatchesT(const ElemArgs& ea, const PatchArgs& pa, const FaceArgs& fa, hipStream_t s)
+{
+    constexpr int FPB = kPatch / PressureTile<4>::kMaxQ;
+    const int nEB = pa.nPatch;
+    const int nFB = fa.nFaces > 0 ? (fa.nFaces + FPB - 1) / FPB : 0;
+    if (nEB + nFB == 0) return;
+    switch (ea.law) {
+        case LAW_NH: kElemTet4Patch<T, LAW_NH><<<nEB + nFB, kPatch, 0, s>>>(ea, pa, fa, nEB); break;
+        case LAW_MR: kElemTet4Patch<T, LAW_MR><<<nEB + nFB, kPatch, 0, s>>>(ea, pa, fa, nEB); break;
+        default: kElemTet4Patch<T, LAW_YEOH><<<nEB + nFB, kPatch, 0, s>>>(ea, pa, fa, nEB);
+    }
+}
+
 template <typename T, int LAW, int TPB>
 void launchTet4FacesTpb(const ElemArgs& ea, const FaceArgs& fa, hipStream_t s)
 {
@@ -350,6 +437,16 @@ __global__ __launch_bounds__(kThreads) void kGather(GatherArgs a)
     const T* es = static_cast<const T*>(a.eslots);
     double se[3] = {0, 0, 0}, sf[3] = {0, 0, 0};
     if (i < a.nFree) {
+        if (a.rptr && a.ce != 0.0) {
+            const double* er = static_cast<const double*>(a.eslots);
+            const int q1 = a.rptr[i + 1];
+            for (int q = a.rptr[i] + sub; q < q1; q += LPN) {
+                const double* v = er + 3 * static_cast<size_t>(q);
+                se[0] += v[0];
+                se[1] += v[1];
+                se[2] += v[2];
+            }
+        }
         const int p1 = a.gptr[i + 1];
         for (int p = a.gptr[i] + sub; p < p1; p += LPN) {
             const int s = a.gslot[p];
@@ -694,6 +791,29 @@ constexpr int KG = kMaxCompact;
 // v[t] = s_loc . y_t, v[NC + t] = s_t . y_loc, v[2 NC] = y_loc . y_loc,
 // v[2 NC + 1 + t] = s_t . r.
 template <int NC>
+__device__ __forceinline__ void gramStep(double (&v)[3 * NC + 1], double ri, double xi, double xpi, double rpi,
+                                         const double* sk, const double* yk, double& si, double& yi)
+{
+    si = xi - xpi;
+    yi = ri - rpi;
+    v[0] += si * yi;
+    v[NC] += si * yi;
+    v[2 * NC] += yi * yi;
+    v[2 * NC + 1] += si * ri;
+#pragma unroll
+    for (int t = 1; t < NC; ++t) {
+        v[t] += si * yk[t - 1];
+        v[NC + t] += sk[t - 1] * yi;
+        v[2 * NC + 1 + t] += sk[t - 1] * ri;
+    }
+}
+
+__device__ __forceinline__ double2 ld2(const double* p, int i) { return *reinterpret_cast<const double2*>(p + i); }
+
+// P2: each lane takes element pairs (2 p, 2 p + 1) with 16-byte loads and
+// stores (n even, every stream 16-byte aligned), the pair's elements summed
+// in index order
+template <int NC, bool P2>
 __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double* __restrict__ Y,
                                                       const double* __restrict__ x, const double* __restrict__ xp,
                                                       const double* __restrict__ r, const double* __restrict__ rp,
@@ -714,27 +834,48 @@ __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double
     }
     double* sl = S + static_cast<size_t>(loc) * n;
     double* yl = Y + static_cast<size_t>(loc) * n;
-    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
-        const double ri = r[i], xi = x[i], xpi = xp[i], rpi = rp[i];
-        double sk[NC > 1 ? NC - 1 : 1], yk[NC > 1 ? NC - 1 : 1];
+    constexpr int NA = NC > 1 ? NC - 1 : 1;
+    if constexpr (P2) {
+        const int np = n >> 1;
+        for (int p = blockIdx.x * kRB + threadIdx.x; p < np; p += gridDim.x * kRB) {
+            const int i = 2 * p;
+            const double2 r2 = ld2(r, i), x2 = ld2(x, i), xp2 = ld2(xp, i), rp2 = ld2(rp, i);
+            double2 sk2[NA], yk2[NA];
 #pragma unroll
-        for (int t = 1; t < NC; ++t) {
-            sk[t - 1] = sAge[t - 1][i];
-            yk[t - 1] = yAge[t - 1][i];
-        }
-        const double si = xi - xpi, yi = ri - rpi;
-        v[0] += si * yi;
-        v[NC] += si * yi;
-        v[2 * NC] += yi * yi;
-        v[2 * NC + 1] += si * ri;
+            for (int t = 1; t < NC; ++t) {
+                sk2[t - 1] = ld2(sAge[t - 1], i);
+                yk2[t - 1] = ld2(yAge[t - 1], i);
+            }
+            double sk[NA], yk[NA], s0, y0, s1, y1;
 #pragma unroll
-        for (int t = 1; t < NC; ++t) {
-            v[t] += si * yk[t - 1];
-            v[NC + t] += sk[t - 1] * yi;
-            v[2 * NC + 1 + t] += sk[t - 1] * ri;
+            for (int t = 0; t < NC - 1; ++t) {
+                sk[t] = sk2[t].x;
+                yk[t] = yk2[t].x;
+            }
+            gramStep<NC>(v, r2.x, x2.x, xp2.x, rp2.x, sk, yk, s0, y0);
+#pragma unroll
+            for (int t = 0; t < NC - 1; ++t) {
+                sk[t] = sk2[t].y;
+                yk[t] = yk2[t].y;
+            }
+            gramStep<NC>(v, r2.y, x2.y, xp2.y, rp2.y, sk, yk, s1, y1);
+            *reinterpret_cast<double2*>(sl + i) = make_double2(s0, s1);
+            *reinterpret_cast<double2*>(yl + i) = make_double2(y0, y1);
         }
-        sl[i] = si;
-        yl[i] = yi;
+    } else {
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
+            const double ri = r[i], xi = x[i], xpi = xp[i], rpi = rp[i];
+            double sk[NA], yk[NA];
+#pragma unroll
+            for (int t = 1; t < NC; ++t) {
+                sk[t - 1] = sAge[t - 1][i];
+                yk[t - 1] = yAge[t - 1][i];
+            }
+            double si, yi;
+            gramStep<NC>(v, ri, xi, xpi, rpi, sk, yk, si, yi);
+            sl[i] = si;
+            yl[i] = yi;
+        }
     }
     writePartials<N, 0u>(v, work);
 }
@@ -832,7 +973,8 @@ __device__ __forceinline__ double hAt(const HSrc& q, int i)
 
 // Second loop of the two-loop recursion: dots of Y_k with q (the H0 output,
 // stored pairs in slots 0..NC-1) and of dw with q in one sweep.
-template <int NC>
+// P2: element pairs with 16-byte loads of R and dw (as kGramFirst)
+template <int NC, bool P2>
 __global__ __launch_bounds__(kRB) void kTwoLoopDots(HSrc q, const double* __restrict__ R, int n,
                                                         const double* __restrict__ dw, double* work)
 {
@@ -840,14 +982,32 @@ __global__ __launch_bounds__(kRB) void kTwoLoopDots(HSrc q, const double* __rest
     double v[NC + 1];
 #pragma unroll
     for (int k = 0; k <= NC; ++k) v[k] = 0.0;
-    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
-        const double qi = hAt(q, i), dwi = dw[i];
-        double rk[NC];
+    if constexpr (P2) {
+        const int np = n >> 1;
+        for (int p = blockIdx.x * kRB + threadIdx.x; p < np; p += gridDim.x * kRB) {
+            const int i = 2 * p;
+            const double q0 = hAt(q, i), q1 = hAt(q, i + 1);
+            const double2 dw2 = ld2(dw, i);
+            double2 rk[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) rk[k] = R[static_cast<size_t>(k) * n + i];
+            for (int k = 0; k < NC; ++k) rk[k] = ld2(R + static_cast<size_t>(k) * n, i);
 #pragma unroll
-        for (int k = 0; k < NC; ++k) v[k] += rk[k] * qi;
-        v[NC] += dwi * qi;
+            for (int k = 0; k < NC; ++k) v[k] += rk[k].x * q0;
+            v[NC] += dw2.x * q0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[k] += rk[k].y * q1;
+            v[NC] += dw2.y * q1;
+        }
+    } else {
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
+            const double qi = hAt(q, i), dwi = dw[i];
+            double rk[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) rk[k] = R[static_cast<size_t>(k) * n + i];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[k] += rk[k] * qi;
+            v[NC] += dwi * qi;
+        }
     }
     writePartials<NC + 1, 0u>(v, work);
 }
@@ -1306,6 +1466,19 @@ void launchElementForce(const ElemArgs& a, hipStream_t s)
         launchElem<double, MODE_FORCE>(a, nullptr, s);
 }
 
+void launchElementPatches(const ElemArgs& a, const PatchArgs& pa, const FaceArgs& f, hipStream_t s)
+{
+    // faces whose Gauss points fit the tile go into the same launch
+    const bool fused = f.nFaces > 0 && f.nen == 4 && f.fq <= PressureTile<4>::kMaxQ;
+    FaceArgs none = f;
+    none.nFaces = 0;
+    if (a.mixed)
+        launchPatchesT<float>(a, pa, fused ? f : none, s);
+    else
+        launchPatchesT<double>(a, pa, fused ? f : none, s);
+    if (f.nFaces > 0 && !fused) launchPressure(f, s);
+}
+
 void launchElementForceFaces(const ElemArgs& a, const FaceArgs& f, hipStream_t s)
 {
     // one launch for the C3D4 kernel and faces whose Gauss points fit the tile
@@ -1494,7 +1667,7 @@ void gatherLaunch(const GatherArgs& a, int lpn, int g, hipStream_t s)
 void launchGather(const GatherArgs& a, hipStream_t s)
 {
     if (a.nFree <= 0) return;
-    const int lpn = tuning().gatherLanes;
+    const int lpn = a.lanes ? a.lanes : tuning().gatherLanes;
     const int g = static_cast<int>((static_cast<long long>(lpn) * a.nFree + kThreads - 1) / kThreads);
     if (!a.dotWork) return gatherLaunch<0>(a, lpn, g, s);
     if (a.dotXt) {
@@ -1585,6 +1758,16 @@ void withPairCount(int nc, F&& f)
 }
 static_assert(kMaxCompact == 16, "withPairCount covers 1..kMaxCompact");
 
+// the history sweeps' paired form: n even and every stream 16-byte aligned
+// (then every history slot, n doubles apart, is too)
+bool pairedStreams(int n, std::initializer_list<const void*> ptrs)
+{
+    if (n & 1) return false;
+    for (const void* p : ptrs)
+        if (reinterpret_cast<uintptr_t>(p) & 15u) return false;
+    return true;
+}
+
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
                      double* alpha, double* rs, double* out, double* work, hipStream_t s, const int* pinv)
@@ -1592,7 +1775,10 @@ void launchGramFirst(double* S, double* Y, const double* x, const double* xp, co
     withPairCount(nc, [&](auto ncc) {
         constexpr int NC = decltype(ncc)::value;
         const int g = historyGrid(n);
-        kGramFirst<NC><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
+        if (pairedStreams(n, {S, Y, x, xp, r, rp}))
+            kGramFirst<NC, true><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
+        else
+            kGramFirst<NC, false><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
         if (mcap <= 8) {
             if constexpr (NC <= 8)
                 kReduceFinish<3 * NC + 1, 0u><<<1, kRF, 0, s>>>(
@@ -1615,7 +1801,10 @@ void launchTwoLoopSecond(HSrc h, const double* S, const double* Y, int n, int nc
     withPairCount(nc, [&](auto ncc) {
         constexpr int NC = decltype(ncc)::value;
         const int g = historyGrid(n);
-        kTwoLoopDots<NC><<<g, kRB, 0, s>>>(h, Y, n, dw, work);
+        if (pairedStreams(n, {Y, dw}))
+            kTwoLoopDots<NC, true><<<g, kRB, 0, s>>>(h, Y, n, dw, work);
+        else
+            kTwoLoopDots<NC, false><<<g, kRB, 0, s>>>(h, Y, n, dw, work);
         if (mcap <= 8) {
             if constexpr (NC <= 8)
                 kReduceFinish<NC + 1, 0u><<<1, kRF, 0, s>>>(

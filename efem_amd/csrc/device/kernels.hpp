@@ -79,8 +79,32 @@ void launchElementForceFaces(const ElemArgs& a, const FaceArgs& f, hipStream_t s
 // load stiffness of the follower pressure applied to a.v, or its transpose
 void launchPressureTangent(const FaceArgs& a, int transpose, hipStream_t s);
 
+// C3D4 element forces assembled per patch in LDS (host: Tet4Patches): one
+// block of kPatch lanes per patch, a lane per element; after a barrier
+// each record (a free node the patch touches) sums its entries in FP64, in
+// entry order, into its node-major position: part[3 pos + c].  The
+// pressure faces (fa.nFaces > 0) follow in the same launch.  A gather with
+// eslots = part (double), rptr and face-only lists then forms r.
+constexpr int kPatch = 256;
+struct PatchArgs {
+    int E = 0, nPatch = 0;
+    const int* conn = nullptr;        // [4][E] free ids, patch order
+    const int* order = nullptr;       // [E] element of each patch position
+    const double* vol = nullptr;      // [E] patch order
+    const int* roff = nullptr;        // [nPatch + 1]
+    const int* rec = nullptr;         // [nRec][4] first entry, count, position, 0
+    const uint16_t* ent = nullptr;    // record entries 4 lane + a
+    double* part = nullptr;           // [nRec][3], node-major
+};
+void launchElementPatches(const ElemArgs& ea, const PatchArgs& pa, const FaceArgs& fa, hipStream_t s);
+
 struct GatherArgs {
     int nFree, nElemSlots, mixed;
+    int lanes = 0;          // lanes per node (0: the tuned value)
+    // optional: node i's element terms are the contiguous FP64 records
+    // [rptr[i], rptr[i + 1]) of eslots; gptr / gslot then list face slots
+    // only (nElemSlots = 0)
+    const int* rptr = nullptr;
     const int* gptr;
     const int* gslot;
     const void* eslots;     // element slots (double or float)

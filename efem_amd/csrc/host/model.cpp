@@ -477,6 +477,117 @@ void scalarLaplacian(const Model& m, Csc& L)
     }
 }
 
+void buildTet4Patches(const Model& m, int P, Tet4Patches& out)
+{
+    const int E = m.nElems, nen = m.nen;
+    out = Tet4Patches{};
+    out.P = P;
+    if (E <= 0 || nen != 4 || P <= 0) return;
+    std::vector<double> cen(3 * static_cast<size_t>(E));
+    for (int e = 0; e < E; ++e)
+        for (int d = 0; d < 3; ++d) {
+            double s = 0.0;
+            for (int a = 0; a < 4; ++a) s += m.X[3 * static_cast<size_t>(m.conn[4 * static_cast<size_t>(e) + a]) + d];
+            cen[3 * static_cast<size_t>(e) + d] = 0.25 * s;
+        }
+    std::vector<int> idx(E);
+    for (int e = 0; e < E; ++e) idx[e] = e;
+    // bisection: [lo, hi) split at lo + P * floor(patches / 2) along the
+    // longest extent of its centroids; a range of at most P is one patch
+    std::vector<std::pair<int, int>> todo{{0, E}}, leaves;
+    while (!todo.empty()) {
+        const auto [lo, hi] = todo.back();
+        todo.pop_back();
+        const int n = hi - lo;
+        if (n <= P) {
+            leaves.push_back({lo, hi});
+            continue;
+        }
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int i = lo; i < hi; ++i)
+            for (int d = 0; d < 3; ++d) {
+                const double v = cen[3 * static_cast<size_t>(idx[i]) + d];
+                mn[d] = std::min(mn[d], v);
+                mx[d] = std::max(mx[d], v);
+            }
+        int ax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+        const int np = (n + P - 1) / P;
+        const int mid = lo + P * (np / 2);
+        std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int a, int b) {
+            const double va = cen[3 * static_cast<size_t>(a) + ax], vb = cen[3 * static_cast<size_t>(b) + ax];
+            return va < vb || (va == vb && a < b);
+        });
+        todo.push_back({mid, hi});
+        todo.push_back({lo, mid});
+    }
+    std::sort(leaves.begin(), leaves.end());
+    for (const auto& [lo, hi] : leaves) std::sort(idx.begin() + lo, idx.begin() + hi);
+    out.nPatch = static_cast<int>(leaves.size());
+    out.order = idx;
+    out.conn.resize(4 * static_cast<size_t>(E));
+    out.vol.resize(E);
+    for (int pe = 0; pe < E; ++pe) {
+        const int e = idx[pe];
+        for (int a = 0; a < 4; ++a) out.conn[static_cast<size_t>(a) * E + pe] = m.connDof[static_cast<size_t>(a) * E + e];
+        out.vol[pe] = m.vol[e];
+    }
+    // records: per patch the free nodes it touches, ascending; entries
+    // lane-major
+    out.roff.assign(1, 0);
+    std::vector<int> recNode;
+    std::vector<std::pair<int, int>> pairs;  // (free id, 4 lane + a)
+    for (int p = 0; p < out.nPatch; ++p) {
+        const int lo = leaves[p].first, hi = leaves[p].second;
+        pairs.clear();
+        for (int pe = lo; pe < hi; ++pe)
+            for (int a = 0; a < 4; ++a) {
+                const int f = out.conn[static_cast<size_t>(a) * E + pe];
+                if (f < m.nFree) pairs.push_back({f, 4 * (pe - lo) + a});
+            }
+        std::sort(pairs.begin(), pairs.end());
+        for (size_t i = 0; i < pairs.size();) {
+            size_t j = i;
+            while (j < pairs.size() && pairs[j].first == pairs[i].first) ++j;
+            const int first = static_cast<int>(out.ent.size());
+            out.rec.push_back(first);
+            out.rec.push_back(static_cast<int>(j - i));
+            out.rec.push_back(0);
+            out.rec.push_back(0);
+            for (size_t k = i; k < j; ++k) out.ent.push_back(static_cast<uint16_t>(pairs[k].second));
+            while (out.ent.size() % 8) out.ent.push_back(0xffff);
+            recNode.push_back(pairs[i].first);
+            i = j;
+        }
+        out.roff.push_back(static_cast<int>(recNode.size()));
+    }
+    out.nRec = static_cast<int>(recNode.size());
+    // node-major positions of the records (patch order within a node)
+    out.rptr.assign(m.nFree + 1, 0);
+    for (int k = 0; k < out.nRec; ++k) out.rptr[recNode[k] + 1]++;
+    for (int i = 0; i < m.nFree; ++i) out.rptr[i + 1] += out.rptr[i];
+    {
+        std::vector<int> pos(out.rptr.begin(), out.rptr.end() - 1);
+        for (int k = 0; k < out.nRec; ++k) out.rec[4 * static_cast<size_t>(k) + 2] = pos[recNode[k]]++;
+    }
+    // face slots per free node, faces ascending
+    out.fptr.assign(m.nFree + 1, 0);
+    for (int fc = 0; fc < m.nFaces; ++fc)
+        for (int a = 0; a < nen; ++a) {
+            const int f = m.fid[m.conn[static_cast<size_t>(m.faceElem[fc]) * nen + a]];
+            if (f >= 0) out.fptr[f + 1]++;
+        }
+    for (int i = 0; i < m.nFree; ++i) out.fptr[i + 1] += out.fptr[i];
+    out.fslot.resize(out.fptr[m.nFree]);
+    std::vector<int> pos(out.fptr.begin(), out.fptr.end() - 1);
+    for (int fc = 0; fc < m.nFaces; ++fc)
+        for (int a = 0; a < nen; ++a) {
+            const int f = m.fid[m.conn[static_cast<size_t>(m.faceElem[fc]) * nen + a]];
+            if (f >= 0) out.fslot[pos[f]++] = fc * nen + a;
+        }
+}
+
 std::vector<double> freeCoords(const Model& m)
 {
     std::vector<double> xyz(3 * static_cast<size_t>(m.nFree));

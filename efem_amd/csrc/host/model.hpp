@@ -5,6 +5,7 @@
 // (src/fe/ElementConstraint.cpp:69-83), ExternalPressureFace::initial
 // (src/fe/ExternalConstraint.cpp:255-287) and the constraint configs.
 #pragma once
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,31 @@ struct Model {
 };
 
 bool buildModel(const efem_problem& pb, Model& m, std::string& msg);
+
+// C3D4 element patches for a residual assembled in LDS.  Recursive
+// coordinate bisection of the element centroids (longest axis, split at a
+// multiple of P elements; ties by element id) groups the elements into
+// patches of P (the last may hold fewer); within a patch the elements keep
+// ascending id.  A patch's records are the free nodes its elements touch,
+// by ascending free id; a record's entries are the patch's (lane, local
+// node) pairs of that node as 4 lane + a, lane-major.  The entries of each
+// record start on a multiple of 8 (padding: 0xffff).  Records are stored
+// node-major: a free node's records, in patch order, at [rptr[i],
+// rptr[i + 1]); rec[k] = {first entry, entry count, stored position, 0}.
+// The face slots (f nen + a, faces ascending) of a free node are listed
+// apart (fptr / fslot).
+struct Tet4Patches {
+    int P = 0, nPatch = 0, nRec = 0;
+    std::vector<int> order;          // [E] patch position -> element
+    std::vector<int> conn;           // [4][E] free ids, patch order
+    std::vector<double> vol;         // [E] patch order
+    std::vector<int> roff;           // [nPatch + 1] first record of a patch
+    std::vector<int> rec;            // [nRec][4]
+    std::vector<uint16_t> ent;       // entries
+    std::vector<int> rptr;           // [nFree + 1] stored records of a node
+    std::vector<int> fptr, fslot;    // free node -> face slots
+};
+void buildTet4Patches(const Model& m, int P, Tet4Patches& out);
 
 // material parameters (layout of efem_problem.params) and the per-element
 // approximate-Jacobian coefficient derived from them

@@ -292,13 +292,22 @@ const char* efem_host_dense_kernel(void);
  *                       nodal forces and gravity -- one constraint's part
  *                       (Constrainable::conEvaluteResidual,
  *                       src/fe/Constraint.h:26); solves use every term
+ *   "tet4_patches"      0 / 1 (default 1; C3D4, single-problem contexts):
+ *                       the residual sums the element forces of patches of
+ *                       256 spatially close elements in LDS and the gather
+ *                       reads one FP64 record per (patch, node) instead of
+ *                       one slot per (element, node); the same forces, the
+ *                       sums in another fixed order
+ *   "patch_gather_lanes" 1, 2, 4, 8 (default 2): lanes per node of that
+ *                       gather
  *   "gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks":
  *                       launch shapes (see efem_autotune) */
 int efem_set_option(efem_ctx* ctx, const char* name, double value);
 
 /* autotuner (no counterpart in the reference): times the candidate launch
  * shapes on this context's problem and keeps the fastest -- gather lanes per
- * node {1,2,4,8} x C3D4 element block size {64,128,256} (residual time), the
+ * node {1,2,4,8} x C3D4 element block size {64,128,256} (residual time; on
+ * the C3D4 patch path the record gather's lanes {1,2,4,8}), the
  * reduction grid {32,64,128,256} blocks (line-search dot time), and the grid
  * of the reductions over the L-BFGS history {32,64,128,256} blocks (one
  * compact two-loop pass pair with m = 8; grows the context's history buffers
@@ -306,7 +315,8 @@ int efem_set_option(efem_ctx* ctx, const char* name, double value);
  * formulation.  On a batch context it also times the persistent solver
  * (40 iterations of every instance from u = 0, with the preconditioner)
  * at 256, 512 and 1024 lanes per instance and keeps the fastest.
- * report[9] (may be NULL) = {lanes, tet4 threads, reduce blocks, residual
+ * report[9] (may be NULL) = {lanes (the record gather's on the C3D4 patch
+ * path), tet4 threads, reduce blocks, residual
  * ms, dot ms, history blocks, history ms, batch lanes (256 when not a
  * batch), batch ms (0 when not a batch)}.  The same shapes can be set by
  * efem_set_option("gather_lanes" | "tet4_threads" | "reduce_blocks" |

@@ -81,6 +81,37 @@ def test_residual_workloads_fp64(which, opts, rng):
         assert rel(ctx.residual(u), om.residual(u)) <= 1e-10
 
 
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_tet4_patch_assembly_matches_element_slots(prec, rng):
+    # C3D4 residual with each patch's element forces summed in LDS and one
+    # FP64 record per (patch, node) (option tet4_patches, the default)
+    # against the per-(element, node) slot gather: the same element forces,
+    # summed in FP64 in another order.  Meshes: 25.3 patches (the last one
+    # partial) and a single partial patch, fixed nodes and pressure faces;
+    # jittered boxes under the Mooney-Rivlin and Yeoh laws; gather lanes 1,
+    # 2 and 4; the element and pressure terms alone and together
+    meshes = [E.Mesh.workload(2, [30, 6, 6, 5, 300.0]), E.Mesh.workload(2, [7, 3, 2, 1, 300.0]),
+              small(4, 1, 0), small(4, 2, 0)]
+    for m in meshes:
+        print(f"mesh ndof={m.ndof} precision={prec}", flush=True)
+        u = smooth_u(m, rng, 0.05)
+        with E.Context(m, prec) as ctx:
+            ctx.set_option("residual_terms", 1)
+            f_int = ctx.residual(u)
+            bound = 1e-12 * np.abs(f_int).max()
+            for terms in (7, 1, 2):
+                ctx.set_option("residual_terms", terms)
+                ctx.set_option("tet4_patches", 0)
+                r_slots = ctx.residual(u)
+                ctx.set_option("tet4_patches", 1)
+                for lanes in (1, 2, 4):
+                    ctx.set_option("patch_gather_lanes", lanes)
+                    r_patch = ctx.residual(u)
+                    assert np.abs(r_patch - r_slots).max() <= bound, (terms, lanes)
+                # the same sums in the same order: bit for bit on a repeat
+                assert np.array_equal(ctx.residual(u), r_patch)
+
+
 def test_residual_full_beam_fp64_and_mixed(rng):
     m = E.Mesh.workload(2)
     om = O.Model(m)
