@@ -28,13 +28,63 @@ __device__ __forceinline__ int xcdBlockOf(int b, int g)
 }
 __device__ __forceinline__ int xcdBlock() { return xcdBlockOf(blockIdx.x, gridDim.x); }
 
-// wave64 butterfly sum (fixed order)
-__device__ __forceinline__ double waveSum(double v)
+// Cross-lane moves within rows of 16 lanes (DPP; every lane of the wave
+// must be active): CTRL 0xB1 quad_perm [1,0,3,2], 0x4E quad_perm
+// [2,3,0,1], 0x141 row_half_mirror, 0x140 row_mirror.
+template <int CTRL>
+__device__ __forceinline__ float dppMove(float v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dppMove(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float laneValue(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double laneValue(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// op over each row of 16 lanes, in every lane of the row (pairs, quads,
+// eights, sixteens; op commutative, so every lane of a row holds the same
+// value).  Every lane must be active.
+template <typename T, typename Op>
+__device__ __forceinline__ T rowReduce16(T v, Op op)
+{
+    v = op(v, dppMove<0xB1>(v));
+    v = op(v, dppMove<0x4E>(v));
+    v = op(v, dppMove<0x141>(v));
+    v = op(v, dppMove<0x140>(v));
     return v;
 }
+// op over the wave, in every lane: the rows, then the four row values in
+// order.  Every lane must be active.  (A shuffle butterfly costs six
+// dependent LDS-crossbar round trips per value.)
+template <typename T, typename Op>
+__device__ __forceinline__ T waveReduce(T v, Op op)
+{
+    v = rowReduce16(v, op);
+    return op(op(laneValue(v, 0), laneValue(v, 16)), op(laneValue(v, 32), laneValue(v, 48)));
+}
+struct OpAdd {
+    template <typename T>
+    __device__ T operator()(T a, T b) const { return a + b; }
+};
+template <typename T>
+__device__ __forceinline__ T waveSumT(T v) { return waveReduce(v, OpAdd{}); }
+template <typename T>
+__device__ __forceinline__ T rowSum16(T v) { return rowReduce16(v, OpAdd{}); }
+
+// wave64 sum (fixed order), every lane active
+__device__ __forceinline__ double waveSum(double v) { return waveSumT(v); }
 
 // max that propagates NaN (fmax drops it): a norm over a vector holding a
 // NaN is NaN, so the non-finite checks of the solve see it
@@ -43,12 +93,10 @@ __device__ __forceinline__ double maxNan(double a, double b)
     return (a != a) ? a : ((b != b) ? b : fmax(a, b));
 }
 
-__device__ __forceinline__ double waveMax(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = maxNan(v, __shfl_xor(v, off, 64));
-    return v;
-}
+struct OpMaxNan {
+    __device__ double operator()(double a, double b) const { return maxNan(a, b); }
+};
+__device__ __forceinline__ double waveMax(double v) { return waveReduce(v, OpMaxNan{}); }
 
 // H = Gu J0^-1 is formed in FP64 in both modes: J - 1 (and with it the
 // near-incompressible volumetric stress (2/D1)(J-1) cof F, 2/D1 = 2e6 here)
@@ -778,12 +826,9 @@ __device__ __forceinline__ void trsvBwdItem(const TrsvArgs& a, int4 d0, int4 d1,
             acc2 += bk * (sg * src[2]);
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc0 += __shfl_xor(acc0, off, 64);
-        acc1 += __shfl_xor(acc1, off, 64);
-        acc2 += __shfl_xor(acc2, off, 64);
-    }
+    acc0 = waveSumT(acc0);
+    acc1 = waveSumT(acc1);
+    acc2 = waveSumT(acc2);
     if (G == 1) {
         if (lane == 0 && k < w) {
             T* zo = z + 3 * static_cast<size_t>(j0 + k);
@@ -896,12 +941,9 @@ __device__ __forceinline__ void trsvBwdStageItem(const TrsvArgs& a, int4 d0, int
             acc2 += bk[i] * L.v[2][q];
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc0 += __shfl_xor(acc0, off, 64);
-        acc1 += __shfl_xor(acc1, off, 64);
-        acc2 += __shfl_xor(acc2, off, 64);
-    }
+    acc0 = waveSumT(acc0);
+    acc1 = waveSumT(acc1);
+    acc2 = waveSumT(acc2);
     if (lane == 0 && own) {
         T* zo = z + 3 * static_cast<size_t>(j0 + k);
         zo[0] = acc0;
@@ -970,12 +1012,9 @@ __device__ __forceinline__ void trsvBwdNarrowItem(const TrsvArgs& a, int4 d0, in
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
-        acc0 += __shfl_xor(acc0, off, 64);
-        acc1 += __shfl_xor(acc1, off, 64);
-        acc2 += __shfl_xor(acc2, off, 64);
-    }
+    acc0 = rowSum16(acc0);
+    acc1 = rowSum16(acc1);
+    acc2 = rowSum16(acc2);
     if (ph == 0 && k < w) {
         T* zo = z + 3 * static_cast<size_t>(j0 + k);
         zo[0] = acc0;

@@ -1416,12 +1416,9 @@ __global__ __launch_bounds__(64 * W) void kTrsvTop(TrsvArgs a, const double* __r
             }
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc0 += __shfl_xor(acc0, off, 64);
-        acc1 += __shfl_xor(acc1, off, 64);
-        acc2 += __shfl_xor(acc2, off, 64);
-    }
+    acc0 = waveSumT(acc0);
+    acc1 = waveSumT(acc1);
+    acc2 = waveSumT(acc2);
     if (lane == 0 && k < t.nT) {
         T* zo = static_cast<T*>(a.z) + 3 * static_cast<size_t>(t.tcol[k]);
         zo[0] = acc0;
