@@ -480,7 +480,8 @@ constexpr int kTermsAll = 7;
 // "residual_terms" of efem_residual)
 int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD = nullptr,
                      const double* dotCb = nullptr, const double* dotXt = nullptr, double* dotOut = nullptr,
-                     int terms = kTermsAll, const double* dotCarry = nullptr)
+                     int terms = kTermsAll, const double* dotCarry = nullptr,
+                     efem::dev::TrialFinish* defer = nullptr)
 {
     const efem::Model& m = c->model;
     const bool f32 = elem32(c) && !c->elemFp64;
@@ -534,6 +535,7 @@ int residualInternal(efem_ctx* c, const double* x, double* r, const double* dotD
         ga.dotWork = c->gdotWork;
         ga.dotOut = dotOut;
         ga.dotCarry = dotCarry;
+        ga.defer = defer;
     }
     efem::dev::launchGather(ga, c->stream);
     HIP_CHECK(c, hipGetLastError());
@@ -1791,8 +1793,29 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     // (kOut + 4).  Writes S/Y slot loc, the Gram row / column of loc, the
     // scalar slots and d2; reads x, xp, r, rp and the rest of the history.
     bool preformed = false;  // the next first trial's x already formed
+    // A first trial's finishing reduction (its scalars into the pinned
+    // slot) is deferred when the look-ahead may form the next direction
+    // right behind it: it then runs as one extra block of that direction's
+    // Gram / first-loop sweep, which records the trial's event; otherwise it
+    // is launched on its own before the host waits for the trial.  A
+    // speculative trial that is rolled back drops it.
+    efem::dev::TrialFinish pendingTF{};
+    int pendingSlot = 0;
+    auto flushPending = [&]() -> int {
+        if (pendingTF.k) {
+            pendingTF.ev = c->trialEv[pendingSlot];
+            efem::dev::launchTrialFinish(pendingTF, c->stream);
+            pendingTF = efem::dev::TrialFinish{};
+            HIP_CHECK(c, hipGetLastError());
+        }
+        return EFEM_OK;
+    };
     auto nextDirection = [&]() -> int {
         preformed = false;
+        if (!compact) {
+            int rcf = flushPending();
+            if (rcf) return rcf;
+        }
         const int loc = ptr % mc;
         double* sl = c->S + static_cast<size_t>(loc) * n;
         double* yl = c->Y + static_cast<size_t>(loc) * n;
@@ -1807,9 +1830,12 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             // writes the permuted right-hand side the sweeps read
             const bool permuted = h0 && !staged;
             double* q0 = staged ? c->fac.in : permuted ? c->fac.bp : c->d2;
+            efem::dev::TrialFinish ride = pendingTF;
+            ride.ev = ride.k ? c->trialEv[pendingSlot] : nullptr;
+            pendingTF = efem::dev::TrialFinish{};
             efem::dev::launchGramFirst(c->S, c->Y, c->x, c->xp, c->r, c->rp, n, ncorr, loc, mc, sc + kGram,
                                        sc + kYs, sc + kYY + loc, sc + kTheta, sc + kAlpha, sc + kRS, q0, c->work,
-                                       c->stream, permuted ? c->fac.pinv : nullptr);
+                                       c->stream, permuted ? c->fac.pinv : nullptr, ride.k ? &ride : nullptr);
             efem::dev::HSrc h{c->d2};
             if (staged) {
                 if ((rc2 = applyH0Graph(c))) return rc2;
@@ -1887,15 +1913,26 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
     // ||x - u_c||_inf, ||x - x_target||^2, slope r . d of the iteration}
     // copied to pinned slot `slot` behind an event
     auto enqueueTrial = [&](double step, int slot) -> int {
+        int rc2 = flushPending();
+        if (rc2) return rc2;
         if (!(preformed && step == 1.0)) efem::dev::launchStep(c->x, c->xp, c->d, step, n, c->stream);
         preformed = false;
         // the trial's scalars come with the residual's gather; without the
         // energy rule its finishing block writes them, and the slope, into
         // the pinned slot itself
         double* pinnedSlot = c->hostScalDev + kTrialSlot + 8 * slot;
-        int rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr,
-                                   energyLs ? sc + kOut : pinnedSlot, kTermsAll, energyLs ? nullptr : sc + kOut + 4);
+        const bool defer = !energyLs && c->lookahead && compact;
+        efem::dev::TrialFinish tf{};
+        rc2 = residualInternal(c, c->x, c->r, c->d, base, prm->x_target ? c->xtarget : nullptr,
+                               energyLs ? sc + kOut : pinnedSlot, kTermsAll, energyLs ? nullptr : sc + kOut + 4,
+                               defer ? &tf : nullptr);
         if (rc2) return rc2;
+        if (tf.k) {
+            // the event is recorded behind the launch that finishes it
+            pendingTF = tf;
+            pendingSlot = slot;
+            return EFEM_OK;
+        }
         if (energyLs) {
             // the potential's change from xp and its rounding scale into
             // slots 5, 6: per-element energy differences (compensated)
@@ -1950,6 +1987,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
             if ((rc = enqueueTrial(1.0, slot ^ 1))) return rc;
             spec = true;
         }
+        if (pendingTF.k && pendingSlot == slot && (rc = flushPending())) return rc;
         HIP_CHECK(c, hipEventSynchronize(c->trialEv[slot]));
         const double* hsc = c->hostScal + kTrialSlot + 8 * slot;
         const double dg = hsc[0];
@@ -1966,7 +2004,10 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
                             std::fabs(hsc[5]) <= kEnergyRtol * hsc[6];
         if (!(proj && armijo) && it + 1 < prm->max_linesearch) {
             // rejected: the next trial of this iteration
-            if (spec) restore(before);
+            if (spec) {
+                restore(before);
+                pendingTF = efem::dev::TrialFinish{};  // the speculative trial's
+            }
             cur *= !armijo ? 0.5 : (dg * dgInit > 0 ? 1.5 : 0.5);
             ++it;
             if ((rc = enqueueTrial(cur, slot))) return rc;
@@ -1997,6 +2038,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
                             rinf <= prm->fp64_switch * rinf0;
         if (spec && (stop || toFp64)) {
             restore(before);
+            pendingTF = efem::dev::TrialFinish{};  // the speculative trial's
             spec = false;
         }
         if (stop) break;
@@ -2018,6 +2060,7 @@ int efem_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_solve_
         rotate();
         if ((rc = enqueueTrial(1.0, slot))) return rc;
     }
+    pendingTF = efem::dev::TrialFinish{};
     HIP_CHECK(c, hipMemcpyAsync(u, c->x, nb, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->elemFp64 = false;

@@ -647,8 +647,7 @@ __device__ __forceinline__ void writePartials(double (&v)[K], double* work)
 // p = epi.prefetch() -- what the epilogue reads besides the sums, loaded
 // before the partials so that the two loads overlap
 template <int K, unsigned MM, typename Epi>
-__global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, Epi epi,
-                                                     int stride = kReduceBlocks)
+__device__ __forceinline__ void reduceFinishBody(const double* __restrict__ work, int nb, Epi epi, int stride)
 {
     const typename Epi::Pre pre = epi.prefetch();
     double v[K];
@@ -674,6 +673,13 @@ __global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ 
     }
     blockReduce<K, kRFW, MM>(v);
     epi(v, pre);
+}
+
+template <int K, unsigned MM, typename Epi>
+__global__ __launch_bounds__(kRF) void kReduceFinish(const double* __restrict__ work, int nb, Epi epi,
+                                                     int stride = kReduceBlocks)
+{
+    reduceFinishBody<K, MM, Epi>(work, nb, epi, stride);
 }
 
 struct NoPrefetch {
@@ -843,8 +849,18 @@ template <int NC, bool P2>
 __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double* __restrict__ Y,
                                                       const double* __restrict__ x, const double* __restrict__ xp,
                                                       const double* __restrict__ r, const double* __restrict__ rp,
-                                                      int n, int loc, int mcap, double* work)
+                                                      int n, int loc, int mcap, double* work, int g1,
+                                                      const double* __restrict__ tfWork, int tfNb, int tfK,
+                                                      double* tfOut, const double* tfCarry)
 {
+    // block g1 (when tfK > 0): a trial's finishing reduction, riding along
+    if (tfK && static_cast<int>(blockIdx.x) == g1) {
+        if (tfK == 4)
+            reduceFinishBody<4, 0x6u>(tfWork, tfNb, EpiStoreCarry<4>{{}, tfOut, tfCarry}, tfNb);
+        else
+            reduceFinishBody<3, 0x6u>(tfWork, tfNb, EpiStoreCarry<3>{{}, tfOut, tfCarry}, tfNb);
+        return;
+    }
     static_assert(NC >= 1 && NC <= kMaxCompact, "pair count");
     constexpr int N = 3 * NC + 1;
     double v[N];
@@ -863,7 +879,7 @@ __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double
     constexpr int NA = NC > 1 ? NC - 1 : 1;
     if constexpr (P2) {
         const int np = n >> 1;
-        for (int p = blockIdx.x * kRB + threadIdx.x; p < np; p += gridDim.x * kRB) {
+        for (int p = blockIdx.x * kRB + threadIdx.x; p < np; p += g1 * kRB) {
             const int i = 2 * p;
             const double2 r2 = ld2(r, i), x2 = ld2(x, i), xp2 = ld2(xp, i), rp2 = ld2(rp, i);
             double2 sk2[NA], yk2[NA];
@@ -889,7 +905,7 @@ __global__ __launch_bounds__(kRB) void kGramFirst(double* __restrict__ S, double
             *reinterpret_cast<double2*>(yl + i) = make_double2(y0, y1);
         }
     } else {
-        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += g1 * kRB) {
             const double ri = r[i], xi = x[i], xpi = xp[i], rpi = rp[i];
             double sk[NA], yk[NA];
 #pragma unroll
@@ -1693,6 +1709,13 @@ void launchGather(const GatherArgs& a, hipStream_t s)
     const int lpn = a.lanes ? a.lanes : tuning().gatherLanes;
     const int g = static_cast<int>((static_cast<long long>(lpn) * a.nFree + kThreads - 1) / kThreads);
     if (!a.dotWork) return gatherLaunch<0>(a, lpn, g, s);
+    if (a.defer && a.dotCarry) {
+        // the finishing reduction runs later (see TrialFinish)
+        if (a.dotXt) gatherLaunch<4>(a, lpn, g, s);
+        else gatherLaunch<3>(a, lpn, g, s);
+        *a.defer = TrialFinish{a.dotWork, g, a.dotXt ? 4 : 3, a.dotOut, a.dotCarry, nullptr};
+        return;
+    }
     if (a.dotXt) {
         gatherLaunch<4>(a, lpn, g, s);
         if (a.dotCarry)
@@ -1706,6 +1729,15 @@ void launchGather(const GatherArgs& a, hipStream_t s)
         else
             kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(a.dotWork, g, EpiStore<3>{{}, a.dotOut}, g);
     }
+}
+
+void launchTrialFinish(const TrialFinish& tf, hipStream_t s)
+{
+    if (tf.k == 4)
+        kReduceFinish<4, 0x6u><<<1, kRF, 0, s>>>(tf.work, tf.nb, EpiStoreCarry<4>{{}, tf.out, tf.carry}, tf.nb);
+    else if (tf.k == 3)
+        kReduceFinish<3, 0x6u><<<1, kRF, 0, s>>>(tf.work, tf.nb, EpiStoreCarry<3>{{}, tf.out, tf.carry}, tf.nb);
+    if (tf.k && tf.ev) (void)hipEventRecord(tf.ev, s);
 }
 
 void launchDotMax(const double* a, const double* b, const double* c, int n, double* work, double* out,
@@ -1793,15 +1825,23 @@ bool pairedStreams(int n, std::initializer_list<const void*> ptrs)
 
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
-                     double* alpha, double* rs, double* out, double* work, hipStream_t s, const int* pinv)
+                     double* alpha, double* rs, double* out, double* work, hipStream_t s, const int* pinv,
+                     const TrialFinish* tf)
 {
     withPairCount(nc, [&](auto ncc) {
         constexpr int NC = decltype(ncc)::value;
         const int g = historyGrid(n);
+        const bool ride = tf && tf->k;
+        const int gl = g + (ride ? 1 : 0);
+        const double* tw = ride ? tf->work : nullptr;
+        const int tnb = ride ? tf->nb : 0, tk = ride ? tf->k : 0;
+        double* to = ride ? tf->out : nullptr;
+        const double* tc = ride ? tf->carry : nullptr;
         if (pairedStreams(n, {S, Y, x, xp, r, rp}))
-            kGramFirst<NC, true><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
+            kGramFirst<NC, true><<<gl, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work, g, tw, tnb, tk, to, tc);
         else
-            kGramFirst<NC, false><<<g, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work);
+            kGramFirst<NC, false><<<gl, kRB, 0, s>>>(S, Y, x, xp, r, rp, n, loc, mcap, work, g, tw, tnb, tk, to, tc);
+        if (ride && tf->ev) (void)hipEventRecord(tf->ev, s);
         if (mcap <= 8) {
             if constexpr (NC <= 8)
                 kReduceFinish<3 * NC + 1, 0u><<<1, kRF, 0, s>>>(

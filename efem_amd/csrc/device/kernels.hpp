@@ -125,7 +125,24 @@ struct GatherArgs {
     // optional: dotOut[4] = *dotCarry as well (dotOut then pinned host
     // memory: the host reads the trial's scalars without a copy)
     const double* dotCarry = nullptr;
+    // optional (with dotCarry): the finishing reduction is not launched but
+    // described here, to run as one extra block of the next history sweep
+    // (launchGramFirst) or on its own (launchTrialFinish)
+    struct TrialFinish* defer = nullptr;
 };
+
+// A trial's finishing reduction of the gather's line-search partials (k = 3
+// or 4 components at work[c nb + b]) into pinned memory out[0..k), with
+// out[4] = *carry; ev (may be null) is recorded behind the launch that runs
+// it.  k = 0: nothing pending.
+struct TrialFinish {
+    const double* work = nullptr;
+    int nb = 0, k = 0;
+    double* out = nullptr;
+    const double* carry = nullptr;
+    hipEvent_t ev = nullptr;
+};
+void launchTrialFinish(const TrialFinish& tf, hipStream_t s);
 void launchGather(const GatherArgs& a, hipStream_t s);
 // doubles of dotWork a gather over nFree nodes needs (any lanes per node)
 size_t gatherDotWork(int nFree);
@@ -236,7 +253,7 @@ void launchTheta(const double* ys, const double* yy, double* theta, hipStream_t 
 void launchGramFirst(double* S, double* Y, const double* x, const double* xp, const double* r, const double* rp,
                      int n, int nc, int loc, int mcap, double* G, double* ys, double* yy, double* theta,
                      double* alpha, double* rs, double* out, double* work, hipStream_t s,
-                     const int* pinv = nullptr);
+                     const int* pinv = nullptr, const TrialFinish* tf = nullptr);
 
 // ---- supernodal triangular solves (3 right-hand sides) ---------------------
 struct TrsvArgs {
