@@ -750,7 +750,7 @@ def main():
     e_us = 1e3 * ems / max(ek, 1)
     t_roof = max(B / (HBM_PEAK_GBS * 1e9), F / (FP32_PEAK_TFLOPS * 1e12))
     out["roofline_element"] = {
-        "kernel": "kElemTet4<float, MR>",
+        "kernel": "kElemTet4Patch<float, MR> (forces summed per patch in LDS, with the pressure faces)",
         "bound": "hbm" if B / HBM_PEAK_GBS > F / (FP32_PEAK_TFLOPS * 1e3) else "valu",
         "achieved_GBs": round(B / (e_us * 1e-6) / 1e9, 1),
         "achieved_TFLOPs": round(F / (e_us * 1e-6) / 1e12, 2),
