@@ -38,6 +38,11 @@ struct DeviceFactor {
     int4* items = nullptr;  // descriptor pairs
     int* instItems = nullptr;  // batches: [instance][level] {fwd off, cnt, bwd off, cnt}
     int instLevels = 0;
+    // batches: the dense Z_i of every instance (see BatchArgs::zArena), or null
+    void* batchZ = nullptr;
+    long long* batchZoff = nullptr;
+    int* batchZld = nullptr;
+    int* batchZtcol = nullptr;
     std::vector<int> fwdOff, fwdCnt, bwdOff, bwdCnt;  // per level, in items
     std::vector<int> fwdWide, bwdWide;               // per level: 16-wave items
     // per level: forward extend-add table (K = 0: none) at fwdTab + fwdTabOff
@@ -169,6 +174,11 @@ struct efem_ctx {
     // sets the gather's lanes per node on that path
     bool tet4Patches = true;
     int patchLanes = 2;
+    // option "batch_dense_h0": batches apply each instance's preconditioner
+    // as one product with its dense Z_i = (L_i L_i^T)^-1 (instances of at
+    // most kBatchDenseMaxN free nodes), formed on the device at each
+    // factorisation; 0: the level-scheduled sweeps per instance
+    bool batchDense = true;
     efem::dev::PatchArgs patch;  // device arrays (patch.part null: no plan)
     int *pRptr = nullptr, *pFptr = nullptr, *pFslot = nullptr;
     int pRec = 0;
@@ -1155,6 +1165,103 @@ void factorStructure(const efem::SupernodalSymbolic& S, efem::SupernodalFactor& 
     F.flops = S.flops;
 }
 
+// Batches: Z_i of every instance over its whole elimination tree, by the
+// fused top's selected inversion (T = all supernodes of the instance) from
+// the FP64 block inverses of the merged factor F (host layout, uploaded for
+// the formation), into one arena; FP32 copy for the FP32 preconditioner.
+// Leaves the level path in place (no dense Z) when an instance is larger
+// than kBatchDenseMaxN free nodes.
+int formBatchZ(efem_ctx* c, const std::vector<efem::SupernodalFactor>& P, const efem::SupernodalFactor& F,
+               const std::vector<int>& snOffset)
+{
+    DeviceFactor& d = c->fac;
+    const int np = static_cast<int>(P.size());
+    std::vector<efem::TopPlan> plans(np);
+    for (int i = 0; i < np; ++i) {
+        if (P[i].n > efem::dev::kBatchDenseMaxN) return EFEM_OK;
+        if (!efem::planTop(P[i], 0, P[i].n, efem::dev::kTopMaxK, efem::dev::kLdAlign, plans[i]) ||
+            plans[i].nT != P[i].n)
+            return EFEM_OK;
+    }
+    AllocInto ai(c, &d.allocs);
+    std::vector<long long> zoff(np);
+    std::vector<int> zld(np), ztcol(static_cast<size_t>(F.n));
+    std::vector<efem::dev::TopSuper> sn;
+    std::vector<int> rtl, snBase(np), rtlBase(np), taskBase(np);
+    std::vector<int4> tasks;
+    long long tot = 0;
+    for (int i = 0; i < np; ++i) {
+        const efem::TopPlan& T = plans[i];
+        const int col = F.first[snOffset[i]];  // the instance's first merged column
+        zoff[i] = tot;
+        zld[i] = T.ld;
+        tot += (static_cast<long long>(T.ld) * T.nT + 63) / 64 * 64;
+        for (int t = 0; t < T.nT; ++t) ztcol[static_cast<size_t>(col) + t] = col + T.tcol[t];
+        snBase[i] = static_cast<int>(sn.size());
+        rtlBase[i] = static_cast<int>(rtl.size());
+        taskBase[i] = static_cast<int>(tasks.size());
+        for (const efem::TopSuperInfo& t : T.super) {
+            efem::dev::TopSuper o;
+            o.cs = t.cs;
+            o.w = t.w;
+            o.m = t.m;
+            o.ld = t.m;
+            o.boff = F.valPtr[snOffset[i] + t.s];
+            o.rOff = t.rOff;
+            o.pad = 0;
+            sn.push_back(o);
+        }
+        rtl.insert(rtl.end(), T.rtl.begin(), T.rtl.end());
+        for (size_t q = 0; q + 3 < T.tasks.size(); q += 4)
+            tasks.push_back(make_int4(T.tasks[q], T.tasks[q + 1], T.tasks[q + 2], T.tasks[q + 3]));
+    }
+    int rc;
+    double *b64 = nullptr, *z64 = nullptr;
+    efem::dev::TopSuper* snd = nullptr;
+    int *rtld = nullptr, *zldd = nullptr, *ztc = nullptr;
+    int4* td = nullptr;
+    long long* zo = nullptr;
+    if ((rc = devAlloc(c, &b64, static_cast<size_t>(std::max(F.Bsize, 1LL)))) || (rc = upload(c, &snd, sn)) ||
+        (rc = upload(c, &rtld, rtl)) || (rc = upload(c, &td, tasks)) || (rc = devAlloc(c, &z64, static_cast<size_t>(tot))) ||
+        (rc = upload(c, &zo, zoff)) || (rc = upload(c, &zldd, zld)) || (rc = upload(c, &ztc, ztcol)))
+        return rc;
+    HIP_CHECK(c, hipMemcpyAsync(b64, F.B.get(), static_cast<size_t>(F.Bsize) * sizeof(double), hipMemcpyHostToDevice,
+                                c->stream));
+    // padding rows stay zero (the dense apply reads whole 16-byte groups)
+    HIP_CHECK(c, hipMemsetAsync(z64, 0, static_cast<size_t>(tot) * sizeof(double), c->stream));
+    for (int i = 0; i < np; ++i) {
+        const efem::TopPlan& T = plans[i];
+        efem::dev::TopBuild b;
+        b.Z = z64 + zoff[i];
+        b.nT = T.nT;
+        b.ldz = T.ld;
+        b.B = b64;
+        b.sn = snd + snBase[i];
+        b.rtl = rtld + rtlBase[i];
+        b.tasks = td + taskBase[i];
+        for (const efem::TopPlan::Step& st : T.steps) {
+            if (st.kind == 0) efem::dev::launchTopGemm(b, st.first, st.count, c->stream);
+            else if (st.kind == 1) efem::dev::launchTopMirror(b, st.p0, st.p1, c->stream);
+            else efem::dev::launchTopDiag(b, st.first, st.count, c->stream);
+        }
+    }
+    HIP_CHECK(c, hipGetLastError());
+    if (fac32(c)) {
+        float* z32 = nullptr;
+        if ((rc = devAlloc(c, &z32, static_cast<size_t>(tot)))) return rc;
+        efem::dev::launchFactorToF32(z64, z32, tot, c->stream);
+        HIP_CHECK(c, hipGetLastError());
+        d.batchZ = z32;
+    } else {
+        d.batchZ = z64;
+    }
+    d.batchZoff = zo;
+    d.batchZld = zldd;
+    d.batchZtcol = ztc;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return EFEM_OK;
+}
+
 int factorInternal(efem_ctx* c, int* failed)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -1166,11 +1273,11 @@ int factorInternal(efem_ctx* c, int* failed)
     efem::SupernodalFactor F;
     std::string msg;
     std::vector<int> snOffset;
+    std::vector<efem::SupernodalFactor> Pparts;  // batches: per-instance structure (blocks merged into F)
     bool ok;
     if (!c->parts.empty()) {
-        std::vector<efem::SupernodalFactor> P;
-        ok = efem::factorModels(c->parts, P, msg, c->denseSubtree);
-        if (ok) efem::mergeFactors(P, F, snOffset);
+        ok = efem::factorModels(c->parts, Pparts, msg, c->denseSubtree);
+        if (ok) efem::mergeFactors(Pparts, F, snOffset);
     } else {
         // element codes e nen^2 + a nen + b must fit the map's ints
         const bool dev = c->devLaplacian && static_cast<long long>(m.nElems) * m.nen * m.nen < (1LL << 31);
@@ -1234,6 +1341,7 @@ int factorInternal(efem_ctx* c, int* failed)
     c->factorPhases[1] = F.seconds - F.numericSeconds;
     c->factorPhases[2] = F.numericSeconds;
     int rc = uploadFactor(c, F, c->parts.empty() ? nullptr : &snOffset);
+    if (rc == EFEM_OK && !c->parts.empty() && c->batchDense) rc = formBatchZ(c, Pparts, F, snOffset);
     if (c->parts.empty()) c->hostWs->giveBlocks(std::move(F.B), static_cast<size_t>(F.Bsize));
     if (rc) return rc;
     c->factorFailed = false;
@@ -2299,6 +2407,12 @@ int efem_set_option(efem_ctx* c, const char* name, double value)
         c->tet4Patches = value != 0.0;
         return EFEM_OK;
     }
+    if (k == "batch_dense_h0") {
+        // takes effect at the next factorisation
+        c->batchDense = value != 0.0;
+        releaseFactor(c);
+        return EFEM_OK;
+    }
     if (k == "patch_gather_lanes" && (iv == 1 || iv == 2 || iv == 4 || iv == 8)) {
         c->patchLanes = iv;
         return EFEM_OK;
@@ -2774,6 +2888,12 @@ int efem_batch_solve(efem_ctx* c, double* u, const efem_lbfgs_params* prm, efem_
     a.items = h0 ? c->fac.items : nullptr;
     a.instItems = h0 ? c->fac.instItems : nullptr;
     a.nlev = h0 ? c->fac.instLevels : 0;
+    if (h0 && c->batchDense && c->fac.batchZ) {
+        a.zArena = c->fac.batchZ;
+        a.zoff = c->fac.batchZoff;
+        a.zld = c->fac.batchZld;
+        a.ztcol = c->fac.batchZtcol;
+    }
     a.ranges = c->dRanges;
     a.x = c->x;
     a.r = c->r;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include <type_traits>
 #include "kernels.hpp"
 
 // This file is compiled once per workgroup width (256, 512 and 1024 lanes,
@@ -54,6 +55,7 @@ union BatchLds {
     ElemTileLds<float, NEN, NG, MODE_FORCE, BT> ef;
     ElemTileLds<double, NEN, NG, MODE_FORCE, BT> ed;
     TrsvFwdLds<TF, BW> tf;
+    TF zf[kBatchDenseMaxN * 3];  // the dense apply's right-hand side
 };
 
 template <typename T, int NEN, int NG, int LAW>
@@ -139,6 +141,78 @@ __device__ void blockH0(const BatchArgs& a, int inst, const BatchRange& I, const
     __syncthreads();
 }
 
+// out = K^-1 in on the instance through its dense Z_i: f_t = in at the
+// instance's factor column t (staged in LDS), then z_k = sum_t Z_i[k, t] f_t
+// (Z_i symmetric: row k is read as column k's transpose, so a wave's loads
+// of one t are one contiguous segment).  Each lane owns RPT consecutive
+// rows (one 16-byte load per t) and a wave 64 RPT rows; waves take the row
+// groups in turn; the sum over t runs in t order.
+template <typename TF>
+__device__ __noinline__ void blockH0Dense(const void* zArena, long long zoff, int ld, const int* __restrict__ tc,
+                                             const int* __restrict__ perm, int n, const double* in, double* out,
+                                             TF* f)
+{
+    constexpr int RPT = 16 / sizeof(TF);
+    using V = typename std::conditional<sizeof(TF) == 4, float4, double2>::type;
+    const TF* Z = static_cast<const TF*>(zArena) + zoff;
+    for (int t = threadIdx.x; t < n; t += BT) {
+        const int o = perm[tc[t]];
+        f[3 * t] = static_cast<TF>(in[3 * o]);
+        f[3 * t + 1] = static_cast<TF>(in[3 * o + 1]);
+        f[3 * t + 2] = static_cast<TF>(in[3 * o + 2]);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int UN = 16;  // loads in flight per lane
+    for (int k0 = wave * 64 * RPT; k0 < n; k0 += BW * 64 * RPT) {
+        const int k = k0 + lane * RPT;  // this lane's first row (< ld: rows n .. ld - 1 are zero)
+        TF acc[RPT][3];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) acc[q][0] = acc[q][1] = acc[q][2] = TF(0);
+        const bool on = k < n;
+        const TF* zk = Z + k;
+        int t = 0;
+        for (; t + UN <= n; t += UN) {
+            V zv[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u)
+                zv[u] = on ? *reinterpret_cast<const V*>(zk + static_cast<size_t>(t + u) * ld) : V{};
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const TF* zz = reinterpret_cast<const TF*>(&zv[u]);
+                const TF f0 = f[3 * (t + u)], f1 = f[3 * (t + u) + 1], f2 = f[3 * (t + u) + 2];
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    acc[q][0] += zz[q] * f0;
+                    acc[q][1] += zz[q] * f1;
+                    acc[q][2] += zz[q] * f2;
+                }
+            }
+        }
+        for (; t < n; ++t) {
+            const V zv = on ? *reinterpret_cast<const V*>(zk + static_cast<size_t>(t) * ld) : V{};
+            const TF* zz = reinterpret_cast<const TF*>(&zv);
+            const TF f0 = f[3 * t], f1 = f[3 * t + 1], f2 = f[3 * t + 2];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                acc[q][0] += zz[q] * f0;
+                acc[q][1] += zz[q] * f1;
+                acc[q][2] += zz[q] * f2;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (k + q < n) {
+                const int o = perm[tc[k + q]];
+                out[3 * o] = static_cast<double>(acc[q][0]);
+                out[3 * o + 1] = static_cast<double>(acc[q][1]);
+                out[3 * o + 2] = static_cast<double>(acc[q][2]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <int NEN, int NG, typename TF>
 __device__ __forceinline__ void residualAt(const BatchArgs& a, const BatchRange& I, bool f32,
                                            BatchLds<NEN, NG, TF>& U)
@@ -198,7 +272,10 @@ __global__ __launch_bounds__(BT) void kBatchLbfgs(BatchArgs a)
         const bool h0 = a.usePrecond != 0;
         // d = -H0 r
         if (h0) {
-            blockH0<TF, NEN, NG>(a, inst, I, r, d, U, bred);
+            if (a.zArena)
+                blockH0Dense<TF>(a.zArena, a.zoff[inst], a.zld[inst], a.ztcol + I.n0, a.trsv.perm, I.n1 - I.n0, r, d,
+                                 U.zf);
+            else blockH0<TF, NEN, NG>(a, inst, I, r, d, U, bred);
             for (int i = d0 + tid; i < d1; i += BT) d[i] = -d[i];
         } else {
             for (int i = d0 + tid; i < d1; i += BT) d[i] = -r[i];
@@ -304,7 +381,10 @@ __global__ __launch_bounds__(BT) void kBatchLbfgs(BatchArgs a)
                 __syncthreads();
             }
             if (h0) {
-                blockH0<TF, NEN, NG>(a, inst, I, d, q, U, bred);
+                if (a.zArena)
+                    blockH0Dense<TF>(a.zArena, a.zoff[inst], a.zld[inst], a.ztcol + I.n0, a.trsv.perm, I.n1 - I.n0, d,
+                                     q, U.zf);
+                else blockH0<TF, NEN, NG>(a, inst, I, d, q, U, bred);
                 for (int i = d0 + tid; i < d1; i += BT) d[i] = q[i];
             } else {
                 for (int i = d0 + tid; i < d1; i += BT) d[i] = d[i] / theta;

@@ -388,7 +388,17 @@ struct BatchArgs {
     const double* xt;            // merged displacement targets, or null
     double targetTol;            // stop an instance at ||x - xt|| <= tol ||xt||
     double* stats;               // [instance][kBatchStats]
+    // dense preconditioner (zArena non-null): per instance Z_i = (L_i
+    // L_i^T)^-1 over its whole tree (TF, column-major, ld zld[i], at
+    // zoff[i]); ztcol[n0 + t] = the merged factor column of the instance's
+    // local index t (n_i = n1 - n0 <= kBatchDenseMaxN)
+    const void* zArena = nullptr;
+    const long long* zoff = nullptr;
+    const int* zld = nullptr;
+    const int* ztcol = nullptr;
 };
+// instances up to this many free nodes take the dense preconditioner
+constexpr int kBatchDenseMaxN = 1024;
 // stats per instance: iterations, r_evals, ls_trials, ||r||_inf, ||x||_inf,
 // non-finite flag, FP64-switch iteration, preconditioned flag, target
 // reached, last ||x - xt|| / ||xt||

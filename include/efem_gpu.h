@@ -300,6 +300,12 @@ const char* efem_host_dense_kernel(void);
  *                       sums in another fixed order
  *   "patch_gather_lanes" 1, 2, 4, 8 (default 2): lanes per node of that
  *                       gather
+ *   "batch_dense_h0"    0 / 1 (default 1; batch contexts): the persistent
+ *                       solver applies each instance's preconditioner as one
+ *                       product with its dense Z_i = (L_i L_i^T)^-1 (formed
+ *                       on the device at each factorisation; instances of at
+ *                       most 1024 free nodes) instead of the level sweeps;
+ *                       releases the factor (the next solve factors again)
  *   "gather_lanes", "tet4_threads", "reduce_blocks", "history_blocks":
  *                       launch shapes (see efem_autotune) */
 int efem_set_option(efem_ctx* ctx, const char* name, double value);

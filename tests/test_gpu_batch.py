@@ -117,6 +117,34 @@ def test_batch_mixed_reaches_fp64_solution():
         assert rel(a, c) <= 1e-6
 
 
+@pytest.mark.parametrize("prec", [E.FP64, E.MIXED])
+def test_batch_dense_preconditioner_matches_level_sweeps(prec):
+    # the dense per-instance Z_i = (L_i L_i^T)^-1 (option batch_dense_h0, the
+    # default) against the level-scheduled sweeps: the same K_i^-1 up to
+    # rounding, so both solves reach the same converged displacements with
+    # comparable iteration counts; full-length beams (675 to 725 free nodes,
+    # loads x 1/64 as in the bench); an instance's result does not depend on
+    # the rest of the batch
+    ms = [E.Mesh.workload(5, [i, 20240531, 1.0 / 64]) for i in (0, 5, 9)]
+    out = {}
+    for dense in (1, 0):
+        with E.Batch(ms, prec) as b:
+            b.set_option("batch_dense_h0", dense)
+            u, st = b.solve(epsilon=1e-9, max_iterations=8000, fp64_switch=1e-3 if prec == E.MIXED else 0.0)
+            out[dense] = (b.split(u), st)
+            if dense:
+                with E.Batch(ms[1:2], prec) as one:
+                    one.set_option("batch_dense_h0", 1)
+                    u1, _ = one.solve(epsilon=1e-9, max_iterations=8000,
+                                      fp64_switch=1e-3 if prec == E.MIXED else 0.0)
+                assert np.array_equal(u1, out[1][0][1]), "dense path not batch-independent"
+    for i in range(len(ms)):
+        assert out[1][1][i]["rinf"] <= 1e-9 and out[0][1][i]["rinf"] <= 1e-9, i
+        assert rel(out[1][0][i], out[0][0][i]) <= 1e-6, i
+        it1, it0 = out[1][1][i]["iterations"], out[0][1][i]["iterations"]
+        assert abs(it1 - it0) <= 0.3 * max(it0, 1), (i, it1, it0)
+
+
 def test_batch_rejects_bad_input():
     ms = beams(range(2))
     with E.Batch(ms) as b:
