@@ -223,11 +223,168 @@ __device__ __forceinline__ void residualAt(const BatchArgs& a, const BatchRange&
         blockResidual<double, NEN, NG, TF>(a, I, U);
 }
 
+// histories of at most kBC pairs take the compact two-loop (one reduction
+// for the new pair's Gram row / column and the first loop's dots, one for
+// the second loop's dots); longer ones the reference's order
+constexpr int kBC = 8;
+
+// The compact two-loop of one instance (the reference's LBFGSMat
+// add_correction + apply_Hv recursion regrouped; pairs by age t: slot
+// (loc - t) mod m, age 0 the new pair).  First half: the new pair s = x -
+// xp, y = r - rp is stored in slot loc and, in the same pass, s.y, y.y, its
+// Gram row / column s_loc.y_k, s_k.y_loc and s_k.r of every pair are
+// summed; then the first loop newest -> oldest (q_0 = -r): alpha_t =
+// (s_t.q_0 - sum_{u newer} alpha_u s_t.y_u) / ys_t, and q = -r - sum_t
+// alpha_t y_t into qOut.  gram (kBC x kBC, s_i.y_k by slot), ys and alpha
+// live in LDS for the whole solve.  Returns theta = y.y / s.y.  Called by
+// every lane of the block (contains barriers).  Its own function (with
+// plain arguments): inlined into the persistent kernel its arrays made
+// the kernel spill.
+__device__ __noinline__ double compactFirst(const double* __restrict__ x, const double* __restrict__ xp,
+                                            const double* __restrict__ r, const double* __restrict__ rp, double* S,
+                                            double* Y, size_t nt, int m, int loc, int ncn, int d0, int d1,
+                                            double* ys, double* alpha, double* gramp, double* redp,
+                                            double* __restrict__ qOut)
+{
+    double(*gram)[kBC] = reinterpret_cast<double(*)[kBC]>(gramp);
+    double(*red)[BW] = reinterpret_cast<double(*)[BW]>(redp);
+    const int tid = threadIdx.x;
+    int slot[kBC];
+#pragma unroll
+    for (int t = 0; t < kBC; ++t) slot[t] = (loc - t + m) % m;
+    double* sl = S + loc * nt;
+    double* yl = Y + loc * nt;
+    constexpr int K = 3 * kBC + 2;
+    double v[K];
+    int op[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        v[c] = 0.0;
+        op[c] = 0;
+    }
+    for (int i = d0 + tid; i < d1; i += BT) {
+        const double ri = r[i];
+        const double si = x[i] - xp[i], yi = ri - rp[i];
+        double sk[kBC], yk[kBC];
+#pragma unroll
+        for (int t = 1; t < kBC; ++t) {
+            sk[t] = t < ncn ? S[slot[t] * nt + i] : 0.0;
+            yk[t] = t < ncn ? Y[slot[t] * nt + i] : 0.0;
+        }
+        sl[i] = si;
+        yl[i] = yi;
+        v[0] += si * yi;
+        v[1] += yi * yi;
+        v[2 + 2 * kBC] += si * ri;
+#pragma unroll
+        for (int t = 1; t < kBC; ++t) {
+            v[2 + t] += si * yk[t];
+            v[2 + kBC + t] += sk[t] * yi;
+            v[2 + 2 * kBC + t] += sk[t] * ri;
+        }
+    }
+    blockReduceK<K>(v, op, red);
+    if (tid == 0) {
+        ys[loc] = v[0];
+        gram[loc][loc] = v[0];
+        for (int t = 1; t < ncn; ++t) {
+            gram[loc][slot[t]] = v[2 + t];
+            gram[slot[t]][loc] = v[2 + kBC + t];
+        }
+    }
+    __syncthreads();
+    double al[kBC];
+#pragma unroll
+    for (int t = 0; t < kBC; ++t) {
+        al[t] = 0.0;
+        if (t < ncn) {
+            double acc = -v[2 + 2 * kBC + t];
+#pragma unroll
+            for (int u = 0; u < t; ++u) acc -= al[u] * gram[slot[t]][slot[u]];
+            al[t] = acc / ys[slot[t]];
+        }
+    }
+    if (tid == 0) {
+#pragma unroll
+        for (int t = 0; t < kBC; ++t)
+            if (t < ncn) alpha[slot[t]] = al[t];
+    }
+    for (int i = d0 + tid; i < d1; i += BT) {
+        double yv[kBC];
+#pragma unroll
+        for (int t = 0; t < kBC; ++t) yv[t] = t < ncn ? Y[slot[t] * nt + i] : 0.0;
+        double qi = -r[i];
+#pragma unroll
+        for (int t = 0; t < kBC; ++t)
+            if (t < ncn) qi -= al[t] * yv[t];
+        qOut[i] = qi;
+    }
+    __syncthreads();
+    return v[1] / v[0];
+}
+
+// Second half of the compact two-loop: with h = H0 q, the dots y_t.h in one
+// pass, then oldest -> newest beta_t = (y_t.h + sum_{u older} coef_u
+// y_t.s_u) / ys_t, coef_t = alpha_t - beta_t, and d = h + sum_t coef_t s_t.
+__device__ __noinline__ void compactSecond(const double* S, const double* Y, size_t nt, int m, int loc, int ncn,
+                                           int d0, int d1, const double* ys, const double* alpha,
+                                           const double* gramp, double* redp, const double* __restrict__ h,
+                                           double* __restrict__ dOut)
+{
+    const double(*gram)[kBC] = reinterpret_cast<const double(*)[kBC]>(gramp);
+    double(*red)[BW] = reinterpret_cast<double(*)[BW]>(redp);
+    const int tid = threadIdx.x;
+    int slot[kBC];
+#pragma unroll
+    for (int t = 0; t < kBC; ++t) slot[t] = (loc - t + m) % m;
+    double w[kBC];
+    int op[kBC];
+#pragma unroll
+    for (int c = 0; c < kBC; ++c) {
+        w[c] = 0.0;
+        op[c] = 0;
+    }
+    for (int i = d0 + tid; i < d1; i += BT) {
+        const double hi = h[i];
+        double yv[kBC];
+#pragma unroll
+        for (int t = 0; t < kBC; ++t) yv[t] = t < ncn ? Y[slot[t] * nt + i] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kBC; ++t) w[t] += yv[t] * hi;
+    }
+    blockReduceK<kBC>(w, op, red);
+    double cf[kBC];
+#pragma unroll
+    for (int t = kBC - 1; t >= 0; --t) {
+        cf[t] = 0.0;
+        if (t < ncn) {
+            double acc = w[t];
+#pragma unroll
+            for (int u = t + 1; u < kBC; ++u)
+                if (u < ncn) acc += cf[u] * gram[slot[u]][slot[t]];
+            cf[t] = alpha[slot[t]] - acc / ys[slot[t]];
+        }
+    }
+    for (int i = d0 + tid; i < d1; i += BT) {
+        double sv[kBC];
+#pragma unroll
+        for (int t = 0; t < kBC; ++t) sv[t] = t < ncn ? S[slot[t] * nt + i] : 0.0;
+        double di = h[i];
+#pragma unroll
+        for (int t = kBC - 1; t >= 0; --t)
+            if (t < ncn) di += cf[t] * sv[t];
+        dOut[i] = di;
+    }
+    __syncthreads();
+}
+
 template <int NEN, int NG, typename TF>
 __global__ __launch_bounds__(BT) void kBatchLbfgs(BatchArgs a)
 {
     __shared__ BatchLds<NEN, NG, TF> U;
-    __shared__ double red[4][BW];
+    // the compact two-loop's reductions carry up to 3 kBC + 2 components
+    __shared__ double red[3 * kBC + 2][BW];
+    __shared__ double gram[kBC][kBC];  // s_i . y_k of the stored pairs (compact two-loop)
     __shared__ TF bred[BW][3];
     __shared__ double ys[kMaxCompact], alpha[kMaxCompact];
     const int inst = blockIdx.x;
@@ -347,6 +504,26 @@ __global__ __launch_bounds__(BT) void kBatchLbfgs(BatchArgs a)
             const int loc = ptr % m;
             double* sl = a.S + loc * nt;
             double* yl = a.Y + loc * nt;
+            if (m <= kBC) {
+                // compact two-loop (the reference's recursion regrouped)
+                const int ncn = ncorr < m ? ncorr + 1 : m;
+                theta = compactFirst(x, a.xp, r, a.rp, a.S, a.Y, nt, m, loc, ncn, d0, d1, ys, alpha, &gram[0][0],
+                                     &red[0][0], d);
+                ncorr = ncn;
+                ptr = loc + 1;
+                if (h0) {
+                    if (a.zArena)
+                        blockH0Dense<TF>(a.zArena, a.zoff[inst], a.zld[inst], a.ztcol + I.n0, a.trsv.perm,
+                                         I.n1 - I.n0, d, q, U.zf);
+                    else blockH0<TF, NEN, NG>(a, inst, I, d, q, U, bred);
+                } else {
+                    for (int i = d0 + tid; i < d1; i += BT) q[i] = d[i] / theta;
+                    __syncthreads();
+                }
+                compactSecond(a.S, a.Y, nt, m, loc, ncn, d0, d1, ys, alpha, &gram[0][0], &red[0][0], q, d);
+                continue;
+            }
+            // reference order (histories longer than kBC)
             {
                 double v[2] = {0.0, 0.0};
                 const int op[2] = {0, 0};
