@@ -179,6 +179,10 @@ struct efem_ctx {
     // most kBatchDenseMaxN free nodes), formed on the device at each
     // factorisation; 0: the level-scheduled sweeps per instance
     bool batchDense = true;
+    // side streams (default priority) and events of the batch Z formation:
+    // the instances' formations are independent, so they run side by side
+    std::vector<hipStream_t> sideStreams;
+    std::vector<hipEvent_t> sideEvents;  // [0]: inputs ready, then one per side stream
     efem::dev::PatchArgs patch;  // device arrays (patch.part null: no plan)
     int *pRptr = nullptr, *pFptr = nullptr, *pFslot = nullptr;
     int pRec = 0;
@@ -1229,7 +1233,25 @@ int formBatchZ(efem_ctx* c, const std::vector<efem::SupernodalFactor>& P, const 
                                 c->stream));
     // padding rows stay zero (the dense apply reads whole 16-byte groups)
     HIP_CHECK(c, hipMemsetAsync(z64, 0, static_cast<size_t>(tot) * sizeof(double), c->stream));
+    // instance i forms on side stream i mod kSide, after the inputs above;
+    // the context stream waits for all of them
+    constexpr int kSide = 8;
+    if (c->sideStreams.empty()) {
+        for (int q = 0; q < kSide; ++q) {
+            hipStream_t st = nullptr;
+            HIP_CHECK(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            c->sideStreams.push_back(st);
+        }
+        for (int q = 0; q <= kSide; ++q) {
+            hipEvent_t e = nullptr;
+            HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c->sideEvents.push_back(e);
+        }
+    }
+    HIP_CHECK(c, hipEventRecord(c->sideEvents[0], c->stream));
+    for (int q = 0; q < kSide; ++q) HIP_CHECK(c, hipStreamWaitEvent(c->sideStreams[q], c->sideEvents[0], 0));
     for (int i = 0; i < np; ++i) {
+        hipStream_t st = c->sideStreams[i % kSide];
         const efem::TopPlan& T = plans[i];
         efem::dev::TopBuild b;
         b.Z = z64 + zoff[i];
@@ -1239,13 +1261,17 @@ int formBatchZ(efem_ctx* c, const std::vector<efem::SupernodalFactor>& P, const 
         b.sn = snd + snBase[i];
         b.rtl = rtld + rtlBase[i];
         b.tasks = td + taskBase[i];
-        for (const efem::TopPlan::Step& st : T.steps) {
-            if (st.kind == 0) efem::dev::launchTopGemm(b, st.first, st.count, c->stream);
-            else if (st.kind == 1) efem::dev::launchTopMirror(b, st.p0, st.p1, c->stream);
-            else efem::dev::launchTopDiag(b, st.first, st.count, c->stream);
+        for (const efem::TopPlan::Step& sp : T.steps) {
+            if (sp.kind == 0) efem::dev::launchTopGemm(b, sp.first, sp.count, st);
+            else if (sp.kind == 1) efem::dev::launchTopMirror(b, sp.p0, sp.p1, st);
+            else efem::dev::launchTopDiag(b, sp.first, sp.count, st);
         }
     }
     HIP_CHECK(c, hipGetLastError());
+    for (int q = 0; q < kSide; ++q) {
+        HIP_CHECK(c, hipEventRecord(c->sideEvents[1 + q], c->sideStreams[q]));
+        HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->sideEvents[1 + q], 0));
+    }
     if (fac32(c)) {
         float* z32 = nullptr;
         if ((rc = devAlloc(c, &z32, static_cast<size_t>(tot)))) return rc;
@@ -1644,6 +1670,11 @@ int efem_destroy(efem_ctx* c)
     if (c->hostScal) (void)hipHostFree(c->hostScal);
     for (hipEvent_t e : c->trialEv)
         if (e) (void)hipEventDestroy(e);
+    for (hipStream_t q : c->sideStreams) {
+        (void)hipStreamSynchronize(q);
+        (void)hipStreamDestroy(q);
+    }
+    for (hipEvent_t e : c->sideEvents) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return EFEM_OK;
