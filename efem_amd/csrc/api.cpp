@@ -179,10 +179,6 @@ struct efem_ctx {
     // most kBatchDenseMaxN free nodes), formed on the device at each
     // factorisation; 0: the level-scheduled sweeps per instance
     bool batchDense = true;
-    // side streams (default priority) and events of the batch Z formation:
-    // the instances' formations are independent, so they run side by side
-    std::vector<hipStream_t> sideStreams;
-    std::vector<hipEvent_t> sideEvents;  // [0]: inputs ready, then one per side stream
     efem::dev::PatchArgs patch;  // device arrays (patch.part null: no plan)
     int *pRptr = nullptr, *pFptr = nullptr, *pFslot = nullptr;
     int pRec = 0;
@@ -990,7 +986,8 @@ int uploadFactor(efem_ctx* c, efem::SupernodalFactor& F, const std::vector<int>*
             o.w = t.w;
             o.m = t.m;
             o.rOff = t.rOff;
-            o.pad = 0;
+            o.ldz = top.ld;
+            o.zoff = 0;
             if (devX) {
                 o.ld = efem::dev::ldOf(t.m);
                 o.boff = dval[t.s];
@@ -1191,7 +1188,7 @@ int formBatchZ(efem_ctx* c, const std::vector<efem::SupernodalFactor>& P, const 
     std::vector<long long> zoff(np);
     std::vector<int> zld(np), ztcol(static_cast<size_t>(F.n));
     std::vector<efem::dev::TopSuper> sn;
-    std::vector<int> rtl, snBase(np), rtlBase(np), taskBase(np);
+    std::vector<int> rtl, snBase(np), rtlBase(np);
     std::vector<int4> tasks;
     long long tot = 0;
     for (int i = 0; i < np; ++i) {
@@ -1203,7 +1200,6 @@ int formBatchZ(efem_ctx* c, const std::vector<efem::SupernodalFactor>& P, const 
         for (int t = 0; t < T.nT; ++t) ztcol[static_cast<size_t>(col) + t] = col + T.tcol[t];
         snBase[i] = static_cast<int>(sn.size());
         rtlBase[i] = static_cast<int>(rtl.size());
-        taskBase[i] = static_cast<int>(tasks.size());
         for (const efem::TopSuperInfo& t : T.super) {
             efem::dev::TopSuper o;
             o.cs = t.cs;
@@ -1211,67 +1207,86 @@ int formBatchZ(efem_ctx* c, const std::vector<efem::SupernodalFactor>& P, const 
             o.m = t.m;
             o.ld = t.m;
             o.boff = F.valPtr[snOffset[i] + t.s];
-            o.rOff = t.rOff;
-            o.pad = 0;
+            o.rOff = rtlBase[i] + t.rOff;
+            o.ldz = T.ld;
+            o.zoff = zoff[i];
             sn.push_back(o);
         }
         rtl.insert(rtl.end(), T.rtl.begin(), T.rtl.end());
-        for (size_t q = 0; q + 3 < T.tasks.size(); q += 4)
-            tasks.push_back(make_int4(T.tasks[q], T.tasks[q + 1], T.tasks[q + 2], T.tasks[q + 3]));
+    }
+    // step j of every tree in one launch: the trees' step sequences follow
+    // one pattern by depth (their kinds agree step by step); tile tasks with
+    // global supernode indices, mirror steps as tile tasks
+    size_t nsteps = 0;
+    for (const efem::TopPlan& T : plans) nsteps = std::max(nsteps, T.steps.size());
+    struct Merged {
+        int kind, first, count;
+    };
+    std::vector<Merged> merged;
+    std::vector<efem::dev::TopMirrorTask> mtasks;
+    for (size_t j = 0; j < nsteps; ++j) {
+        int kind = -1;
+        for (const efem::TopPlan& T : plans)
+            if (j < T.steps.size()) {
+                if (kind >= 0 && T.steps[j].kind != kind) return EFEM_OK;  // the level path stays
+                kind = T.steps[j].kind;
+            }
+        Merged mg{kind, 0, 0};
+        if (kind == 1) {
+            mg.first = static_cast<int>(mtasks.size());
+            for (int i = 0; i < np; ++i) {
+                if (j >= plans[i].steps.size()) continue;
+                const int p0 = plans[i].steps[j].p0, p1 = plans[i].steps[j].p1;
+                if (p0 <= 0 || p1 <= p0) continue;
+                constexpr int TB = efem::dev::kTopTile;
+                for (int i0 = 0; i0 < p0; i0 += TB)
+                    for (int j0 = p0; j0 < p1; j0 += TB)
+                        mtasks.push_back(efem::dev::TopMirrorTask{zoff[i], plans[i].ld, p0, p1, i0, j0, 0});
+            }
+            mg.count = static_cast<int>(mtasks.size()) - mg.first;
+        } else {
+            mg.first = static_cast<int>(tasks.size());
+            for (int i = 0; i < np; ++i) {
+                if (j >= plans[i].steps.size()) continue;
+                const efem::TopPlan::Step& st = plans[i].steps[j];
+                for (int q = st.first; q < st.first + st.count; ++q)
+                    tasks.push_back(make_int4(plans[i].tasks[4 * q] + snBase[i], plans[i].tasks[4 * q + 1],
+                                              plans[i].tasks[4 * q + 2], plans[i].tasks[4 * q + 3]));
+            }
+            mg.count = static_cast<int>(tasks.size()) - mg.first;
+        }
+        merged.push_back(mg);
     }
     int rc;
     double *b64 = nullptr, *z64 = nullptr;
     efem::dev::TopSuper* snd = nullptr;
     int *rtld = nullptr, *zldd = nullptr, *ztc = nullptr;
     int4* td = nullptr;
+    efem::dev::TopMirrorTask* md = nullptr;
     long long* zo = nullptr;
     if ((rc = devAlloc(c, &b64, static_cast<size_t>(std::max(F.Bsize, 1LL)))) || (rc = upload(c, &snd, sn)) ||
-        (rc = upload(c, &rtld, rtl)) || (rc = upload(c, &td, tasks)) || (rc = devAlloc(c, &z64, static_cast<size_t>(tot))) ||
+        (rc = upload(c, &rtld, rtl)) || (rc = upload(c, &td, tasks)) || (rc = upload(c, &md, mtasks)) ||
+        (rc = devAlloc(c, &z64, static_cast<size_t>(tot))) ||
         (rc = upload(c, &zo, zoff)) || (rc = upload(c, &zldd, zld)) || (rc = upload(c, &ztc, ztcol)))
         return rc;
     HIP_CHECK(c, hipMemcpyAsync(b64, F.B.get(), static_cast<size_t>(F.Bsize) * sizeof(double), hipMemcpyHostToDevice,
                                 c->stream));
     // padding rows stay zero (the dense apply reads whole 16-byte groups)
     HIP_CHECK(c, hipMemsetAsync(z64, 0, static_cast<size_t>(tot) * sizeof(double), c->stream));
-    // instance i forms on side stream i mod kSide, after the inputs above;
-    // the context stream waits for all of them
-    constexpr int kSide = 8;
-    if (c->sideStreams.empty()) {
-        for (int q = 0; q < kSide; ++q) {
-            hipStream_t st = nullptr;
-            HIP_CHECK(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            c->sideStreams.push_back(st);
-        }
-        for (int q = 0; q <= kSide; ++q) {
-            hipEvent_t e = nullptr;
-            HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            c->sideEvents.push_back(e);
-        }
-    }
-    HIP_CHECK(c, hipEventRecord(c->sideEvents[0], c->stream));
-    for (int q = 0; q < kSide; ++q) HIP_CHECK(c, hipStreamWaitEvent(c->sideStreams[q], c->sideEvents[0], 0));
-    for (int i = 0; i < np; ++i) {
-        hipStream_t st = c->sideStreams[i % kSide];
-        const efem::TopPlan& T = plans[i];
-        efem::dev::TopBuild b;
-        b.Z = z64 + zoff[i];
-        b.nT = T.nT;
-        b.ldz = T.ld;
-        b.B = b64;
-        b.sn = snd + snBase[i];
-        b.rtl = rtld + rtlBase[i];
-        b.tasks = td + taskBase[i];
-        for (const efem::TopPlan::Step& sp : T.steps) {
-            if (sp.kind == 0) efem::dev::launchTopGemm(b, sp.first, sp.count, st);
-            else if (sp.kind == 1) efem::dev::launchTopMirror(b, sp.p0, sp.p1, st);
-            else efem::dev::launchTopDiag(b, sp.first, sp.count, st);
-        }
+    efem::dev::TopBuild b;
+    b.Z = z64;
+    b.nT = 0;
+    b.ldz = 0;
+    b.B = b64;
+    b.sn = snd;
+    b.rtl = rtld;
+    b.tasks = td;
+    for (const Merged& mg : merged) {
+        if (mg.kind == 0) efem::dev::launchTopGemm(b, mg.first, mg.count, c->stream);
+        else if (mg.kind == 1) efem::dev::launchTopMirrorTasks(z64, md + mg.first, mg.count, c->stream);
+        else efem::dev::launchTopDiag(b, mg.first, mg.count, c->stream);
     }
     HIP_CHECK(c, hipGetLastError());
-    for (int q = 0; q < kSide; ++q) {
-        HIP_CHECK(c, hipEventRecord(c->sideEvents[1 + q], c->sideStreams[q]));
-        HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->sideEvents[1 + q], 0));
-    }
     if (fac32(c)) {
         float* z32 = nullptr;
         if ((rc = devAlloc(c, &z32, static_cast<size_t>(tot)))) return rc;
@@ -1670,11 +1685,6 @@ int efem_destroy(efem_ctx* c)
     if (c->hostScal) (void)hipHostFree(c->hostScal);
     for (hipEvent_t e : c->trialEv)
         if (e) (void)hipEventDestroy(e);
-    for (hipStream_t q : c->sideStreams) {
-        (void)hipStreamSynchronize(q);
-        (void)hipStreamDestroy(q);
-    }
-    for (hipEvent_t e : c->sideEvents) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return EFEM_OK;

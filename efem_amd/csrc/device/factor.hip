@@ -414,6 +414,7 @@ __global__ __launch_bounds__(256) void kTopGemm(TopBuild b, int first)
     __shared__ double As[TB][TB + 1], Bs[TB][TB + 1];
     const int4 t4 = b.tasks[first + blockIdx.x];
     const TopSuper sn = b.sn[t4.x];
+    double* const Zs = b.Z + sn.zoff;
     const int r0 = t4.y, j0 = t4.z;
     const int nr = sn.m - sn.w;
     const int arows = min(TB, t4.w - r0), bcols = min(TB, sn.w - j0);
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(256) void kTopGemm(TopBuild b, int first)
         __syncthreads();
         for (int e = tid; e < TB * TB; e += 256) {
             const int i = e % TB, j = e / TB;
-            As[i][j] = (i < arows && j < kc) ? b.Z[(r0 + i) + static_cast<size_t>(rt[k0 + j]) * b.ldz] : 0.0;
+            As[i][j] = (i < arows && j < kc) ? Zs[(r0 + i) + static_cast<size_t>(rt[k0 + j]) * sn.ldz] : 0.0;
             Bs[j][i] = (i < kc && j < bcols) ? B21[(k0 + i) + static_cast<size_t>(j0 + j) * sn.ld] : 0.0;
         }
         __syncthreads();
@@ -449,11 +450,28 @@ __global__ __launch_bounds__(256) void kTopGemm(TopBuild b, int first)
         for (int q = 0; q < 4; ++q) {
             const int i = 4 * tr + p, j = 4 * tc + q;
             if (i < arows && j < bcols)
-                b.Z[(r0 + i) + static_cast<size_t>(sn.cs + j0 + j) * b.ldz] = -acc.v[p][q];
+                Zs[(r0 + i) + static_cast<size_t>(sn.cs + j0 + j) * sn.ldz] = -acc.v[p][q];
         }
 }
 
 // Z[j, i] = Z[i, j] for i < p0 <= j < p1 (64 x 64 tiles through LDS)
+// one 64 x 64 tile of a mirror step per block, the trees' steps together
+__global__ __launch_bounds__(256) void kTopMirrorTasks(double* Z, const TopMirrorTask* __restrict__ tasks)
+{
+    __shared__ double t[TB][TB + 1];
+    const TopMirrorTask m = tasks[blockIdx.x];
+    double* const Zt = Z + m.zoff;
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int i = e % TB, j = e / TB;
+        if (m.i0 + i < m.p0 && m.j0 + j < m.p1) t[i][j] = Zt[(m.i0 + i) + static_cast<size_t>(m.j0 + j) * m.ldz];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int j = e % TB, i = e / TB;
+        if (m.i0 + i < m.p0 && m.j0 + j < m.p1) Zt[(m.j0 + j) + static_cast<size_t>(m.i0 + i) * m.ldz] = t[i][j];
+    }
+}
+
 __global__ __launch_bounds__(256) void kTopMirror(TopBuild b, int p0, int p1)
 {
     __shared__ double t[TB][TB + 1];
@@ -478,6 +496,7 @@ __global__ __launch_bounds__(256) void kTopDiag(TopBuild b, int first)
     __shared__ double As[TB][TB + 1], Vs[TB][TB + 1];
     const int4 t4 = b.tasks[first + blockIdx.x];
     const TopSuper sn = b.sn[t4.x];
+    double* const Zs = b.Z + sn.zoff;
     const int i0 = t4.y, j0 = t4.z;
     const int ni = min(TB, sn.w - i0), nj = min(TB, sn.w - j0);
     const int* rt = b.rtl + sn.rOff;
@@ -493,7 +512,7 @@ __global__ __launch_bounds__(256) void kTopDiag(TopBuild b, int first)
             double v = 0.0;
             if (kk < kc && c < nj)
                 v = r < sn.w ? Bp[r + static_cast<size_t>(j0 + c) * sn.ld]
-                             : -b.Z[rt[r - sn.w] + static_cast<size_t>(sn.cs + j0 + c) * b.ldz];
+                             : -Zs[rt[r - sn.w] + static_cast<size_t>(sn.cs + j0 + c) * sn.ldz];
             Vs[c][kk] = v;
         }
         __syncthreads();
@@ -516,7 +535,7 @@ __global__ __launch_bounds__(256) void kTopDiag(TopBuild b, int first)
         for (int q = 0; q < 4; ++q) {
             const int i = 4 * tr + p, j = 4 * tc + q;
             if (i < ni && j < nj)
-                b.Z[(sn.cs + i0 + i) + static_cast<size_t>(sn.cs + j0 + j) * b.ldz] = acc.v[p][q];
+                Zs[(sn.cs + i0 + i) + static_cast<size_t>(sn.cs + j0 + j) * sn.ldz] = acc.v[p][q];
         }
 }
 
@@ -530,6 +549,11 @@ void launchTopGemm(const TopBuild& b, int first, int count, hipStream_t s)
 void launchTopMirror(const TopBuild& b, int p0, int p1, hipStream_t s)
 {
     if (p0 > 0 && p1 > p0) kTopMirror<<<dim3((p0 + TB - 1) / TB, (p1 - p0 + TB - 1) / TB), 256, 0, s>>>(b, p0, p1);
+}
+
+void launchTopMirrorTasks(double* Z, const TopMirrorTask* tasks, int count, hipStream_t s)
+{
+    if (count > 0) kTopMirrorTasks<<<count, 256, 0, s>>>(Z, tasks);
 }
 
 void launchTopDiag(const TopBuild& b, int first, int count, hipStream_t s)

@@ -320,8 +320,18 @@ void launchTrsvTop(const TrsvArgs& a, const double* bp, const TopArgs& t, hipStr
 struct TopSuper {
     int cs, w, m, ld;         // first T-local column, width, rows, block leading dimension
     long long boff;           // block offset in the source
-    int rOff, pad;            // rows below the columns as T-local indices: rtl[rOff .. rOff + m - w)
+    int rOff, ldz;            // rows below the columns as T-local indices: rtl[rOff .. rOff + m - w);
+                              // Z's leading dimension
+    long long zoff;           // Z of this supernode's tree at TopBuild::Z + zoff (0: one Z)
 };
+// one tile of a mirror step of several trees: Z[j, i] = Z[i, j] for rows i
+// < p0 and columns p0 <= j < p1 of the tree's Z at zoff (ld ldz)
+struct TopMirrorTask {
+    long long zoff;
+    int ldz, p0, p1, i0, j0, pad;
+};
+void launchTopMirrorTasks(double* Z, const TopMirrorTask* tasks, int count, hipStream_t s);
+constexpr int kTopTile = 64;  // the top kernels' tile
 struct TopBuild {
     double* Z;
     int nT, ldz;
